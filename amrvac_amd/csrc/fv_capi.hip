@@ -1,0 +1,591 @@
+// fv_capi.hip -- the C-ABI of include/amrvac_fv.h: device-resident level
+// state, kernel selection, and the auxiliary device kernels (CFL dt
+// reduction, volume averages, ghost-cell fill for download).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/amrvac_fv.h"
+#include "fv_launch.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(AMRVAC_FV_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+constexpr double kBig = 1.0e99;     // src/mod_constants.t:11
+constexpr double kSmall = 1.0e-12;  // src/mod_constants.t:8
+
+}  // namespace
+
+struct amrvac_fv_ctx {
+  amrvac_fv_config cfg{};
+  int nw = 0;
+  int nnb = 0;
+  int nb = 0;
+  int bnz = 1;
+  size_t cells = 0;  // interior cells per block
+  size_t cells_g = 0;  // ghosted cells per block
+  int nxg[3] = {1, 1, 1};
+  double dx[3] = {1, 1, 1};
+  int bc[6] = {5, 5, 5, 5, 5, 5};
+  double* slot[3] = {nullptr, nullptr, nullptr};
+  int* d_nbr = nullptr;
+  int* d_crash = nullptr;
+  double* d_red = nullptr;    // per-block reduction results
+  double* d_red2 = nullptr;   // final reduction
+  double* d_stage = nullptr;  // ghosted staging for download
+  std::vector<double> h_stage;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  fvc::FVParams fp{};
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  size_t nev = 0;
+};
+
+namespace fv {
+
+template <class P>
+__global__ void __launch_bounds__(NTHREADS) getdt_kernel(const double* __restrict__ w, int nblocks, int bnz,
+                                                         FVParams fp, double* out) {
+  const int b = blockIdx.x;
+  if (b >= nblocks) return;
+  const size_t cells = (size_t)BX * BX * bnz;
+  const double* wb = w + (size_t)b * P::NW * cells;
+  double maxtot = -1.0e99, cmaxb = 0.0;
+  for (size_t c = threadIdx.x; c < cells; c += NTHREADS) {
+    double cons[P::NW], pr[P::NW];
+#pragma unroll
+    for (int iw = 0; iw < P::NW; ++iw) cons[iw] = wb[iw * cells + c];
+    to_prim<P>(cons, pr, fp);
+    // getdt_courant, maxsum, slab_uniform (src/mod_dt.t:203-221)
+    double cm = cell_cmax<P, 0>(pr, fp);
+    cmaxb = fmax(cmaxb, cm);
+    double tot = cm * fp.dxinv[0];
+    if constexpr (P::NDIM > 1) {
+      cm = cell_cmax<P, 1>(pr, fp);
+      cmaxb = fmax(cmaxb, cm);
+      tot = tot + cm * fp.dxinv[1];
+    }
+    if constexpr (P::NDIM > 2) {
+      cm = cell_cmax<P, 2>(pr, fp);
+      cmaxb = fmax(cmaxb, cm);
+      tot = tot + cm * fp.dxinv[2];
+    }
+    maxtot = fmax(maxtot, tot);
+  }
+  // wave64 shuffle max, then across the 4 waves through LDS
+  for (int off = 32; off > 0; off >>= 1) {
+    maxtot = fmax(maxtot, __shfl_xor(maxtot, off, 64));
+    cmaxb = fmax(cmaxb, __shfl_xor(cmaxb, off, 64));
+  }
+  __shared__ double s[2][NTHREADS / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { s[0][wv] = maxtot; s[1][wv] = cmaxb; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = s[0][0], c = s[1][0];
+    for (int i = 1; i < NTHREADS / 64; ++i) { m = fmax(m, s[0][i]); c = fmax(c, s[1][i]); }
+    out[2 * b] = m;
+    out[2 * b + 1] = c;
+  }
+}
+
+__global__ void __launch_bounds__(NTHREADS) reduce_max2_kernel(const double* __restrict__ in, int n,
+                                                               double* out) {
+  double m = -1.0e99, c = 0.0;
+  for (int i = threadIdx.x; i < n; i += NTHREADS) {
+    m = fmax(m, in[2 * i]);
+    c = fmax(c, in[2 * i + 1]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    m = fmax(m, __shfl_xor(m, off, 64));
+    c = fmax(c, __shfl_xor(c, off, 64));
+  }
+  __shared__ double s[2][NTHREADS / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { s[0][wv] = m; s[1][wv] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < NTHREADS / 64; ++i) { s[0][0] = fmax(s[0][0], s[0][i]); s[1][0] = fmax(s[1][0], s[1][i]); }
+    out[0] = s[0][0];
+    out[1] = s[1][0];
+  }
+}
+
+// per-block sums of w and w^2 over the interior, [nb][2][nw]
+__global__ void __launch_bounds__(NTHREADS) volsum_kernel(const double* __restrict__ w, int nblocks, int nw,
+                                                          size_t cells, double* out) {
+  const int b = blockIdx.x;
+  if (b >= nblocks) return;
+  __shared__ double s[NTHREADS / 64];
+  for (int iw = 0; iw < nw; ++iw) {
+    const double* q = w + ((size_t)b * nw + iw) * cells;
+    for (int p = 1; p <= 2; ++p) {
+      double acc = 0.0;
+      for (size_t c = threadIdx.x; c < cells; c += NTHREADS) {
+        const double v = q[c];
+        acc += p == 1 ? v : v * v;
+      }
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+      const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+      if (lane == 0) s[wv] = acc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < NTHREADS / 64; ++i) t += s[i];
+        out[((size_t)b * 2 + (p - 1)) * nw + iw] = t;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ghost-cell fill for download: the ghosted reference layout of every block,
+// same-level copies and physical boundaries as getbc
+// (src/mod_ghostcells_update.t:350-566, src/mod_boundary_conditions.t:12-120)
+template <int NDIM>
+__global__ void __launch_bounds__(NTHREADS) fill_ghosted_kernel(SweepArgs a, int nw, int ng, int nxg0, int nxg1,
+                                                                int nxg2, double* out) {
+  const int b = blockIdx.y;
+  const size_t cg = (size_t)nxg0 * nxg1 * nxg2;
+  const size_t cells = (size_t)BX * BX * a.bnz;
+  for (size_t c = blockIdx.x * (size_t)NTHREADS + threadIdx.x; c < cg; c += (size_t)gridDim.x * NTHREADS) {
+    int i = int(c % nxg0);
+    int j = int((c / nxg0) % nxg1);
+    int k = int(c / ((size_t)nxg0 * nxg1));
+    int x = i - ng, y = NDIM > 1 ? j - ng : 0, z = NDIM > 2 ? k - ng : 0;
+    double sign;
+    const int sb = locate<NDIM>(a, b, x, y, z, sign);
+    const size_t so = (size_t)sb * nw * cells + ((size_t)z * BX + y) * BX + x;
+    for (int iw = 0; iw < nw; ++iw) out[((size_t)b * nw + iw) * cg + c] = sign * a.src[so + iw * cells];
+  }
+}
+
+template <class P>
+hipError_t launch_getdt(const double* w, int nb, int bnz, const FVParams& fp, double* out, hipStream_t s) {
+  hipLaunchKernelGGL((getdt_kernel<P>), dim3(nb), dim3(NTHREADS), 0, s, w, nb, bnz, fp, out);
+  return hipGetLastError();
+}
+
+}  // namespace fv
+
+namespace {
+
+fvc::KernelKey make_key(const amrvac_fv_ctx* c, int riem) {
+  fvc::KernelKey k{};
+  k.ndim = c->cfg.ndim;
+  k.physics = c->cfg.physics;
+  k.ndir = c->cfg.ndir;
+  k.divb = c->cfg.physics == 2 ? c->cfg.divb : 0;
+  k.mixed = c->cfg.precision == AMRVAC_FV_PREC_MIXED ? 1 : 0;
+  k.riem = riem;
+  k.weno = c->cfg.limiter == 16 ? 1 : 0;
+  k.limiter = c->cfg.limiter;
+  return k;
+}
+
+hipError_t dispatch_sweep(const amrvac_fv_ctx* c, const fvc::KernelKey& k, const fvc::SweepArgs& a, int grid) {
+  const bool exact = c->cfg.precision == 2;
+  if (k.ndim == 2 && k.physics == 1)
+    return exact ? fvx::launch_sweep_2d_hd(k, a, c->fp, grid, c->stream)
+                 : fv::launch_sweep_2d_hd(k, a, c->fp, grid, c->stream);
+  if (k.ndim == 2 && k.physics == 2)
+    return exact ? fvx::launch_sweep_2d_mhd(k, a, c->fp, grid, c->stream)
+                 : fv::launch_sweep_2d_mhd(k, a, c->fp, grid, c->stream);
+  if (k.ndim == 3 && k.physics == 1)
+    return exact ? fvx::launch_sweep_3d_hd(k, a, c->fp, grid, c->stream)
+                 : fv::launch_sweep_3d_hd(k, a, c->fp, grid, c->stream);
+  if (k.ndim == 3 && k.physics == 2)
+    return exact ? fvx::launch_sweep_3d_mhd(k, a, c->fp, grid, c->stream)
+                 : fv::launch_sweep_3d_mhd(k, a, c->fp, grid, c->stream);
+  return hipErrorInvalidValue;
+}
+
+template <template <class> class F, class... Args>
+hipError_t with_phys(const amrvac_fv_ctx* c, Args... args) {
+  using namespace fv;
+  const int nd = c->cfg.ndim, ph = c->cfg.physics, nr = c->cfg.ndir, db = c->cfg.divb;
+  if (nd == 2 && ph == 1 && nr == 2) return F<Phys<2, 2, false, DIVB_NONE>>::run(args...);
+  if (nd == 2 && ph == 2 && nr == 3 && db == DIVB_GLM) return F<Phys<2, 3, true, DIVB_GLM>>::run(args...);
+  if (nd == 2 && ph == 2 && nr == 3 && db == DIVB_LINDE) return F<Phys<2, 3, true, DIVB_LINDE>>::run(args...);
+  if (nd == 2 && ph == 2 && nr == 3 && db == DIVB_NONE) return F<Phys<2, 3, true, DIVB_NONE>>::run(args...);
+  if (nd == 3 && ph == 1 && nr == 3) return F<Phys<3, 3, false, DIVB_NONE>>::run(args...);
+  if (nd == 3 && ph == 2 && nr == 3 && db == DIVB_GLM) return F<Phys<3, 3, true, DIVB_GLM>>::run(args...);
+  if (nd == 3 && ph == 2 && nr == 3 && db == DIVB_NONE) return F<Phys<3, 3, true, DIVB_NONE>>::run(args...);
+  return hipErrorInvalidValue;
+}
+
+template <class P>
+struct GetdtOp {
+  static hipError_t run(const double* w, int nb, int bnz, const fvc::FVParams* fp, double* out, hipStream_t s) {
+    return fv::launch_getdt<P>(w, nb, bnz, *fp, out, s);
+  }
+};
+
+bool check_slot(int s) { return s >= 0 && s <= 2; }
+
+}  // namespace
+
+extern "C" {
+
+const char* amrvac_fv_last_error(void) { return g_err.c_str(); }
+
+int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
+  if (!cfg || !out) return fail(AMRVAC_FV_EINVAL, "null argument");
+  const amrvac_fv_config& c = *cfg;
+  if (c.ndim < 2 || c.ndim > 3) return fail(AMRVAC_FV_EINVAL, "ndim must be 2 or 3");
+  if (c.ndir < c.ndim || c.ndir > 3) return fail(AMRVAC_FV_EINVAL, "need ndim <= ndir <= 3");
+  if (c.physics != 1 && c.physics != 2) return fail(AMRVAC_FV_EINVAL, "physics must be 1 (hd) or 2 (mhd)");
+  if (c.physics == 1 && c.ndir != c.ndim) return fail(AMRVAC_FV_EINVAL, "hd: ndir must equal ndim");
+  if (c.physics == 2 && c.ndir != 3) return fail(AMRVAC_FV_EINVAL, "mhd: ndir must be 3");
+  if (c.physics == 2 && c.divb != 0 && c.divb != 1 && c.divb != 4)
+    return fail(AMRVAC_FV_EINVAL, "typedivbfix must be none (0), glm (1) or linde (4)");
+  if (c.physics == 2 && c.divb == 4 && c.ndim != 2)
+    return fail(AMRVAC_FV_EINVAL, "Linde div B cleaning is supported for ndim = 2 only");
+  if (c.block_nx[0] != 16 || c.block_nx[1] != 16)
+    return fail(AMRVAC_FV_EINVAL, "block_nx(1) and block_nx(2) must be 16");
+  if (c.ndim == 3 && c.block_nx[2] < 4) return fail(AMRVAC_FV_EINVAL, "block_nx(3) must be at least 4");
+  const bool weno = c.limiter == 16;
+  if (!weno && !(c.limiter >= 1 && c.limiter <= 7) && c.limiter != 9)
+    return fail(AMRVAC_FV_EINVAL, "unsupported limiter code");
+  if (c.nghost != (weno ? 3 : 2))
+    return fail(AMRVAC_FV_EINVAL, weno ? "weno5 needs nghostcells = 3" : "nghostcells must be 2");
+  if (c.boundspeed < 1 || c.boundspeed > 3) return fail(AMRVAC_FV_EINVAL, "boundspeed must be 1, 2 or 3");
+  if (c.precision < 0 || c.precision > 2) return fail(AMRVAC_FV_EINVAL, "precision must be 0, 1 or 2");
+  if (c.gamma <= 0.0 || c.gamma == 1.0) return fail(AMRVAC_FV_EINVAL, "gamma must be > 0 and != 1");
+  amrvac_fv_ctx* x = new amrvac_fv_ctx();
+  x->cfg = c;
+  x->nw = c.physics == 1 ? c.ndir + 2 : 2 * c.ndir + 2 + (c.divb == 1 ? 1 : 0);
+  x->nnb = c.ndim == 2 ? 9 : 27;
+  x->bnz = c.ndim == 3 ? c.block_nx[2] : 1;
+  x->cells = (size_t)16 * 16 * x->bnz;
+  for (int d = 0; d < 3; ++d) x->nxg[d] = d < c.ndim ? c.block_nx[d] + 2 * c.nghost : 1;
+  x->cells_g = (size_t)x->nxg[0] * x->nxg[1] * x->nxg[2];
+  hipError_t e = hipStreamCreate(&x->stream);
+  if (e != hipSuccess) {
+    delete x;
+    return fail(AMRVAC_FV_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  x->own_stream = true;
+  *out = x;
+  return AMRVAC_FV_OK;
+}
+
+static void free_level(amrvac_fv_ctx* c) {
+  for (int s = 0; s < 3; ++s) { if (c->slot[s]) (void)hipFree(c->slot[s]); c->slot[s] = nullptr; }
+  if (c->d_nbr) (void)hipFree(c->d_nbr);
+  if (c->d_crash) (void)hipFree(c->d_crash);
+  if (c->d_red) (void)hipFree(c->d_red);
+  if (c->d_red2) (void)hipFree(c->d_red2);
+  if (c->d_stage) (void)hipFree(c->d_stage);
+  c->d_nbr = nullptr; c->d_crash = nullptr; c->d_red = nullptr; c->d_red2 = nullptr; c->d_stage = nullptr;
+}
+
+int amrvac_fv_finalize(amrvac_fv_ctx* c) {
+  if (!c) return AMRVAC_FV_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_level(c);
+  for (auto& ev : c->events) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_set_stream(amrvac_fv_ctx* c, void* stream) {
+  if (!c) return fail(AMRVAC_FV_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (stream) {
+    if (c->own_stream) HIPCHK(hipStreamDestroy(c->stream));
+    c->stream = (hipStream_t)stream;
+    c->own_stream = false;
+  } else if (!c->own_stream) {
+    HIPCHK(hipStreamCreate(&c->stream));
+    c->own_stream = true;
+  }
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_set_mesh(amrvac_fv_ctx* c, int nblocks, const double* dx, const int* neighbor,
+                       const double* xlohi, const int* bc) {
+  if (!c || !dx || !neighbor || !bc) return fail(AMRVAC_FV_EINVAL, "null argument");
+  (void)xlohi;
+  if (nblocks < 1) return fail(AMRVAC_FV_EINVAL, "nblocks must be positive");
+  const int nd = c->cfg.ndim;
+  for (int i = 0; i < nblocks * c->nnb; ++i)
+    if (neighbor[i] >= nblocks) return fail(AMRVAC_FV_EINVAL, "neighbour index out of range");
+  for (int b = 0; b < nblocks; ++b)
+    if (neighbor[b * c->nnb + (c->nnb - 1) / 2] != b)
+      return fail(AMRVAC_FV_EINVAL, "neighbour table centre entry must be the block itself");
+  for (int s = 0; s < 2 * nd; ++s)
+    if (bc[s] != 2 && bc[s] != 3 && bc[s] != 4 && bc[s] != 5)
+      return fail(AMRVAC_FV_EINVAL, "boundary type must be cont, symm, asymm or periodic");
+  for (int d = 0; d < nd; ++d)
+    if (!(dx[d] > 0.0)) return fail(AMRVAC_FV_EINVAL, "dx must be positive");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  free_level(c);
+  c->nb = nblocks;
+  for (int d = 0; d < 3; ++d) c->dx[d] = d < nd ? dx[d] : 1.0;
+  for (int s = 0; s < 6; ++s) c->bc[s] = s < 2 * nd ? bc[s] : 5;
+  const size_t n = (size_t)nblocks * c->nw * c->cells;
+  for (int s = 0; s < 3; ++s) {
+    if (hipMalloc(&c->slot[s], n * sizeof(double)) != hipSuccess) {
+      free_level(c);
+      return fail(AMRVAC_FV_ENOMEM, "state slot allocation failed");
+    }
+    HIPCHK(hipMemsetAsync(c->slot[s], 0, n * sizeof(double), c->stream));
+  }
+  if (hipMalloc(&c->d_nbr, sizeof(int) * nblocks * c->nnb) != hipSuccess ||
+      hipMalloc(&c->d_crash, sizeof(int)) != hipSuccess ||
+      hipMalloc(&c->d_red, sizeof(double) * (size_t)nblocks * 2 * (c->nw > 1 ? c->nw : 1)) != hipSuccess ||
+      hipMalloc(&c->d_red2, sizeof(double) * 2) != hipSuccess ||
+      hipMalloc(&c->d_stage, sizeof(double) * (size_t)nblocks * c->nw * c->cells_g) != hipSuccess) {
+    free_level(c);
+    return fail(AMRVAC_FV_ENOMEM, "work buffer allocation failed");
+  }
+  HIPCHK(hipMemcpyAsync(c->d_nbr, neighbor, sizeof(int) * nblocks * c->nnb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->d_crash, 0, sizeof(int), c->stream));
+  // scheme constants that depend on the mesh
+  fvc::FVParams& fp = c->fp;
+  const amrvac_fv_config& g = c->cfg;
+  fp.gamma = g.gamma;
+  fp.gamma_1 = g.gamma - 1.0;
+  fp.inv_gamma_1 = 1.0 / (g.gamma - 1.0);
+  fp.glm_alpha = g.glm_alpha;
+  fp.glm_extended = g.glm_extended;
+  fp.divbdiff = g.divbdiff;
+  fp.limiter = g.limiter;
+  fp.boundspeed = g.boundspeed;
+  fp.tvdlfeps = g.tvdlfeps;
+  fp.small_pressure = g.small_pressure;
+  fp.small_density = g.small_density;
+  fp.check_small = g.check_small_values;
+  fp.invds2 = 0.0;
+  fp.mindx = c->dx[0];
+  for (int d = 0; d < 3; ++d) {
+    fp.dx[d] = c->dx[d];
+    fp.dxinv[d] = 1.0 / c->dx[d];
+    const double r = g.cada3_radius * c->dx[d];
+    fp.rdelinv[d] = 1.0 / (r * r);
+    if (d < nd) {
+      fp.invds2 = fp.invds2 + 1.0 / (c->dx[d] * c->dx[d]);
+      if (c->dx[d] < fp.mindx) fp.mindx = c->dx[d];
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_upload(amrvac_fv_ctx* c, int slot, const double* w) {
+  if (!c || !w || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  const int ng = c->cfg.nghost, nd = c->cfg.ndim;
+  const size_t n = (size_t)c->nb * c->nw * c->cells;
+  c->h_stage.resize(n);
+  for (int b = 0; b < c->nb; ++b)
+    for (int iw = 0; iw < c->nw; ++iw) {
+      const double* src = w + ((size_t)b * c->nw + iw) * c->cells_g;
+      double* dst = c->h_stage.data() + ((size_t)b * c->nw + iw) * c->cells;
+      for (int k = 0; k < c->bnz; ++k)
+        for (int j = 0; j < 16; ++j) {
+          const int kg = nd > 2 ? k + ng : 0;
+          const double* row = src + ((size_t)kg * c->nxg[1] + (j + ng)) * c->nxg[0] + ng;
+          std::memcpy(dst + ((size_t)k * 16 + j) * 16, row, 16 * sizeof(double));
+        }
+    }
+  HIPCHK(hipMemcpyAsync(c->slot[slot], c->h_stage.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMRVAC_FV_OK;
+}
+
+static fvc::SweepArgs base_args(const amrvac_fv_ctx* c) {
+  fvc::SweepArgs a{};
+  a.nbr = c->d_nbr;
+  a.nblocks = c->nb;
+  a.bnz = c->bnz;
+  a.per_xcd = (c->nb + 7) / 8;
+  for (int s = 0; s < 6; ++s) a.bc[s] = c->bc[s];
+  a.crash = c->d_crash;
+  return a;
+}
+
+int amrvac_fv_download(amrvac_fv_ctx* c, int slot, double* w) {
+  if (!c || !w || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  if (c->nb > 65535) return fail(AMRVAC_FV_EINVAL, "download supports at most 65535 blocks");
+  fvc::SweepArgs a = base_args(c);
+  a.src = c->slot[slot];
+  const int blocks_x = (int)((c->cells_g + fvc::NTHREADS - 1) / fvc::NTHREADS);
+  dim3 grid(blocks_x, c->nb);
+  if (c->cfg.ndim == 2)
+    hipLaunchKernelGGL((fv::fill_ghosted_kernel<2>), grid, dim3(fvc::NTHREADS), 0, c->stream, a, c->nw,
+                       c->cfg.nghost, c->nxg[0], c->nxg[1], c->nxg[2], c->d_stage);
+  else
+    hipLaunchKernelGGL((fv::fill_ghosted_kernel<3>), grid, dim3(fvc::NTHREADS), 0, c->stream, a, c->nw,
+                       c->cfg.nghost, c->nxg[0], c->nxg[1], c->nxg[2], c->d_stage);
+  HIPCHK(hipGetLastError());
+  const size_t n = (size_t)c->nb * c->nw * c->cells_g;
+  HIPCHK(hipMemcpyAsync(w, c->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_getdt(amrvac_fv_ctx* c, int slot, double courantpar, double* dtmin_out, double* cmax_out) {
+  if (!c || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  hipError_t e = with_phys<GetdtOp>(c, (const double*)c->slot[slot], c->nb, c->bnz,
+                                    (const fvc::FVParams*)&c->fp, c->d_red, c->stream);
+  if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("getdt launch: ") + hipGetErrorString(e));
+  hipLaunchKernelGGL(fv::reduce_max2_kernel, dim3(1), dim3(fvc::NTHREADS), 0, c->stream, c->d_red, c->nb, c->d_red2);
+  HIPCHK(hipGetLastError());
+  double r[2];
+  HIPCHK(hipMemcpyAsync(r, c->d_red2, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // min over blocks of courantpar/maxtot equals courantpar/max(maxtot): the
+  // correctly rounded quotient is monotone in the divisor
+  const double dt = r[0] > kSmall ? courantpar / r[0] : kBig;
+  if (dtmin_out) *dtmin_out = dt < kBig ? dt : kBig;
+  if (cmax_out) *cmax_out = r[1];
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_advect1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, double cmax_global, int psa,
+                      int psb, int comb_slot, double comb_a, double comb_b, int* crash_out) {
+  if (!c || !check_slot(psa) || !check_slot(psb) || psa == psb) return fail(AMRVAC_FV_EINVAL, "bad slots");
+  if (comb_slot >= 0 && !check_slot(comb_slot)) return fail(AMRVAC_FV_EINVAL, "bad combination slot");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  int riem;
+  switch (method) {
+    case 1: riem = fv::RIEM_HLL; break;
+    case 2: riem = fv::RIEM_HLLC; break;
+    case 5: riem = fv::RIEM_TVDLF; break;
+    case 8: riem = fv::RIEM_HANCOCK; break;
+    default: return fail(AMRVAC_FV_EINVAL, "unsupported flux scheme");
+  }
+  if (riem == fv::RIEM_HLLC && c->cfg.physics != 1) return fail(AMRVAC_FV_EINVAL, "hllc is provided for hd");
+  if (riem == fv::RIEM_HANCOCK && c->cfg.limiter == 16)
+    return fail(AMRVAC_FV_EINVAL, "the Hancock predictor takes a TVD limiter");
+  if (c->cfg.physics == 2 && c->cfg.divb == 1 && !(cmax_global > 0.0))
+    return fail(AMRVAC_FV_EINVAL, "GLM needs cmax_global > 0");
+  fvc::SweepArgs a = base_args(c);
+  a.src = c->slot[psa];
+  a.dst = c->slot[psb];
+  a.base = comb_slot >= 0 ? c->slot[comb_slot] : nullptr;
+  a.comb_mode = comb_slot < 0 ? 0 : (comb_b == 0.0 ? 1 : 2);
+  a.comb_a = comb_a;
+  a.comb_b = comb_b;
+  const double qdt = dtfactor * dt;
+  a.qdt = qdt;
+  for (int d = 0; d < 3; ++d) a.dq[d] = -qdt / c->dx[d];
+  a.ch = cmax_global;
+  // add_source_glm psi decay factor, src/mhd/mod_mhd_phys.t:5491-5495
+  a.glm_decay = std::exp(-qdt * cmax_global * c->cfg.glm_alpha / c->fp.mindx);
+  const fvc::KernelKey k = make_key(c, riem);
+  if (crash_out) HIPCHK(hipMemsetAsync(c->d_crash, 0, sizeof(int), c->stream));
+  const int grid = 8 * a.per_xcd;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    if (c->nev == c->events.size()) {
+      hipEvent_t x0, x1;
+      HIPCHK(hipEventCreate(&x0));
+      HIPCHK(hipEventCreate(&x1));
+      c->events.push_back({x0, x1});
+    }
+    e0 = c->events[c->nev].first;
+    e1 = c->events[c->nev].second;
+    ++c->nev;
+    HIPCHK(hipEventRecord(e0, c->stream));
+  }
+  hipError_t e = dispatch_sweep(c, k, a, grid);
+  if (e == hipErrorInvalidValue) return fail(AMRVAC_FV_EINVAL, "no stage kernel for this configuration");
+  if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("stage kernel: ") + hipGetErrorString(e));
+  if (c->timing) HIPCHK(hipEventRecord(e1, c->stream));
+  if (crash_out) {
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *crash_out = h;
+  }
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_copy(amrvac_fv_ctx* c, int dst, int src) {
+  if (!c || !check_slot(dst) || !check_slot(src)) return fail(AMRVAC_FV_EINVAL, "bad slots");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  if (dst == src) return AMRVAC_FV_OK;
+  HIPCHK(hipMemcpyAsync(c->slot[dst], c->slot[src], (size_t)c->nb * c->nw * c->cells * sizeof(double),
+                        hipMemcpyDeviceToDevice, c->stream));
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_volume_average(amrvac_fv_ctx* c, int slot, double* modes) {
+  if (!c || !modes || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  hipLaunchKernelGGL(fv::volsum_kernel, dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, c->slot[slot], c->nb,
+                     c->nw, c->cells, c->d_red);
+  HIPCHK(hipGetLastError());
+  std::vector<double> h((size_t)c->nb * 2 * c->nw);
+  HIPCHK(hipMemcpyAsync(h.data(), c->d_red, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // src/io/mod_input_output.t:2694-2729: volume-weighted sums, uniform cells
+  double dvol = 1.0;
+  for (int d = 0; d < c->cfg.ndim; ++d) dvol = dvol * c->dx[d];
+  const double vol = dvol * (double)c->cells * c->nb;
+  for (int p = 0; p < 2; ++p)
+    for (int iw = 0; iw < c->nw; ++iw) {
+      double s = 0.0;
+      for (int b = 0; b < c->nb; ++b) s += dvol * h[((size_t)b * 2 + p) * c->nw + iw];
+      modes[p * c->nw + iw] = s / vol;
+    }
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_slot_pointer(amrvac_fv_ctx* c, int slot, void** dptr, long long* nbytes) {
+  if (!c || !check_slot(slot) || !dptr) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  *dptr = c->slot[slot];
+  if (nbytes) *nbytes = (long long)((size_t)c->nb * c->nw * c->cells * sizeof(double));
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_sync(amrvac_fv_ctx* c) {
+  if (!c) return fail(AMRVAC_FV_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_sweep_timing(amrvac_fv_ctx* c, int reset, double* avg_ms, int* count) {
+  if (!c) return fail(AMRVAC_FV_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  for (size_t i = 0; i < c->nev; ++i) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c->events[i].first, c->events[i].second));
+    tot += ms;
+  }
+  if (avg_ms) *avg_ms = c->nev ? tot / (double)c->nev : 0.0;
+  if (count) *count = (int)c->nev;
+  if (reset) {
+    c->nev = 0;
+    c->timing = true;
+  }
+  return AMRVAC_FV_OK;
+}
+
+}  // extern "C"

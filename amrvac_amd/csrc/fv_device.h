@@ -1,0 +1,541 @@
+// fv_device.h -- device-side physics, limiters and Riemann solvers of the
+// finite-volume block update, templated on the physics (HD / ideal MHD with
+// GLM or Linde div B control), the direction of the face (compile-time, so
+// every per-variable array stays in registers) and the working precision R
+// of reconstruction / fluxes / wave speeds.
+//
+// Each routine restates the reference routine cited beside it; operand
+// order follows the reference's Fortran so that FP64 results agree with the
+// reference to rounding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#ifdef FV_EXACT
+#define FV_NS fvx
+#else
+#define FV_NS fv
+#endif
+
+namespace fvc {
+// scalar parameters of the scheme (kernel argument, by value)
+struct FVParams {
+  double gamma, gamma_1, inv_gamma_1;
+  double glm_alpha;
+  int glm_extended;
+  double divbdiff;
+  int limiter;
+  int boundspeed;
+  double tvdlfeps;
+  double small_pressure, small_density;
+  int check_small;
+  double dx[3];
+  double dxinv[3];     // 1/dx
+  double rdelinv[3];   // 1/(cada3_radius*dx)^2, src/limiter/mod_limiter.t:254
+  double invds2;       // sum_d 1/dx_d^2 (Linde)
+  double mindx;        // minval(dxlevel) (GLM decay)
+};
+
+}  // namespace fvc
+
+namespace FV_NS {
+using fvc::FVParams;
+
+// ---------------------------------------------------------------------------
+// physics traits: variable indices as in hd_phys_init / mhd_phys_init
+// (rho_, mom(:), e_, mag(:), psi_; src/hd/mod_hd_phys.t:217-228,
+// src/mhd/mod_mhd_phys.t:478-503)
+// ---------------------------------------------------------------------------
+enum { DIVB_NONE = 0, DIVB_GLM = 1, DIVB_LINDE = 4 };
+enum { RIEM_HLL = 1, RIEM_HLLC = 2, RIEM_TVDLF = 5, RIEM_HANCOCK = 8 };
+enum {
+  LIM_MINMOD = 1, LIM_WOODWARD = 2, LIM_MCBETA = 3, LIM_SUPERBEE = 4,
+  LIM_VANLEER = 5, LIM_ALBADA = 6, LIM_KOREN = 7, LIM_CADA3 = 9, LIM_WENO5 = 16
+};
+
+template <int NDIM_, int NDIR_, bool MHD_, int DIVB_>
+struct Phys {
+  static constexpr int NDIM = NDIM_;
+  static constexpr int NDIR = NDIR_;
+  static constexpr bool MHD = MHD_;
+  static constexpr bool GLM = MHD_ && DIVB_ == DIVB_GLM;
+  static constexpr bool LINDE = MHD_ && DIVB_ == DIVB_LINDE;
+  static constexpr int RHO = 0;
+  static constexpr int MOM = 1;
+  static constexpr int E = NDIR_ + 1;
+  static constexpr int MAG = NDIR_ + 2;
+  static constexpr int PSI = 2 * NDIR_ + 2;
+  static constexpr int NW = MHD_ ? (2 * NDIR_ + 2 + (GLM ? 1 : 0)) : (NDIR_ + 2);
+};
+
+// ---------------------------------------------------------------------------
+// conversions
+// ---------------------------------------------------------------------------
+
+// hd_to_primitive src/hd/mod_hd_phys.t:703-734,
+// mhd_to_primitive_origin src/mhd/mod_mhd_phys.t:1741-1765 (always FP64:
+// the pressure recovery from total energy cancels at low beta)
+template <class P>
+__device__ __forceinline__ void to_prim(const double* c, double* w, const FVParams& fp) {
+#pragma unroll
+  for (int iw = 0; iw < P::NW; ++iw) w[iw] = c[iw];
+  const double inv_rho = 1.0 / w[P::RHO];
+#pragma unroll
+  for (int d = 0; d < P::NDIR; ++d) w[P::MOM + d] = w[P::MOM + d] * inv_rho;
+  double vsq = w[P::MOM] * w[P::MOM];
+#pragma unroll
+  for (int d = 1; d < P::NDIR; ++d) vsq = vsq + w[P::MOM + d] * w[P::MOM + d];
+  if constexpr (!P::MHD) {
+    w[P::E] = (fp.gamma - 1.0) * (w[P::E] - 0.5 * w[P::RHO] * vsq);
+  } else {
+    double bsq = w[P::MAG] * w[P::MAG];
+#pragma unroll
+    for (int d = 1; d < P::NDIR; ++d) bsq = bsq + w[P::MAG + d] * w[P::MAG + d];
+    w[P::E] = fp.gamma_1 * (w[P::E] - 0.5 * (w[P::RHO] * vsq + bsq));
+  }
+}
+
+// hd_to_conserved src/hd/mod_hd_phys.t:677-700,
+// mhd_to_conserved_origin src/mhd/mod_mhd_phys.t:1562-1579
+template <class P, class R>
+__device__ __forceinline__ void to_cons(R* w, R inv_gamma_1) {
+  R vsq = w[P::MOM] * w[P::MOM];
+#pragma unroll
+  for (int d = 1; d < P::NDIR; ++d) vsq = vsq + w[P::MOM + d] * w[P::MOM + d];
+  if constexpr (!P::MHD) {
+    w[P::E] = w[P::E] * inv_gamma_1 + R(0.5) * vsq * w[P::RHO];
+  } else {
+    R bsq = w[P::MAG] * w[P::MAG];
+#pragma unroll
+    for (int d = 1; d < P::NDIR; ++d) bsq = bsq + w[P::MAG + d] * w[P::MAG + d];
+    w[P::E] = w[P::E] * inv_gamma_1 + R(0.5) * (vsq * w[P::RHO] + bsq);
+  }
+#pragma unroll
+  for (int d = 0; d < P::NDIR; ++d) w[P::MOM + d] = w[P::RHO] * w[P::MOM + d];
+}
+
+// hd_get_flux src/hd/mod_hd_phys.t:1097-1146,
+// mhd_get_flux src/mhd/mod_mhd_phys.t:3810-3889 (GLM :3868-3874)
+template <class P, class R, int ID>
+__device__ __forceinline__ void phys_flux(const R* wC, const R* w, R ch, R* f) {
+  const R vn = w[P::MOM + ID];
+  if constexpr (!P::MHD) {
+    f[P::RHO] = vn * w[P::RHO];
+#pragma unroll
+    for (int d = 0; d < P::NDIR; ++d) f[P::MOM + d] = vn * wC[P::MOM + d];
+    f[P::MOM + ID] = f[P::MOM + ID] + w[P::E];
+    f[P::E] = vn * (wC[P::E] + w[P::E]);
+  } else {
+    const R bn = w[P::MAG + ID];
+    f[P::RHO] = vn * w[P::RHO];
+#pragma unroll
+    for (int d = 0; d < P::NDIR; ++d) f[P::MOM + d] = wC[P::MOM + ID] * w[P::MOM + d] - bn * w[P::MAG + d];
+    R bsq = w[P::MAG] * w[P::MAG];
+#pragma unroll
+    for (int d = 1; d < P::NDIR; ++d) bsq = bsq + w[P::MAG + d] * w[P::MAG + d];
+    const R ptotal = w[P::E] + R(0.5) * bsq;
+    f[P::MOM + ID] = f[P::MOM + ID] + ptotal;
+    R bv = w[P::MAG] * w[P::MOM];
+#pragma unroll
+    for (int d = 1; d < P::NDIR; ++d) bv = bv + w[P::MAG + d] * w[P::MOM + d];
+    f[P::E] = vn * (wC[P::E] + ptotal) - bn * bv;
+#pragma unroll
+    for (int d = 0; d < P::NDIR; ++d) f[P::MAG + d] = vn * w[P::MAG + d] - bn * w[P::MOM + d];
+    if constexpr (P::GLM) {
+      f[P::MAG + ID] = w[P::PSI];
+      f[P::PSI] = ch * ch * w[P::MAG + ID];
+    }
+  }
+}
+
+// mhd_get_csound_prim src/mhd/mod_mhd_phys.t:3329-3379 (no B0, no Hall)
+template <class P, class R, int ID>
+__device__ __forceinline__ R fast_speed(const R* w, R gamma) {
+  const R inv_rho = R(1) / w[P::RHO];
+  const R cs = gamma * w[P::E] * inv_rho;
+  R b2 = w[P::MAG] * w[P::MAG];
+#pragma unroll
+  for (int d = 1; d < P::NDIR; ++d) b2 = b2 + w[P::MAG + d] * w[P::MAG + d];
+  const R cfast2 = b2 * inv_rho + cs;
+  const R bn = w[P::MAG + ID];
+  R av = cfast2 * cfast2 - R(4) * cs * (bn * bn) * inv_rho;
+  if (av < R(0)) av = R(0);
+  return sqrt(R(0.5) * (cfast2 + sqrt(av)));
+}
+
+// hd_get_cbounds src/hd/mod_hd_phys.t:891-1001,
+// mhd_get_cbounds src/mhd/mod_mhd_phys.t:3037-3127
+template <class P, class R, int ID, bool WANT_CMIN>
+__device__ __forceinline__ void cbounds(const R* wLC, const R* wRC, const R* wLp, const R* wRp,
+                                        const FVParams& fp, R& cmax, R& cmin, int& crash) {
+  const R g = R(fp.gamma);
+  constexpr int MN = P::MOM + ID;
+  if constexpr (!P::MHD) {
+    if (fp.boundspeed == 1) {
+      const R tmp1 = sqrt(wLp[P::RHO]);
+      const R tmp2 = sqrt(wRp[P::RHO]);
+      const R tmp3 = R(1) / (tmp1 + tmp2);
+      const R umean = (wLp[MN] * tmp1 + wRp[MN] * tmp2) * tmp3;
+      const R csL = g * wLp[P::E] / wLp[P::RHO];
+      const R csR = g * wRp[P::E] / wRp[P::RHO];
+      const R dv = wRp[MN] - wLp[MN];
+      R dmean = (tmp1 * csL + tmp2 * csR) * tmp3 + R(0.5) * tmp1 * tmp2 * (tmp3 * tmp3) * (dv * dv);
+      dmean = sqrt(dmean);
+      if constexpr (WANT_CMIN) {
+        cmin = umean - dmean;
+        cmax = umean + dmean;
+      } else {
+        cmax = fabs(umean) + dmean;
+      }
+    } else if (fp.boundspeed == 2) {
+      // conserved mean, hd_get_csound2 -> hd_get_pthermal (:1021-1064)
+      const R rho = R(0.5) * (wLC[P::RHO] + wRC[P::RHO]);
+      const R mn = R(0.5) * (wLC[MN] + wRC[MN]);
+      R msq = R(0);
+#pragma unroll
+      for (int d = 0; d < P::NDIR; ++d) {
+        const R m = R(0.5) * (wLC[P::MOM + d] + wRC[P::MOM + d]);
+        msq = msq + m * m;
+      }
+      const R e = R(0.5) * (wLC[P::E] + wRC[P::E]);
+      const R tmp1 = mn / rho;
+      const R pth = (g - R(1)) * (e - R(0.5) * msq / rho);
+      if (fp.check_small && pth < R(fp.small_pressure)) crash = 1;
+      const R cs = sqrt(g * pth / rho);
+      if constexpr (WANT_CMIN) {
+        cmax = fmax(tmp1 + cs, R(0));
+        cmin = fmin(tmp1 - cs, R(0));
+      } else {
+        cmax = fabs(tmp1) + cs;
+      }
+    } else {
+      const R csL = g * wLp[P::E] / wLp[P::RHO];
+      const R csR = g * wRp[P::E] / wRp[P::RHO];
+      const R cs = fmax(sqrt(csL), sqrt(csR));
+      if constexpr (WANT_CMIN) cmin = fmin(wLp[MN], wRp[MN]) - cs;
+      cmax = fmax(wLp[MN], wRp[MN]) + cs;
+    }
+  } else {
+    if (fp.boundspeed == 1) {
+      const R csL = fast_speed<P, R, ID>(wLp, g);
+      const R csR = fast_speed<P, R, ID>(wRp, g);
+      const R tmp1 = sqrt(wLp[P::RHO]);
+      const R tmp2 = sqrt(wRp[P::RHO]);
+      const R tmp3 = R(1) / (tmp1 + tmp2);
+      const R umean = (wLp[MN] * tmp1 + wRp[MN] * tmp2) * tmp3;
+      const R dv = wRp[MN] - wLp[MN];
+      const R dmean = sqrt((tmp1 * (csL * csL) + tmp2 * (csR * csR)) * tmp3 +
+                           R(0.5) * tmp1 * tmp2 * (tmp3 * tmp3) * (dv * dv));
+      if constexpr (WANT_CMIN) {
+        cmin = umean - dmean;
+        cmax = umean + dmean;
+      } else {
+        cmax = fabs(umean) + dmean;
+      }
+    } else if (fp.boundspeed == 2) {
+      R wm[P::NW];
+#pragma unroll
+      for (int iw = 0; iw < P::NW; ++iw) wm[iw] = R(0.5) * (wLp[iw] + wRp[iw]);
+      const R cs = fast_speed<P, R, ID>(wm, g);
+      if constexpr (WANT_CMIN) {
+        cmax = fmax(wm[MN] + cs, R(0));
+        cmin = fmin(wm[MN] - cs, R(0));
+      } else {
+        cmax = fabs(wm[MN]) + cs;
+      }
+    } else {
+      const R csL = fast_speed<P, R, ID>(wLp, g);
+      const R csR = fast_speed<P, R, ID>(wRp, g);
+      const R cs = fmax(csL, csR);
+      if constexpr (WANT_CMIN) cmin = fmin(wLp[MN], wRp[MN]) - cs;
+      cmax = fmax(wLp[MN], wRp[MN]) + cs;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// slope limiters, dwlimiter2 src/limiter/mod_limiter.t:128-334 for one cell:
+// a = dwC(i-1), b = dwC(i); gives ldw(i) and rdw(i)
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ __forceinline__ R mn2(R a, R b) { return a < b ? a : b; }
+template <class R>
+__device__ __forceinline__ R mx2(R a, R b) { return a > b ? a : b; }
+
+template <class R>
+__device__ __forceinline__ void cada3_side(R th, R eta, R& out) {
+  const R cadalfa = R(0.5), cadbeta = R(2), cadgamma = R(1.6);
+  const R A = (R(2) + th) * R(1.0 / 3.0);
+  const R lim = mx2(R(0), mn2(A, mx2(-cadalfa * th, mn2(mn2(cadbeta * th, A), cadgamma))));
+  if (eta <= R(1) - R(1.0e-14)) {
+    out = A;
+  } else if (eta >= R(1) + R(1.0e-14)) {
+    out = lim;
+  } else {
+    const R t2 = (eta - R(1)) * R(1.0e14);
+    out = R(0.5) * ((R(1) - t2) * A + (R(1) + t2) * lim);
+  }
+}
+
+template <class R, int LIM>
+__device__ __forceinline__ void dwlimiter(int lim_rt, R a, R b, R rdelinv, R& ldw, R& rdw) {
+  const int lim = LIM > 0 ? LIM : lim_rt;
+  if (lim == LIM_MINMOD) {
+    R t = copysign(R(1), b);
+    t = t * mx2(R(0), mn2(fabs(b), t * a));
+    ldw = t; rdw = t;
+  } else if (lim == LIM_VANLEER) {
+    const R t = R(2) * mx2(a * b, R(0)) / (b + a + R(1.0e-12));
+    ldw = t; rdw = t;
+  } else if (lim == LIM_CADA3) {
+    const R eps = R(1.4901161193847656e-08);  // sqrt(epsilon(1d0))
+    const R eta = (b * b + a * a) * rdelinv;
+    R l, r;
+    cada3_side(a / (b + copysign(eps, b)), eta, l);
+    cada3_side(b / (a + copysign(eps, a)), eta, r);
+    ldw = l * b;
+    rdw = r * a;
+  } else if (lim == LIM_KOREN) {
+    const R t = copysign(R(1), b);
+    const R t2 = mn2(R(2) * fabs(b), R(2) * t * a);
+    ldw = t * mx2(R(0), mn2(t2, (a * t + R(2) * fabs(b)) * R(1.0 / 3.0)));
+    rdw = t * mx2(R(0), mn2(t2, (R(2) * a * t + fabs(b)) * R(1.0 / 3.0)));
+  } else if (lim == LIM_WOODWARD) {
+    R t = copysign(R(1), b);
+    t = R(2) * t * mx2(R(0), mn2(mn2(fabs(b), t * a), t * R(0.25) * (a + b)));
+    ldw = t; rdw = t;
+  } else if (lim == LIM_MCBETA) {
+    R t = copysign(R(1), b);
+    t = t * mx2(R(0), mn2(mn2(R(1.4) * fabs(b), R(1.4) * t * a), t * R(0.5) * (a + b)));
+    ldw = t; rdw = t;
+  } else if (lim == LIM_SUPERBEE) {
+    R t = copysign(R(1), b);
+    t = t * mx2(mx2(R(0), mn2(R(2) * fabs(b), t * a)), mn2(fabs(b), R(2) * t * a));
+    ldw = t; rdw = t;
+  } else {  // albada
+    const R t = (a * (b * b + R(1.0e-12)) + b * (a * a + R(1.0e-12))) / (b * b + a * a + R(2.0e-12));
+    ldw = t; rdw = t;
+  }
+}
+
+// WENO5limiter var=1 src/limiter/mod_weno.t:254-377, one variable, face
+// between cells 0 and 1 of the stencil s[-2..3] (passed as s0..s5 =
+// w(i-2)..w(i+3))
+template <class R>
+__device__ __forceinline__ void weno5(R wmm, R wm, R w0, R wp, R wpp, R wppp, R& wl, R& wr) {
+  const R bc1 = R(1.0833333333333333), bc2 = R(0.25);
+  const R d1 = R(1.0 / 10.0), d2 = R(3.0 / 5.0), d3 = R(3.0 / 10.0);
+  const R u11 = R(1.0 / 3.0), u12 = R(-7.0 / 6.0), u13 = R(11.0 / 6.0);
+  const R u21 = R(-1.0 / 6.0), u22 = R(5.0 / 6.0), u23 = R(1.0 / 3.0);
+  const R u31 = R(1.0 / 3.0), u32 = R(5.0 / 6.0), u33 = R(-1.0 / 6.0);
+  const R weps = R(1.0e-18);
+  {
+    const R f1 = u11 * wmm + u12 * wm + u13 * w0;
+    const R f2 = u21 * wm + u22 * w0 + u23 * wp;
+    const R f3 = u31 * w0 + u32 * wp + u33 * wpp;
+    R t = wmm + w0 - R(2) * wm, q = wmm - R(4) * wm + R(3) * w0;
+    const R b1 = bc1 * (t * t) + bc2 * (q * q);
+    t = wm + wp - R(2) * w0; q = wm - wp;
+    const R b2 = bc1 * (t * t) + bc2 * (q * q);
+    t = w0 + wpp - R(2) * wp; q = R(3) * w0 - R(4) * wp + wpp;
+    const R b3 = bc1 * (t * t) + bc2 * (q * q);
+    t = b1 + weps; const R a1 = d1 / (t * t);
+    t = b2 + weps; const R a2 = d2 / (t * t);
+    t = b3 + weps; const R a3 = d3 / (t * t);
+    const R as = a1 + a2 + a3;
+    wl = f1 * a1 / as + f2 * a2 / as + f3 * a3 / as;
+  }
+  {
+    const R f1 = u11 * wppp + u12 * wpp + u13 * wp;
+    const R f2 = u21 * wpp + u22 * wp + u23 * w0;
+    const R f3 = u31 * wp + u32 * w0 + u33 * wm;
+    R t = wppp + wp - R(2) * wpp, q = wppp - R(4) * wpp + R(3) * wp;
+    const R b1 = bc1 * (t * t) + bc2 * (q * q);
+    t = wpp + w0 - R(2) * wp; q = wpp - w0;
+    const R b2 = bc1 * (t * t) + bc2 * (q * q);
+    t = wp + wm - R(2) * w0; q = R(3) * wp - R(4) * w0 + wm;
+    const R b3 = bc1 * (t * t) + bc2 * (q * q);
+    t = b1 + weps; const R a1 = d1 / (t * t);
+    t = b2 + weps; const R a2 = d2 / (t * t);
+    t = b3 + weps; const R a3 = d3 / (t * t);
+    const R as = a1 + a2 + a3;
+    wr = f1 * a1 / as + f2 * a2 / as + f3 * a3 / as;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// one face: reconstruct_LR (src/mod_finite_volume.t:1110-1228), modify_wLR
+// (GLM, src/mhd/mod_mhd_phys.t:6326-6377), fluxes, bounds and the Riemann
+// solver (src/mod_finite_volume.t:348-504, src/hd/mod_hd_hllc.t:56-257).
+// st[k][iw] are primitive values of cells i-NG+1+k (face between i and i+1,
+// NG = 2: st = w(i-1), w(i), w(i+1), w(i+2); NG = 3 adds w(i-2), w(i+3)).
+// ---------------------------------------------------------------------------
+template <class P, class R, int RIEM, int ID, int LIM, int NG>
+__device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& fp, R ch, R* fC,
+                                          int& crash) {
+  constexpr int NW = P::NW;
+  constexpr int C0 = NG - 1;  // index of cell i in st
+  R wLp[NW], wRp[NW];
+  if constexpr (LIM == LIM_WENO5) {
+    R tl[NW], tr[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      wLp[iw] = st[C0][iw];
+      wRp[iw] = st[C0 + 1][iw];
+      weno5(st[C0 - 2][iw], st[C0 - 1][iw], st[C0][iw], st[C0 + 1][iw], st[C0 + 2][iw], st[C0 + 3][iw],
+            tl[iw], tr[iw]);
+    }
+    // fix_limiter src/limiter/mod_weno.t:100-133 with hd/mhd check_w
+    const bool bad = tl[P::RHO] < R(fp.small_density) || tl[P::E] < R(fp.small_pressure) ||
+                     tr[P::RHO] < R(fp.small_density) || tr[P::E] < R(fp.small_pressure);
+    if (!bad) {
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) { wLp[iw] = tl[iw]; wRp[iw] = tr[iw]; }
+    }
+  } else {
+    const R rdel = R(fp.rdelinv[ID]);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const R dm = st[C0][iw] - st[C0 - 1][iw];
+      const R d0 = st[C0 + 1][iw] - st[C0][iw];
+      const R dp = st[C0 + 2][iw] - st[C0 + 1][iw];
+      R ldw, rdw, l2, r2;
+      dwlimiter<R, LIM>(fp.limiter, dm, d0, rdel, ldw, r2);
+      dwlimiter<R, LIM>(fp.limiter, d0, dp, rdel, l2, rdw);
+      wLp[iw] = st[C0][iw] + R(0.5) * ldw;
+      wRp[iw] = st[C0 + 1][iw] - R(0.5) * rdw;
+    }
+  }
+  R wLC[NW], wRC[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) { wLC[iw] = wLp[iw]; wRC[iw] = wRp[iw]; }
+  const R inv_g1 = R(fp.inv_gamma_1);
+  to_cons<P, R>(wLC, inv_g1);
+  to_cons<P, R>(wRC, inv_g1);
+  if constexpr (P::GLM) {
+    constexpr int B = P::MAG + ID, S = P::PSI, E = P::E;
+    const R dB = wRp[B] - wLp[B];
+    const R dPsi = wRp[S] - wLp[S];
+    wLp[B] = R(0.5) * (wRp[B] + wLp[B] - dPsi / ch);
+    wLp[S] = R(0.5) * (wRp[S] + wLp[S] - dB * ch);
+    wRp[B] = wLp[B];
+    wRp[S] = wLp[S];
+    wRC[E] = wRC[E] - R(0.5) * (wRC[B] * wRC[B]);
+    wLC[E] = wLC[E] - R(0.5) * (wLC[B] * wLC[B]);
+    wRC[B] = wLp[B]; wRC[S] = wLp[S];
+    wLC[B] = wLp[B]; wLC[S] = wLp[S];
+    wRC[E] = wRC[E] + R(0.5) * (wRC[B] * wRC[B]);
+    wLC[E] = wLC[E] + R(0.5) * (wLC[B] * wLC[B]);
+  }
+  R fL[NW], fR[NW];
+  phys_flux<P, R, ID>(wLC, wLp, ch, fL);
+  phys_flux<P, R, ID>(wRC, wRp, ch, fR);
+  R cmax, cmin = R(0);
+  if constexpr (RIEM == RIEM_TVDLF) {
+    cbounds<P, R, ID, false>(wLC, wRC, wLp, wRp, fp, cmax, cmin, crash);
+    const R fac = R(-0.5) * R(fp.tvdlfeps) * cmax;
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      fC[iw] = R(0.5) * (fL[iw] + fR[iw]);
+      fC[iw] = fC[iw] + fac * (wRC[iw] - wLC[iw]);
+    }
+  } else {
+    cbounds<P, R, ID, true>(wLC, wRC, wLp, wRp, fp, cmax, cmin, crash);
+    // flux_type: normal B gets Lax-Friedrichs dissipation only without GLM
+    // (src/mhd/mod_mhd_phys.t:580-588)
+    constexpr int BTV = (P::MHD && !P::GLM && P::NDIM > 1) ? (P::MAG + ID) : -1;
+    if constexpr (RIEM == RIEM_HLL) {
+      const R inv = R(1) / (cmax - cmin);
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) {
+        if (iw == BTV) {
+          fC[iw] = -R(fp.tvdlfeps) * R(0.5) * mx2(cmax, R(fabs(cmin))) * (wRC[iw] - wLC[iw]);
+        } else if (cmin >= R(0)) {
+          fC[iw] = fL[iw];
+        } else if (cmax <= R(0)) {
+          fC[iw] = fR[iw];
+        } else {
+          fC[iw] = (cmax * fL[iw] - cmin * fR[iw] + cmin * cmax * (wRC[iw] - wLC[iw])) / (cmax - cmin);
+        }
+      }
+      (void)inv;
+    } else {  // HLLC (hydrodynamics)
+      constexpr int RHO = P::RHO, MN = P::MOM + ID, E = P::E;
+      int patchf = 1;
+      if (cmin >= R(0)) patchf = -2;
+      else if (cmax <= R(0)) patchf = 2;
+      R lambdaCD = R(0);
+      R Fhll[NW];
+      if (patchf == 1) {
+        R whll_m = R(0), whll_r = R(0);
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) {
+          Fhll[iw] = (cmax * fL[iw] - cmin * fR[iw] + cmin * cmax * (wRC[iw] - wLC[iw])) / (cmax - cmin);
+          const R wh = (cmax * wRC[iw] - cmin * wLC[iw] + fL[iw] - fR[iw]) / (cmax - cmin);
+          if (iw == MN) whll_m = wh;
+          if (iw == RHO) whll_r = wh;
+        }
+        lambdaCD = whll_m / whll_r;
+        bool cond = true;
+        if (cmin < R(0) && lambdaCD > R(0) && lambdaCD < cmax) {
+          patchf = -1;
+        } else if (cmax > R(0) && lambdaCD < R(0) && lambdaCD > cmin) {
+          patchf = 1;
+        } else if (lambdaCD >= cmax || lambdaCD <= cmin) {
+          lambdaCD = R(0);
+          patchf = 3;
+          cond = false;
+        }
+        if (cond && lambdaCD == R(0)) patchf = (-cmin >= cmax) ? 1 : -1;
+      } else {
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) Fhll[iw] = R(0);
+      }
+      if (patchf == 1 || patchf == -1) {
+        const bool right = patchf == 1;
+        const R cspeed = right ? cmax : cmin;
+        R wSub[NW], fSub[NW], wCD[NW];
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) {
+          wSub[iw] = right ? wRC[iw] : wLC[iw];
+          fSub[iw] = right ? fR[iw] : fL[iw];
+        }
+        const R vSub = wSub[MN] / wSub[RHO];
+        const R csmls = R(1) / (cspeed - lambdaCD);
+        wCD[RHO] = wSub[RHO] * (cspeed - vSub) * csmls;
+#pragma unroll
+        for (int d = 0; d < P::NDIR; ++d) {
+          if (d != ID) wCD[P::MOM + d] = (cspeed * wSub[P::MOM + d] - fSub[P::MOM + d]) * csmls;
+          else wCD[P::MOM + d] = wCD[RHO] * lambdaCD;
+        }
+        const R pCD = wSub[RHO] * (cspeed - vSub) * (lambdaCD - vSub) + fSub[MN] - wSub[MN] * vSub;
+        wCD[E] = (cspeed * wSub[E] - fSub[E] + lambdaCD * pCD) * csmls;
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) fC[iw] = fSub[iw] + cspeed * (wCD[iw] - wSub[iw]);
+      } else if (patchf == -2) {
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) fC[iw] = fL[iw];
+      } else if (patchf == 2) {
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) fC[iw] = fR[iw];
+      } else {
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) fC[iw] = Fhll[iw];
+      }
+    }
+  }
+}
+
+// cell-centred cmax for dt: hd_get_cmax src/hd/mod_hd_phys.t:787-811,
+// mhd_get_cmax_origin src/mhd/mod_mhd_phys.t:2482-2550 (FP64, primitive)
+template <class P, int ID>
+__device__ __forceinline__ double cell_cmax(const double* w, const FVParams& fp) {
+  if constexpr (!P::MHD) {
+    return fabs(w[P::MOM + ID]) + sqrt(fp.gamma * w[P::E] / w[P::RHO]);
+  } else {
+    return fabs(w[P::MOM + ID]) + fast_speed<P, double, ID>(w, fp.gamma);
+  }
+}
+
+}  // namespace FV_NS

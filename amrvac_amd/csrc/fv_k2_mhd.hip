@@ -1,0 +1,12 @@
+// 2-D ideal MHD stage kernels (ndir = 3: 2.5-D), GLM or Linde div B control.
+#include "fv_launch.h"
+namespace FV_NS {
+hipError_t launch_sweep_2d_mhd(const KernelKey& k, const SweepArgs& a, const FVParams& fp,
+                                        int grid, hipStream_t s) {
+  if (k.ndir != 3) return hipErrorInvalidValue;
+  if (k.divb == DIVB_GLM) return launch_prec<Phys<2, 3, true, DIVB_GLM>>(k, a, fp, grid, s);
+  if (k.divb == DIVB_LINDE) return launch_prec<Phys<2, 3, true, DIVB_LINDE>>(k, a, fp, grid, s);
+  if (k.divb == DIVB_NONE) return launch_prec<Phys<2, 3, true, DIVB_NONE>>(k, a, fp, grid, s);
+  return hipErrorInvalidValue;
+}
+}  // namespace FV_NS

@@ -1,0 +1,83 @@
+// fv_launch.h -- kernel selection between the C-ABI layer and the
+// per-physics translation units (each compiled twice: with FMA contraction,
+// and with -ffp-contract=off for the bit-reproducible FP64 mode).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fv_sweep.h"
+
+namespace fvc {
+struct KernelKey {
+  int ndim, physics, ndir, divb;
+  int mixed;   // 0: FP64 reconstruction/Riemann, 1: FP32
+  int riem;    // RIEM_*
+  int weno;    // 1: WENO5 (NG = 3), 0: TVD limiters (NG = 2)
+  int limiter; // runtime limiter code (TVD path)
+};
+
+// Returns hipSuccess on launch, hipErrorInvalidValue when no kernel matches.
+using SweepLauncher = hipError_t (*)(const KernelKey&, const SweepArgs&, const FVParams&, int grid,
+                                     hipStream_t);
+}  // namespace fvc
+
+// launchers, defined once per namespace (fv: FMA contraction; fvx:
+// -ffp-contract=off, FP64 only)
+#define FV_DECLARE_LAUNCHERS(NS)                                                                     \
+  namespace NS {                                                                                     \
+  hipError_t launch_sweep_2d_hd(const fvc::KernelKey&, const fvc::SweepArgs&, const fvc::FVParams&, \
+                                int, hipStream_t);                                                   \
+  hipError_t launch_sweep_2d_mhd(const fvc::KernelKey&, const fvc::SweepArgs&, const fvc::FVParams&,\
+                                 int, hipStream_t);                                                  \
+  hipError_t launch_sweep_3d_hd(const fvc::KernelKey&, const fvc::SweepArgs&, const fvc::FVParams&, \
+                                int, hipStream_t);                                                   \
+  hipError_t launch_sweep_3d_mhd(const fvc::KernelKey&, const fvc::SweepArgs&, const fvc::FVParams&,\
+                                 int, hipStream_t);                                                  \
+  }
+FV_DECLARE_LAUNCHERS(fv)
+FV_DECLARE_LAUNCHERS(fvx)
+
+namespace FV_NS {
+using fvc::KernelKey;
+
+
+template <class P, class R, int RIEM, int LIM, int NG>
+hipError_t launch_one(const SweepArgs& a, const FVParams& fp, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((sweep_kernel<P, R, RIEM, LIM, NG>), dim3(grid), dim3(NTHREADS), 0, s, a, fp);
+  return hipGetLastError();
+}
+
+// TVD (runtime limiter) or WENO5 for one physics / precision / solver
+template <class P, class R, int RIEM>
+hipError_t launch_lim(const KernelKey& k, const SweepArgs& a, const FVParams& fp, int grid, hipStream_t s) {
+  if (k.weno) {
+    if constexpr (RIEM == RIEM_HANCOCK) return hipErrorInvalidValue;
+    else return launch_one<P, R, RIEM, LIM_WENO5, 3>(a, fp, grid, s);
+  }
+  return launch_one<P, R, RIEM, 0, 2>(a, fp, grid, s);
+}
+
+template <class P, class R>
+hipError_t launch_riem(const KernelKey& k, const SweepArgs& a, const FVParams& fp, int grid, hipStream_t s) {
+  switch (k.riem) {
+    case RIEM_HLL: return launch_lim<P, R, RIEM_HLL>(k, a, fp, grid, s);
+    case RIEM_TVDLF: return launch_lim<P, R, RIEM_TVDLF>(k, a, fp, grid, s);
+    case RIEM_HANCOCK: return launch_lim<P, R, RIEM_HANCOCK>(k, a, fp, grid, s);
+    case RIEM_HLLC:
+      if constexpr (!P::MHD) return launch_lim<P, R, RIEM_HLLC>(k, a, fp, grid, s);
+      else return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class P>
+hipError_t launch_prec(const KernelKey& k, const SweepArgs& a, const FVParams& fp, int grid, hipStream_t s) {
+#ifdef FV_EXACT
+  if (k.mixed) return hipErrorInvalidValue;
+#else
+  if (k.mixed) return launch_riem<P, float>(k, a, fp, grid, s);
+#endif
+  return launch_riem<P, double>(k, a, fp, grid, s);
+}
+
+}  // namespace FV_NS

@@ -1,0 +1,204 @@
+"""ctypes binding of libamrvac_fv.so (include/amrvac_fv.h).
+
+``DeviceFV`` implements the backend protocol of :mod:`amrvac_amd.host`
+(getdt_courant / advect1 / volume_average / ...) on the HIP kernels, so the
+same reference-shaped time loop drives either the device path or, in tests,
+the CPU oracle.  There is no CPU fallback: constructing a ``DeviceFV``
+without the built library or without a GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .host import BOUNDSPEED, DIVB, LIMITERS, Par
+from .mesh import UniformMesh
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libamrvac_fv.so")
+
+PREC = {"fp64": 0, "mixed": 1, "fp64_exact": 2}
+
+# every symbol include/amrvac_fv.h declares
+EXPORTS = ("amrvac_fv_init", "amrvac_fv_finalize", "amrvac_fv_set_stream", "amrvac_fv_set_mesh",
+           "amrvac_fv_upload", "amrvac_fv_download", "amrvac_fv_getdt", "amrvac_fv_advect1",
+           "amrvac_fv_copy", "amrvac_fv_volume_average", "amrvac_fv_slot_pointer",
+           "amrvac_fv_sync", "amrvac_fv_sweep_timing", "amrvac_fv_last_error")
+
+
+class FVConfig(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int), ("ndir", ctypes.c_int), ("physics", ctypes.c_int),
+        ("gamma", ctypes.c_double), ("divb", ctypes.c_int), ("glm_alpha", ctypes.c_double),
+        ("glm_extended", ctypes.c_int), ("divbdiff", ctypes.c_double), ("limiter", ctypes.c_int),
+        ("boundspeed", ctypes.c_int), ("tvdlfeps", ctypes.c_double), ("cada3_radius", ctypes.c_double),
+        ("small_pressure", ctypes.c_double), ("small_density", ctypes.c_double),
+        ("check_small_values", ctypes.c_int), ("nghost", ctypes.c_int),
+        ("block_nx", ctypes.c_int * 3), ("precision", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the in-tree library (build it with ``amrvac_amd.build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: run `python -m amrvac_amd.build` first")
+    L = ctypes.CDLL(path)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    L.amrvac_fv_init.argtypes = [ctypes.POINTER(FVConfig), ctypes.POINTER(vp)]
+    L.amrvac_fv_finalize.argtypes = [vp]
+    L.amrvac_fv_set_stream.argtypes = [vp, vp]
+    L.amrvac_fv_set_mesh.argtypes = [vp, ctypes.c_int, dp, ip, dp, ip]
+    L.amrvac_fv_upload.argtypes = [vp, ctypes.c_int, dp]
+    L.amrvac_fv_download.argtypes = [vp, ctypes.c_int, dp]
+    L.amrvac_fv_getdt.argtypes = [vp, ctypes.c_int, ctypes.c_double, dp, dp]
+    L.amrvac_fv_advect1.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_double, ip]
+    L.amrvac_fv_copy.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.amrvac_fv_volume_average.argtypes = [vp, ctypes.c_int, dp]
+    L.amrvac_fv_slot_pointer.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_longlong)]
+    L.amrvac_fv_sync.argtypes = [vp]
+    L.amrvac_fv_sweep_timing.argtypes = [vp, ctypes.c_int, dp, ip]
+    L.amrvac_fv_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def make_config(par: Par, mesh: UniformMesh, precision: str | None = None) -> FVConfig:
+    c = FVConfig()
+    c.ndim = par.ndim
+    c.ndir = par.ndir
+    c.physics = 1 if par.physics == "hd" else 2
+    c.gamma = par.gamma
+    c.divb = DIVB[par.typedivbfix] if par.physics == "mhd" else 0
+    c.glm_alpha = par.mhd_glm_alpha
+    c.glm_extended = 1 if par.mhd_glm_extended else 0
+    c.divbdiff = par.divbdiff
+    c.limiter = LIMITERS[par.limiter]
+    c.boundspeed = BOUNDSPEED[par.typeboundspeed]
+    c.tvdlfeps = par.tvdlfeps
+    c.cada3_radius = par.cada3_radius
+    c.small_pressure = par.small_pressure
+    c.small_density = par.small_density
+    c.check_small_values = 1 if par.check_small_values else 0
+    c.nghost = par.nghost
+    for d in range(3):
+        c.block_nx[d] = mesh.block_nx[d] if d < par.ndim else 1
+    c.precision = PREC[precision or par.precision]
+    return c
+
+
+class FVError(RuntimeError):
+    pass
+
+
+class DeviceFV:
+    """Backend protocol of amrvac_amd.host on the HIP kernels."""
+
+    def __init__(self, par: Par, mesh: UniformMesh, precision: str | None = None):
+        self.L = load_library()
+        self.par = par
+        self.mesh = mesh
+        self.nw = par.nw
+        self.shape = (mesh.nblocks, self.nw) + mesh.nxg[::-1]
+        cfg = make_config(par, mesh, precision)
+        h = ctypes.c_void_p()
+        self._check(self.L.amrvac_fv_init(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        dx = np.ascontiguousarray(mesh.dx, dtype=np.float64)
+        nbr = np.ascontiguousarray(mesh.neighbor, dtype=np.int32)
+        xl = np.ascontiguousarray(mesh.xlohi, dtype=np.float64)
+        bc = np.ascontiguousarray(mesh.bc_codes, dtype=np.int32)
+        self._check(self.L.amrvac_fv_set_mesh(self.h, mesh.nblocks, _dp(dx), _ip(nbr), _dp(xl), _ip(bc)))
+        self._crash = False
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            msg = self.L.amrvac_fv_last_error().decode()
+            raise FVError(f"amrvac_fv error {rc}: {msg}")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.amrvac_fv_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- backend protocol ---------------------------------------------------
+    def set_state(self, slot: int, w: np.ndarray) -> None:
+        a = np.ascontiguousarray(w, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"state shape {a.shape} != {self.shape}")
+        self._check(self.L.amrvac_fv_upload(self.h, slot, _dp(a)))
+
+    def get_state(self, slot: int) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        self._check(self.L.amrvac_fv_download(self.h, slot, _dp(out)))
+        return out
+
+    def getbc(self, slot: int) -> None:
+        """Ghost cells of the device state are read from the neighbours by
+        every sweep and materialised by get_state; nothing to do."""
+
+    def getdt_courant(self, slot: int, courantpar: float):
+        dt = ctypes.c_double()
+        cm = ctypes.c_double()
+        self._check(self.L.amrvac_fv_getdt(self.h, slot, courantpar, ctypes.byref(dt), ctypes.byref(cm)))
+        return dt.value, cm.value
+
+    def copy(self, dst: int, src: int) -> None:
+        self._check(self.L.amrvac_fv_copy(self.h, dst, src))
+
+    def advect1(self, method, dtfactor, dt, cmax_global, psa, psb, comb, check_crash: bool = False) -> None:
+        if comb is None:
+            cs, ca, cb = -1, 0.0, 0.0
+        elif comb[0] == "copy":
+            cs, ca, cb = comb[1], 1.0, 0.0
+        else:
+            _, ca, cs, cb = comb
+        crash = ctypes.c_int(0)
+        self._check(self.L.amrvac_fv_advect1(self.h, method, dtfactor, dt, cmax_global, psa, psb, cs, ca, cb,
+                                             ctypes.byref(crash) if check_crash else None))
+        if crash.value:
+            self._crash = True
+
+    def volume_average(self, slot: int) -> np.ndarray:
+        out = np.zeros(2 * self.nw, dtype=np.float64)
+        self._check(self.L.amrvac_fv_volume_average(self.h, slot, _dp(out)))
+        return out
+
+    def crash(self) -> bool:
+        return self._crash
+
+    # -- extras ---------------------------------------------------------------
+    def sync(self) -> None:
+        self._check(self.L.amrvac_fv_sync(self.h))
+
+    def slot_pointer(self, slot: int) -> tuple[int, int]:
+        p = ctypes.c_void_p()
+        n = ctypes.c_longlong()
+        self._check(self.L.amrvac_fv_slot_pointer(self.h, slot, ctypes.byref(p), ctypes.byref(n)))
+        return int(p.value), int(n.value)
+
+    def sweep_timing(self, reset: bool = False) -> tuple[float, int]:
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int()
+        self._check(self.L.amrvac_fv_sweep_timing(self.h, 1 if reset else 0, ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value

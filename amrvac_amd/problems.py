@@ -1,0 +1,242 @@
+"""Initial conditions (the user-module side of the reference,
+``usr_init_one_grid``) for the workloads this package runs.
+
+Each function returns the conserved state of every block in the reference
+layout ``w[ib, iw, (k,) j, i]`` with ghost cells, evaluated at the cell
+centres of ``UniformMesh.cell_centers``.  Seeded noise is a counter-based
+hash of (seed, level, global cell index, component), so it does not depend on
+block order.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .mesh import UniformMesh
+
+DPI = 3.141592653589793238462643383279502884197169399375105  # src/mod_constants.t:21
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = (x + np.uint64(0x9E3779B97F4A7C15)) & _M64
+        z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _M64
+        z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _M64
+        return z ^ (z >> np.uint64(31))
+
+
+def hashed_uniform(seed: int, level: int, gidx: np.ndarray, comp: int) -> np.ndarray:
+    """Uniform noise in [-1, 1) for global cell indices ``gidx[nd, ...]``."""
+    h = _splitmix64(np.full(gidx.shape[1:], np.uint64(seed & 0xFFFFFFFFFFFFFFFF)))
+    vals = [level] + [gidx[d] for d in range(gidx.shape[0])] + [0] * (3 - gidx.shape[0]) + [comp]
+    for v in vals:
+        hv = np.asarray(v).astype(np.int64).astype(np.uint64)
+        h = _splitmix64(h ^ hv)
+    u = (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return 2.0 * u - 1.0
+
+
+def hd_to_conserved(prim: np.ndarray, gamma: float, ndir: int) -> np.ndarray:
+    """hd_to_conserved, src/hd/mod_hd_phys.t:677-700 (w[..., iw] order)."""
+    w = prim.copy()
+    inv_g1 = 1.0 / (gamma - 1.0)
+    vsq = w[:, 1] * w[:, 1]
+    for c in range(1, ndir):
+        vsq = vsq + w[:, 1 + c] * w[:, 1 + c]
+    e = ndir + 1
+    w[:, e] = w[:, e] * inv_g1 + 0.5 * vsq * w[:, 0]
+    for c in range(ndir):
+        w[:, 1 + c] = w[:, 0] * w[:, 1 + c]
+    return w
+
+
+def mhd_to_conserved(prim: np.ndarray, gamma: float, ndir: int) -> np.ndarray:
+    """mhd_to_conserved_origin, src/mhd/mod_mhd_phys.t:1562-1579."""
+    w = prim.copy()
+    inv_g1 = 1.0 / (gamma - 1.0)
+    vsq = w[:, 1] * w[:, 1]
+    for c in range(1, ndir):
+        vsq = vsq + w[:, 1 + c] * w[:, 1 + c]
+    mag0 = ndir + 2
+    bsq = w[:, mag0] * w[:, mag0]
+    for c in range(1, ndir):
+        bsq = bsq + w[:, mag0 + c] * w[:, mag0 + c]
+    e = ndir + 1
+    w[:, e] = w[:, e] * inv_g1 + 0.5 * (vsq * w[:, 0] + bsq)
+    for c in range(ndir):
+        w[:, 1 + c] = w[:, 0] * w[:, 1 + c]
+    return w
+
+
+def alfven_wave_2p5d(mesh: UniformMesh, gamma: float = 5.0 / 3.0) -> np.ndarray:
+    """Circularly polarised Alfven wave, tests/mhd/Alfven_wave_2.5D/mod_usr.t
+    (2-D, ndir=3, Linde cleaning: nw = 8)."""
+    x = mesh.cell_centers()
+    A = 0.1
+    ca = 1.0
+    k1 = 2.0 * DPI
+    alpha = np.arctan(2.0)
+    beta = 0.0
+    k2 = k1 * np.tan(alpha)
+    phi = k1 * x[:, 0] + k2 * x[:, 1]
+    rho = np.ones_like(phi)
+    p = 0.1 * np.ones_like(phi)
+    v = [np.zeros_like(phi), A * np.sin(phi), A * np.cos(phi)]
+    b = [np.sqrt(rho) * ca, -np.sqrt(rho) * ca * A * np.sin(phi),
+         -np.sqrt(rho) * ca * A * np.cos(phi)]
+
+    def rotate(u):
+        g = np.arctan(np.cos(alpha) * np.tan(beta))
+        r0 = np.cos(g) * np.cos(alpha) * u[0] - np.sin(alpha) * u[1] - np.sin(g) * np.cos(alpha) * u[2]
+        r1 = np.cos(g) * np.sin(alpha) * u[0] + np.cos(alpha) * u[1] - np.sin(g) * np.sin(alpha) * u[2]
+        r2 = np.sin(g) * u[0] + np.cos(g) * u[2]
+        return [r0, r1, r2]
+
+    vr = rotate(v)
+    br = rotate(b)
+    prim = np.stack([rho, vr[0], vr[1], vr[2], p, br[0], br[1], br[2]], axis=1)
+    return mhd_to_conserved(prim, gamma, 3)
+
+
+def kelvin_helmholtz_2d(mesh: UniformMesh, gamma: float = 1.4, seed: int = 20261202,
+                        amp: float = 1.0e-3) -> np.ndarray:
+    """2-D Euler Kelvin-Helmholtz: rho=2, vx=+0.5 for |y-0.5|<0.25, else
+    rho=1, vx=-0.5; p=2.5; vy=0.01 sin(4 pi x); seeded uniform noise of
+    amplitude ``amp`` on vx and vy."""
+    x = mesh.cell_centers()
+    g = mesh.global_index()
+    inside = np.abs(x[:, 1] - 0.5) < 0.25
+    rho = np.where(inside, 2.0, 1.0)
+    vx = np.where(inside, 0.5, -0.5) + amp * hashed_uniform(seed, 1, g.transpose(1, 0, *range(2, g.ndim)), 0)
+    vy = 0.01 * np.sin(4.0 * DPI * x[:, 0]) + amp * hashed_uniform(seed, 1, g.transpose(1, 0, *range(2, g.ndim)), 1)
+    p = 2.5 * np.ones_like(rho)
+    prim = np.stack([rho, vx, vy, p], axis=1)
+    return hd_to_conserved(prim, gamma, 2)
+
+
+def orszag_tang_3d(mesh: UniformMesh, gamma: float = 5.0 / 3.0, seed: int = 20261202,
+                   amp: float = 1.0e-3, b_amp: float = 1.0, glm: bool = True,
+                   rho_field: np.ndarray | None = None) -> np.ndarray:
+    """3-D Orszag-Tang vortex: rho=25/(36 pi), p=5/(12 pi),
+    v=(-sin 2 pi y, sin 2 pi x, vz), B=b_amp/sqrt(4 pi) (-sin 2 pi y,
+    sin 4 pi x, 0), vz = ``amp`` seeded noise; psi=0 when GLM is on."""
+    x = mesh.cell_centers()
+    g = mesh.global_index()
+    gi = g.transpose(1, 0, *range(2, g.ndim))
+    rho0 = 25.0 / 36.0 / DPI
+    p0 = 5.0 / 12.0 / DPI
+    b0 = b_amp / np.sqrt(4.0 * DPI)
+    rho = rho0 * np.ones_like(x[:, 0])
+    if rho_field is not None:
+        rho = rho * (1.0 + rho_field)
+    vx = -np.sin(2.0 * DPI * x[:, 1])
+    vy = np.sin(2.0 * DPI * x[:, 0])
+    vz = amp * hashed_uniform(seed, 1, gi, 2)
+    p = p0 * np.ones_like(rho)
+    bx = -b0 * np.sin(2.0 * DPI * x[:, 1])
+    by = b0 * np.sin(4.0 * DPI * x[:, 0])
+    bz = np.zeros_like(rho)
+    comps = [rho, vx, vy, vz, p, bx, by, bz]
+    if glm:
+        comps.append(np.zeros_like(rho))
+    prim = np.stack(comps, axis=1)
+    return mhd_to_conserved(prim, gamma, 3)
+
+
+def _gradient_c(q: np.ndarray, axis: int, dx: float) -> np.ndarray:
+    """Central gradient of src/mod_geometry.t:329-403 (nth=1) on a
+    [..., (k,) j, i] array; returns values for interior+ghost cells with the
+    outermost layer left at zero."""
+    out = np.zeros_like(q)
+    n = q.shape[axis]
+    sl_c = [slice(None)] * q.ndim
+    sl_p = [slice(None)] * q.ndim
+    sl_m = [slice(None)] * q.ndim
+    sl_c[axis] = slice(1, n - 1)
+    sl_p[axis] = slice(2, n)
+    sl_m[axis] = slice(0, n - 2)
+    out[tuple(sl_c)] = (q[tuple(sl_p)] - q[tuple(sl_m)]) * 0.5
+    return out / dx
+
+
+def field_loop_az(mesh: UniformMesh, x: np.ndarray, v0, t: float,
+                  A0: float = 1.0e-3, R0: float = 0.3) -> np.ndarray:
+    """vectorpot_z of tests/mhd/field_loop_2.5D/mod_usr.t (periodic shift)."""
+    lo, hi = mesh.xprobmin, mesh.xprobmax
+    x1 = x[:, 0] - v0[0] * t - lo[0]
+    x1 = lo[0] + (x1 - np.floor(x1 / (hi[0] - lo[0])) * (hi[0] - lo[0]))
+    x2 = x[:, 1] - v0[1] * t - lo[1]
+    x2 = lo[1] + (x2 - np.floor(x2 / (hi[1] - lo[1])) * (hi[1] - lo[1]))
+    r2 = x1 ** 2 + x2 ** 2
+    return np.where(r2 < R0 ** 2, A0 * (R0 - np.sqrt(r2)), 0.0)
+
+
+def field_loop_bfield(mesh: UniformMesh, x: np.ndarray, v0, t: float):
+    """bfield_solution (init_from_vectorpot branch): B = curl(Az z) with the
+    central gradient, evaluated on the interior."""
+    az = field_loop_az(mesh, x, v0, t)
+    b1 = _gradient_c(az, az.ndim - 2, mesh.dx[1])   # d/dy
+    b2 = -_gradient_c(az, az.ndim - 1, mesh.dx[0])  # -d/dx
+    return b1, b2
+
+
+def field_loop_2p5d(mesh: UniformMesh, gamma: float = 5.0 / 3.0) -> np.ndarray:
+    """Advected field loop, tests/mhd/field_loop_2.5D/mod_usr.t (iprob=1):
+    rho=1, v=(2,1,0), p=1, B from the vector potential; the 8 MHD
+    variables only (the three diagnostic variables of that module are
+    computed at output time)."""
+    x = mesh.cell_centers()
+    v0 = (2.0, 1.0)
+    b1, b2 = field_loop_bfield(mesh, x, v0, 0.0)
+    one = np.ones_like(b1)
+    prim = np.stack([one, 2.0 * one, 1.0 * one, 0.0 * one, 1.0 * one, b1, b2, 0.0 * one], axis=1)
+    return mhd_to_conserved(prim, gamma, 3)
+
+
+def field_loop_output_vars(mesh: UniformMesh, w: np.ndarray, t: float) -> np.ndarray:
+    """set_output_vars of the field-loop module: div B (2nd and 4th order,
+    src/mod_geometry.t:582-655) and |B - B_exact|; returns [nb, 3, ...]."""
+    ng = mesh.nghost
+    dx, dy = mesh.dx
+    bx, by = w[:, 5], w[:, 6]
+    ny, nx = w.shape[-2:]
+    d2 = np.zeros_like(bx)
+    d4 = np.zeros_like(bx)
+    I = slice(ng, nx - ng)
+    J = slice(ng, ny - ng)
+    div = np.zeros_like(bx[:, J, I])
+    div = div + 0.5 * (bx[:, J, ng + 1:nx - ng + 1] - bx[:, J, ng - 1:nx - ng - 1]) * (1.0 / dx)
+    div = div + 0.5 * (by[:, ng + 1:ny - ng + 1, I] - by[:, ng - 1:ny - ng - 1, I]) * (1.0 / dy)
+    d2[:, J, I] = div
+    div4 = np.zeros_like(bx[:, J, I])
+    div4 = div4 + (-bx[:, J, ng + 2:nx - ng + 2] + 8.0 * bx[:, J, ng + 1:nx - ng + 1]
+                   - 8.0 * bx[:, J, ng - 1:nx - ng - 1] + bx[:, J, ng - 2:nx - ng - 2]) / (12.0 * dx)
+    div4 = div4 + (-by[:, ng + 2:ny - ng + 2, I] + 8.0 * by[:, ng + 1:ny - ng + 1, I]
+                   - 8.0 * by[:, ng - 1:ny - ng - 1, I] + by[:, ng - 2:ny - ng - 2, I]) / (12.0 * dy)
+    d4[:, J, I] = div4
+    x = mesh.cell_centers()
+    e1, e2 = field_loop_bfield(mesh, x, (2.0, 1.0), t)
+    err = np.sqrt((bx - e1) ** 2 + (by - e2) ** 2)
+    berr = np.zeros_like(bx)
+    berr[:, J, I] = err[:, J, I]
+    return np.stack([d2, d4, berr], axis=1)
+
+
+def blast_wave(mesh: UniformMesh, gamma: float = 5.0 / 3.0, seed: int = 20261202,
+               rho0: float = 1.0, p_out: float = 1.0e-2, p_in: float = 10.0,
+               radius: float = 0.05, noise: float = 0.01) -> np.ndarray:
+    """Hydrodynamic blast wave: rho=1 with seeded relative density noise,
+    v=0, p=p_in inside r<radius around the domain centre, p_out outside."""
+    x = mesh.cell_centers()
+    g = mesh.global_index()
+    gi = g.transpose(1, 0, *range(2, g.ndim))
+    nd = mesh.ndim
+    r2 = np.zeros_like(x[:, 0])
+    for d in range(nd):
+        c = 0.5 * (mesh.xprobmin[d] + mesh.xprobmax[d])
+        r2 = r2 + (x[:, d] - c) ** 2
+    rho = rho0 * (1.0 + noise * hashed_uniform(seed, 1, gi, 0))
+    p = np.where(r2 < radius ** 2, p_in, p_out)
+    comps = [rho] + [np.zeros_like(rho)] * nd + [p]
+    return hd_to_conserved(np.stack(comps, axis=1), gamma, nd)

@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X finite-volume block update on the 3-D ideal MHD
+benchmark (Orszag-Tang + GLM, 256^3 in 16^3 blocks, ssprk3 / HLL / cada3,
+CFL 0.6), in Mcell-updates/s: one cell-update = one interior cell advanced by
+one RK stage (the reference's own counter, src/amrvac.t:351,375).
+
+One "step" = setdt (device CFL reduction) + the three ssprk3 stages, on a
+level resident in HBM.  Prints one JSON line; see --help.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def workload_par(precision: str):
+    from amrvac_amd.host import Par
+    return Par(physics="mhd", ndim=3, ndir=3, gamma=5.0 / 3.0, typedivbfix="glm", time_stepper="threestep",
+               flux_scheme="hll", limiter="cada3", typeboundspeed="Einfeldt", courantpar=0.6,
+               precision=precision)
+
+
+def workload_mesh(n: int):
+    from amrvac_amd.mesh import UniformMesh
+    return UniformMesh(3, (n, n, n), (16, 16, 16), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), ("periodic",) * 6)
+
+
+def run_device(par, mesh, w0, steps, warmup, label):
+    from amrvac_amd.fv import DeviceFV
+    from amrvac_amd.host import TimeIntegrator
+    be = DeviceFV(par, mesh)
+    be.set_state(0, w0)
+    ti = TimeIntegrator(par, mesh, be)
+    for _ in range(warmup):
+        ti.step()
+    be.sync()
+    be.sweep_timing(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ti.step()
+    be.sync()
+    t1 = time.perf_counter()
+    avg_ms, nlaunch = be.sweep_timing()
+    print(f"[bench] {label}: {steps} steps in {t1 - t0:.4f} s, sweep avg {avg_ms:.4f} ms x {nlaunch}",
+          file=sys.stderr, flush=True)
+    return be, ti, t1 - t0, avg_ms, nlaunch
+
+
+def cpu_baseline(seconds: float, n: int = 64):
+    """The FP64 CPU oracle on a bounded sample of the same workload."""
+    from oracle.oracle import OracleFV
+    from amrvac_amd import problems
+    from amrvac_amd.host import TimeIntegrator
+    threads = os.cpu_count() or 1
+    par = workload_par("fp64")
+    mesh = workload_mesh(n)
+    be = OracleFV(par, mesh, threads=threads)
+    be.set_state(0, problems.orszag_tang_3d(mesh))
+    be.getbc(0)
+    ti = TimeIntegrator(par, mesh, be)
+    ti.step()  # warm-up step (page faults, thread start)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        ti.step()
+        k += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    cells = n ** 3
+    return {"value": cells * 3 * k / dt / 1e6, "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{k} ssprk3 steps of the same 3-D MHD-GLM setup at {n}^3 in 16^3 blocks, "
+                      f"FP64 CPU restatement, OpenMP over {threads} threads"}
+
+
+def load_traffic(name: str):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            return json.load(fh).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=256, help="cells per side (multiple of 16)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline budget (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the block update runs on a single MI355X in this build "
+              "(multi-GPU sharding is a design note, not an implemented path).", file=sys.stderr)
+        return 2
+    from amrvac_amd import problems
+    n = args.size
+    mesh = workload_mesh(n)
+    t0 = time.perf_counter()
+    w0 = problems.orszag_tang_3d(mesh)
+    print(f"[bench] initial condition {n}^3: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    cells = n ** 3
+    stages = 3
+    nw = 9
+    b_alg = 3 * nw * 8  # stage input + RK base + stage output, FP64 state
+
+    par64 = workload_par("fp64")
+    be64, ti64, t64, ms64, _ = run_device(par64, mesh, w0, args.steps, args.warmup, "fp64")
+    value64 = cells * stages * args.steps / t64 / 1e6
+    achieved = b_alg * cells / (ms64 * 1e-3) / 1e9
+    result = {
+        "metric": "Mcell-updates/s 3D MHD 256^3, FP64 vs mixed @L1<=1e-5; adjoint/forward cost",
+        "value": round(value64, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t64 / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Orszag-Tang initial condition)",
+        "config": {"workload": f"3-D ideal MHD + GLM, Orszag-Tang, {n}^3 in 16^3 blocks "
+                               f"({mesh.nblocks} blocks), ssprk3 / HLL / cada3 / Einfeldt, CFL 0.6",
+                   "cells": cells, "stages_per_step": stages},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("fp64"),
+                     "kernel": "fv::sweep_kernel<MHD-GLM 3-D, double, HLL>", "avg_launch_ms": round(ms64, 4),
+                     "alg_bytes_per_launch": b_alg * cells},
+    }
+    if not args.no_mixed:
+        w64 = mesh.interior(be64.get_state(0))
+        del be64
+        parmx = workload_par("mixed")
+        bemx, timx, tmx, msmx, _ = run_device(parmx, mesh, w0, args.steps, args.warmup, "mixed")
+        wmx = mesh.interior(bemx.get_state(0))
+        del bemx
+        l1 = [float(np.abs(wmx[:, iw] - w64[:, iw]).sum() / max(np.abs(w64[:, iw]).sum(), 1e-300))
+              for iw in range(nw)]
+        valmx = cells * stages * args.steps / tmx / 1e6
+        achmx = b_alg * cells / (msmx * 1e-3) / 1e9
+        result["mixed"] = {"value": round(valmx, 2), "speedup_vs_fp64": round(valmx / value64, 3),
+                           "rel_l1_vs_fp64": [float(f"{v:.3e}") for v in l1],
+                           "mass_drift": float(f"{abs(wmx[:, 0].sum() - w64[:, 0].sum()) / w64[:, 0].sum():.3e}"),
+                           "roofline_frac": round(achmx / HBM_PEAK_GBS, 4), "avg_launch_ms": round(msmx, 4),
+                           "traffic": load_traffic("mixed")}
+        del w64, wmx
+    if args.cpu_seconds > 0:
+        try:
+            result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        except Exception as exc:  # the oracle is optional on a GPU box
+            result["cpu_baseline"] = {"value": None, "error": str(exc)}
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

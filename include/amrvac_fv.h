@@ -1,0 +1,145 @@
+/*
+ * amrvac_fv.h -- C-ABI of the MI355X-native finite-volume block update.
+ *
+ * Drop-in boundary: the reference's advect1 (src/mod_advance.t:611-666) and
+ * the routines it and setdt reach on the hot path.  A Fortran host binds
+ * these entry points through ISO_C_BINDING; every
+ * argument is a plain C scalar, pointer or size.  All blocks of one level
+ * live in HBM as one batched array per state slot (0 = ps, 1 = ps1,
+ * 2 = ps2, src/mod_physicaldata.t:111-122).
+ *
+ * Conventions
+ *  - Return value 0 on success, a negative AMRVAC_FV_E* code otherwise; the
+ *    host maps a nonzero return onto its mpistop (src/mod_comm_lib.t:207-218).
+ *    amrvac_fv_last_error() gives the message.
+ *  - Codes for flux schemes, limiters and boundary types are the reference's
+ *    own integer parameters (src/mod_global_parameters.t:457-469, :541-550;
+ *    src/limiter/mod_limiter.t:15-40).
+ *  - Host arrays of block state use the reference layout w(ixG^S,1:nw) per
+ *    block (Fortran order: x fastest, then y, z, then the variable), blocks
+ *    concatenated in the order given to amrvac_fv_set_mesh.
+ *  - Never call from inside an OpenMP parallel region; calls on one context
+ *    are serialised on that context's HIP stream.
+ */
+#ifndef AMRVAC_FV_H
+#define AMRVAC_FV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMRVAC_FV_OK 0
+#define AMRVAC_FV_EINVAL (-1)   /* bad argument or unsupported option */
+#define AMRVAC_FV_EHIP (-2)     /* HIP runtime error */
+#define AMRVAC_FV_ESTATE (-3)   /* call out of order (e.g. no mesh yet) */
+#define AMRVAC_FV_ENOMEM (-4)   /* device allocation failed */
+
+/* precision of reconstruction / Riemann / wave speeds; the conserved state,
+   flux-divergence accumulation, pressure recovery and dt stay FP64 */
+#define AMRVAC_FV_PREC_FP64 0
+#define AMRVAC_FV_PREC_MIXED 1
+
+typedef struct amrvac_fv_config {
+  int ndim;            /* 2 or 3 (compile-time ndim of the reference) */
+  int ndir;            /* vector components, ndim <= ndir <= 3 */
+  int physics;         /* 1 = hd (src/hd/mod_hd_phys.t), 2 = mhd (src/mhd/mod_mhd_phys.t) */
+  double gamma;        /* hd_gamma / mhd_gamma */
+  int divb;            /* typedivbfix: 0 none, 1 glm, 4 linde */
+  double glm_alpha;    /* mhd_glm_alpha (0.5) */
+  int glm_extended;    /* mhd_glm_extended (1) */
+  double divbdiff;     /* Linde coefficient (0.8) */
+  int limiter;         /* minmod 1 .. cada3 9, weno5 16 */
+  int boundspeed;      /* typeboundspeed: 1 Einfeldt, 2 cmaxmean, 3 cmaxleftright */
+  double tvdlfeps;     /* 1.0 */
+  double cada3_radius; /* 0.1 */
+  double small_pressure, small_density; /* 0.0 */
+  int check_small_values;               /* 1 */
+  int nghost;          /* 2, or 3 for weno5 */
+  int block_nx[3];     /* interior cells per block; x and y must be 16 */
+  int precision;       /* AMRVAC_FV_PREC_* */
+} amrvac_fv_config;
+
+typedef struct amrvac_fv_ctx amrvac_fv_ctx;
+
+/* Replaces: the physics/method set-up of hd_phys_init / mhd_phys_init and
+   read_par_files (src/hd/mod_hd_phys.t:159-370, src/mhd/mod_mhd_phys.t:
+   296-949, src/io/mod_input_output.t:292-1080) as far as the block update
+   reads it.  Selects the current HIP device's kernels; allocates nothing on
+   the device yet. */
+int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** ctx);
+int amrvac_fv_finalize(amrvac_fv_ctx* ctx);
+
+/* HIP stream (hipStream_t passed as void*) for all work of this context;
+   NULL selects the context's own stream. */
+int amrvac_fv_set_stream(amrvac_fv_ctx* ctx, void* stream);
+
+/* Replaces: the level's connectivity after build_connectivity
+   (src/mod_functions_connectivity.t:55) and alloc_node geometry
+   (src/amr/mod_amr_solution_node.t:60-186) for one level.
+   dx[ndim]: cell size.  neighbor[nb*3^ndim]: same-level neighbour block
+   for direction n = (i1+1)+3(i2+1)+9(i3+1), negative = physical boundary.
+   xlohi[nb*ndim*2]: block extents.  bc[2*ndim]: boundary type per side.
+   Allocates the three state slots (interior cells only, FP64). */
+int amrvac_fv_set_mesh(amrvac_fv_ctx* ctx, int nblocks, const double* dx,
+                       const int* neighbor, const double* xlohi, const int* bc);
+
+/* Host <-> device copies of one slot in the reference layout (ghost cells
+   included).  upload reads the interior; download fills the ghost cells as
+   getbc would (same-level copies and physical boundaries,
+   src/mod_ghostcells_update.t:350-566) with a device kernel. */
+int amrvac_fv_upload(amrvac_fv_ctx* ctx, int slot, const double* w_host);
+int amrvac_fv_download(amrvac_fv_ctx* ctx, int slot, double* w_host);
+
+/* Replaces: getdt_courant, maxsum (src/mod_dt.t:160-287) over every block,
+   with the min over blocks (the MPI_ALLREDUCE of :81-86 on one rank).
+   dtmin_out = min_blocks courantpar / max_cells sum_d cmax_d/dx_d;
+   cmax_out = max over cells and directions of cmax (cmax_global for GLM,
+   :128-129). */
+int amrvac_fv_getdt(amrvac_fv_ctx* ctx, int slot, double courantpar,
+                    double* dtmin_out, double* cmax_out);
+
+/* Replaces: advect1(method, dtfactor, idim^LIM, qtC, psa, qt, psb)
+   (src/mod_advance.t:611-666) for idimmin=1, idimmax=ndim on one level:
+   finite_volume (fs_hll/fs_hllc/fs_tvdlf) or hancock (fs_hancock) for every
+   block from slot psa into slot psb with qdt = dtfactor*dt, followed by the
+   ghost-cell update of psb (on a uniform level ghost cells are read from
+   the neighbours by the next sweep, so nothing is stored).
+   Fused RK combination (the whole-array loops of src/mod_advance.t:251-267):
+     comb_slot < 0                     psb holds the initial wnew;
+     comb_slot >= 0, comb_b == 0       psb = ps[comb_slot] first (ps1 = ps);
+     comb_slot >= 0, comb_b != 0       psb = comb_a*ps[comb_slot] + comb_b*ps[psa].
+   cmax_global is the GLM speed c_h.  *crash_out (may be NULL) is set to 1
+   when a small-value check of the reference would set crash. */
+int amrvac_fv_advect1(amrvac_fv_ctx* ctx, int method, double dtfactor,
+                      double dt, double cmax_global, int psa, int psb,
+                      int comb_slot, double comb_a, double comb_b,
+                      int* crash_out);
+
+/* ps[dst] = ps[src] (interior cells) */
+int amrvac_fv_copy(amrvac_fv_ctx* ctx, int dst, int src);
+
+/* Replaces: get_volume_average for power 1 and 2
+   (src/io/mod_input_output.t:2694-2729); modes[2*nw] */
+int amrvac_fv_volume_average(amrvac_fv_ctx* ctx, int slot, double* modes);
+
+/* Device pointer and byte size of a slot (interior-only layout
+   [nb][nw][bnz][bny][bnx]), for zero-copy use by PyTorch on the same
+   stream. */
+int amrvac_fv_slot_pointer(amrvac_fv_ctx* ctx, int slot, void** dptr,
+                           long long* nbytes);
+
+/* Wait for the context's stream. */
+int amrvac_fv_sync(amrvac_fv_ctx* ctx);
+
+/* Average duration (ms) of the sweep kernel launches since the last reset,
+   measured with HIP events on the context's stream; count = launches. */
+int amrvac_fv_sweep_timing(amrvac_fv_ctx* ctx, int reset, double* avg_ms,
+                           int* count);
+
+/* Message of the last failing call on this thread. */
+const char* amrvac_fv_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

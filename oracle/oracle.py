@@ -1,0 +1,173 @@
+"""ctypes binding of the FP64 CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke`` and bench.py's CPU-baseline leg may
+import this module; the product path never does.  ``OracleFV`` implements
+the backend protocol of :mod:`amrvac_amd.host` on the C restatement in
+amrvac_oracle.c.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liborc.so")
+
+PHYS = {"hd": 1, "mhd": 2}
+DIVB = {"none": 0, "glm": 1, "linde": 4}
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int), ("ndir", ctypes.c_int), ("phys", ctypes.c_int),
+        ("gamma", ctypes.c_double), ("divb", ctypes.c_int),
+        ("glm_alpha", ctypes.c_double), ("glm_extended", ctypes.c_int),
+        ("divbdiff", ctypes.c_double), ("limiter", ctypes.c_int),
+        ("boundspeed", ctypes.c_int), ("tvdlfeps", ctypes.c_double),
+        ("cada3_radius", ctypes.c_double), ("small_pressure", ctypes.c_double),
+        ("small_density", ctypes.c_double), ("check_small_values", ctypes.c_int),
+        ("nghost", ctypes.c_int), ("block_nx", ctypes.c_int * 3),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile liborc.so with the Makefile beside this file."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "-s"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_create.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_int, dp, ip, dp, ip,
+                                 ctypes.POINTER(vp)]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_set_w.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_get_w.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_get_x.argtypes = [vp, dp]
+        L.orc_getdt_courant.argtypes = [vp, ctypes.c_int, ctypes.c_double, dp, dp]
+        L.orc_advect1.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_axpby.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                ctypes.c_double, ctypes.c_int]
+        L.orc_copy.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.orc_getbc.argtypes = [vp, ctypes.c_int]
+        L.orc_volume_average.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_crash_flag.argtypes = [vp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def make_params(par, block_nx) -> OrcParams:
+    from amrvac_amd.host import BOUNDSPEED, LIMITERS
+    p = OrcParams()
+    p.ndim = par.ndim
+    p.ndir = par.ndir
+    p.phys = PHYS[par.physics]
+    p.gamma = par.gamma
+    p.divb = DIVB[par.typedivbfix] if par.physics == "mhd" else 0
+    p.glm_alpha = par.mhd_glm_alpha
+    p.glm_extended = 1 if par.mhd_glm_extended else 0
+    p.divbdiff = par.divbdiff
+    p.limiter = LIMITERS[par.limiter]
+    p.boundspeed = BOUNDSPEED[par.typeboundspeed]
+    p.tvdlfeps = par.tvdlfeps
+    p.cada3_radius = par.cada3_radius
+    p.small_pressure = par.small_pressure
+    p.small_density = par.small_density
+    p.check_small_values = 1 if par.check_small_values else 0
+    p.nghost = par.nghost
+    for d in range(3):
+        p.block_nx[d] = block_nx[d] if d < par.ndim else 1
+    return p
+
+
+class OracleFV:
+    """Backend protocol of amrvac_amd.host on the C oracle."""
+
+    def __init__(self, par, mesh, threads: int = 0):
+        self.par = par
+        self.mesh = mesh
+        self.L = lib()
+        self.L.orc_set_threads(threads)
+        p = make_params(par, mesh.block_nx)
+        h = ctypes.c_void_p()
+        dx = np.ascontiguousarray(mesh.dx, dtype=np.float64)
+        nbr = np.ascontiguousarray(mesh.neighbor, dtype=np.int32)
+        xl = np.ascontiguousarray(mesh.xlohi, dtype=np.float64)
+        bc = np.ascontiguousarray(mesh.bc_codes, dtype=np.int32)
+        rc = self.L.orc_create(ctypes.byref(p), mesh.nblocks, _dptr(dx), _iptr(nbr), _dptr(xl),
+                               _iptr(bc), ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"orc_create failed: {rc}")
+        self.h = h
+        self.nw = par.nw
+        self.shape = (mesh.nblocks, self.nw) + mesh.nxg[::-1]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def set_state(self, slot: int, w: np.ndarray) -> None:
+        a = np.ascontiguousarray(w, dtype=np.float64)
+        assert a.shape == self.shape, (a.shape, self.shape)
+        self.L.orc_set_w(self.h, slot, _dptr(a))
+
+    def get_state(self, slot: int) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        self.L.orc_get_w(self.h, slot, _dptr(out))
+        return out
+
+    def getbc(self, slot: int) -> None:
+        self.L.orc_getbc(self.h, slot)
+
+    def getdt_courant(self, slot: int, courantpar: float):
+        dt = ctypes.c_double()
+        cm = ctypes.c_double()
+        self.L.orc_getdt_courant(self.h, slot, courantpar, ctypes.byref(dt), ctypes.byref(cm))
+        return dt.value, cm.value
+
+    def copy(self, dst: int, src: int) -> None:
+        self.L.orc_copy(self.h, dst, src)
+
+    def advect1(self, method, dtfactor, dt, cmax_global, psa, psb, comb) -> None:
+        if comb is not None:
+            if comb[0] == "copy":
+                self.L.orc_copy(self.h, psb, comb[1])
+            else:
+                _, a, x, b = comb
+                self.L.orc_axpby(self.h, psb, a, x, b, psa)
+        rc = self.L.orc_advect1(self.h, method, dtfactor, dt, cmax_global, psa, psb)
+        if rc != 0:
+            raise RuntimeError(f"orc_advect1 failed: {rc}")
+
+    def volume_average(self, slot: int) -> np.ndarray:
+        out = np.zeros(2 * self.nw, dtype=np.float64)
+        self.L.orc_volume_average(self.h, slot, _dptr(out))
+        return out
+
+    def crash(self) -> bool:
+        return bool(self.L.orc_crash_flag(self.h))
