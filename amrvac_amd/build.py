@@ -49,7 +49,10 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
         src = os.path.join(CSRC, u + ".hip")
         tasks.append((src, os.path.join(BUILD, u + ".o"), []))
         tasks.append((src, os.path.join(BUILD, u + "_x.o"), ["-DFV_EXACT", "-ffp-contract=off"]))
-    tasks.append((os.path.join(CSRC, "fv_capi.hip"), os.path.join(BUILD, "fv_capi.o"), []))
+    # the dt / volume-average / ghost-fill kernels are memory bound; they are
+    # built without contraction so that dt and cmax_global round like the
+    # CPU restatement's in every precision mode
+    tasks.append((os.path.join(CSRC, "fv_capi.hip"), os.path.join(BUILD, "fv_capi.o"), ["-ffp-contract=off"]))
     jobs = jobs or min(len(tasks), max(1, (os.cpu_count() or 4)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         results = list(ex.map(lambda t: _compile(*t), tasks))

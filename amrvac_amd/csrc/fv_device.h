@@ -42,6 +42,43 @@ struct FVParams {
 namespace FV_NS {
 using fvc::FVParams;
 
+// Arithmetic helpers.  The contraction-free build (FV_EXACT) keeps IEEE
+// division / square root and the reference's MIN/MAX semantics, so that every
+// operation rounds like the CPU restatement; the production builds use the
+// hardware min/max and, in FP32, the single-instruction reciprocal and square
+// root (about 1 ulp).
+#ifdef FV_EXACT
+constexpr bool kExact = true;
+#else
+constexpr bool kExact = false;
+#endif
+
+template <class R>
+__device__ __forceinline__ R mn2(R a, R b) {
+  if constexpr (kExact) return a < b ? a : b;
+  else return fmin(a, b);
+}
+template <class R>
+__device__ __forceinline__ R mx2(R a, R b) {
+  if constexpr (kExact) return a > b ? a : b;
+  else return fmax(a, b);
+}
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) {
+  if constexpr (kExact) return a / b;
+  else return a * __builtin_amdgcn_rcpf(b);
+}
+__device__ __forceinline__ double frcp(double a) { return 1.0 / a; }
+__device__ __forceinline__ float frcp(float a) {
+  if constexpr (kExact) return 1.0f / a;
+  else return __builtin_amdgcn_rcpf(a);
+}
+__device__ __forceinline__ double fsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float fsqrt(float a) {
+  if constexpr (kExact) return sqrtf(a);
+  else return __builtin_amdgcn_sqrtf(a);
+}
+
 // ---------------------------------------------------------------------------
 // physics traits: variable indices as in hd_phys_init / mhd_phys_init
 // (rho_, mom(:), e_, mag(:), psi_; src/hd/mod_hd_phys.t:217-228,
@@ -152,7 +189,7 @@ __device__ __forceinline__ void phys_flux(const R* wC, const R* w, R ch, R* f) {
 // mhd_get_csound_prim src/mhd/mod_mhd_phys.t:3329-3379 (no B0, no Hall)
 template <class P, class R, int ID>
 __device__ __forceinline__ R fast_speed(const R* w, R gamma) {
-  const R inv_rho = R(1) / w[P::RHO];
+  const R inv_rho = frcp(w[P::RHO]);
   const R cs = gamma * w[P::E] * inv_rho;
   R b2 = w[P::MAG] * w[P::MAG];
 #pragma unroll
@@ -161,7 +198,7 @@ __device__ __forceinline__ R fast_speed(const R* w, R gamma) {
   const R bn = w[P::MAG + ID];
   R av = cfast2 * cfast2 - R(4) * cs * (bn * bn) * inv_rho;
   if (av < R(0)) av = R(0);
-  return sqrt(R(0.5) * (cfast2 + sqrt(av)));
+  return fsqrt(R(0.5) * (cfast2 + fsqrt(av)));
 }
 
 // hd_get_cbounds src/hd/mod_hd_phys.t:891-1001,
@@ -173,15 +210,15 @@ __device__ __forceinline__ void cbounds(const R* wLC, const R* wRC, const R* wLp
   constexpr int MN = P::MOM + ID;
   if constexpr (!P::MHD) {
     if (fp.boundspeed == 1) {
-      const R tmp1 = sqrt(wLp[P::RHO]);
-      const R tmp2 = sqrt(wRp[P::RHO]);
-      const R tmp3 = R(1) / (tmp1 + tmp2);
+      const R tmp1 = fsqrt(wLp[P::RHO]);
+      const R tmp2 = fsqrt(wRp[P::RHO]);
+      const R tmp3 = frcp(tmp1 + tmp2);
       const R umean = (wLp[MN] * tmp1 + wRp[MN] * tmp2) * tmp3;
-      const R csL = g * wLp[P::E] / wLp[P::RHO];
-      const R csR = g * wRp[P::E] / wRp[P::RHO];
+      const R csL = fdiv(g * wLp[P::E], wLp[P::RHO]);
+      const R csR = fdiv(g * wRp[P::E], wRp[P::RHO]);
       const R dv = wRp[MN] - wLp[MN];
       R dmean = (tmp1 * csL + tmp2 * csR) * tmp3 + R(0.5) * tmp1 * tmp2 * (tmp3 * tmp3) * (dv * dv);
-      dmean = sqrt(dmean);
+      dmean = fsqrt(dmean);
       if constexpr (WANT_CMIN) {
         cmin = umean - dmean;
         cmax = umean + dmean;
@@ -199,33 +236,33 @@ __device__ __forceinline__ void cbounds(const R* wLC, const R* wRC, const R* wLp
         msq = msq + m * m;
       }
       const R e = R(0.5) * (wLC[P::E] + wRC[P::E]);
-      const R tmp1 = mn / rho;
-      const R pth = (g - R(1)) * (e - R(0.5) * msq / rho);
+      const R tmp1 = fdiv(mn, rho);
+      const R pth = (g - R(1)) * (e - fdiv(R(0.5) * msq, rho));
       if (fp.check_small && pth < R(fp.small_pressure)) crash = 1;
-      const R cs = sqrt(g * pth / rho);
+      const R cs = fsqrt(fdiv(g * pth, rho));
       if constexpr (WANT_CMIN) {
-        cmax = fmax(tmp1 + cs, R(0));
-        cmin = fmin(tmp1 - cs, R(0));
+        cmax = mx2(tmp1 + cs, R(0));
+        cmin = mn2(tmp1 - cs, R(0));
       } else {
         cmax = fabs(tmp1) + cs;
       }
     } else {
-      const R csL = g * wLp[P::E] / wLp[P::RHO];
-      const R csR = g * wRp[P::E] / wRp[P::RHO];
-      const R cs = fmax(sqrt(csL), sqrt(csR));
-      if constexpr (WANT_CMIN) cmin = fmin(wLp[MN], wRp[MN]) - cs;
-      cmax = fmax(wLp[MN], wRp[MN]) + cs;
+      const R csL = fdiv(g * wLp[P::E], wLp[P::RHO]);
+      const R csR = fdiv(g * wRp[P::E], wRp[P::RHO]);
+      const R cs = mx2(fsqrt(csL), fsqrt(csR));
+      if constexpr (WANT_CMIN) cmin = mn2(wLp[MN], wRp[MN]) - cs;
+      cmax = mx2(wLp[MN], wRp[MN]) + cs;
     }
   } else {
     if (fp.boundspeed == 1) {
       const R csL = fast_speed<P, R, ID>(wLp, g);
       const R csR = fast_speed<P, R, ID>(wRp, g);
-      const R tmp1 = sqrt(wLp[P::RHO]);
-      const R tmp2 = sqrt(wRp[P::RHO]);
-      const R tmp3 = R(1) / (tmp1 + tmp2);
+      const R tmp1 = fsqrt(wLp[P::RHO]);
+      const R tmp2 = fsqrt(wRp[P::RHO]);
+      const R tmp3 = frcp(tmp1 + tmp2);
       const R umean = (wLp[MN] * tmp1 + wRp[MN] * tmp2) * tmp3;
       const R dv = wRp[MN] - wLp[MN];
-      const R dmean = sqrt((tmp1 * (csL * csL) + tmp2 * (csR * csR)) * tmp3 +
+      const R dmean = fsqrt((tmp1 * (csL * csL) + tmp2 * (csR * csR)) * tmp3 +
                            R(0.5) * tmp1 * tmp2 * (tmp3 * tmp3) * (dv * dv));
       if constexpr (WANT_CMIN) {
         cmin = umean - dmean;
@@ -239,17 +276,17 @@ __device__ __forceinline__ void cbounds(const R* wLC, const R* wRC, const R* wLp
       for (int iw = 0; iw < P::NW; ++iw) wm[iw] = R(0.5) * (wLp[iw] + wRp[iw]);
       const R cs = fast_speed<P, R, ID>(wm, g);
       if constexpr (WANT_CMIN) {
-        cmax = fmax(wm[MN] + cs, R(0));
-        cmin = fmin(wm[MN] - cs, R(0));
+        cmax = mx2(wm[MN] + cs, R(0));
+        cmin = mn2(wm[MN] - cs, R(0));
       } else {
         cmax = fabs(wm[MN]) + cs;
       }
     } else {
       const R csL = fast_speed<P, R, ID>(wLp, g);
       const R csR = fast_speed<P, R, ID>(wRp, g);
-      const R cs = fmax(csL, csR);
-      if constexpr (WANT_CMIN) cmin = fmin(wLp[MN], wRp[MN]) - cs;
-      cmax = fmax(wLp[MN], wRp[MN]) + cs;
+      const R cs = mx2(csL, csR);
+      if constexpr (WANT_CMIN) cmin = mn2(wLp[MN], wRp[MN]) - cs;
+      cmax = mx2(wLp[MN], wRp[MN]) + cs;
     }
   }
 }
@@ -258,11 +295,6 @@ __device__ __forceinline__ void cbounds(const R* wLC, const R* wRC, const R* wLp
 // slope limiters, dwlimiter2 src/limiter/mod_limiter.t:128-334 for one cell:
 // a = dwC(i-1), b = dwC(i); gives ldw(i) and rdw(i)
 // ---------------------------------------------------------------------------
-template <class R>
-__device__ __forceinline__ R mn2(R a, R b) { return a < b ? a : b; }
-template <class R>
-__device__ __forceinline__ R mx2(R a, R b) { return a > b ? a : b; }
-
 template <class R>
 __device__ __forceinline__ void cada3_side(R th, R eta, R& out) {
   const R cadalfa = R(0.5), cadbeta = R(2), cadgamma = R(1.6);
@@ -286,14 +318,14 @@ __device__ __forceinline__ void dwlimiter(int lim_rt, R a, R b, R rdelinv, R& ld
     t = t * mx2(R(0), mn2(fabs(b), t * a));
     ldw = t; rdw = t;
   } else if (lim == LIM_VANLEER) {
-    const R t = R(2) * mx2(a * b, R(0)) / (b + a + R(1.0e-12));
+    const R t = fdiv(R(2) * mx2(a * b, R(0)), b + a + R(1.0e-12));
     ldw = t; rdw = t;
   } else if (lim == LIM_CADA3) {
     const R eps = R(1.4901161193847656e-08);  // sqrt(epsilon(1d0))
     const R eta = (b * b + a * a) * rdelinv;
     R l, r;
-    cada3_side(a / (b + copysign(eps, b)), eta, l);
-    cada3_side(b / (a + copysign(eps, a)), eta, r);
+    cada3_side(fdiv(a, b + copysign(eps, b)), eta, l);
+    cada3_side(fdiv(b, a + copysign(eps, a)), eta, r);
     ldw = l * b;
     rdw = r * a;
   } else if (lim == LIM_KOREN) {
@@ -314,9 +346,31 @@ __device__ __forceinline__ void dwlimiter(int lim_rt, R a, R b, R rdelinv, R& ld
     t = t * mx2(mx2(R(0), mn2(R(2) * fabs(b), t * a)), mn2(fabs(b), R(2) * t * a));
     ldw = t; rdw = t;
   } else {  // albada
-    const R t = (a * (b * b + R(1.0e-12)) + b * (a * a + R(1.0e-12))) / (b * b + a * a + R(2.0e-12));
+    const R t = fdiv(a * (b * b + R(1.0e-12)) + b * (a * a + R(1.0e-12)), b * b + a * a + R(2.0e-12));
     ldw = t; rdw = t;
   }
+}
+
+// Limited slopes of one face: ldw(i) from (dwC(i-1), dwC(i)) and rdw(i+1)
+// from (dwC(i), dwC(i+1)).  For cada3 both quotients share the denominator
+// dwC(i) + sign(eps, dwC(i)); outside the bit-reproducible build it is
+// inverted once.
+template <class R, int LIM>
+__device__ __forceinline__ void face_limit(int lim_rt, R dm, R d0, R dp, R rdelinv, R& ldw, R& rdw) {
+  const int lim = LIM > 0 ? LIM : lim_rt;
+  if (!kExact && lim == LIM_CADA3) {
+    const R eps = R(1.4901161193847656e-08);
+    const R inv = frcp(d0 + copysign(eps, d0));
+    R l, r;
+    cada3_side(dm * inv, (d0 * d0 + dm * dm) * rdelinv, l);
+    cada3_side(dp * inv, (dp * dp + d0 * d0) * rdelinv, r);
+    ldw = l * d0;
+    rdw = r * d0;
+    return;
+  }
+  R r2, l2;
+  dwlimiter<R, LIM>(lim_rt, dm, d0, rdelinv, ldw, r2);
+  dwlimiter<R, LIM>(lim_rt, d0, dp, rdelinv, l2, rdw);
 }
 
 // WENO5limiter var=1 src/limiter/mod_weno.t:254-377, one variable, face
@@ -340,11 +394,11 @@ __device__ __forceinline__ void weno5(R wmm, R wm, R w0, R wp, R wpp, R wppp, R&
     const R b2 = bc1 * (t * t) + bc2 * (q * q);
     t = w0 + wpp - R(2) * wp; q = R(3) * w0 - R(4) * wp + wpp;
     const R b3 = bc1 * (t * t) + bc2 * (q * q);
-    t = b1 + weps; const R a1 = d1 / (t * t);
-    t = b2 + weps; const R a2 = d2 / (t * t);
-    t = b3 + weps; const R a3 = d3 / (t * t);
+    t = b1 + weps; const R a1 = fdiv(d1, t * t);
+    t = b2 + weps; const R a2 = fdiv(d2, t * t);
+    t = b3 + weps; const R a3 = fdiv(d3, t * t);
     const R as = a1 + a2 + a3;
-    wl = f1 * a1 / as + f2 * a2 / as + f3 * a3 / as;
+    wl = fdiv(f1 * a1, as) + fdiv(f2 * a2, as) + fdiv(f3 * a3, as);
   }
   {
     const R f1 = u11 * wppp + u12 * wpp + u13 * wp;
@@ -356,11 +410,11 @@ __device__ __forceinline__ void weno5(R wmm, R wm, R w0, R wp, R wpp, R wppp, R&
     const R b2 = bc1 * (t * t) + bc2 * (q * q);
     t = wp + wm - R(2) * w0; q = R(3) * wp - R(4) * w0 + wm;
     const R b3 = bc1 * (t * t) + bc2 * (q * q);
-    t = b1 + weps; const R a1 = d1 / (t * t);
-    t = b2 + weps; const R a2 = d2 / (t * t);
-    t = b3 + weps; const R a3 = d3 / (t * t);
+    t = b1 + weps; const R a1 = fdiv(d1, t * t);
+    t = b2 + weps; const R a2 = fdiv(d2, t * t);
+    t = b3 + weps; const R a3 = fdiv(d3, t * t);
     const R as = a1 + a2 + a3;
-    wr = f1 * a1 / as + f2 * a2 / as + f3 * a3 / as;
+    wr = fdiv(f1 * a1, as) + fdiv(f2 * a2, as) + fdiv(f3 * a3, as);
   }
 }
 
@@ -400,9 +454,8 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
       const R dm = st[C0][iw] - st[C0 - 1][iw];
       const R d0 = st[C0 + 1][iw] - st[C0][iw];
       const R dp = st[C0 + 2][iw] - st[C0 + 1][iw];
-      R ldw, rdw, l2, r2;
-      dwlimiter<R, LIM>(fp.limiter, dm, d0, rdel, ldw, r2);
-      dwlimiter<R, LIM>(fp.limiter, d0, dp, rdel, l2, rdw);
+      R ldw, rdw;
+      face_limit<R, LIM>(fp.limiter, dm, d0, dp, rdel, ldw, rdw);
       wLp[iw] = st[C0][iw] + R(0.5) * ldw;
       wRp[iw] = st[C0 + 1][iw] - R(0.5) * rdw;
     }
@@ -417,7 +470,7 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
     constexpr int B = P::MAG + ID, S = P::PSI, E = P::E;
     const R dB = wRp[B] - wLp[B];
     const R dPsi = wRp[S] - wLp[S];
-    wLp[B] = R(0.5) * (wRp[B] + wLp[B] - dPsi / ch);
+    wLp[B] = R(0.5) * (wRp[B] + wLp[B] - fdiv(dPsi, ch));
     wLp[S] = R(0.5) * (wRp[S] + wLp[S] - dB * ch);
     wRp[B] = wLp[B];
     wRp[S] = wLp[S];
@@ -446,7 +499,7 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
     // (src/mhd/mod_mhd_phys.t:580-588)
     constexpr int BTV = (P::MHD && !P::GLM && P::NDIM > 1) ? (P::MAG + ID) : -1;
     if constexpr (RIEM == RIEM_HLL) {
-      const R inv = R(1) / (cmax - cmin);
+      const R inv = frcp(cmax - cmin);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) {
         if (iw == BTV) {
@@ -456,10 +509,11 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
         } else if (cmax <= R(0)) {
           fC[iw] = fR[iw];
         } else {
-          fC[iw] = (cmax * fL[iw] - cmin * fR[iw] + cmin * cmax * (wRC[iw] - wLC[iw])) / (cmax - cmin);
+          const R num = cmax * fL[iw] - cmin * fR[iw] + cmin * cmax * (wRC[iw] - wLC[iw]);
+          if constexpr (kExact) fC[iw] = num / (cmax - cmin);
+          else fC[iw] = num * inv;
         }
       }
-      (void)inv;
     } else {  // HLLC (hydrodynamics)
       constexpr int RHO = P::RHO, MN = P::MOM + ID, E = P::E;
       int patchf = 1;
@@ -469,14 +523,23 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
       R Fhll[NW];
       if (patchf == 1) {
         R whll_m = R(0), whll_r = R(0);
+        const R inv = frcp(cmax - cmin);
 #pragma unroll
         for (int iw = 0; iw < NW; ++iw) {
-          Fhll[iw] = (cmax * fL[iw] - cmin * fR[iw] + cmin * cmax * (wRC[iw] - wLC[iw])) / (cmax - cmin);
-          const R wh = (cmax * wRC[iw] - cmin * wLC[iw] + fL[iw] - fR[iw]) / (cmax - cmin);
+          const R nf = cmax * fL[iw] - cmin * fR[iw] + cmin * cmax * (wRC[iw] - wLC[iw]);
+          const R nw = cmax * wRC[iw] - cmin * wLC[iw] + fL[iw] - fR[iw];
+          R wh;
+          if constexpr (kExact) {
+            Fhll[iw] = nf / (cmax - cmin);
+            wh = nw / (cmax - cmin);
+          } else {
+            Fhll[iw] = nf * inv;
+            wh = nw * inv;
+          }
           if (iw == MN) whll_m = wh;
           if (iw == RHO) whll_r = wh;
         }
-        lambdaCD = whll_m / whll_r;
+        lambdaCD = fdiv(whll_m, whll_r);
         bool cond = true;
         if (cmin < R(0) && lambdaCD > R(0) && lambdaCD < cmax) {
           patchf = -1;
@@ -501,8 +564,8 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
           wSub[iw] = right ? wRC[iw] : wLC[iw];
           fSub[iw] = right ? fR[iw] : fL[iw];
         }
-        const R vSub = wSub[MN] / wSub[RHO];
-        const R csmls = R(1) / (cspeed - lambdaCD);
+        const R vSub = fdiv(wSub[MN], wSub[RHO]);
+        const R csmls = frcp(cspeed - lambdaCD);
         wCD[RHO] = wSub[RHO] * (cspeed - vSub) * csmls;
 #pragma unroll
         for (int d = 0; d < P::NDIR; ++d) {

@@ -54,7 +54,14 @@ hipError_t launch_lim(const KernelKey& k, const SweepArgs& a, const FVParams& fp
     if constexpr (RIEM == RIEM_HANCOCK) return hipErrorInvalidValue;
     else return launch_one<P, R, RIEM, LIM_WENO5, 3>(a, fp, grid, s);
   }
-  return launch_one<P, R, RIEM, 0, 2>(a, fp, grid, s);
+  // compile-time limiter for the schemes of the benchmark configurations,
+  // a runtime switch for the others
+  switch (k.limiter) {
+    case LIM_MINMOD: return launch_one<P, R, RIEM, LIM_MINMOD, 2>(a, fp, grid, s);
+    case LIM_VANLEER: return launch_one<P, R, RIEM, LIM_VANLEER, 2>(a, fp, grid, s);
+    case LIM_CADA3: return launch_one<P, R, RIEM, LIM_CADA3, 2>(a, fp, grid, s);
+    default: return launch_one<P, R, RIEM, 0, 2>(a, fp, grid, s);
+  }
 }
 
 template <class P, class R>

@@ -240,3 +240,22 @@ def blast_wave(mesh: UniformMesh, gamma: float = 5.0 / 3.0, seed: int = 20261202
     p = np.where(r2 < radius ** 2, p_in, p_out)
     comps = [rho] + [np.zeros_like(rho)] * nd + [p]
     return hd_to_conserved(np.stack(comps, axis=1), gamma, nd)
+
+
+def orszag_tang_2p5d(mesh: UniformMesh, gamma: float = 5.0 / 3.0, seed: int = 20261202,
+                     amp: float = 1.0e-3, glm: bool = True) -> np.ndarray:
+    """2-D Orszag-Tang vortex with three vector components (as
+    tests/mhd/Orszag_Tang_2D/mod_usr.t, plus a seeded vz perturbation)."""
+    x = mesh.cell_centers()
+    g = mesh.global_index()
+    gi = g.transpose(1, 0, *range(2, g.ndim))
+    rho0 = 25.0 / 36.0 / DPI
+    p0 = 5.0 / 12.0 / DPI
+    b0 = 1.0 / np.sqrt(4.0 * DPI)
+    one = np.ones_like(x[:, 0])
+    comps = [rho0 * one, -np.sin(2.0 * DPI * x[:, 1]), np.sin(2.0 * DPI * x[:, 0]),
+             amp * hashed_uniform(seed, 1, gi, 2), p0 * one, -b0 * np.sin(2.0 * DPI * x[:, 1]),
+             b0 * np.sin(4.0 * DPI * x[:, 0]), 0.0 * one]
+    if glm:
+        comps.append(0.0 * one)
+    return mhd_to_conserved(np.stack(comps, axis=1), gamma, 3)

@@ -102,6 +102,9 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256, help="cells per side (multiple of 16)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline budget (0: skip)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed region, copy one state slot device-to-device (a plain copy of "
+                         "known size, the reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the block update runs on a single MI355X in this build "
@@ -121,6 +124,10 @@ def main() -> int:
     par64 = workload_par("fp64")
     be64, ti64, t64, ms64, _ = run_device(par64, mesh, w0, args.steps, args.warmup, "fp64")
     value64 = cells * stages * args.steps / t64 / 1e6
+    if args.copy_probe:
+        be64.copy(1, 0)
+        be64.sync()
+        print(f"[bench] copy probe: {mesh.nblocks * nw * 4096 * 8} bytes read and written", file=sys.stderr)
     achieved = b_alg * cells / (ms64 * 1e-3) / 1e9
     result = {
         "metric": "Mcell-updates/s 3D MHD 256^3, FP64 vs mixed @L1<=1e-5; adjoint/forward cost",

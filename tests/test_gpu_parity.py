@@ -78,6 +78,18 @@ CASES = {
         mesh=dict(ndim=3, domain_nx=(32, 16, 32), block_nx=(16, 16, 8), xprobmin=(0.0,) * 3,
                   xprobmax=(1.0, 0.5, 1.0), bc=("periodic",) * 6),
         ic="ot3d", steps=3),
+    "alfven2d_linde_hll_cada3": dict(
+        par=dict(physics="mhd", ndim=2, ndir=3, typedivbfix="linde", time_stepper="threestep",
+                 flux_scheme="hll", limiter="cada3", typeboundspeed="cmaxmean", courantpar=0.9),
+        mesh=dict(ndim=2, domain_nx=(64, 32), block_nx=(16, 16), xprobmin=(0.0, 0.0), xprobmax=(1.0, 0.5),
+                  bc=("periodic",) * 4),
+        ic="alfven", steps=6),
+    "ot2d_glm_hll_vanleer": dict(
+        par=dict(physics="mhd", ndim=2, ndir=3, typedivbfix="glm", time_stepper="threestep",
+                 flux_scheme="hll", limiter="vanleer", courantpar=0.8),
+        mesh=dict(ndim=2, domain_nx=(64, 64), block_nx=(16, 16), xprobmin=(0.0, 0.0), xprobmax=(1.0, 1.0),
+                  bc=("periodic",) * 4),
+        ic="ot2d", steps=6),
     "hd3d_hllc_minmod": dict(
         par=dict(physics="hd", ndim=3, ndir=3, gamma=5.0 / 3.0, time_stepper="threestep", flux_scheme="hllc",
                  limiter="minmod", courantpar=0.6),
@@ -92,6 +104,10 @@ def make_ic(kind, par, mesh):
         return problems.kelvin_helmholtz_2d(mesh, gamma=par.gamma)
     if kind == "ot3d":
         return problems.orszag_tang_3d(mesh, gamma=par.gamma)
+    if kind == "alfven":
+        return problems.alfven_wave_2p5d(mesh, gamma=par.gamma)
+    if kind == "ot2d":
+        return problems.orszag_tang_2p5d(mesh, gamma=par.gamma)
     if kind == "blast":
         return problems.blast_wave(mesh, gamma=par.gamma)
     if kind == "blast3d":
