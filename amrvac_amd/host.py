@@ -132,6 +132,9 @@ class TimeIntegrator:
     def setdt(self) -> float:
         par, st = self.par, self.state
         dtmin_mype, cmax = self.be.getdt_courant(0, par.courantpar)
+        if not math.isfinite(dtmin_mype):
+            st.crash = True
+            raise RuntimeError("Error: non-finite time step, run crash.")
         if dtmin_mype < par.dtmin:
             st.crash = True
         if par.slowsteps > st.it - self.it_init + 1:

@@ -1177,3 +1177,59 @@ int orc_volume_average(const orc_sim* s, int slot, double* modes) {
 }
 
 int orc_crash_flag(const orc_sim* s) { return s->crash; }
+
+/* ------------------------------------------------------------------ */
+/* unit-test hooks (single face / single cell)                         */
+/* ------------------------------------------------------------------ */
+
+int orc_dwlimiter(const orc_sim* s, int idim, double a, double b, double* ldw, double* rdw) {
+  dwlimiter_cell(s, idim, a, b, ldw, rdw);
+  return 0;
+}
+
+int orc_weno5_face(const double* w6, double* wl, double* wr) {
+  /* w6 = w(i-2) .. w(i+3); face between i and i+1 */
+  weno5_face(w6 + 2, 0, wl, wr);
+  return 0;
+}
+
+/* one face of finite_volume: lw[iw*nl + k] primitive values of the nl
+   cells around the face (face between cells nl/2-1 and nl/2) */
+int orc_face_flux(orc_sim* s, int method, int idim, int nl, const double* lw_in, double ch,
+                  double* fC, double* cmax_out, double* cmin_out) {
+  double lw[MAXW][MAXLINE];
+  double wLp[MAXLINE][MAXW], wRp[MAXLINE][MAXW];
+  if (nl > MAXLINE || nl < 4) return 1;
+  for (int iw = 0; iw < s->nw; ++iw)
+    for (int k = 0; k < nl; ++k) lw[iw][k] = lw_in[iw * nl + k];
+  const int i = nl / 2 - 1;
+  reconstruct_line(s, idim, lw, i, i, wLp, wRp);
+  double wLC[MAXW], wRC[MAXW], fL[MAXW], fR[MAXW], cmax = 0.0, cmin = 0.0;
+  int crash = 0;
+  for (int iw = 0; iw < s->nw; ++iw) { wLC[iw] = wLp[i][iw]; wRC[iw] = wRp[i][iw]; }
+  to_conserved_cell(s, wLC);
+  to_conserved_cell(s, wRC);
+  if (s->p.phys == ORC_PHYS_MHD && s->p.divb == ORC_DIVB_GLM)
+    modify_wLR_glm(s, wLC, wRC, wLp[i], wRp[i], idim, ch);
+  get_flux(s, wLC, wLp[i], idim, ch, fL);
+  get_flux(s, wRC, wRp[i], idim, ch, fR);
+  if (method == ORC_FS_TVDLF) {
+    get_cbounds(s, wLC, wRC, wLp[i], wRp[i], idim, 0, &cmax, &cmin, &crash);
+    riemann_tvdlf(s, wLC, wRC, fL, fR, cmax, fC);
+  } else {
+    get_cbounds(s, wLC, wRC, wLp[i], wRp[i], idim, 1, &cmax, &cmin, &crash);
+    if (method == ORC_FS_HLLC) riemann_hllc(s, idim, wLC, wRC, fL, fR, cmax, cmin, fC);
+    else riemann_hll(s, idim, wLC, wRC, fL, fR, cmax, cmin, fC);
+  }
+  if (cmax_out) *cmax_out = cmax;
+  if (cmin_out) *cmin_out = cmin;
+  return 0;
+}
+
+/* cell cmax of a conserved state (getdt_courant per cell) */
+double orc_cell_cmax(const orc_sim* s, const double* wcons, int idim) {
+  double cell[MAXW];
+  for (int iw = 0; iw < s->nw; ++iw) cell[iw] = wcons[iw];
+  to_primitive_cell(s, cell);
+  return get_cmax(s, cell, idim);
+}

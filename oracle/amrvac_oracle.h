@@ -121,6 +121,16 @@ int orc_volume_average(const orc_sim* s, int slot, double* modes);
 /* set by any small-value check that the reference would flag */
 int orc_crash_flag(const orc_sim* s);
 
+/* unit-test hooks: dwlimiter2 for one cell (a = dwC(i-1), b = dwC(i));
+   WENO5 for one face from w6 = w(i-2..i+3); one face of finite_volume from
+   nl primitive cells lw[iw*nl + k] around the face (between nl/2-1 and
+   nl/2); the cell cmax of a conserved state */
+int orc_dwlimiter(const orc_sim* s, int idim, double a, double b, double* ldw, double* rdw);
+int orc_weno5_face(const double* w6, double* wl, double* wr);
+int orc_face_flux(orc_sim* s, int method, int idim, int nl, const double* lw, double ch,
+                  double* fC, double* cmax, double* cmin);
+double orc_cell_cmax(const orc_sim* s, const double* wcons, int idim);
+
 /* number of OpenMP threads the oracle uses (0 = library default) */
 void orc_set_threads(int n);
 

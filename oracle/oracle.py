@@ -41,14 +41,17 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
-_lib = None
+_libs: dict = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        build()
-        L = ctypes.CDLL(_LIB_PATH)
+def lib(path: str | None = None):
+    """The oracle library (default: liborc.so beside this file; a different
+    path loads another build of the same source, e.g. a planted mutant)."""
+    path = path or _LIB_PATH
+    if path not in _libs:
+        if path == _LIB_PATH:
+            build()
+        L = ctypes.CDLL(path)
         vp = ctypes.c_void_p
         dp = ctypes.POINTER(ctypes.c_double)
         ip = ctypes.POINTER(ctypes.c_int)
@@ -68,8 +71,13 @@ def lib():
         L.orc_volume_average.argtypes = [vp, ctypes.c_int, dp]
         L.orc_crash_flag.argtypes = [vp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
-        _lib = L
-    return _lib
+        L.orc_dwlimiter.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp]
+        L.orc_weno5_face.argtypes = [dp, dp, dp]
+        L.orc_face_flux.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, dp, dp, dp]
+        L.orc_cell_cmax.argtypes = [vp, dp, ctypes.c_int]
+        L.orc_cell_cmax.restype = ctypes.c_double
+        _libs[path] = L
+    return _libs[path]
 
 
 def _dptr(a: np.ndarray):
@@ -107,10 +115,10 @@ def make_params(par, block_nx) -> OrcParams:
 class OracleFV:
     """Backend protocol of amrvac_amd.host on the C oracle."""
 
-    def __init__(self, par, mesh, threads: int = 0):
+    def __init__(self, par, mesh, threads: int = 0, lib_path: str | None = None):
         self.par = par
         self.mesh = mesh
-        self.L = lib()
+        self.L = lib(lib_path)
         self.L.orc_set_threads(threads)
         p = make_params(par, mesh.block_nx)
         h = ctypes.c_void_p()
@@ -171,3 +179,32 @@ class OracleFV:
 
     def crash(self) -> bool:
         return bool(self.L.orc_crash_flag(self.h))
+
+    # -- single face / cell hooks for closed-form checks ----------------------
+    def limiter(self, a: float, b: float, idim: int = 0):
+        l, r = ctypes.c_double(), ctypes.c_double()
+        self.L.orc_dwlimiter(self.h, idim, a, b, ctypes.byref(l), ctypes.byref(r))
+        return l.value, r.value
+
+    def face_flux(self, method: int, idim: int, prims: np.ndarray, ch: float = 0.0):
+        """prims[k, iw] for the cells around one face (face between the
+        middle two); returns (flux[nw], cmax, cmin)."""
+        lw = np.ascontiguousarray(np.asarray(prims, dtype=np.float64).T)
+        f = np.zeros(self.nw)
+        cmax, cmin = ctypes.c_double(), ctypes.c_double()
+        rc = self.L.orc_face_flux(self.h, method, idim, lw.shape[1], _dptr(lw), ch, _dptr(f),
+                                  ctypes.byref(cmax), ctypes.byref(cmin))
+        if rc != 0:
+            raise RuntimeError("orc_face_flux failed")
+        return f, cmax.value, cmin.value
+
+    def cell_cmax(self, wcons: np.ndarray, idim: int) -> float:
+        a = np.ascontiguousarray(wcons, dtype=np.float64)
+        return self.L.orc_cell_cmax(self.h, _dptr(a), idim)
+
+
+def weno5_face(w6, lib_path: str | None = None):
+    a = np.ascontiguousarray(w6, dtype=np.float64)
+    l, r = ctypes.c_double(), ctypes.c_double()
+    lib(lib_path).orc_weno5_face(_dptr(a), ctypes.byref(l), ctypes.byref(r))
+    return l.value, r.value

@@ -1,0 +1,460 @@
+"""Closed-form and invariant checks of the CPU oracle, for the parts of the
+scheme no reproducible golden log covers (HLLC, GLM, Einfeldt MHD bounds,
+WENO5, the limiters one by one).
+
+Every check is a function ``check_*(lib_path)`` returning a message (empty
+when it passes), so that tests/test_oracle_mutants.py can run the same
+checks against builds of the oracle with planted mistakes and show that each
+mistake is caught.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from amrvac_amd import problems
+from amrvac_amd.host import FS, Par, TimeIntegrator
+from amrvac_amd.mesh import UniformMesh
+from oracle.oracle import OracleFV, weno5_face
+
+import exact_riemann
+
+
+def _sim(par, mesh=None, lib_path=None):
+    mesh = mesh or UniformMesh(par.ndim, (16,) * par.ndim, (16,) * par.ndim, (0.0,) * par.ndim,
+                               (1.0,) * par.ndim, ("periodic",) * (2 * par.ndim), nghost=par.nghost)
+    return OracleFV(par, mesh, lib_path=lib_path)
+
+
+def _prims_const(left, right, n=4):
+    """n cells, the left half = left, the right half = right."""
+    return np.array([left] * (n // 2) + [right] * (n // 2), dtype=np.float64)
+
+
+# --------------------------------------------------------------------------
+# Riemann problem of the Euler equations (Toro ch. 4)
+# --------------------------------------------------------------------------
+def shock_tube(par: Par, nx: int, t_end: float, left, right, lib_path=None):
+    mesh = UniformMesh(2, (nx, 16), (16, 16), (0.0, 0.0), (1.0, 16.0 / nx), ("cont", "cont", "periodic", "periodic"),
+                       nghost=par.nghost)
+    x = mesh.cell_centers()
+    L = np.where(x[:, 0] < 0.5, 1.0, 0.0)
+    rho = L * left[0] + (1 - L) * right[0]
+    u = L * left[1] + (1 - L) * right[1]
+    p = L * left[2] + (1 - L) * right[2]
+    w = problems.hd_to_conserved(np.stack([rho, u, 0 * u, p], axis=1), par.gamma, 2)
+    be = OracleFV(par, mesh, lib_path=lib_path)
+    be.set_state(0, w)
+    be.getbc(0)
+    import dataclasses
+    par2 = dataclasses.replace(par, time_max=t_end)
+    TimeIntegrator(par2, mesh, be).run()
+    g = mesh.to_global(be.get_state(0))
+    xc = (np.arange(nx) + 0.5) / nx
+    rho_num = g[0][0]
+    rho_ex, _, _ = exact_riemann.solution(xc, t_end, 0.5, left, right, par.gamma)
+    return float(np.abs(rho_num - rho_ex).mean()), rho_num
+
+
+SOD = ((1.0, 0.0, 1.0), (0.125, 0.0, 0.1))
+
+SCHEMES = {
+    "3step_hllc_vanleer": dict(time_stepper="threestep", flux_scheme="hllc", limiter="vanleer"),
+    "3step_hll_cada3": dict(time_stepper="threestep", flux_scheme="hll", limiter="cada3"),
+    "2step_tvdlf_minmod": dict(time_stepper="twostep", flux_scheme="tvdlf", limiter="minmod"),
+    "3step_tvdlf_weno5": dict(time_stepper="threestep", flux_scheme="tvdlf", limiter="weno5"),
+    "3step_hll_koren_einfeldt": dict(time_stepper="threestep", flux_scheme="hll", limiter="koren"),
+}
+
+
+def check_sod(scheme: str, lib_path=None) -> str:
+    par = Par(physics="hd", ndim=2, ndir=2, gamma=1.4, courantpar=0.4, **SCHEMES[scheme])
+    e1, _ = shock_tube(par, 128, 0.2, *SOD, lib_path=lib_path)
+    e2, _ = shock_tube(par, 256, 0.2, *SOD, lib_path=lib_path)
+    if not (e2 < 0.006 and e1 / e2 > 1.6):
+        return f"{scheme}: L1(rho) {e1:.4g} (128), {e2:.4g} (256)"
+    return ""
+
+
+@pytest.mark.parametrize("scheme", sorted(SCHEMES))
+def test_sod_shock_tube_converges_to_exact_solution(scheme):
+    assert check_sod(scheme) == ""
+
+
+# --------------------------------------------------------------------------
+# Smooth advection: order of accuracy
+# --------------------------------------------------------------------------
+def density_wave_error(par: Par, nx: int, lib_path=None, t_end: float = 1.0) -> float:
+    """L1 error of a density wave 1 + 0.2 sin(2 pi x) carried once around a
+    periodic unit domain by u = 1 at uniform pressure (an exact solution of
+    the Euler equations: pure advection of the contact)."""
+    import dataclasses
+    mesh = UniformMesh(2, (nx, 16), (16, 16), (0.0, 0.0), (1.0, 16.0 / nx), ("periodic",) * 4, nghost=par.nghost)
+    x = mesh.cell_centers()[:, 0]
+    dx = 1.0 / nx
+    # exact cell averages
+    rho = 1.0 + 0.2 * (np.cos(2 * np.pi * (x - dx / 2)) - np.cos(2 * np.pi * (x + dx / 2))) / (2 * np.pi * dx)
+    w = problems.hd_to_conserved(np.stack([rho, 0 * rho + 1.0, 0 * rho, 0 * rho + 1.0], axis=1), par.gamma, 2)
+    be = OracleFV(par, mesh, lib_path=lib_path)
+    be.set_state(0, w)
+    be.getbc(0)
+    TimeIntegrator(dataclasses.replace(par, time_max=t_end), mesh, be).run()
+    return float(np.abs(mesh.to_global(be.get_state(0))[0] - mesh.to_global(w)[0]).mean())
+
+
+# scheme: (settings, minimum error ratio 64 -> 128 cells, maximum error at 128)
+SMOOTH = {
+    "twostep_tvdlf_vanleer": (dict(time_stepper="twostep", flux_scheme="tvdlf", limiter="vanleer"), 3.3, 8e-4),
+    "threestep_hllc_vanleer": (dict(time_stepper="threestep", flux_scheme="hllc", limiter="vanleer"), 3.3, 6e-4),
+    "threestep_hll_cada3": (dict(time_stepper="threestep", flux_scheme="hll", limiter="cada3"), 6.0, 2e-5),
+    "threestep_tvdlf_weno5": (dict(time_stepper="threestep", flux_scheme="tvdlf", limiter="weno5"), 16.0, 1.5e-7),
+}
+
+
+def check_smooth_order(scheme: str, lib_path=None) -> str:
+    settings, min_ratio, max_err = SMOOTH[scheme]
+    par = Par(physics="hd", ndim=2, ndir=2, gamma=1.4, courantpar=0.6, **settings)
+    e1 = density_wave_error(par, 64, lib_path)
+    e2 = density_wave_error(par, 128, lib_path)
+    if not (e2 < max_err and e1 / e2 > min_ratio):
+        return f"{scheme}: L1(rho) {e1:.4g} (64), {e2:.4g} (128), ratio {e1 / e2:.3g}"
+    return ""
+
+
+@pytest.mark.parametrize("scheme", sorted(SMOOTH))
+def test_smooth_density_wave_order_of_accuracy(scheme):
+    """Second order for the TVD limiters (Hancock predictor-corrector and
+    SSPRK3), third order for cada3, fifth for WENO5 in space."""
+    assert check_smooth_order(scheme) == ""
+
+
+def check_hllc_stationary_contact(lib_path=None) -> str:
+    """HLLC keeps an isolated stationary contact exactly; HLL smears it."""
+    out = {}
+    for fs in ("hllc", "hll"):
+        par = Par(physics="hd", ndim=2, ndir=2, gamma=1.4, time_stepper="threestep", flux_scheme=fs,
+                  limiter="minmod", courantpar=0.5)
+        mesh = UniformMesh(2, (64, 16), (16, 16), (0.0, 0.0), (1.0, 0.25), ("cont", "cont", "periodic", "periodic"))
+        x = mesh.cell_centers()
+        rho = np.where(x[:, 0] < 0.5, 1.0, 0.1)
+        w = problems.hd_to_conserved(np.stack([rho, 0 * rho, 0 * rho, 0 * rho + 1.0], axis=1), 1.4, 2)
+        be = OracleFV(par, mesh, lib_path=lib_path)
+        be.set_state(0, w)
+        be.getbc(0)
+        TimeIntegrator(par, mesh, be).run(nsteps=20)
+        out[fs] = float(np.abs(mesh.interior(be.get_state(0))[:, 0] - mesh.interior(w)[:, 0]).max())
+    if not (out["hllc"] < 1e-13 and out["hll"] > 1e-3):
+        return f"contact: hllc {out['hllc']:.3g}, hll {out['hll']:.3g}"
+    return ""
+
+
+def test_hllc_keeps_a_stationary_contact():
+    assert check_hllc_stationary_contact() == ""
+
+
+# --------------------------------------------------------------------------
+# Single-face identities
+# --------------------------------------------------------------------------
+def euler_flux(prim, gamma, idim, ndir):
+    """Textbook flux of the Euler equations in direction idim."""
+    rho, v, p = prim[0], prim[1:1 + ndir], prim[1 + ndir]
+    e = p / (gamma - 1.0) + 0.5 * rho * np.dot(v, v)
+    f = np.zeros(ndir + 2)
+    f[0] = rho * v[idim]
+    f[1:1 + ndir] = rho * v[idim] * v
+    f[1 + idim] += p
+    f[1 + ndir] = (e + p) * v[idim]
+    return f
+
+
+def mhd_flux(prim, gamma, idim, ndir=3):
+    """Textbook flux of ideal MHD (no GLM) in direction idim."""
+    rho, v, p, B = prim[0], prim[1:4], prim[4], prim[5:8]
+    e = p / (gamma - 1.0) + 0.5 * rho * np.dot(v, v) + 0.5 * np.dot(B, B)
+    pt = p + 0.5 * np.dot(B, B)
+    f = np.zeros(8)
+    f[0] = rho * v[idim]
+    f[1:4] = rho * v[idim] * v - B[idim] * B
+    f[1 + idim] += pt
+    f[4] = (e + pt) * v[idim] - B[idim] * np.dot(B, v)
+    f[5:8] = v[idim] * B - B[idim] * v
+    return f
+
+
+def check_flux_consistency(lib_path=None) -> str:
+    """F_Riemann(u, u) = f(u) for every solver, HD and MHD, all directions."""
+    rng = np.random.default_rng(7)
+    msgs = []
+    hd = _sim(Par(physics="hd", ndim=3, ndir=3, gamma=1.4, limiter="minmod"), lib_path=lib_path)
+    mhd = _sim(Par(physics="mhd", ndim=3, ndir=3, gamma=5 / 3, typedivbfix="none", limiter="minmod"),
+               lib_path=lib_path)
+    for _ in range(20):
+        pr = np.concatenate([[0.5 + rng.random()], rng.normal(size=3), [0.2 + rng.random()]])
+        for fs in ("tvdlf", "hll", "hllc"):
+            for d in range(3):
+                f, _, _ = hd.face_flux(FS[fs], d, _prims_const(pr, pr))
+                ref = euler_flux(pr, 1.4, d, 3)
+                if not np.allclose(f, ref, rtol=1e-12, atol=1e-12):
+                    msgs.append(f"hd {fs} dim {d}: {f} vs {ref}")
+        pm = np.concatenate([[0.5 + rng.random()], rng.normal(size=3), [0.2 + rng.random()], rng.normal(size=3)])
+        for fs in ("tvdlf", "hll"):
+            for d in range(3):
+                f, _, _ = mhd.face_flux(FS[fs], d, _prims_const(pm, pm))
+                ref = mhd_flux(pm, 5 / 3, d)
+                if not np.allclose(f, ref, rtol=1e-12, atol=1e-12):
+                    msgs.append(f"mhd {fs} dim {d}: {f} vs {ref}")
+    return "; ".join(msgs[:3])
+
+
+def test_riemann_fluxes_are_consistent():
+    assert check_flux_consistency() == ""
+
+
+def check_signal_speeds(lib_path=None) -> str:
+    """Einfeldt bounds of identical states are u -+ c (Roe average of equal
+    states); the MHD fast speed is max(c_s, c_A) for B along the normal and
+    sqrt(c_s^2 + c_A^2) for B across it."""
+    msgs = []
+    g = 1.4
+    hd = _sim(Par(physics="hd", ndim=2, ndir=2, gamma=g, limiter="minmod", typeboundspeed="Einfeldt"),
+              lib_path=lib_path)
+    pr = np.array([0.7, 0.3, -0.2, 1.3])
+    _, cmax, cmin = hd.face_flux(FS["hll"], 0, _prims_const(pr, pr))
+    c = math.sqrt(g * 1.3 / 0.7)
+    if not (abs(cmax - (0.3 + c)) < 1e-14 and abs(cmin - (0.3 - c)) < 1e-14):
+        msgs.append(f"hd Einfeldt {cmin},{cmax} vs {0.3 - c},{0.3 + c}")
+    gm = 5 / 3
+    mhd = _sim(Par(physics="mhd", ndim=3, ndir=3, gamma=gm, typedivbfix="none", limiter="minmod",
+                   typeboundspeed="Einfeldt"), lib_path=lib_path)
+    rho, p = 0.9, 0.4
+    cs = math.sqrt(gm * p / rho)
+    for B, expect in (([1.2, 0, 0], max(cs, 1.2 / math.sqrt(rho))),
+                      ([0, 0.8, 0.6], math.sqrt(cs ** 2 + 1.0 / rho))):
+        pm = np.array([rho, 0.0, 0.0, 0.0, p] + B)
+        _, cmax, cmin = mhd.face_flux(FS["hll"], 0, _prims_const(pm, pm))
+        if not (abs(cmax - expect) < 1e-12 and abs(cmin + expect) < 1e-12):
+            msgs.append(f"mhd fast speed B={B}: {cmin},{cmax} vs +-{expect}")
+        cons = problems.mhd_to_conserved(pm[None, :], gm, 3)[0]
+        cm = mhd.cell_cmax(cons, 0)
+        if abs(cm - expect) > 1e-12:
+            msgs.append(f"mhd cell cmax B={B}: {cm} vs {expect}")
+    return "; ".join(msgs)
+
+
+def test_signal_speeds_special_cases():
+    assert check_signal_speeds() == ""
+
+
+def check_glm_interface_state(lib_path=None) -> str:
+    """mhd_modify_wLR solves the linear (B_n, psi) Riemann problem: the face
+    values carry the characteristic variables psi + c_h B_n from the left and
+    psi - c_h B_n from the right (Dedner et al. 2002, eq. 42).  The face
+    fluxes of B_n and psi are psi* and c_h^2 B_n*."""
+    gm = 5 / 3
+    ch = 2.3
+    sim = _sim(Par(physics="mhd", ndim=3, ndir=3, gamma=gm, typedivbfix="glm", limiter="minmod"),
+               lib_path=lib_path)
+    msgs = []
+    rng = np.random.default_rng(3)
+    for d in range(3):
+        pl = np.concatenate([[1.0, 0.1, -0.2, 0.05, 0.6], rng.normal(size=3), [rng.normal()]])
+        pr = np.concatenate([[0.8, 0.0, 0.1, -0.1, 0.5], rng.normal(size=3), [rng.normal()]])
+        f, _, _ = sim.face_flux(FS["hll"], d, _prims_const(pl, pr), ch=ch)
+        psi_s, bn_s = f[5 + d], f[8] / ch ** 2
+        if abs((psi_s + ch * bn_s) - (pl[8] + ch * pl[5 + d])) > 1e-12:
+            msgs.append(f"dim {d}: right-going invariant")
+        if abs((psi_s - ch * bn_s) - (pr[8] - ch * pr[5 + d])) > 1e-12:
+            msgs.append(f"dim {d}: left-going invariant")
+    return "; ".join(msgs)
+
+
+def test_glm_interface_riemann_solution():
+    assert check_glm_interface_state() == ""
+
+
+def check_limiters(lib_path=None) -> str:
+    """dwlimiter2 identities: minmod/vanleer/koren/cada3 vanish for opposite
+    signs and are symmetric in scaling; minmod picks the smaller slope;
+    vanleer is the harmonic mean; cada3 in smooth regions (eta < 1) is the
+    third-order slope (2+theta)/3 dwC(i)."""
+    msgs = []
+    base = dict(physics="hd", ndim=2, ndir=2, gamma=1.4)
+    sims = {lim: _sim(Par(limiter=lim, **base), lib_path=lib_path)
+            for lim in ("minmod", "vanleer", "koren", "cada3", "woodward", "superbee", "mcbeta")}
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        a, b = rng.normal(size=2)
+        for lim, s in sims.items():
+            l, r = s.limiter(a, b)
+            if a * b < 0 and lim != "cada3" and (l != 0.0 or r != 0.0):
+                msgs.append(f"{lim} not zero for opposite signs")
+        l, r = sims["minmod"].limiter(a, b)
+        if a * b > 0 and l != math.copysign(min(abs(a), abs(b)), b):
+            msgs.append("minmod value")
+        l, _ = sims["vanleer"].limiter(a, b)
+        if a * b > 0 and abs(l - 2 * a * b / (a + b)) > 1e-9 * abs(l):
+            msgs.append("vanleer value")
+        # TVD bounds: 0 <= phi <= 2 min(1, theta)
+        for lim in ("minmod", "vanleer", "koren", "superbee", "mcbeta", "woodward"):
+            l, _ = sims[lim].limiter(a, b)
+            if a * b > 0 and not (0 <= l / b <= 2 * min(1.0, a / b) + 1e-12):
+                msgs.append(f"{lim} outside TVD region")
+    # cada3 (Cada & Torrilhon 2009): where the local variation is small
+    # against cada3_radius*dx (eta < 1) the slope is the third-order one,
+    # (2 dwC(i+1) + dwC(i))/3 to the left face and (2 dwC(i) + dwC(i+1))/3 to
+    # the right (up to the sqrt(eps) guard in theta); with a=b large it is b.
+    s = sims["cada3"]
+    for a, b in ((1e-3, 2e-3), (-3e-3, 1e-3), (4e-3, -1e-3), (2e-3, 2e-3)):
+        l, r = s.limiter(a, b)
+        if abs(l - (2 * b + a) / 3) > 1e-4 * abs(b) or abs(r - (2 * a + b) / 3) > 1e-4 * abs(a):
+            msgs.append(f"cada3 smooth {a},{b}: {l},{r}")
+    l, r = s.limiter(0.5, 0.5)
+    if abs(l - 0.5) > 1e-7 or abs(r - 0.5) > 1e-7:
+        msgs.append(f"cada3 equal slopes {l},{r}")
+    l, r = s.limiter(0.5, 0.05)
+    # theta=10 in the limited region: min(A, max(.., min(2 theta, A, 1.6))) = 1.6
+    if abs(l - 1.6 * 0.05) > 1e-9:
+        msgs.append(f"cada3 limited {l}")
+    return "; ".join(msgs[:4])
+
+
+def test_limiter_identities():
+    assert check_limiters() == ""
+
+
+def check_weno5_polynomials(lib_path=None) -> str:
+    """WENO5 face values are exact for cell averages of polynomials of degree
+    <= 2, whatever the nonlinear weights."""
+    rng = np.random.default_rng(5)
+    msgs = []
+    for _ in range(50):
+        c = rng.normal(size=3)
+        h = 0.1
+        # cell averages over [x_j - h/2, x_j + h/2] of c0 + c1 x + c2 x^2, x_j = (j-2) h
+        xs = (np.arange(6) - 2) * h
+        avg = c[0] + c[1] * xs + c[2] * (xs ** 2 + h * h / 12.0)
+        xf = 0.5 * h
+        exact = c[0] + c[1] * xf + c[2] * xf * xf
+        wl, wr = weno5_face(avg, lib_path=lib_path)
+        if abs(wl - exact) > 1e-12 or abs(wr - exact) > 1e-12:
+            msgs.append(f"weno5 {wl} {wr} vs {exact}")
+    return "; ".join(msgs[:2])
+
+
+def test_weno5_exact_for_quadratics():
+    assert check_weno5_polynomials() == ""
+
+
+# --------------------------------------------------------------------------
+# Whole-step invariants
+# --------------------------------------------------------------------------
+def check_glm_decay(lib_path=None) -> str:
+    """A uniform MHD state with uniform psi: fluxes cancel and the only change
+    is the implicit psi decay exp(-qdt c_h alpha / dx) of add_source_glm,
+    combined by ssprk3."""
+    par = Par(physics="mhd", ndim=3, ndir=3, gamma=5 / 3, typedivbfix="glm", time_stepper="threestep",
+              flux_scheme="hll", limiter="cada3", courantpar=0.6)
+    mesh = UniformMesh(3, (16, 16, 16), (16, 16, 16), (0.0,) * 3, (1.0,) * 3, ("periodic",) * 6)
+    prim = np.array([1.0, 0.3, -0.1, 0.2, 0.5, 0.4, 0.2, -0.3, 0.01])
+    w = problems.mhd_to_conserved(np.broadcast_to(prim[None, :, None, None, None],
+                                                  (1, 9) + mesh.nxg[::-1]).copy(), 5 / 3, 3)
+    be = OracleFV(par, mesh, lib_path=lib_path)
+    be.set_state(0, w)
+    be.getbc(0)
+    ti = TimeIntegrator(par, mesh, be)
+    ti.step()
+    dt, ch = ti.state.dt, ti.state.cmax_global
+    alpha, dx = 0.5, 1.0 / 16
+    d = [math.exp(-b * dt * ch * alpha / dx) for b in (1.0, 0.25, 2.0 / 3.0)]
+    psi1 = d[0] * 0.01
+    psi2 = d[1] * (0.75 * 0.01 + (1.0 - 0.75) * psi1)
+    psi3 = d[2] * (1.0 / 3.0 * 0.01 + (1.0 - 1.0 / 3.0) * psi2)
+    got = mesh.interior(be.get_state(0))
+    msgs = []
+    if abs(got[0, 8].max() - psi3) > 1e-15 or abs(got[0, 8].min() - psi3) > 1e-15:
+        msgs.append(f"psi {got[0, 8].max()} vs {psi3}")
+    if np.abs(got[0, :8] - mesh.interior(w)[0, :8]).max() > 1e-14:
+        msgs.append("non-psi variables changed")
+    return "; ".join(msgs)
+
+
+def test_glm_psi_decay_closed_form():
+    assert check_glm_decay() == ""
+
+
+def check_conservation_and_uniform_states(lib_path=None) -> str:
+    msgs = []
+    # periodic KH: mass, momentum and energy totals to round-off
+    par = Par(physics="hd", ndim=2, ndir=2, gamma=1.4, time_stepper="threestep", flux_scheme="hllc",
+              limiter="vanleer", courantpar=0.8)
+    mesh = UniformMesh(2, (64, 64), (16, 16), (0.0, 0.0), (1.0, 1.0), ("periodic",) * 4)
+    w0 = problems.kelvin_helmholtz_2d(mesh)
+    be = OracleFV(par, mesh, lib_path=lib_path)
+    be.set_state(0, w0)
+    be.getbc(0)
+    TimeIntegrator(par, mesh, be).run(nsteps=10)
+    t0 = mesh.interior(w0).sum(axis=(0, 2, 3))
+    t1 = mesh.interior(be.get_state(0)).sum(axis=(0, 2, 3))
+    if np.abs(t1 - t0).max() > 1e-10 * np.abs(t0).max():
+        msgs.append(f"HD totals drift {t1 - t0}")
+    # periodic OT + GLM: mass to round-off
+    par = Par(physics="mhd", ndim=3, ndir=3, typedivbfix="glm", time_stepper="threestep", flux_scheme="hll",
+              limiter="cada3", courantpar=0.6)
+    mesh = UniformMesh(3, (32, 32, 32), (16, 16, 16), (0.0,) * 3, (1.0,) * 3, ("periodic",) * 6)
+    w0 = problems.orszag_tang_3d(mesh)
+    be = OracleFV(par, mesh, lib_path=lib_path)
+    be.set_state(0, w0)
+    be.getbc(0)
+    TimeIntegrator(par, mesh, be).run(nsteps=2)
+    m0 = mesh.interior(w0)[:, 0].sum()
+    m1 = mesh.interior(be.get_state(0))[:, 0].sum()
+    if abs(m1 - m0) > 1e-12 * m0:
+        msgs.append(f"MHD mass drift {m1 - m0}")
+    # uniform moving states stay uniform
+    for fs, lim, phys in (("hllc", "vanleer", "hd"), ("hll", "cada3", "mhd"), ("tvdlf", "weno5", "hd")):
+        if phys == "hd":
+            par = Par(physics="hd", ndim=2, ndir=2, gamma=1.4, time_stepper="threestep", flux_scheme=fs,
+                      limiter=lim, courantpar=0.5)
+            mesh = UniformMesh(2, (32, 32), (16, 16), (0.0, 0.0), (1.0, 1.0), ("periodic",) * 4, nghost=par.nghost)
+            prim = np.array([1.3, 0.4, -0.7, 2.0])
+            w0 = problems.hd_to_conserved(np.broadcast_to(prim[None, :, None, None], (mesh.nblocks, 4) + mesh.nxg[::-1]).copy(), 1.4, 2)
+        else:
+            par = Par(physics="mhd", ndim=2, ndir=3, typedivbfix="linde", time_stepper="threestep", flux_scheme=fs,
+                      limiter=lim, courantpar=0.5)
+            mesh = UniformMesh(2, (32, 32), (16, 16), (0.0, 0.0), (1.0, 1.0), ("periodic",) * 4)
+            prim = np.array([1.3, 0.4, -0.7, 0.1, 2.0, 0.3, 0.5, -0.2])
+            w0 = problems.mhd_to_conserved(np.broadcast_to(prim[None, :, None, None], (mesh.nblocks, 8) + mesh.nxg[::-1]).copy(), 5 / 3, 3)
+        be = OracleFV(par, mesh, lib_path=lib_path)
+        be.set_state(0, w0)
+        be.getbc(0)
+        TimeIntegrator(par, mesh, be).run(nsteps=3)
+        dev = np.abs(mesh.interior(be.get_state(0)) - mesh.interior(w0)).max()
+        if dev > 1e-13:
+            msgs.append(f"uniform {phys} {fs} {lim} changed by {dev:.3g}")
+    return "; ".join(msgs)
+
+
+def test_conservation_and_uniform_states():
+    assert check_conservation_and_uniform_states() == ""
+
+
+ALL_CHECKS = {
+    "sod_hllc_vanleer": lambda p: check_sod("3step_hllc_vanleer", p),
+    "sod_hll_cada3": lambda p: check_sod("3step_hll_cada3", p),
+    "sod_tvdlf_minmod": lambda p: check_sod("2step_tvdlf_minmod", p),
+    "sod_tvdlf_weno5": lambda p: check_sod("3step_tvdlf_weno5", p),
+    "smooth_twostep_vanleer": lambda p: check_smooth_order("twostep_tvdlf_vanleer", p),
+    "smooth_hllc_vanleer": lambda p: check_smooth_order("threestep_hllc_vanleer", p),
+    "smooth_hll_cada3": lambda p: check_smooth_order("threestep_hll_cada3", p),
+    "smooth_tvdlf_weno5": lambda p: check_smooth_order("threestep_tvdlf_weno5", p),
+    "hllc_contact": check_hllc_stationary_contact,
+    "flux_consistency": check_flux_consistency,
+    "signal_speeds": check_signal_speeds,
+    "glm_interface": check_glm_interface_state,
+    "limiters": check_limiters,
+    "weno5_polynomials": check_weno5_polynomials,
+    "glm_decay": check_glm_decay,
+    "conservation": check_conservation_and_uniform_states,
+}
