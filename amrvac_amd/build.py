@@ -33,7 +33,7 @@ def common_flags() -> list[str]:
     return ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             # packed FP32 arithmetic stays off (v_pk_* forms are avoided)
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-Wno-unused-command-line-argument", "-I", os.path.join(HERE, "..", "include")]
+            "-Wno-unused-command-line-argument", "-I", os.path.abspath(os.path.join(HERE, "..", "include"))]
 
 
 def _compile(src: str, obj: str, extra: list[str]) -> tuple[str, int, str]:
@@ -42,17 +42,20 @@ def _compile(src: str, obj: str, extra: list[str]) -> tuple[str, int, str]:
     return obj, p.returncode, p.stdout + p.stderr
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose: bool = False, jobs: int | None = None, csrc: str = CSRC, build_dir: str = BUILD,
+          lib: str = LIB) -> str:
+    """Compile every unit under ``csrc`` and link ``lib`` (defaults: the
+    package's own sources and in-tree library)."""
+    os.makedirs(build_dir, exist_ok=True)
     tasks = []
     for u in KERNEL_UNITS:
-        src = os.path.join(CSRC, u + ".hip")
-        tasks.append((src, os.path.join(BUILD, u + ".o"), []))
-        tasks.append((src, os.path.join(BUILD, u + "_x.o"), ["-DFV_EXACT", "-ffp-contract=off"]))
+        src = os.path.join(csrc, u + ".hip")
+        tasks.append((src, os.path.join(build_dir, u + ".o"), []))
+        tasks.append((src, os.path.join(build_dir, u + "_x.o"), ["-DFV_EXACT", "-ffp-contract=off"]))
     # the dt / volume-average / ghost-fill kernels are memory bound; they are
     # built without contraction so that dt and cmax_global round like the
     # CPU restatement's in every precision mode
-    tasks.append((os.path.join(CSRC, "fv_capi.hip"), os.path.join(BUILD, "fv_capi.o"), ["-ffp-contract=off"]))
+    tasks.append((os.path.join(csrc, "fv_capi.hip"), os.path.join(build_dir, "fv_capi.o"), ["-ffp-contract=off"]))
     jobs = jobs or min(len(tasks), max(1, (os.cpu_count() or 4)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         results = list(ex.map(lambda t: _compile(*t), tasks))
@@ -64,13 +67,13 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     if failed:
         raise RuntimeError("HIP compilation failed: " + ", ".join(os.path.basename(f[0]) for f in failed))
     objs = [t[1] for t in tasks]
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("link failed:\n" + p.stdout + p.stderr)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

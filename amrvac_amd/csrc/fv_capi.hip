@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/amrvac_fv.h"
+#include "amrvac_fv.h"
 #include "fv_launch.h"
 
 namespace {
@@ -487,7 +487,7 @@ int amrvac_fv_advect1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
   a.src = c->slot[psa];
   a.dst = c->slot[psb];
   a.base = comb_slot >= 0 ? c->slot[comb_slot] : nullptr;
-  a.comb_mode = comb_slot < 0 ? 0 : (comb_b == 0.0 ? 1 : 2);
+  a.comb_mode = comb_slot < 0 ? 0 : (comb_b == 0.0 ? (comb_slot == psa ? 3 : 1) : 2);
   a.comb_a = comb_a;
   a.comb_b = comb_b;
   const double qdt = dtfactor * dt;

@@ -46,7 +46,11 @@ using fvc::FVParams;
 // division / square root and the reference's MIN/MAX semantics, so that every
 // operation rounds like the CPU restatement; the production builds use the
 // hardware min/max and, in FP32, the single-instruction reciprocal and square
-// root (about 1 ulp).
+// root (about 1 ulp).  In FP64 they refine the hardware reciprocal and
+// reciprocal square root (about 23 correct bits) by Newton / Goldschmidt
+// steps to within an ulp or two, in half the instructions of the IEEE
+// sequences; arguments are finite and positive (densities, sound speeds,
+// wave-speed spans), or zero for the square root.
 #ifdef FV_EXACT
 constexpr bool kExact = true;
 #else
@@ -63,17 +67,44 @@ __device__ __forceinline__ R mx2(R a, R b) {
   if constexpr (kExact) return a > b ? a : b;
   else return fmax(a, b);
 }
-__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
-__device__ __forceinline__ float fdiv(float a, float b) {
-  if constexpr (kExact) return a / b;
-  else return a * __builtin_amdgcn_rcpf(b);
+__device__ __forceinline__ double frcp(double a) {
+  if constexpr (kExact) {
+    return 1.0 / a;
+  } else {
+    double r = __builtin_amdgcn_rcp(a);
+    double e = fma(-a, r, 1.0);  // relative error of r, about 2^-23
+    r = fma(r, e, r);            // about 2^-46
+    e = fma(-a, r, 1.0);
+    return fma(r, e, r);
+  }
 }
-__device__ __forceinline__ double frcp(double a) { return 1.0 / a; }
 __device__ __forceinline__ float frcp(float a) {
   if constexpr (kExact) return 1.0f / a;
   else return __builtin_amdgcn_rcpf(a);
 }
-__device__ __forceinline__ double fsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ double fdiv(double a, double b) {
+  if constexpr (kExact) return a / b;
+  else return a * frcp(b);
+}
+__device__ __forceinline__ float fdiv(float a, float b) {
+  if constexpr (kExact) return a / b;
+  else return a * __builtin_amdgcn_rcpf(b);
+}
+__device__ __forceinline__ double fsqrt(double a) {
+  if constexpr (kExact) {
+    return sqrt(a);
+  } else {
+    // Goldschmidt: g -> sqrt(a), h -> 1/(2 sqrt(a)) from y = rsq(a)
+    const double y = __builtin_amdgcn_rsq(a);
+    double g = a * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    const double d = fma(-g, g, a);
+    g = fma(d, h, g);
+    return a == 0.0 ? 0.0 : g;
+  }
+}
 __device__ __forceinline__ float fsqrt(float a) {
   if constexpr (kExact) return sqrtf(a);
   else return __builtin_amdgcn_sqrtf(a);

@@ -34,7 +34,8 @@ struct SweepArgs {
   int nblocks;
   int bnz;             // block_nx(3) (1 in 2-D)
   int per_xcd;         // ceil(nblocks/8)
-  int comb_mode;       // 0: init = dst, 1: init = base, 2: init = a*base + b*src
+  int comb_mode;       // 0: init = dst, 1: init = base, 2: init = a*base + b*src,
+                       // 3: init = src (base is the stage input itself)
   double comb_a, comb_b;
   double dq[3];        // -qdt/dx_d
   double qdt;
@@ -122,23 +123,71 @@ __device__ __forceinline__ void load_cons(const double* __restrict__ w, int b, i
   for (int iw = 0; iw < P::NW; ++iw) c[iw] = w[off + iw * cells];
 }
 
-// primitive state of an arbitrary (possibly halo) cell
+// conserved state of an arbitrary (possibly halo) cell, with the sign of an
+// antisymmetric boundary applied
 template <class P, int NDIM>
-__device__ __forceinline__ void load_prim(const SweepArgs& a, const FVParams& fp, int b, int x, int y,
-                                          int z, double* w) {
+__device__ __forceinline__ void load_cons_at(const SweepArgs& a, int b, int x, int y, int z, double* c) {
   double sign;
   const int sb = locate<NDIM>(a, b, x, y, z, sign);
-  double c[P::NW];
   load_cons<P>(a.src, sb, x, y, z, a.bnz, c);
   if (sign < 0.0) {
 #pragma unroll
     for (int iw = 0; iw < P::NW; ++iw) c[iw] = -c[iw];
   }
-  to_prim<P>(c, w, fp);
 }
 
+// primitive state (FP64 conversion, rounded to R) of conserved state c
+template <class P, class R>
+__device__ __forceinline__ void prim_of(const double* c, R* out, const FVParams& fp) {
+  double w[P::NW];
+  to_prim<P>(c, w, fp);
+#pragma unroll
+  for (int iw = 0; iw < P::NW; ++iw) out[iw] = R(w[iw]);
+}
+
+// primitive state of an arbitrary (possibly halo) cell
+template <class P, class R, int NDIM>
+__device__ __forceinline__ void load_prim(const SweepArgs& a, const FVParams& fp, int b, int x, int y,
+                                          int z, R* w) {
+  double c[P::NW];
+  load_cons_at<P, NDIM>(a, b, x, y, z, c);
+  prim_of<P, R>(c, w, fp);
+}
+
+// The RK base and the output are touched once per launch: they stream
+// through L2 (non-temporal) so that they do not displace the stage input,
+// whose planes are re-read as halos by neighbouring workgroups.
+__device__ __forceinline__ double ld_stream(const double* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_stream(double v, double* p) { __builtin_nontemporal_store(v, p); }
+
+// Cell t of the four NG-deep halo strips of a plane (x and y block-local,
+// outside the block on one side), t < 4*NG*BX; then the 4 NG x NG corners.
+template <int NG>
+__device__ __forceinline__ void strip_cell(int t, int& x, int& y) {
+  constexpr int NSTRIP = 4 * NG * BX;
+  if (t < NSTRIP) {
+    const int s = t / (NG * BX), r = t % (NG * BX);
+    const int along = r % BX, depth = r / BX;
+    if (s == 0) { x = -NG + depth; y = along; }
+    else if (s == 1) { x = BX + depth; y = along; }
+    else if (s == 2) { x = along; y = -NG + depth; }
+    else { x = along; y = BX + depth; }
+  } else {
+    const int u = t - NSTRIP;
+    const int q = u / (NG * NG), r = u % (NG * NG);
+    const int ox = r % NG, oy = r / NG;
+    x = (q & 1) ? BX + ox : -NG + ox;
+    y = (q & 2) ? BX + oy : -NG + oy;
+  }
+}
+
+// Occupancy: FP64 kernels run 2 waves per SIMD (LDS and registers), FP32
+// kernels are held to 3 (at most 168 VGPRs).
+template <class R>
+constexpr int kMinWaves = std::is_same<R, float>::value ? 3 : 2;
+
 template <class P, class R, int RIEM, int LIM, int NG>
-__global__ void __launch_bounds__(NTHREADS)
+__global__ void __launch_bounds__(NTHREADS, kMinWaves<R>)
 sweep_kernel(SweepArgs a, FVParams fp) {
   constexpr int NDIM = P::NDIM;
   constexpr int NW = P::NW;
@@ -146,10 +195,19 @@ sweep_kernel(SweepArgs a, FVParams fp) {
   constexpr bool CORNERS = P::LINDE;  // Linde needs div B one cell out
   constexpr bool HANCOCK = RIEM == RIEM_HANCOCK;
   static_assert(!(P::LINDE && NDIM == 3), "Linde cleaning is 2-D only on the device path");
+  constexpr int NSTRIP = 4 * NG * BX;
+  constexpr int NHALO = NSTRIP + (CORNERS ? 4 * NG * NG : 0);
+  // 3-D, FP32 reconstruction: the halo strips of the NG planes ahead are
+  // loaded together with the window's lookahead plane and kept in an LDS ring
+  // (the face neighbours read the same planes at the same time, so the loads
+  // hit L2).  The FP64 kernel keeps to 2 workgroups per CU within 160 KB of
+  // LDS by loading the halo of the current plane instead.
+  constexpr bool HRING = NDIM == 3 && std::is_same<R, float>::value;
 
   __shared__ R sp[NW][PX][PX];          // primitive plane with halo
   __shared__ R fxs[HANCOCK ? 1 : NW][BX][BX + 1];  // x faces: fxs[iw][y][f]
   __shared__ R fys[HANCOCK ? 1 : NW][BX + 1][BX];  // y faces: fys[iw][f][x]
+  __shared__ R hring[HRING ? NG : 1][HRING ? NW : 1][HRING ? NSTRIP : 1];
 
   const int b = wg_block(a.per_xcd);
   if (b >= a.nblocks) return;
@@ -161,91 +219,136 @@ sweep_kernel(SweepArgs a, FVParams fp) {
   const size_t cells = (size_t)BX * BX * a.bnz;
   const size_t boff = (size_t)b * NW * cells;
 
-  // z window (3-D): W[m] = primitive plane zb+m of this lane's column
-  constexpr int NWIN = NDIM == 3 ? 2 * NG : 1;
-  R W[NWIN][NW];
+  // z window (3-D): Wp[m] = primitive plane k-NG+1+m (m < NG, the current
+  // plane last), Wc[m] = conserved plane k+1+m as stored (FP64).  The current
+  // plane's conserved state is the RK input of comb modes 2 and 3, so the
+  // block's own planes are read from HBM once.
+  constexpr int NWP = NDIM == 3 ? NG : 1;
+  R Wp[NWP][NW];
+  double Wc[NWP][NW];
   R fz_carry[NW];
+  // primitive planes k-NG+1 .. k+NG of this lane's column
+  auto zstencil = [&](R (*Z)[NW]) {
+#pragma unroll
+    for (int m = 0; m < NG; ++m)
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) Z[m][iw] = Wp[m][iw];
+#pragma unroll
+    for (int m = 0; m < NG; ++m) prim_of<P, R>(Wc[m], Z[NG + m], fp);
+  };
   if constexpr (NDIM == 3) {
-    // prologue: planes -NG .. NG-1, carried face -1/2
+    // prologue: planes -NG .. NG-1, halo ring of planes 0 .. NG-1, carried
+    // face -1/2
 #pragma unroll
-    for (int m = 0; m < 2 * NG; ++m) {
-      double w[NW];
-      load_prim<P, NDIM>(a, fp, b, tx, ty, m - NG, w);
+    for (int m = 0; m < NG; ++m) load_prim<P, R, NDIM>(a, fp, b, tx, ty, m - NG, Wp[m]);
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) W[m][iw] = R(w[iw]);
+    for (int m = 0; m < NG; ++m) load_cons_at<P, NDIM>(a, b, tx, ty, m, Wc[m]);
+    if constexpr (HRING) {
+      for (int p = 0; p < NG && p < a.bnz; ++p)
+        for (int t = tid; t < NSTRIP; t += NTHREADS) {
+          int x, y;
+          strip_cell<NG>(t, x, y);
+          R w[NW];
+          load_prim<P, R, NDIM>(a, fp, b, x, y, p, w);
+#pragma unroll
+          for (int iw = 0; iw < NW; ++iw) hring[p][iw][t] = w[iw];
+        }
     }
-    if constexpr (!HANCOCK) face_flux<P, R, RIEM, 2, LIM, NG>(W, fp, ch, fz_carry, crash);
+    if constexpr (!HANCOCK) {
+      R Z[2 * NG][NW];
+      zstencil(Z);
+      face_flux<P, R, RIEM, 2, LIM, NG>(Z, fp, ch, fz_carry, crash);
+    }
+    if constexpr (HRING) __syncthreads();
   }
 
   const int nz = NDIM == 3 ? a.bnz : 1;
   for (int k = 0; k < nz; ++k) {
-    // ---- stage plane k (primitive, with x/y halo) in LDS -----------------
-    double cons_k[NW];  // wCT of this lane's cell (for init when comb uses it)
-    if constexpr (NDIM == 3) {
-      // advance the window: drop plane k-NG, load plane k+NG, so that
-      // W[m] = plane k-NG+1+m and plane k = W[NG-1]
+    const size_t o = boff + ((size_t)k * BX + ty) * BX + tx;
+    double cons_k[NW];  // wCT of this lane's cell
+    double acc[NW];
+    // RK base (src/mod_advance.t:251-267), streamed; the combination with
+    // the stage input is formed once the base has arrived
+    auto rk_init = [&]() {
+      if (a.comb_mode == 0) {
 #pragma unroll
-      for (int m = 0; m < 2 * NG - 1; ++m)
+        for (int iw = 0; iw < NW; ++iw) acc[iw] = ld_stream(a.dst + o + iw * cells);
+      } else if (a.comb_mode == 1) {
 #pragma unroll
-        for (int iw = 0; iw < NW; ++iw) W[m][iw] = W[m + 1][iw];
-      {
-        double w[NW];
-        load_prim<P, NDIM>(a, fp, b, tx, ty, k + NG, w);
+        for (int iw = 0; iw < NW; ++iw) acc[iw] = ld_stream(a.base + o + iw * cells);
+      } else if (a.comb_mode == 2) {
 #pragma unroll
-        for (int iw = 0; iw < NW; ++iw) W[2 * NG - 1][iw] = R(w[iw]);
+        for (int iw = 0; iw < NW; ++iw) acc[iw] = a.comb_a * ld_stream(a.base + o + iw * cells) + a.comb_b * cons_k[iw];
+      } else {
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) acc[iw] = cons_k[iw];
       }
+    };
+    // ---- stage plane k (primitive, with x/y halo) in LDS -----------------
+    if constexpr (NDIM == 3) {
+      // advance the window and issue the lookahead plane k+NG
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) sp[iw][ty + NG][tx + NG] = W[NG - 1][iw];
-      load_cons<P>(a.src, b, tx, ty, k, a.bnz, cons_k);
+      for (int iw = 0; iw < NW; ++iw) cons_k[iw] = Wc[0][iw];
+#pragma unroll
+      for (int m = 0; m < NG - 1; ++m)
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) Wp[m][iw] = Wp[m + 1][iw];
+      prim_of<P, R>(cons_k, Wp[NG - 1], fp);
+#pragma unroll
+      for (int m = 0; m < NG - 1; ++m)
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) Wc[m][iw] = Wc[m + 1][iw];
+      load_cons_at<P, NDIM>(a, b, tx, ty, k + NG, Wc[NG - 1]);
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) sp[iw][ty + NG][tx + NG] = Wp[NG - 1][iw];
     } else {
       load_cons<P>(a.src, b, tx, ty, 0, 1, cons_k);
-      double w[NW];
-      to_prim<P>(cons_k, w, fp);
+      R w[NW];
+      prim_of<P, R>(cons_k, w, fp);
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) sp[iw][ty + NG][tx + NG] = R(w[iw]);
+      for (int iw = 0; iw < NW; ++iw) sp[iw][ty + NG][tx + NG] = w[iw];
     }
-    // halo strips: 4 strips of NG x 16 cells (+ 4 NG x NG corners for Linde)
-    {
-      constexpr int NSTRIP = 4 * NG * BX;
-      constexpr int NCORN = CORNERS ? 4 * NG * NG : 0;
-      for (int t = tid; t < NSTRIP + NCORN; t += NTHREADS) {
+    if constexpr (HRING) {
+      const int slot = k % NG;
+      for (int t = tid; t < NSTRIP; t += NTHREADS) {
         int x, y;
-        if (t < NSTRIP) {
-          const int s = t / (NG * BX), r = t % (NG * BX);
-          const int along = r % BX, depth = r / BX;
-          if (s == 0) { x = -NG + depth; y = along; }
-          else if (s == 1) { x = BX + depth; y = along; }
-          else if (s == 2) { x = along; y = -NG + depth; }
-          else { x = along; y = BX + depth; }
-        } else {
-          const int u = t - NSTRIP;
-          const int q = u / (NG * NG), r = u % (NG * NG);
-          const int ox = r % NG, oy = r / NG;
-          x = (q & 1) ? BX + ox : -NG + ox;
-          y = (q & 2) ? BX + oy : -NG + oy;
-        }
-        double w[NW];
-        load_prim<P, NDIM>(a, fp, b, x, y, k, w);
+        strip_cell<NG>(t, x, y);
 #pragma unroll
-        for (int iw = 0; iw < NW; ++iw) sp[iw][y + NG][x + NG] = R(w[iw]);
+        for (int iw = 0; iw < NW; ++iw) sp[iw][y + NG][x + NG] = hring[slot][iw][t];
+      }
+    } else {
+      // halo strips (+ corners for Linde) of plane k
+      for (int t = tid; t < NHALO; t += NTHREADS) {
+        int x, y;
+        strip_cell<NG>(t, x, y);
+        R w[NW];
+        load_prim<P, R, NDIM>(a, fp, b, x, y, k, w);
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) sp[iw][y + NG][x + NG] = w[iw];
       }
     }
     __syncthreads();
-
-    double acc[NW];
-    // ---- init: the RK combination (src/mod_advance.t:251-267) ------------
-    {
-      const size_t o = boff + ((size_t)k * BX + ty) * BX + tx;
-      if (a.comb_mode == 0) {
+    if constexpr (HRING) {
+      // halo of plane k+NG into the ring slot just copied out
+      if (k + NG < nz) {
+        const int slot = k % NG;
+        for (int t = tid; t < NSTRIP; t += NTHREADS) {
+          int x, y;
+          strip_cell<NG>(t, x, y);
+          R w[NW];
+          load_prim<P, R, NDIM>(a, fp, b, x, y, k + NG, w);
 #pragma unroll
-        for (int iw = 0; iw < NW; ++iw) acc[iw] = a.dst[o + iw * cells];
-      } else if (a.comb_mode == 1) {
-#pragma unroll
-        for (int iw = 0; iw < NW; ++iw) acc[iw] = a.base[o + iw * cells];
-      } else {
-#pragma unroll
-        for (int iw = 0; iw < NW; ++iw) acc[iw] = a.comb_a * a.base[o + iw * cells] + a.comb_b * cons_k[iw];
+          for (int iw = 0; iw < NW; ++iw) hring[slot][iw][t] = w[iw];
+        }
       }
+    }
+
+    // z neighbours of this lane's cell for the sources and the Hancock
+    // predictor: primitive planes k-1 and k+1
+    R zm[NW], zp[NW];
+    if constexpr (NDIM == 3) {
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) zm[iw] = Wp[NG - 2][iw];
     }
 
     if constexpr (!HANCOCK) {
@@ -279,8 +382,16 @@ sweep_kernel(SweepArgs a, FVParams fp) {
           for (int iw = 0; iw < NW; ++iw) fys[iw][f][x] = fc[iw];
         }
       }
+      rk_init();
+      // z face k+1/2 last, so that the lookahead load has landed
       R fz_new[NW];
-      if constexpr (NDIM == 3) face_flux<P, R, RIEM, 2, LIM, NG>(W, fp, ch, fz_new, crash);
+      if constexpr (NDIM == 3) {
+        R Z[2 * NG][NW];
+        zstencil(Z);
+        face_flux<P, R, RIEM, 2, LIM, NG>(Z, fp, ch, fz_new, crash);
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) zp[iw] = Z[NG][iw];
+      }
       __syncthreads();
       // ---- flux divergence, dimension by dimension (:269-284) ------------
 #pragma unroll
@@ -294,6 +405,8 @@ sweep_kernel(SweepArgs a, FVParams fp) {
       }
     } else {
       // ---- Hancock predictor (src/mod_finite_volume.t:19-113) -------------
+      rk_init();
+      if constexpr (NDIM == 3) prim_of<P, R>(Wc[0], zp, fp);
       const R inv_g1 = R(fp.inv_gamma_1);
       auto hancock_dim = [&](auto IDc, const R* wm, const R* w0, const R* wp) {
         constexpr int ID = decltype(IDc)::value;
@@ -329,8 +442,7 @@ sweep_kernel(SweepArgs a, FVParams fp) {
         wp[iw] = sp[iw][ty + NG + 1][tx + NG];
       }
       hancock_dim(std::integral_constant<int, 1>(), wm, w0, wp);
-      if constexpr (NDIM == 3)
-        hancock_dim(std::integral_constant<int, 2>(), W[NG - 2], W[NG - 1], W[NG]);
+      if constexpr (NDIM == 3) hancock_dim(std::integral_constant<int, 2>(), zm, w0, zp);
     }
 
     // ---- sources: addsource2 -> mhd_add_source (src/mod_source.t:130-160,
@@ -348,9 +460,8 @@ sweep_kernel(SweepArgs a, FVParams fp) {
         divb = divb + 0.5 * (double(sp[P::MAG][cy][cx + 1]) - double(sp[P::MAG][cy][cx - 1])) * fp.dxinv[0];
         divb = divb + 0.5 * (double(sp[P::MAG + 1][cy + 1][cx]) - double(sp[P::MAG + 1][cy - 1][cx])) * fp.dxinv[1];
         if constexpr (NDIM == 3) {
-          const int c = NG - 1;
-          gp[2] = ((double(W[c + 1][S]) - double(W[c - 1][S])) * 0.5) / fp.dx[2];
-          divb = divb + 0.5 * (double(W[c + 1][P::MAG + 2]) - double(W[c - 1][P::MAG + 2])) * fp.dxinv[2];
+          gp[2] = ((double(zp[S]) - double(zm[S])) * 0.5) / fp.dx[2];
+          divb = divb + 0.5 * (double(zp[P::MAG + 2]) - double(zm[P::MAG + 2])) * fp.dxinv[2];
         }
 #pragma unroll
         for (int d = 0; d < P::NDIR; ++d) bn[d] = double(sp[P::MAG + d][cy][cx]);
@@ -381,11 +492,8 @@ sweep_kernel(SweepArgs a, FVParams fp) {
       acc[P::E] = acc[P::E] + double(sp[P::MAG + 1][cy][cx]) * g;
     }
 
-    {
-      const size_t o = boff + ((size_t)k * BX + ty) * BX + tx;
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) a.dst[o + iw * cells] = acc[iw];
-    }
+    for (int iw = 0; iw < NW; ++iw) st_stream(acc[iw], a.dst + o + iw * cells);
     __syncthreads();  // LDS plane is rewritten next iteration
   }
   if (crash) *a.crash = 1;

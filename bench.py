@@ -119,7 +119,10 @@ def main() -> int:
     cells = n ** 3
     stages = 3
     nw = 9
-    b_alg = 3 * nw * 8  # stage input + RK base + stage output, FP64 state
+    # algorithmic HBM bytes per cell and launch, FP64 state: stage 1 reads the
+    # stage input (also its RK base) and writes the output; stages 2 and 3
+    # also read the RK base: (2 + 3 + 3) / 3 slot passes on average
+    b_alg = (2 + 3 + 3) * nw * 8 / 3
 
     par64 = workload_par("fp64")
     be64, ti64, t64, ms64, _ = run_device(par64, mesh, w0, args.steps, args.warmup, "fp64")
@@ -148,7 +151,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("fp64"),
                      "kernel": "fv::sweep_kernel<MHD-GLM 3-D, double, HLL>", "avg_launch_ms": round(ms64, 4),
-                     "alg_bytes_per_launch": b_alg * cells},
+                     "alg_bytes_per_launch": int(b_alg * cells)},
     }
     if not args.no_mixed:
         w64 = mesh.interior(be64.get_state(0))
