@@ -54,6 +54,25 @@ struct amrvac_fv_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   fvc::FVParams fp{};
+  // block-structured AMR: virtual blocks after the nb leaves hold the ghost
+  // cells across level jumps (restricted from finer or prolonged from
+  // coarser neighbours); the sweep's neighbour table points at them
+  bool amr = false;
+  bool fix_conserve = false;
+  int nv = 0;                // virtual blocks
+  int nface = 0;             // register stride (transverse cells of a face)
+  int* d_level = nullptr;    // [nb] level-1
+  int* d_ntype = nullptr;    // [nb][nnb]
+  int* d_nchild = nullptr;   // [nb][nnb][nch]
+  int4* d_rtask = nullptr;   // restriction: (coarse block, direction, virtual block, -)
+  int4* d_ptask = nullptr;   // prolongation: (fine block, direction, virtual block, coarse block)
+  int2* d_ptask2 = nullptr;  // prolongation: (packed position in parent, packed direction coarse->parent)
+  int* d_ftask = nullptr;    // refluxing: coarse blocks with a finer face neighbour
+  int* d_fregoff = nullptr;  // [nb][2*ndim]
+  double* d_freg = nullptr;  // flux registers
+  double* d_xlo = nullptr;   // [nb][ndim] block lower corners
+  int nrt = 0, npt = 0, nft = 0;
+  std::vector<int> h_level;
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
   size_t nev = 0;
@@ -63,9 +82,14 @@ namespace fv {
 
 template <class P>
 __global__ void __launch_bounds__(NTHREADS) getdt_kernel(const double* __restrict__ w, int nblocks, int bnz,
-                                                         FVParams fp, double* out) {
+                                                         FVParams fp, const int* __restrict__ blevel,
+                                                         double* out) {
   const int b = blockIdx.x;
   if (b >= nblocks) return;
+  if (blevel) {  // the block's level: 1/dx scales by 2^level exactly
+    const double sc = double(1 << blevel[b]);
+    for (int d = 0; d < 3; ++d) fp.dxinv[d] = fp.dxinv[d] * sc;
+  }
   const size_t cells = (size_t)BX * BX * bnz;
   const double* wb = w + (size_t)b * P::NW * cells;
   double maxtot = -1.0e99, cmaxb = 0.0;
@@ -178,9 +202,267 @@ __global__ void __launch_bounds__(NTHREADS) fill_ghosted_kernel(SweepArgs a, int
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// block-structured AMR: ghost cells across level jumps and refluxing
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void decode_dir(int k, int ndim, int* i) {
+  for (int d = 0; d < 3; ++d) {
+    if (d < ndim) { i[d] = k % 3 - 1; k /= 3; } else i[d] = 0;
+  }
+}
+
+// Ghost cells of a coarse block facing a finer neighbour: the mean of the
+// 2^ndim fine cells each covers, summed in array-element order
+// (coarsen_grid src/amr/mod_coarsen.t:12-79 into the coarse representation,
+// bc_fill_restrict src/mod_ghostcells_update.t:855-898).  Written into the
+// direction's virtual block in the neighbour's frame.
+template <int NDIM>
+__global__ void __launch_bounds__(NTHREADS) restrict_kernel(double* w, int nw, int bnz, int ng, int nnb,
+                                                            const int4* task, const int* nchild) {
+  const int4 t = task[blockIdx.x];
+  const int C = t.x, k = t.y, V = t.z;
+  int i[3];
+  decode_dir(k, NDIM, i);
+  const int n[3] = {BX, BX, bnz};
+  int lo[3] = {0, 0, 0}, cnt[3] = {1, 1, 1};
+  for (int d = 0; d < NDIM; ++d) {
+    lo[d] = i[d] == -1 ? -ng : (i[d] == 0 ? 0 : n[d]);
+    cnt[d] = i[d] == 0 ? n[d] : ng;
+  }
+  const size_t cells = (size_t)BX * BX * bnz;
+  const double cofi = 1.0 / double(1 << NDIM);
+  const int total = cnt[0] * cnt[1] * cnt[2];
+  for (int c = threadIdx.x; c < total; c += NTHREADS) {
+    const int g[3] = {lo[0] + c % cnt[0], lo[1] + (c / cnt[0]) % cnt[1], lo[2] + c / (cnt[0] * cnt[1])};
+    int gn[3] = {0, 0, 0}, f[3] = {0, 0, 0}, cidx = 0;
+    for (int d = 0; d < NDIM; ++d) {
+      gn[d] = g[d] - i[d] * n[d];
+      const int ic = gn[d] >= n[d] / 2 ? 2 : 1;
+      cidx += (ic - 1) << d;
+      f[d] = 2 * (gn[d] - (ic - 1) * (n[d] / 2));
+    }
+    const int F = nchild[((size_t)C * nnb + k) * (1 << NDIM) + cidx];
+    const double* wf = w + (size_t)F * nw * cells;
+    double* wv = w + (size_t)V * nw * cells + ((size_t)gn[2] * BX + gn[1]) * BX + gn[0];
+    for (int iw = 0; iw < nw; ++iw) {
+      double sum = 0.0;
+      bool first = true;
+      for (int dk = 0; dk <= (NDIM > 2 ? 1 : 0); ++dk)
+        for (int dj = 0; dj <= 1; ++dj)
+          for (int di = 0; di <= 1; ++di) {
+            const double v = wf[iw * cells + ((size_t)(f[2] + dk) * BX + (f[1] + dj)) * BX + f[0] + di];
+            sum = first ? v : sum + v;
+            first = false;
+          }
+      wv[iw * cells] = sum * cofi;
+    }
+  }
+}
+
+// one cell of the coarse representation of fine block F (1-based index p of
+// the (n/2 + 2ng)^ndim coarse grid around F), variable iw, as bc_prolong
+// sees it: physical boundary rule on the coarse grid where F touches the
+// boundary (bc_phys), the mean of F's own fine cells inside F, and the
+// coarse neighbour's data (its interior or, through its neighbour table,
+// its same-level or restricted ghost cells) elsewhere
+template <int NDIM>
+__device__ double psc_value(const SweepArgs& a, int nw, int ng, int F, int C, const int* icF, const int* iC,
+                            const bool (*bnd)[2], int p0, int p1, int p2, int iw) {
+  const int n[3] = {BX, BX, a.bnz};
+  int q[3] = {p0, p1, p2};
+  double sign = 1.0;
+  bool inside = true;
+  for (int d = 0; d < NDIM; ++d) {
+    const int clo = ng + 1, chi = ng + n[d] / 2;
+    if (q[d] < clo && bnd[d][0]) {
+      const int tp = a.bc[2 * d];
+      if (tp == 2) q[d] = clo;
+      else { q[d] = 2 * clo - 1 - q[d]; if (tp == 4) sign = -sign; }
+    } else if (q[d] > chi && bnd[d][1]) {
+      const int tp = a.bc[2 * d + 1];
+      if (tp == 2) q[d] = chi;
+      else { q[d] = 2 * chi + 1 - q[d]; if (tp == 4) sign = -sign; }
+    }
+    if (q[d] < clo || q[d] > chi) inside = false;
+  }
+  const size_t cells = (size_t)BX * BX * a.bnz;
+  if (inside) {
+    int f[3] = {0, 0, 0};
+    for (int d = 0; d < NDIM; ++d) f[d] = 2 * (q[d] - (ng + 1));
+    const double* wf = a.src + (size_t)F * nw * cells + iw * cells;
+    double sum = 0.0;
+    bool first = true;
+    for (int dk = 0; dk <= (NDIM > 2 ? 1 : 0); ++dk)
+      for (int dj = 0; dj <= 1; ++dj)
+        for (int di = 0; di <= 1; ++di) {
+          const double v = wf[((size_t)(f[2] + dk) * BX + (f[1] + dj)) * BX + f[0] + di];
+          sum = first ? v : sum + v;
+          first = false;
+        }
+    return sign * (sum * (1.0 / double(1 << NDIM)));
+  }
+  int x[3] = {0, 0, 0};
+  for (int d = 0; d < NDIM; ++d) x[d] = q[d] + (icF[d] - 1) * (n[d] / 2) + n[d] * iC[d] - ng - 1;
+  double s2;
+  const int sb = locate<NDIM>(a, C, x[0], x[1], x[2], s2);
+  return sign * s2 * a.src[(size_t)sb * nw * cells + iw * cells + ((size_t)x[2] * BX + x[1]) * BX + x[0]];
+}
+
+// Ghost cells of a fine block facing a coarser neighbour: interpolation_linear
+// (src/mod_ghostcells_update.t:1381-1525, slab_uniform, conserved variables)
+// from the coarse representation with limited slopes, as bc_prolong
+// (:1261-1339) and bc_fill_prolong (:1103-1154) assemble it.
+template <int NDIM>
+__global__ void __launch_bounds__(NTHREADS) prolong_kernel(SweepArgs a, double* w, int nw, int ng, int nnb,
+                                                           const int4* task, const int2* task2, const int* ntype,
+                                                           const int* level, const double* xlo, FVParams fp) {
+  const int4 t = task[blockIdx.x];
+  const int2 t2 = task2[blockIdx.x];
+  const int F = t.x, k = t.y, V = t.z, C = t.w;
+  int i[3], icF[3] = {1, 1, 1}, iC[3] = {0, 0, 0};
+  decode_dir(k, NDIM, i);
+  for (int d = 0; d < NDIM; ++d) {
+    icF[d] = (t2.x >> (2 * d)) & 3;
+    iC[d] = ((t2.y >> (2 * d)) & 3) - 1;
+  }
+  const int n[3] = {BX, BX, a.bnz};
+  bool bnd[3][2] = {{false, false}, {false, false}, {false, false}};
+  for (int d = 0; d < NDIM; ++d)
+    for (int sd = 0; sd < 2; ++sd) {
+      int v[3] = {0, 0, 0};
+      v[d] = sd == 0 ? -1 : 1;
+      int kk = 0, m = 1;
+      for (int e = 0; e < NDIM; ++e) { kk += (v[e] + 1) * m; m *= 3; }
+      bnd[d][sd] = ntype[(size_t)F * nnb + kk] == 1;
+    }
+  int lo[3] = {0, 0, 0}, cnt[3] = {1, 1, 1};
+  double dxFi[3] = {1, 1, 1}, dxCo[3] = {1, 1, 1}, invdxCo[3] = {1, 1, 1}, xFimin[3] = {0, 0, 0},
+         xComin[3] = {0, 0, 0};
+  const double sc = double(1 << level[F]);
+  for (int d = 0; d < NDIM; ++d) {
+    lo[d] = i[d] == -1 ? -ng : (i[d] == 0 ? 0 : n[d]);
+    cnt[d] = i[d] == 0 ? n[d] : ng;
+    dxFi[d] = fp.dx[d] / sc;
+    dxCo[d] = 2.0 * dxFi[d];
+    invdxCo[d] = 1.0 / dxCo[d];
+    const double rpxmin = xlo[(size_t)F * NDIM + d];
+    xFimin[d] = rpxmin - double(ng) * dxFi[d];
+    xComin[d] = rpxmin - double(ng) * dxCo[d];
+  }
+  const size_t cells = (size_t)BX * BX * a.bnz;
+  const int total = cnt[0] * cnt[1] * cnt[2];
+  for (int c = threadIdx.x; c < total; c += NTHREADS) {
+    const int g[3] = {lo[0] + c % cnt[0], lo[1] + (c / cnt[0]) % cnt[1], lo[2] + c / (cnt[0] * cnt[1])};
+    int co[3] = {1, 1, 1};
+    double eta[3] = {0, 0, 0};
+    for (int d = 0; d < NDIM; ++d) {
+      const double xFi = xFimin[d] + (double(g[d] + ng + 1) - 0.5) * dxFi[d];
+      co[d] = int((xFi - xComin[d]) * invdxCo[d]) + 1;
+      const double xCo = xComin[d] + (double(co[d]) - 0.5) * dxCo[d];
+      eta[d] = (xFi - xCo) * invdxCo[d];
+    }
+    int gn[3] = {0, 0, 0};
+    for (int d = 0; d < NDIM; ++d) gn[d] = g[d] - i[d] * n[d];
+    double* wv = w + (size_t)V * nw * cells + ((size_t)gn[2] * BX + gn[1]) * BX + gn[0];
+    for (int iw = 0; iw < nw; ++iw) {
+      const double c0 = psc_value<NDIM>(a, nw, ng, F, C, icF, iC, bnd, co[0], co[1], co[2], iw);
+      double val = c0;
+      for (int d = 0; d < NDIM; ++d) {
+        int h[3] = {co[0], co[1], co[2]}, p[3] = {co[0], co[1], co[2]};
+        h[d] -= 1;
+        p[d] += 1;
+        const double slopeL = c0 - psc_value<NDIM>(a, nw, ng, F, C, icF, iC, bnd, h[0], h[1], h[2], iw);
+        const double slopeR = psc_value<NDIM>(a, nw, ng, F, C, icF, iC, bnd, p[0], p[1], p[2], iw) - c0;
+        const double slopeC = 0.5 * (slopeR + slopeL);
+        const double signC = copysign(1.0, slopeC);
+        const double m1 = fabs(slopeC) < signC * slopeL ? fabs(slopeC) : signC * slopeL;
+        const double m2 = m1 < signC * slopeR ? m1 : signC * slopeR;
+        const double slope = signC * (0.0 > m2 ? 0.0 : m2);
+        val = val + slope * eta[d];
+      }
+      wv[iw * cells] = val;
+    }
+  }
+}
+
+// fix_conserve (src/mod_fix_conserve.t:465-595) with the registers of
+// store_flux (:597-648): a coarse block's cells next to a finer neighbour
+// drop their own face flux and take the sum of the fine fluxes
+template <int NDIM>
+__global__ void __launch_bounds__(NTHREADS) fixcons_kernel(double* w, int nw, int bnz, int nnb, const int* ftask,
+                                                           const int* ntype, const int* nchild,
+                                                           const int* fregoff, const double* freg, int nface) {
+  const int C = ftask[blockIdx.x];
+  const int n[3] = {BX, BX, bnz};
+  const size_t cells = (size_t)BX * BX * bnz;
+  const double cofi = 1.0 / double(1 << NDIM);
+  double* wc = w + (size_t)C * nw * cells;
+  for (int idims = 0; idims < NDIM; ++idims) {
+    int ta = -1, tb = -1;
+    for (int d = 0; d < NDIM; ++d) {
+      if (d == idims) continue;
+      if (ta < 0) ta = d; else tb = d;
+    }
+    const int na = n[ta], nbt = tb >= 0 ? n[tb] : 1;
+    for (int iside = 1; iside <= 2; ++iside) {
+      int v[3] = {0, 0, 0};
+      v[idims] = 2 * iside - 3;
+      int k = 0, m = 1;
+      for (int e = 0; e < NDIM; ++e) { k += (v[e] + 1) * m; m *= 3; }
+      if (ntype[(size_t)C * nnb + k] != 4) continue;
+      const int ix = iside == 1 ? 0 : n[idims] - 1;
+      const double* own = freg + fregoff[C * 2 * NDIM + 2 * idims + iside - 1];
+      for (int t = threadIdx.x; t < nw * na * nbt; t += NTHREADS) {
+        const int iw = t / (na * nbt), r = t % (na * nbt), a2 = r % na, b2 = r / na;
+        int c[3] = {0, 0, 0};
+        c[idims] = ix;
+        c[ta] = a2;
+        if (tb >= 0) c[tb] = b2;
+        const double f = own[(size_t)iw * nface + a2 + (size_t)na * b2];
+        const double pflux = iside == 1 ? -f : f;
+        double& x = wc[iw * cells + ((size_t)c[2] * BX + c[1]) * BX + c[0]];
+        x = x - pflux;
+      }
+      __syncthreads();
+      const int iother = 3 - iside;
+      const int nca = na / 2, ncb = tb >= 0 ? nbt / 2 : 1;
+      const int nkid = 1 << (NDIM - 1);
+      for (int t = threadIdx.x; t < nkid * nw * nca * ncb; t += NTHREADS) {
+        const int kid = t / (nw * nca * ncb), r0 = t % (nw * nca * ncb);
+        const int iw = r0 / (nca * ncb), r = r0 % (nca * ncb), a2 = r % nca, b2 = r / nca;
+        int ic[3] = {1, 1, 1};
+        ic[idims] = v[idims] == -1 ? 2 : 1;
+        ic[ta] = 1 + (kid & 1);
+        if (tb >= 0) ic[tb] = 1 + ((kid >> 1) & 1);
+        int cidx = 0;
+        for (int d = 0; d < NDIM; ++d) cidx += (ic[d] - 1) << d;
+        const int F = nchild[((size_t)C * nnb + k) * (1 << NDIM) + cidx];
+        const double* q = freg + fregoff[F * 2 * NDIM + 2 * idims + iother - 1] + (size_t)iw * nface;
+        const int fa = 2 * a2, fb = tb >= 0 ? 2 * b2 : 0;
+        double sum = q[fa + (size_t)na * fb];
+        sum = sum + q[fa + 1 + (size_t)na * fb];
+        if (tb >= 0) {
+          sum = sum + q[fa + (size_t)na * (fb + 1)];
+          sum = sum + q[fa + 1 + (size_t)na * (fb + 1)];
+        }
+        const double pf = iother == 1 ? sum : -sum;
+        int c[3] = {0, 0, 0};
+        c[idims] = ix;
+        c[ta] = (ic[ta] - 1) * nca + a2;
+        if (tb >= 0) c[tb] = (ic[tb] - 1) * ncb + b2;
+        double& x = wc[iw * cells + ((size_t)c[2] * BX + c[1]) * BX + c[0]];
+        x = x + pf * cofi;
+      }
+      __syncthreads();
+    }
+  }
+}
+
 template <class P>
-hipError_t launch_getdt(const double* w, int nb, int bnz, const FVParams& fp, double* out, hipStream_t s) {
-  hipLaunchKernelGGL((getdt_kernel<P>), dim3(nb), dim3(NTHREADS), 0, s, w, nb, bnz, fp, out);
+hipError_t launch_getdt(const double* w, int nb, int bnz, const FVParams& fp, const int* blevel, double* out,
+                        hipStream_t s) {
+  hipLaunchKernelGGL((getdt_kernel<P>), dim3(nb), dim3(NTHREADS), 0, s, w, nb, bnz, fp, blevel, out);
   return hipGetLastError();
 }
 
@@ -234,8 +516,9 @@ hipError_t with_phys(const amrvac_fv_ctx* c, Args... args) {
 
 template <class P>
 struct GetdtOp {
-  static hipError_t run(const double* w, int nb, int bnz, const fvc::FVParams* fp, double* out, hipStream_t s) {
-    return fv::launch_getdt<P>(w, nb, bnz, *fp, out, s);
+  static hipError_t run(const double* w, int nb, int bnz, const fvc::FVParams* fp, const int* blevel, double* out,
+                        hipStream_t s) {
+    return fv::launch_getdt<P>(w, nb, bnz, *fp, blevel, out, s);
   }
 };
 
@@ -296,6 +579,17 @@ static void free_level(amrvac_fv_ctx* c) {
   if (c->d_red2) (void)hipFree(c->d_red2);
   if (c->d_stage) (void)hipFree(c->d_stage);
   c->d_nbr = nullptr; c->d_crash = nullptr; c->d_red = nullptr; c->d_red2 = nullptr; c->d_stage = nullptr;
+  void* amr_bufs[] = {c->d_level, c->d_ntype, c->d_nchild, c->d_rtask, c->d_ptask, c->d_ptask2,
+                      c->d_ftask, c->d_fregoff, c->d_freg, c->d_xlo};
+  for (void* p : amr_bufs)
+    if (p) (void)hipFree(p);
+  c->d_level = nullptr; c->d_ntype = nullptr; c->d_nchild = nullptr; c->d_rtask = nullptr;
+  c->d_ptask = nullptr; c->d_ptask2 = nullptr; c->d_ftask = nullptr; c->d_fregoff = nullptr;
+  c->d_freg = nullptr; c->d_xlo = nullptr;
+  c->amr = false;
+  c->nv = 0;
+  c->nrt = c->npt = c->nft = 0;
+  c->h_level.clear();
 }
 
 int amrvac_fv_finalize(amrvac_fv_ctx* c) {
@@ -322,6 +616,9 @@ int amrvac_fv_set_stream(amrvac_fv_ctx* c, void* stream) {
   return AMRVAC_FV_OK;
 }
 
+static int alloc_level(amrvac_fv_ctx* c, int nblocks, int capacity, const double* dx, const int* neighbor,
+                       const int* bc);
+
 int amrvac_fv_set_mesh(amrvac_fv_ctx* c, int nblocks, const double* dx, const int* neighbor,
                        const double* xlohi, const int* bc) {
   if (!c || !dx || !neighbor || !bc) return fail(AMRVAC_FV_EINVAL, "null argument");
@@ -340,10 +637,19 @@ int amrvac_fv_set_mesh(amrvac_fv_ctx* c, int nblocks, const double* dx, const in
     if (!(dx[d] > 0.0)) return fail(AMRVAC_FV_EINVAL, "dx must be positive");
   HIPCHK(hipStreamSynchronize(c->stream));
   free_level(c);
+  return alloc_level(c, nblocks, nblocks, dx, neighbor, bc);
+}
+
+// state slots with room for `capacity` blocks (leaves + virtual ghost
+// blocks), work buffers, the sweep's neighbour table and the scheme
+// constants of the level-1 cell size dx
+static int alloc_level(amrvac_fv_ctx* c, int nblocks, int capacity, const double* dx, const int* neighbor,
+                       const int* bc) {
+  const int nd = c->cfg.ndim;
   c->nb = nblocks;
   for (int d = 0; d < 3; ++d) c->dx[d] = d < nd ? dx[d] : 1.0;
   for (int s = 0; s < 6; ++s) c->bc[s] = s < 2 * nd ? bc[s] : 5;
-  const size_t n = (size_t)nblocks * c->nw * c->cells;
+  const size_t n = (size_t)capacity * c->nw * c->cells;
   for (int s = 0; s < 3; ++s) {
     if (hipMalloc(&c->slot[s], n * sizeof(double)) != hipSuccess) {
       free_level(c);
@@ -425,6 +731,61 @@ static fvc::SweepArgs base_args(const amrvac_fv_ctx* c) {
   return a;
 }
 
+
+}  // extern "C"
+
+// ghost cells across level jumps of a slot: restriction into the coarse
+// blocks' virtual blocks, then prolongation into the fine blocks' (which
+// reads the former through the coarse block's neighbour table)
+static int getbc_amr(amrvac_fv_ctx* c, int slot) {
+  if (!c->amr) return AMRVAC_FV_OK;
+  const int nd = c->cfg.ndim, ng = c->cfg.nghost;
+  if (c->nrt) {
+    if (nd == 2)
+      hipLaunchKernelGGL((fv::restrict_kernel<2>), dim3(c->nrt), dim3(fvc::NTHREADS), 0, c->stream, c->slot[slot],
+                         c->nw, c->bnz, ng, c->nnb, c->d_rtask, c->d_nchild);
+    else
+      hipLaunchKernelGGL((fv::restrict_kernel<3>), dim3(c->nrt), dim3(fvc::NTHREADS), 0, c->stream, c->slot[slot],
+                         c->nw, c->bnz, ng, c->nnb, c->d_rtask, c->d_nchild);
+    HIPCHK(hipGetLastError());
+  }
+  if (c->npt) {
+    fvc::SweepArgs a = base_args(c);
+    a.src = c->slot[slot];
+    if (nd == 2)
+      hipLaunchKernelGGL((fv::prolong_kernel<2>), dim3(c->npt), dim3(fvc::NTHREADS), 0, c->stream, a, c->slot[slot],
+                         c->nw, ng, c->nnb, c->d_ptask, c->d_ptask2, c->d_ntype, c->d_level, c->d_xlo, c->fp);
+    else
+      hipLaunchKernelGGL((fv::prolong_kernel<3>), dim3(c->npt), dim3(fvc::NTHREADS), 0, c->stream, a, c->slot[slot],
+                         c->nw, ng, c->nnb, c->d_ptask, c->d_ptask2, c->d_ntype, c->d_level, c->d_xlo, c->fp);
+    HIPCHK(hipGetLastError());
+  }
+  return AMRVAC_FV_OK;
+}
+
+static int fix_conserve_amr(amrvac_fv_ctx* c, int slot) {
+  if (!c->amr || !c->fix_conserve || !c->nft) return AMRVAC_FV_OK;
+  if (c->cfg.ndim == 2)
+    hipLaunchKernelGGL((fv::fixcons_kernel<2>), dim3(c->nft), dim3(fvc::NTHREADS), 0, c->stream, c->slot[slot],
+                       c->nw, c->bnz, c->nnb, c->d_ftask, c->d_ntype, c->d_nchild, c->d_fregoff, c->d_freg,
+                       c->nface);
+  else
+    hipLaunchKernelGGL((fv::fixcons_kernel<3>), dim3(c->nft), dim3(fvc::NTHREADS), 0, c->stream, c->slot[slot],
+                       c->nw, c->bnz, c->nnb, c->d_ftask, c->d_ntype, c->d_nchild, c->d_fregoff, c->d_freg,
+                       c->nface);
+  HIPCHK(hipGetLastError());
+  return AMRVAC_FV_OK;
+}
+
+template <class T>
+static bool upload_vec(amrvac_fv_ctx* c, T** dst, const std::vector<T>& v) {
+  if (v.empty()) return true;
+  if (hipMalloc(dst, sizeof(T) * v.size()) != hipSuccess) return false;
+  return hipMemcpyAsync(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream) == hipSuccess;
+}
+
+extern "C" {
+
 int amrvac_fv_download(amrvac_fv_ctx* c, int slot, double* w) {
   if (!c || !w || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
@@ -450,7 +811,7 @@ int amrvac_fv_getdt(amrvac_fv_ctx* c, int slot, double courantpar, double* dtmin
   if (!c || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
   hipError_t e = with_phys<GetdtOp>(c, (const double*)c->slot[slot], c->nb, c->bnz,
-                                    (const fvc::FVParams*)&c->fp, c->d_red, c->stream);
+                                    (const fvc::FVParams*)&c->fp, (const int*)c->d_level, c->d_red, c->stream);
   if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("getdt launch: ") + hipGetErrorString(e));
   hipLaunchKernelGGL(fv::reduce_max2_kernel, dim3(1), dim3(fvc::NTHREADS), 0, c->stream, c->d_red, c->nb, c->d_red2);
   HIPCHK(hipGetLastError());
@@ -496,6 +857,19 @@ int amrvac_fv_advect1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
   a.ch = cmax_global;
   // add_source_glm psi decay factor, src/mhd/mod_mhd_phys.t:5491-5495
   a.glm_decay = std::exp(-qdt * cmax_global * c->cfg.glm_alpha / c->fp.mindx);
+  if (c->amr) {
+    a.blevel = c->d_level;
+    double mindx = c->fp.mindx;
+    for (int l = 0; l < fvc::MAXLEV; ++l) {
+      a.glm_decay_lev[l] = std::exp(-qdt * cmax_global * c->cfg.glm_alpha / mindx);
+      mindx = mindx * 0.5;
+    }
+    if (c->fix_conserve && c->d_freg && riem != fv::RIEM_HANCOCK) {
+      a.freg = c->d_freg;
+      a.fregoff = c->d_fregoff;
+      a.nface = c->nface;
+    }
+  }
   const fvc::KernelKey k = make_key(c, riem);
   if (crash_out) HIPCHK(hipMemsetAsync(c->d_crash, 0, sizeof(int), c->stream));
   const int grid = 8 * a.per_xcd;
@@ -516,6 +890,15 @@ int amrvac_fv_advect1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
   if (e == hipErrorInvalidValue) return fail(AMRVAC_FV_EINVAL, "no stage kernel for this configuration");
   if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("stage kernel: ") + hipGetErrorString(e));
   if (c->timing) HIPCHK(hipEventRecord(e1, c->stream));
+  if (c->amr) {
+    // refluxing, then the ghost cells of psb (advect1: fix_conserve, getbc)
+    if (riem != fv::RIEM_HANCOCK) {
+      const int rc = fix_conserve_amr(c, psb);
+      if (rc != AMRVAC_FV_OK) return rc;
+    }
+    const int rc = getbc_amr(c, psb);
+    if (rc != AMRVAC_FV_OK) return rc;
+  }
   if (crash_out) {
     int h = 0;
     HIPCHK(hipMemcpyAsync(&h, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -543,14 +926,21 @@ int amrvac_fv_volume_average(amrvac_fv_ctx* c, int slot, double* modes) {
   std::vector<double> h((size_t)c->nb * 2 * c->nw);
   HIPCHK(hipMemcpyAsync(h.data(), c->d_red, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  // src/io/mod_input_output.t:2694-2729: volume-weighted sums, uniform cells
-  double dvol = 1.0;
-  for (int d = 0; d < c->cfg.ndim; ++d) dvol = dvol * c->dx[d];
-  const double vol = dvol * (double)c->cells * c->nb;
+  // src/io/mod_input_output.t:2694-2729: volume-weighted sums; the cells of
+  // a block share one volume (dx of its level)
+  std::vector<double> dvol(c->nb);
+  double vol = 0.0;
+  for (int b = 0; b < c->nb; ++b) {
+    double v = 1.0;
+    const double sc = c->amr ? std::ldexp(1.0, -(c->h_level[b] - 1)) : 1.0;
+    for (int d = 0; d < c->cfg.ndim; ++d) v = v * (c->dx[d] * sc);
+    dvol[b] = v;
+    vol += v * (double)c->cells;
+  }
   for (int p = 0; p < 2; ++p)
     for (int iw = 0; iw < c->nw; ++iw) {
       double s = 0.0;
-      for (int b = 0; b < c->nb; ++b) s += dvol * h[((size_t)b * 2 + p) * c->nw + iw];
+      for (int b = 0; b < c->nb; ++b) s += dvol[b] * h[((size_t)b * 2 + p) * c->nw + iw];
       modes[p * c->nw + iw] = s / vol;
     }
   return AMRVAC_FV_OK;
@@ -561,6 +951,139 @@ int amrvac_fv_slot_pointer(amrvac_fv_ctx* c, int slot, void** dptr, long long* n
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
   *dptr = c->slot[slot];
   if (nbytes) *nbytes = (long long)((size_t)c->nb * c->nw * c->cells * sizeof(double));
+  return AMRVAC_FV_OK;
+}
+
+
+int amrvac_fv_set_mesh_amr(amrvac_fv_ctx* c, int nblocks, const double* dx1, const int* level, const int* ntype,
+                           const int* nbr, const int* nchild, const int* icpos, const double* xlohi, const int* bc) {
+  if (!c || !dx1 || !level || !ntype || !nbr || !nchild || !icpos || !xlohi || !bc)
+    return fail(AMRVAC_FV_EINVAL, "null argument");
+  if (nblocks < 1) return fail(AMRVAC_FV_EINVAL, "nblocks must be positive");
+  const int nd = c->cfg.ndim, nnb = c->nnb, nch = 1 << nd;
+  for (int b = 0; b < nblocks; ++b) {
+    if (level[b] < 1 || level[b] > fvc::MAXLEV) return fail(AMRVAC_FV_EINVAL, "level out of range (1..8)");
+    for (int d = 0; d < nd; ++d)
+      if (icpos[b * nd + d] != 1 && icpos[b * nd + d] != 2) return fail(AMRVAC_FV_EINVAL, "icpos must be 1 or 2");
+    for (int k = 0; k < nnb; ++k) {
+      const int t = ntype[b * nnb + k], m = nbr[b * nnb + k];
+      if (k == (nnb - 1) / 2) {
+        if (m != b) return fail(AMRVAC_FV_EINVAL, "neighbour table centre entry must be the block itself");
+        continue;
+      }
+      if (t < 1 || t > 4) return fail(AMRVAC_FV_EINVAL, "neighbour type must be 1..4");
+      if ((t == 2 || t == 3) && (m < 0 || m >= nblocks)) return fail(AMRVAC_FV_EINVAL, "neighbour index out of range");
+      if (t == 3 && level[m] != level[b]) return fail(AMRVAC_FV_EINVAL, "sibling on another level");
+      if (t == 2 && level[m] != level[b] - 1) return fail(AMRVAC_FV_EINVAL, "coarse neighbour not one level up");
+      if (t == 4) {
+        int i[3] = {0, 0, 0}, kk = k;
+        for (int d = 0; d < nd; ++d) { i[d] = kk % 3 - 1; kk /= 3; }
+        for (int j = 0; j < nch; ++j) {
+          bool adjacent = true;  // children of the neighbour node touching block b
+          for (int d = 0; d < nd; ++d) {
+            const int ic = 1 + ((j >> d) & 1);
+            if ((i[d] == -1 && ic != 2) || (i[d] == 1 && ic != 1)) adjacent = false;
+          }
+          if (!adjacent) continue;
+          const int f = nchild[((size_t)b * nnb + k) * nch + j];
+          if (f < 0 || f >= nblocks || level[f] != level[b] + 1)
+            return fail(AMRVAC_FV_EINVAL, "fine neighbour child index or level out of range");
+        }
+      }
+    }
+  }
+  for (int s = 0; s < 2 * nd; ++s)
+    if (bc[s] != 2 && bc[s] != 3 && bc[s] != 4 && bc[s] != 5)
+      return fail(AMRVAC_FV_EINVAL, "boundary type must be cont, symm, asymm or periodic");
+  for (int d = 0; d < nd; ++d)
+    if (!(dx1[d] > 0.0)) return fail(AMRVAC_FV_EINVAL, "dx must be positive");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  free_level(c);
+  // virtual blocks and the sweep's neighbour table
+  std::vector<int> snbr((size_t)nblocks * nnb);
+  std::vector<int4> rt, pt;
+  std::vector<int2> pt2;
+  int nv = 0;
+  for (int b = 0; b < nblocks; ++b)
+    for (int k = 0; k < nnb; ++k) {
+      const int t = ntype[b * nnb + k];
+      if (k == (nnb - 1) / 2) { snbr[(size_t)b * nnb + k] = b; continue; }
+      if (t == 3) { snbr[(size_t)b * nnb + k] = nbr[b * nnb + k]; continue; }
+      if (t == 1) { snbr[(size_t)b * nnb + k] = -1; continue; }
+      const int v = nblocks + nv++;
+      snbr[(size_t)b * nnb + k] = v;
+      if (t == 4) {
+        rt.push_back(make_int4(b, k, v, 0));
+      } else {
+        int i[3] = {0, 0, 0}, kk = k;
+        for (int d = 0; d < nd; ++d) { i[d] = kk % 3 - 1; kk /= 3; }
+        int pic = 0, pdir = 0;
+        for (int d = 0; d < nd; ++d) {
+          const int ic = icpos[b * nd + d];
+          // direction from the coarse neighbour to the block's parent
+          const int q = ic - 1 + i[d];
+          const int iC = -(q >= 0 ? q / 2 : -((-q + 1) / 2));
+          pic |= ic << (2 * d);
+          pdir |= (iC + 1) << (2 * d);
+        }
+        pt.push_back(make_int4(b, k, v, nbr[b * nnb + k]));
+        pt2.push_back(make_int2(pic, pdir));
+      }
+    }
+  int rc = alloc_level(c, nblocks, nblocks + nv, dx1, snbr.data(), bc);
+  if (rc != AMRVAC_FV_OK) return rc;
+  c->amr = true;
+  c->nv = nv;
+  c->nrt = (int)rt.size();
+  c->npt = (int)pt.size();
+  // refluxing registers on faces with a level jump, coarse blocks to fix
+  c->nface = 16 * (c->bnz > 16 ? c->bnz : 16);
+  std::vector<int> fregoff((size_t)nblocks * 2 * nd, -1), ftask;
+  int nreg = 0;
+  for (int b = 0; b < nblocks; ++b) {
+    bool fixes = false;
+    for (int d = 0; d < nd; ++d)
+      for (int sd = 0; sd < 2; ++sd) {
+        int v[3] = {0, 0, 0};
+        v[d] = sd == 0 ? -1 : 1;
+        int k = 0, m = 1;
+        for (int e = 0; e < nd; ++e) { k += (v[e] + 1) * m; m *= 3; }
+        const int t = ntype[b * nnb + k];
+        if (t == 2 || t == 4) fregoff[(size_t)b * 2 * nd + 2 * d + sd] = (nreg++) * c->nw * c->nface;
+        if (t == 4) fixes = true;
+      }
+    if (fixes) ftask.push_back(b);
+  }
+  c->nft = (int)ftask.size();
+  c->h_level.assign(level, level + nblocks);
+  std::vector<int> lev0(nblocks);
+  for (int b = 0; b < nblocks; ++b) lev0[b] = level[b] - 1;
+  std::vector<int> vtype(ntype, ntype + (size_t)nblocks * nnb), vchild(nchild, nchild + (size_t)nblocks * nnb * nch);
+  std::vector<double> xlo((size_t)nblocks * nd);
+  for (int b = 0; b < nblocks; ++b)
+    for (int d = 0; d < nd; ++d) xlo[(size_t)b * nd + d] = xlohi[((size_t)b * nd + d) * 2];
+  bool ok = upload_vec(c, &c->d_level, lev0) && upload_vec(c, &c->d_ntype, vtype) &&
+            upload_vec(c, &c->d_nchild, vchild) && upload_vec(c, &c->d_rtask, rt) && upload_vec(c, &c->d_ptask, pt) &&
+            upload_vec(c, &c->d_ptask2, pt2) && upload_vec(c, &c->d_ftask, ftask) &&
+            upload_vec(c, &c->d_fregoff, fregoff) && upload_vec(c, &c->d_xlo, xlo);
+  if (ok && nreg > 0) ok = hipMalloc(&c->d_freg, sizeof(double) * (size_t)nreg * c->nw * c->nface) == hipSuccess;
+  if (!ok) {
+    free_level(c);
+    return fail(AMRVAC_FV_ENOMEM, "AMR table allocation failed");
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_getbc(amrvac_fv_ctx* c, int slot) {
+  if (!c || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  return getbc_amr(c, slot);
+}
+
+int amrvac_fv_set_fix_conserve(amrvac_fv_ctx* c, int on) {
+  if (!c) return fail(AMRVAC_FV_EINVAL, "null context");
+  c->fix_conserve = on != 0;
   return AMRVAC_FV_OK;
 }
 

@@ -43,7 +43,15 @@ struct SweepArgs {
   double glm_decay;    // exp(-qdt*ch*alpha/min dx)
   int bc[6];           // physical boundary type per side
   int* crash;
+  // block-structured AMR (all nullptr / unused on a uniform level)
+  const int* blevel;        // level-1 of each block: cell size dx/2^blevel
+  double glm_decay_lev[8];  // psi decay factor per level
+  double* freg;             // scaled block-boundary face fluxes (refluxing)
+  const int* fregoff;       // [nb][2*ndim] register offset of a face, -1: none
+  int nface;                // transverse cells of a face (register stride)
 };
+
+constexpr int MAXLEV = 8;
 
 }  // namespace fvc
 
@@ -186,9 +194,32 @@ __device__ __forceinline__ void strip_cell(int t, int& x, int& y) {
 template <class R>
 constexpr int kMinWaves = std::is_same<R, float>::value ? 3 : 2;
 
+// Scheme constants of a block's level: every length scales by 2^-lev
+// (dx(level) = dx(level-1)/2, src/mod_initialize.t:101), so the products
+// below are exact and match a level-wise evaluation bit for bit.
+__device__ __forceinline__ void level_params(const SweepArgs& a, int b, FVParams& fp, double* dq,
+                                             double& glm_decay) {
+  dq[0] = a.dq[0]; dq[1] = a.dq[1]; dq[2] = a.dq[2];
+  glm_decay = a.glm_decay;
+  if (!a.blevel) return;
+  const int lev = a.blevel[b];
+  if (lev == 0) return;
+  const double sc = double(1 << lev), sc2 = sc * sc;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    fp.dx[d] = fp.dx[d] / sc;
+    fp.dxinv[d] = fp.dxinv[d] * sc;
+    fp.rdelinv[d] = fp.rdelinv[d] * sc2;
+    dq[d] = dq[d] * sc;
+  }
+  fp.invds2 = fp.invds2 * sc2;
+  fp.mindx = fp.mindx / sc;
+  glm_decay = a.glm_decay_lev[lev];
+}
+
 template <class P, class R, int RIEM, int LIM, int NG>
 __global__ void __launch_bounds__(NTHREADS, kMinWaves<R>)
-sweep_kernel(SweepArgs a, FVParams fp) {
+sweep_kernel(SweepArgs a, FVParams fp0) {
   constexpr int NDIM = P::NDIM;
   constexpr int NW = P::NW;
   constexpr int PX = BX + 2 * NG;
@@ -211,6 +242,13 @@ sweep_kernel(SweepArgs a, FVParams fp) {
 
   const int b = wg_block(a.per_xcd);
   if (b >= a.nblocks) return;
+  FVParams fp = fp0;
+  double dq[3], glm_decay;
+  level_params(a, b, fp, dq, glm_decay);
+  // refluxing registers of the block's faces (x-, x+, y-, y+, z-, z+)
+  int freg[2 * NDIM];
+#pragma unroll
+  for (int f = 0; f < 2 * NDIM; ++f) freg[f] = a.fregoff ? a.fregoff[b * 2 * NDIM + f] : -1;
   const int tid = threadIdx.x;
   const int tx = tid & (BX - 1);
   const int ty = tid >> 4;
@@ -393,13 +431,33 @@ sweep_kernel(SweepArgs a, FVParams fp) {
         for (int iw = 0; iw < NW; ++iw) zp[iw] = Z[NG][iw];
       }
       __syncthreads();
+      // ---- store_flux (src/mod_fix_conserve.t:597-648): the scaled fluxes
+      //      of the block-boundary faces, for refluxing --------------------
+      if (a.freg) {
+        const size_t nf = (size_t)a.nface;
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) {
+          if (tx == 0 && freg[0] >= 0)
+            a.freg[freg[0] + iw * nf + ty + BX * k] = dq[0] * double(fxs[iw][ty][0]);
+          if (tx == BX - 1 && freg[1] >= 0)
+            a.freg[freg[1] + iw * nf + ty + BX * k] = dq[0] * double(fxs[iw][ty][BX]);
+          if (ty == 0 && freg[2] >= 0)
+            a.freg[freg[2] + iw * nf + tx + BX * k] = dq[1] * double(fys[iw][0][tx]);
+          if (ty == BX - 1 && freg[3] >= 0)
+            a.freg[freg[3] + iw * nf + tx + BX * k] = dq[1] * double(fys[iw][BX][tx]);
+          if constexpr (NDIM == 3) {
+            if (k == 0 && freg[4] >= 0) a.freg[freg[4] + iw * nf + tx + BX * ty] = dq[2] * double(fz_carry[iw]);
+            if (k == nz - 1 && freg[5] >= 0) a.freg[freg[5] + iw * nf + tx + BX * ty] = dq[2] * double(fz_new[iw]);
+          }
+        }
+      }
       // ---- flux divergence, dimension by dimension (:269-284) ------------
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) {
-        acc[iw] = acc[iw] + a.dq[0] * double(fxs[iw][ty][tx + 1]) - a.dq[0] * double(fxs[iw][ty][tx]);
-        acc[iw] = acc[iw] + a.dq[1] * double(fys[iw][ty + 1][tx]) - a.dq[1] * double(fys[iw][ty][tx]);
+        acc[iw] = acc[iw] + dq[0] * double(fxs[iw][ty][tx + 1]) - dq[0] * double(fxs[iw][ty][tx]);
+        acc[iw] = acc[iw] + dq[1] * double(fys[iw][ty + 1][tx]) - dq[1] * double(fys[iw][ty][tx]);
         if constexpr (NDIM == 3) {
-          acc[iw] = acc[iw] + a.dq[2] * double(fz_new[iw]) - a.dq[2] * double(fz_carry[iw]);
+          acc[iw] = acc[iw] + dq[2] * double(fz_new[iw]) - dq[2] * double(fz_carry[iw]);
           fz_carry[iw] = fz_new[iw];
         }
       }
@@ -426,7 +484,7 @@ sweep_kernel(SweepArgs a, FVParams fp) {
         phys_flux<P, R, ID>(wRC, wR, ch, fR);
         phys_flux<P, R, ID>(wLC, wL, ch, fL);
 #pragma unroll
-        for (int iw = 0; iw < NW; ++iw) acc[iw] = acc[iw] + a.dq[ID] * double(fL[iw] - fR[iw]);
+        for (int iw = 0; iw < NW; ++iw) acc[iw] = acc[iw] + dq[ID] * double(fL[iw] - fR[iw]);
       };
       R wm[NW], w0[NW], wp[NW];
 #pragma unroll
@@ -450,7 +508,7 @@ sweep_kernel(SweepArgs a, FVParams fp) {
     if constexpr (P::GLM) {
       // add_source_glm src/mhd/mod_mhd_phys.t:5472-5531
       constexpr int S = P::PSI;
-      acc[S] = a.glm_decay * acc[S];
+      acc[S] = glm_decay * acc[S];
       if (fp.glm_extended) {
         const int cx = tx + NG, cy = ty + NG;
         double gp[3], bn[3];

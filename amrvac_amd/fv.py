@@ -25,7 +25,8 @@ PREC = {"fp64": 0, "mixed": 1, "fp64_exact": 2}
 EXPORTS = ("amrvac_fv_init", "amrvac_fv_finalize", "amrvac_fv_set_stream", "amrvac_fv_set_mesh",
            "amrvac_fv_upload", "amrvac_fv_download", "amrvac_fv_getdt", "amrvac_fv_advect1",
            "amrvac_fv_copy", "amrvac_fv_volume_average", "amrvac_fv_slot_pointer",
-           "amrvac_fv_sync", "amrvac_fv_sweep_timing", "amrvac_fv_last_error")
+           "amrvac_fv_sync", "amrvac_fv_sweep_timing", "amrvac_fv_last_error",
+           "amrvac_fv_set_mesh_amr", "amrvac_fv_getbc", "amrvac_fv_set_fix_conserve")
 
 
 class FVConfig(ctypes.Structure):
@@ -68,6 +69,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.amrvac_fv_sync.argtypes = [vp]
     L.amrvac_fv_sweep_timing.argtypes = [vp, ctypes.c_int, dp, ip]
     L.amrvac_fv_last_error.restype = ctypes.c_char_p
+    L.amrvac_fv_set_mesh_amr.argtypes = [vp, ctypes.c_int, dp, ip, ip, ip, ip, ip, dp, ip]
+    L.amrvac_fv_getbc.argtypes = [vp, ctypes.c_int]
+    L.amrvac_fv_set_fix_conserve.argtypes = [vp, ctypes.c_int]
     _lib = L
     return L
 
@@ -121,11 +125,22 @@ class DeviceFV:
         h = ctypes.c_void_p()
         self._check(self.L.amrvac_fv_init(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
-        dx = np.ascontiguousarray(mesh.dx, dtype=np.float64)
-        nbr = np.ascontiguousarray(mesh.neighbor, dtype=np.int32)
         xl = np.ascontiguousarray(mesh.xlohi, dtype=np.float64)
         bc = np.ascontiguousarray(mesh.bc_codes, dtype=np.int32)
-        self._check(self.L.amrvac_fv_set_mesh(self.h, mesh.nblocks, _dp(dx), _ip(nbr), _dp(xl), _ip(bc)))
+        self.amr = hasattr(mesh, "ntype")
+        if self.amr:  # an AMR forest (amrvac_amd.amr.AMRForest)
+            dx1 = np.ascontiguousarray(mesh.dx[1], dtype=np.float64)
+            lev = np.ascontiguousarray(mesh.level, dtype=np.int32)
+            nt = np.ascontiguousarray(mesh.ntype, dtype=np.int32)
+            nbr = np.ascontiguousarray(mesh.nbr, dtype=np.int32)
+            nch = np.ascontiguousarray(mesh.nchild, dtype=np.int32)
+            icp = np.ascontiguousarray(mesh.icpos, dtype=np.int32)
+            self._check(self.L.amrvac_fv_set_mesh_amr(self.h, mesh.nblocks, _dp(dx1), _ip(lev), _ip(nt), _ip(nbr),
+                                                      _ip(nch), _ip(icp), _dp(xl), _ip(bc)))
+        else:
+            dx = np.ascontiguousarray(mesh.dx, dtype=np.float64)
+            nbr = np.ascontiguousarray(mesh.neighbor, dtype=np.int32)
+            self._check(self.L.amrvac_fv_set_mesh(self.h, mesh.nblocks, _dp(dx), _ip(nbr), _dp(xl), _ip(bc)))
         self._crash = False
 
     def _check(self, rc: int) -> None:
@@ -154,8 +169,13 @@ class DeviceFV:
         return out
 
     def getbc(self, slot: int) -> None:
-        """Ghost cells of the device state are read from the neighbours by
-        every sweep and materialised by get_state; nothing to do."""
+        """On a uniform level ghost cells are read from the neighbours by
+        every sweep and materialised by get_state (nothing to do); on an AMR
+        mesh the ghost cells across level jumps are filled on the device."""
+        self._check(self.L.amrvac_fv_getbc(self.h, slot))
+
+    def set_fix_conserve(self, on: bool) -> None:
+        self._check(self.L.amrvac_fv_set_fix_conserve(self.h, 1 if on else 0))
 
     def getdt_courant(self, slot: int, courantpar: float):
         dt = ctypes.c_double()

@@ -259,3 +259,28 @@ def orszag_tang_2p5d(mesh: UniformMesh, gamma: float = 5.0 / 3.0, seed: int = 20
     if glm:
         comps.append(0.0 * one)
     return mhd_to_conserved(np.stack(comps, axis=1), gamma, 3)
+
+
+def blast_wave_blocks(forest, blocks, gamma: float = 5.0 / 3.0, rho0: float = 1.0, p_out: float = 1.0,
+                      p_in: float = 100.0, radius: float = 0.2, noise: float = 0.0,
+                      seed: int = 20261202) -> np.ndarray:
+    """Blast wave on the given leaves of an AMR forest (ghosted conserved
+    state [len(blocks), nw, ...]): rho=rho0 (times 1 + seeded relative noise
+    hashed on the level and the level's global cell index), v=0, p=p_in for
+    r < radius around the domain centre, p_out outside.  With the defaults
+    it is the initial condition of tests/hd/blast_wave_Cartesian_2D/3D
+    (mod_usr.t: p=100 inside r<0.2, 1 outside)."""
+    x = forest.cell_centers(blocks)
+    nd = forest.ndim
+    r2 = np.zeros_like(x[:, 0])
+    for d in range(nd):
+        c = 0.5 * (forest.xprobmin[d] + forest.xprobmax[d])
+        r2 = r2 + (x[:, d] - c) ** 2
+    rho = rho0 * np.ones_like(r2)
+    if noise:
+        for k, b in enumerate(blocks):
+            gi = forest.global_index(b)
+            rho[k] = rho0 * (1.0 + noise * hashed_uniform(seed, int(forest.level[b]), gi, 0))
+    p = np.where(r2 < radius ** 2, p_in, p_out)
+    comps = [rho] + [np.zeros_like(rho)] * nd + [p]
+    return hd_to_conserved(np.stack(comps, axis=1), gamma, nd)

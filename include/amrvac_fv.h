@@ -83,6 +83,36 @@ int amrvac_fv_set_stream(amrvac_fv_ctx* ctx, void* stream);
 int amrvac_fv_set_mesh(amrvac_fv_ctx* ctx, int nblocks, const double* dx,
                        const int* neighbor, const double* xlohi, const int* bc);
 
+/* Replaces: the connectivity of a block-structured AMR mesh after
+   build_connectivity (src/mod_functions_connectivity.t:55-300) and the
+   level geometry of alloc_node (src/amr/mod_amr_solution_node.t:150-186).
+   The leaves of all levels form one batched array.  dx1[ndim]: level-1
+   cell size (level l has dx1/2^(l-1)); level[nb] (1..8);
+   ntype[nb*3^ndim]: neighbour type per direction (1 boundary, 2 coarse,
+   3 sibling, 4 fine; the centre entry is ignored); nbr[nb*3^ndim]:
+   sibling or coarse neighbour leaf (the block itself at the centre);
+   nchild[nb*3^ndim*2^ndim]: the children of a fine neighbour (child
+   index sum_d (ic_d-1)*2^(d-1)); icpos[nb*ndim]: position within the
+   parent (1 or 2); xlohi[nb*ndim*2]: block extents.  Ghost cells across
+   level jumps follow getbc (restriction, prolongation with
+   interpolation_linear, src/mod_ghostcells_update.t:350-1525) and are kept
+   in device-side virtual blocks. */
+int amrvac_fv_set_mesh_amr(amrvac_fv_ctx* ctx, int nblocks, const double* dx1, const int* level,
+                           const int* ntype, const int* nbr, const int* nchild, const int* icpos,
+                           const double* xlohi, const int* bc);
+
+/* Replaces: getbc (src/mod_ghostcells_update.t:350-566) of one slot on an
+   AMR mesh (after upload or regrid); no-op on a uniform level, whose ghost
+   cells are read from the neighbours by the sweep.  advect1 calls it for
+   psb itself. */
+int amrvac_fv_getbc(amrvac_fv_ctx* ctx, int slot);
+
+/* Replaces: fix_conserve_at_step (src/mod_advance.t:63-111): when on,
+   advect1 on an AMR mesh stores the block-boundary fluxes (store_flux,
+   src/mod_fix_conserve.t:597-648) and corrects the coarse cells next to
+   finer blocks (fix_conserve, :465-595) before the ghost-cell update. */
+int amrvac_fv_set_fix_conserve(amrvac_fv_ctx* ctx, int on);
+
 /* Host <-> device copies of one slot in the reference layout (ghost cells
    included).  upload reads the interior; download fills the ghost cells as
    getbc would (same-level copies and physical boundaries,

@@ -53,8 +53,21 @@ struct orc_sim {
   int nb;
   double dx[3];
   int nnb;
-  int* neighbor;
+  int* neighbor;   /* [nb][nnb] same-level neighbour, -1 otherwise */
   double* xlohi;
+  double* bdx;     /* [nb][3] cell size of each block */
+  int amr;         /* level jumps present: AMR ghost cells and refluxing */
+  int fix_conserve;
+  int* ntype;      /* [nb][nnb]: 1 boundary, 2 coarse, 3 sibling, 4 fine */
+  int* nbr;        /* [nb][nnb] sibling or coarse leaf */
+  int* nchild;     /* [nb][nnb][nch] children of a fine neighbour */
+  int* icpos;      /* [nb][ndim] position within the parent, 1 or 2 */
+  int nch;
+  int nxc[3];      /* coarse representation (psc) with ghost cells */
+  long ncellc;
+  double* psc;     /* [nb][nw][ncellc] */
+  int nface;       /* transverse cells of one block face */
+  double* bflux;   /* [nb][ndim][2][nw][nface] scaled boundary face fluxes */
   int bc[6];
   double* w[3];
   int crash;
@@ -761,7 +774,8 @@ static void add_sources(const orc_sim* s, double qdt, const double* wCT,
 /* finite_volume src/mod_finite_volume.t:116-339 (slab_uniform, no local
    timestep, no stagger grid, fix_small_values off) */
 static void finite_volume(const orc_sim* s, int method, double qdt,
-                          const double* wCT, double* wnew, double ch, int* crash) {
+                          const double* wCT, double* wnew, double ch, int* crash,
+                          double* bf) {
   const int nw = s->nw;
   const long n = s->ncell;
   double* wprim = (double*)malloc(sizeof(double) * (size_t)(nw * n));
@@ -802,6 +816,11 @@ static void finite_volume(const orc_sim* s, int method, double qdt,
         /* flux-divergence update :269-284 */
         for (int iw = 0; iw < nw; ++iw) {
           for (int i = ilo; i <= ihi; ++i) fC[i][iw] = dxinv * fC[i][iw];
+          if (bf) { /* the block-boundary faces, for store_flux */
+            const long t = (long)(a - alo) + (long)(ahi - alo + 1) * (b - blo);
+            bf[((long)(idim * 2 + 0) * nw + iw) * s->nface + t] = fC[ilo][iw];
+            bf[((long)(idim * 2 + 1) * nw + iw) * s->nface + t] = fC[ihi][iw];
+          }
           for (int i = ilo + 1; i <= ihi; ++i) {
             long c = line_cell(s, idim, a, b, i);
             wnew[iw * n + c] = wnew[iw * n + c] + fC[i][iw] - fC[i - 1][iw];
@@ -861,7 +880,16 @@ static void hancock(const orc_sim* s, double qdt, const double* wCT,
 
 /* bc_fill_srl src/mod_ghostcells_update.t:812-852 (same-level copy) and
    fill_boundary_after_gc :579-615 -> bc_phys src/mod_boundary_conditions.t:12-120 */
+static void getbc_phys(orc_sim* s, int slot, int ib);
+static void getbc_block_srl(orc_sim* s, int slot, int ib);
+
 static void getbc_block(orc_sim* s, int slot, int ib) {
+  getbc_block_srl(s, slot, ib);
+  getbc_phys(s, slot, ib);
+}
+
+/* bc_fill_srl: ghost cells from same-level neighbours */
+static void getbc_block_srl(orc_sim* s, int slot, int ib) {
   const int nd = s->p.ndim, ng = s->ng;
   const long n = s->ncell;
   double* w = BLK(s, slot, ib);
@@ -890,7 +918,14 @@ static void getbc_block(orc_sim* s, int slot, int ib) {
           for (int i = lo[0]; i <= hi[0]; ++i)
             w[iw * n + IX(s, i, j, k)] = wn[iw * n + IX(s, i + sh[0], j + sh[1], k + sh[2])];
   }
-  /* physical boundaries */
+}
+
+/* fill_boundary_after_gc src/mod_ghostcells_update.t:579-615: bc_phys on
+   every physical side, dimension by dimension */
+static void getbc_phys(orc_sim* s, int slot, int ib) {
+  const int nd = s->p.ndim, ng = s->ng;
+  const long n = s->ncell;
+  double* w = BLK(s, slot, ib);
   for (int idim = 0; idim < nd; ++idim) {
     int bmin[3], bmax[3];
     for (int d = 0; d < 3; ++d) {
@@ -905,7 +940,7 @@ static void getbc_block(orc_sim* s, int slot, int ib) {
             nidx += (v + 1) * mul;
             mul *= 3;
           }
-          int isb = s->neighbor[(long)ib * s->nnb + nidx] < 0;
+          int isb = s->ntype[(long)ib * s->nnb + nidx] == 1;
           if (side == 0) face_lo = isb; else face_hi = isb;
         }
         bmin[d] = face_lo ? ng : 0;
@@ -919,7 +954,7 @@ static void getbc_block(orc_sim* s, int slot, int ib) {
         nidx += (v + 1) * mul;
         mul *= 3;
       }
-      if (s->neighbor[(long)ib * s->nnb + nidx] >= 0) continue;
+      if (s->ntype[(long)ib * s->nnb + nidx] != 1) continue;
       int type = s->bc[2 * idim + side];
       int lo[3], hi[3];
       for (int d = 0; d < 3; ++d) { lo[d] = bmin[d]; hi[d] = bmax[d]; }
@@ -944,7 +979,416 @@ static void getbc_block(orc_sim* s, int slot, int ib) {
   }
 }
 
+
+/* ------------------------------------------------------------------ */
+/* AMR: ghost cells across level jumps, refluxing                       */
+/* ------------------------------------------------------------------ */
+
+/* Fortran-style 1-based cell access of a block array (nx cells per dim) */
+static inline long IX1(const int* nx, int i, int j, int k) {
+  return (long)(i - 1) + (long)nx[0] * ((long)(j - 1) + (long)nx[1] * (long)(k - 1));
+}
+
+/* index ranges of init_bc, src/mod_ghostcells_update.t:116-180, per
+   dimension d (1-based; interpolation_order 2) */
+typedef struct { int lo, hi; } orc_rng;
+
+static orc_rng rng_R_srl(const orc_sim* s, int d, int i) {
+  const int ng = s->ng, mlo = ng + 1, mhi = ng + s->bnx[d];
+  orc_rng r;
+  if (i == -1) { r.lo = 1; r.hi = ng; }
+  else if (i == 0) { r.lo = mlo; r.hi = mhi; }
+  else { r.lo = mhi + 1; r.hi = s->nx[d]; }
+  return r;
+}
+static orc_rng rng_S_r(const orc_sim* s, int d, int i) {
+  const int ng = s->ng, clo = ng + 1, chi = ng + s->bnx[d] / 2;
+  orc_rng r;
+  if (i == -1) { r.lo = clo; r.hi = clo - 1 + ng; }
+  else if (i == 0) { r.lo = clo; r.hi = chi; }
+  else { r.lo = chi + 1 - ng; r.hi = chi; }
+  return r;
+}
+static orc_rng rng_R_r(const orc_sim* s, int d, int inc) {
+  const int ng = s->ng, mlo = ng + 1, mhi = ng + s->bnx[d], nxco = s->bnx[d] / 2;
+  orc_rng r;
+  if (inc == 0) { r.lo = 1; r.hi = ng; }
+  else if (inc == 1) { r.lo = mlo; r.hi = mlo - 1 + nxco; }
+  else if (inc == 2) { r.lo = mlo + nxco; r.hi = mhi; }
+  else { r.lo = mhi + 1; r.hi = s->nx[d]; }
+  return r;
+}
+static orc_rng rng_S_p(const orc_sim* s, int d, int inc) {
+  const int ng = s->ng, mlo = ng + 1, mhi = ng + s->bnx[d], nxco = s->bnx[d] / 2;
+  const int ngco = (ng + 1) / 2;
+  orc_rng r;
+  if (inc == 0) { r.lo = mlo - 1; r.hi = mlo - 1 + ngco + 1; }
+  else if (inc == 1) { r.lo = mlo - 1; r.hi = mlo - 1 + nxco + ngco + 1; }
+  else if (inc == 2) { r.lo = mlo + nxco - ngco - 1; r.hi = mhi + 1; }
+  else { r.lo = mhi + 1 - ngco - 1; r.hi = mhi + 1; }
+  return r;
+}
+static orc_rng rng_R_p(const orc_sim* s, int d, int inc) {
+  const int ng = s->ng, clo = ng + 1, chi = ng + s->bnx[d] / 2;
+  const int ngco = (ng + 1) / 2;
+  orc_rng r;
+  if (inc == 0) { r.lo = clo - ngco - 1; r.hi = ng + 1; }
+  else if (inc == 1) { r.lo = clo - 1; r.hi = chi + ngco + 1; }
+  else if (inc == 2) { r.lo = clo - ngco - 1; r.hi = chi + 1; }
+  else { r.lo = chi + 1 - 1; r.hi = chi + ngco + 1; }
+  return r;
+}
+
+static int dir_nidx(const orc_sim* s, const int* i) {
+  int k = 0, m = 1;
+  for (int d = 0; d < s->p.ndim; ++d) { k += (i[d] + 1) * m; m *= 3; }
+  return k;
+}
+
+static double* PSC(const orc_sim* s, int ib) { return s->psc + (long)ib * s->nw * s->ncellc; }
+
+/* coarsen_grid src/amr/mod_coarsen.t:12-79 (slab_uniform): mean of the
+   2^ndim fine cells, summed in array-element order; fine interior ixM of
+   wFi (layout nxf) into coarse cells [clo, chi] of wCo (layout nxc), fine
+   cell of coarse cell c: 2*(c - clo) + flo */
+static void coarsen_cells(const orc_sim* s, const double* wFi, const int* nxf, long ncf, int flo_[3],
+                          double* wCo, const int* nxc, long ncc, const int* clo, const int* chi) {
+  const int nd = s->p.ndim;
+  const double cofi = one / (double)(1 << nd);
+  for (int iw = 0; iw < s->nw; ++iw)
+    for (int kc = clo[2]; kc <= chi[2]; ++kc)
+      for (int jc = clo[1]; jc <= chi[1]; ++jc)
+        for (int ic = clo[0]; ic <= chi[0]; ++ic) {
+          int f[3] = {2 * (ic - clo[0]) + flo_[0], 2 * (jc - clo[1]) + flo_[1], 2 * (kc - clo[2]) + flo_[2]};
+          double sum = 0.0;
+          int first = 1;
+          for (int dk = 0; dk <= (nd > 2); ++dk)
+            for (int dj = 0; dj <= (nd > 1); ++dj)
+              for (int di = 0; di <= 1; ++di) {
+                double v = wFi[iw * ncf + IX1(nxf, f[0] + di, f[1] + dj, f[2] + dk)];
+                if (first) { sum = v; first = 0; } else sum = sum + v;
+              }
+          wCo[iw * ncc + IX1(nxc, ic, jc, kc)] = sum * cofi;
+        }
+}
+
+/* bc_phys src/mod_boundary_conditions.t:12-120 on an arbitrary array
+   (layout nxa), side iside (1 low, 2 high) of dimension idim over the
+   1-based range [blo, bhi] */
+static void bc_phys_range(const orc_sim* s, double* w, const int* nxa, long nca, int iside, int idim,
+                          const int* blo, const int* bhi) {
+  const int ng = s->ng;
+  int olo[3], ohi[3];
+  for (int d = 0; d < 3; ++d) { olo[d] = blo[d]; ohi[d] = bhi[d]; }
+  if (iside == 2) olo[idim] = bhi[idim] + 1 - ng;
+  else ohi[idim] = blo[idim] - 1 + ng;
+  const int type = s->bc[2 * idim + (iside - 1)];
+  if (type == ORC_BC_PERIODIC) return;
+  for (int iw = 0; iw < s->nw; ++iw)
+    for (int k = olo[2]; k <= ohi[2]; ++k)
+      for (int j = olo[1]; j <= ohi[1]; ++j)
+        for (int i = olo[0]; i <= ohi[0]; ++i) {
+          int c[3] = {i, j, k}, src[3] = {i, j, k};
+          if (type == ORC_BC_CONT) {
+            src[idim] = (iside == 2) ? olo[idim] - 1 : ohi[idim] + 1;
+          } else if (iside == 2) {
+            src[idim] = 2 * olo[idim] - 1 - c[idim]; /* olo-1, olo-2, ... */
+          } else {
+            src[idim] = 2 * ohi[idim] + 1 - c[idim];
+          }
+          double v = w[iw * nca + IX1(nxa, src[0], src[1], src[2])];
+          if (type == ORC_BC_ASYMM) v = -v;
+          w[iw * nca + IX1(nxa, c[0], c[1], c[2])] = v;
+        }
+}
+
+/* interpolation_linear src/mod_ghostcells_update.t:1381-1525 (slab_uniform,
+   conserved variables): fine ghost cells [flo, fhi] of block ib from its
+   coarse representation */
+static void interpolation_linear(orc_sim* s, int slot, int ib, const int* flo, const int* fhi) {
+  const int nd = s->p.ndim, ng = s->ng;
+  double* w = BLK(s, slot, ib);
+  const double* pc = PSC(s, ib);
+  double dxFi[3], dxCo[3], invdxCo[3], xFimin[3], xComin[3];
+  for (int d = 0; d < nd; ++d) {
+    dxFi[d] = s->bdx[ib * 3 + d];
+    dxCo[d] = two * dxFi[d];
+    invdxCo[d] = 1.0 / dxCo[d];
+    const double rpxmin = s->xlohi[((long)ib * nd + d) * 2];
+    xFimin[d] = rpxmin - (double)ng * dxFi[d];
+    xComin[d] = rpxmin - (double)ng * dxCo[d];
+  }
+  for (int k = flo[2]; k <= fhi[2]; ++k)
+    for (int j = flo[1]; j <= fhi[1]; ++j)
+      for (int i = flo[0]; i <= fhi[0]; ++i) {
+        int f[3] = {i, j, k}, co[3] = {1, 1, 1};
+        double eta[3] = {0, 0, 0};
+        for (int d = 0; d < nd; ++d) {
+          const double xFi = xFimin[d] + ((double)f[d] - half) * dxFi[d];
+          co[d] = (int)((xFi - xComin[d]) * invdxCo[d]) + 1;
+          const double xCo = xComin[d] + ((double)co[d] - half) * dxCo[d];
+          eta[d] = (xFi - xCo) * invdxCo[d];
+        }
+        for (int iw = 0; iw < s->nw; ++iw) {
+          const double* q = pc + iw * s->ncellc;
+          const double c0 = q[IX1(s->nxc, co[0], co[1], co[2])];
+          double val = c0;
+          for (int d = 0; d < nd; ++d) {
+            int h[3] = {co[0], co[1], co[2]}, p[3] = {co[0], co[1], co[2]};
+            h[d] -= 1; p[d] += 1;
+            const double slopeL = c0 - q[IX1(s->nxc, h[0], h[1], h[2])];
+            const double slopeR = q[IX1(s->nxc, p[0], p[1], p[2])] - c0;
+            const double slopeC = half * (slopeR + slopeL);
+            const double signC = copysign(one, slopeC);
+            const double slope = signC * MAX2(zero, MIN2(MIN2(fabs(slopeC), signC * slopeL), signC * slopeR));
+            val = val + slope * eta[d];
+          }
+          w[iw * s->ncell + IX1(s->nx, f[0], f[1], f[2])] = val;
+        }
+      }
+}
+
+/* bc_prolong src/mod_ghostcells_update.t:1261-1339 (no stretching, no
+   primitive prolongation) */
+static void bc_prolong(orc_sim* s, int slot, int ib, const int* i) {
+  const int nd = s->p.ndim, ng = s->ng;
+  int flo[3] = {1, 1, 1}, fhi[3] = {1, 1, 1};
+  for (int d = 0; d < nd; ++d) {
+    orc_rng r = rng_R_srl(s, d, i[d]);
+    flo[d] = r.lo; fhi[d] = r.hi;
+  }
+  int phybound = 0;
+  for (int k = 0; k < s->nnb; ++k)
+    if (s->ntype[(long)ib * s->nnb + k] == 1) phybound = 1;
+  if (phybound) {
+    double dxFi[3], invdxCo[3], xFimin[3], xComin[3];
+    for (int d = 0; d < nd; ++d) {
+      dxFi[d] = s->bdx[ib * 3 + d];
+      const double dxCo = two * dxFi[d];
+      invdxCo[d] = 1.0 / dxCo;
+      const double rpxmin = s->xlohi[((long)ib * nd + d) * 2];
+      xFimin[d] = rpxmin - (double)ng * dxFi[d];
+      xComin[d] = rpxmin - (double)ng * dxCo;
+    }
+    for (int idims = 0; idims < nd; ++idims) {
+      int clo[3] = {1, 1, 1}, chi[3] = {1, 1, 1};
+      for (int d = 0; d < nd; ++d) {
+        clo[d] = (int)((xFimin[d] + ((double)flo[d] - half) * dxFi[d] - xComin[d]) * invdxCo[d]) + 1 - 1;
+        chi[d] = (int)((xFimin[d] + ((double)fhi[d] - half) * dxFi[d] - xComin[d]) * invdxCo[d]) + 1 + 1;
+      }
+      if (nd == 3) {
+        /* avoid undetermined ghost cells at physical boundary edges */
+        const int cmlo = ng + 1, cmhi = ng + s->bnx[0] / 2;
+        int face[3][2];
+        for (int d = 0; d < 3; ++d)
+          for (int sd = 0; sd < 2; ++sd) {
+            int v[3] = {0, 0, 0};
+            v[d] = sd == 0 ? -1 : 1;
+            face[d][sd] = s->ntype[(long)ib * s->nnb + dir_nidx(s, v)] == 1;
+          }
+        if (idims == 0 && (face[0][0] || face[0][1])) {
+          if (face[1][0]) clo[1] = cmlo;
+          if (face[2][0]) clo[2] = cmlo;
+          if (face[1][1]) chi[1] = ng + s->bnx[1] / 2;
+          if (face[2][1]) chi[2] = ng + s->bnx[2] / 2;
+        } else if (idims == 1 && (face[1][0] || face[1][1])) {
+          if (face[2][0]) clo[2] = cmlo;
+          if (face[2][1]) chi[2] = ng + s->bnx[2] / 2;
+        }
+        (void)cmhi;
+      }
+      for (int iside = 1; iside <= 2; ++iside) {
+        int v[3] = {0, 0, 0};
+        v[idims] = 2 * iside - 3;
+        if (s->ntype[(long)ib * s->nnb + dir_nidx(s, v)] != 1) continue;
+        if ((iside == 1 && clo[idims] < 1 + ng) || (iside == 2 && chi[idims] > s->nxc[idims] - ng)) {
+          int blo[3], bhi[3];
+          for (int d = 0; d < 3; ++d) { blo[d] = clo[d]; bhi[d] = chi[d]; }
+          blo[idims] = 1;
+          bhi[idims] = s->nxc[idims];
+          bc_phys_range(s, PSC(s, ib), s->nxc, s->ncellc, iside, idims, blo, bhi);
+        }
+      }
+    }
+  }
+  interpolation_linear(s, slot, ib, flo, fhi);
+}
+
+/* getbc src/mod_ghostcells_update.t:350-566 with level jumps, one rank:
+   coarse representations, sibling copies and restriction to coarser
+   neighbours, coarse values for finer neighbours, prolongation, then the
+   physical boundaries */
+static void getbc_amr(orc_sim* s, int slot) {
+  const int nd = s->p.ndim, ng = s->ng;
+  const int nb = s->nb;
+  /* coarse values of blocks with a coarser neighbour */
+  for (int ib = 0; ib < nb; ++ib) {
+    int has_coarse = 0;
+    for (int k = 0; k < s->nnb; ++k)
+      if (s->ntype[(long)ib * s->nnb + k] == 2) has_coarse = 1;
+    if (!has_coarse) continue;
+    int flo[3] = {1, 1, 1}, clo[3] = {1, 1, 1}, chi[3] = {1, 1, 1};
+    for (int d = 0; d < nd; ++d) { flo[d] = ng + 1; clo[d] = ng + 1; chi[d] = ng + s->bnx[d] / 2; }
+    coarsen_cells(s, BLK(s, slot, ib), s->nx, s->ncell, flo, PSC(s, ib), s->nxc, s->ncellc, clo, chi);
+  }
+  /* sibling copies (bc_fill_srl) and restriction (bc_fill_restrict) */
+  for (int ib = 0; ib < nb; ++ib) getbc_block_srl(s, slot, ib);
+  for (int ib = 0; ib < nb; ++ib) {
+    for (int k = 0; k < s->nnb; ++k) {
+      if (s->ntype[(long)ib * s->nnb + k] != 2) continue;
+      int i[3] = {0, 0, 0}, t = k;
+      for (int d = 0; d < nd; ++d) { i[d] = t % 3 - 1; t /= 3; }
+      const int* ic = s->icpos + (long)ib * nd;
+      int ok = 1;
+      for (int d = 0; d < nd; ++d)
+        if (!(i[d] == 0 || i[d] == 2 * ic[d] - 3)) ok = 0;
+      if (!ok) continue;
+      const int C = s->nbr[(long)ib * s->nnb + k];
+      orc_rng S[3], R[3];
+      for (int d = 0; d < 3; ++d) {
+        if (d < nd) { S[d] = rng_S_r(s, d, i[d]); R[d] = rng_R_r(s, d, -2 * i[d] + ic[d]); }
+        else { S[d].lo = S[d].hi = 1; R[d].lo = R[d].hi = 1; }
+      }
+      const double* src = PSC(s, ib);
+      double* dst = BLK(s, slot, C);
+      for (int iw = 0; iw < s->nw; ++iw)
+        for (int kk = 0; kk <= S[2].hi - S[2].lo; ++kk)
+          for (int jj = 0; jj <= S[1].hi - S[1].lo; ++jj)
+            for (int ii = 0; ii <= S[0].hi - S[0].lo; ++ii)
+              dst[iw * s->ncell + IX1(s->nx, R[0].lo + ii, R[1].lo + jj, R[2].lo + kk)] =
+                  src[iw * s->ncellc + IX1(s->nxc, S[0].lo + ii, S[1].lo + jj, S[2].lo + kk)];
+    }
+  }
+  /* coarse values for finer neighbours (bc_fill_prolong) */
+  for (int ib = 0; ib < nb; ++ib) {
+    for (int k = 0; k < s->nnb; ++k) {
+      if (s->ntype[(long)ib * s->nnb + k] != 4) continue;
+      int i[3] = {0, 0, 0}, t = k;
+      for (int d = 0; d < nd; ++d) { i[d] = t % 3 - 1; t /= 3; }
+      int iclo[3] = {1, 1, 1}, ichi[3] = {1, 1, 1};
+      for (int d = 0; d < nd; ++d) {
+        iclo[d] = i[d] == -1 ? 2 : 1;
+        ichi[d] = i[d] == 1 ? 1 : 2;
+      }
+      for (int c2 = iclo[2]; c2 <= ichi[2]; ++c2)
+        for (int c1 = iclo[1]; c1 <= ichi[1]; ++c1)
+          for (int c0 = iclo[0]; c0 <= ichi[0]; ++c0) {
+            int ic[3] = {c0, c1, c2};
+            int cidx = 0;
+            for (int d = 0; d < nd; ++d) cidx += (ic[d] - 1) << d;
+            const int F = s->nchild[((long)ib * s->nnb + k) * s->nch + cidx];
+            orc_rng S[3], R[3];
+            for (int d = 0; d < 3; ++d) {
+              if (d < nd) { S[d] = rng_S_p(s, d, 2 * i[d] + ic[d]); R[d] = rng_R_p(s, d, ic[d] - i[d]); }
+              else { S[d].lo = S[d].hi = 1; R[d].lo = R[d].hi = 1; }
+            }
+            const double* src = BLK(s, slot, ib);
+            double* dst = PSC(s, F);
+            for (int iw = 0; iw < s->nw; ++iw)
+              for (int kk = 0; kk <= S[2].hi - S[2].lo; ++kk)
+                for (int jj = 0; jj <= S[1].hi - S[1].lo; ++jj)
+                  for (int ii = 0; ii <= S[0].hi - S[0].lo; ++ii)
+                    dst[iw * s->ncellc + IX1(s->nxc, R[0].lo + ii, R[1].lo + jj, R[2].lo + kk)] =
+                        src[iw * s->ncell + IX1(s->nx, S[0].lo + ii, S[1].lo + jj, S[2].lo + kk)];
+          }
+    }
+  }
+  /* prolongation (gc_prolong: directions with dimension 1 fastest) */
+  for (int ib = 0; ib < nb; ++ib)
+    for (int k = 0; k < s->nnb; ++k) {
+      if (s->ntype[(long)ib * s->nnb + k] != 2) continue;
+      int i[3] = {0, 0, 0}, t = k;
+      for (int d = 0; d < nd; ++d) { i[d] = t % 3 - 1; t /= 3; }
+      bc_prolong(s, slot, ib, i);
+    }
+  /* physical boundaries */
+  for (int ib = 0; ib < nb; ++ib) getbc_phys(s, slot, ib);
+}
+
+/* fix_conserve src/mod_fix_conserve.t:465-595 with store_flux :597-648
+   (slab_uniform, one rank): the coarse cells next to a finer neighbour get
+   the sum of the fine fluxes instead of their own */
+static void fix_conserve(orc_sim* s, int slot) {
+  const int nd = s->p.ndim, ng = s->ng, nw = s->nw;
+  const double cofi = one / (double)(1 << nd);
+  const long bstride = (long)nd * 2 * nw * s->nface;
+  for (int ib = 0; ib < s->nb; ++ib) {
+    double* w = BLK(s, slot, ib);
+    for (int idims = 0; idims < nd; ++idims) {
+      int ta = -1, tb = -1;
+      for (int d = 0; d < nd; ++d) {
+        if (d == idims) continue;
+        if (ta < 0) ta = d; else tb = d;
+      }
+      const int na = ta >= 0 ? s->bnx[ta] : 1, nbt = tb >= 0 ? s->bnx[tb] : 1;
+      for (int iside = 1; iside <= 2; ++iside) {
+        int v[3] = {0, 0, 0};
+        v[idims] = 2 * iside - 3;
+        const int k = dir_nidx(s, v);
+        if (s->ntype[(long)ib * s->nnb + k] != 4) continue;
+        const int ix = iside == 1 ? ng + 1 : ng + s->bnx[idims];
+        /* own flux: pflux(1) = -fC(low face), pflux(2) = fC(high face) */
+        const double* own = s->bflux + ib * bstride + (long)(idims * 2 + (iside - 1)) * nw * s->nface;
+        for (int iw = 0; iw < nw; ++iw)
+          for (int b2 = 0; b2 < nbt; ++b2)
+            for (int a2 = 0; a2 < na; ++a2) {
+              int c[3] = {1, 1, 1};
+              c[idims] = ix;
+              if (ta >= 0) c[ta] = ng + 1 + a2;
+              if (tb >= 0) c[tb] = ng + 1 + b2;
+              const double f = own[(long)iw * s->nface + a2 + (long)na * b2];
+              const double pflux = iside == 1 ? -f : f;
+              long cc = iw * s->ncell + IX1(s->nx, c[0], c[1], c[2]);
+              w[cc] = w[cc] - pflux;
+            }
+        /* children across the face */
+        int iclo[3] = {1, 1, 1}, ichi[3] = {1, 1, 1};
+        for (int d = 0; d < nd; ++d) {
+          iclo[d] = v[d] == -1 ? 2 : 1;
+          ichi[d] = v[d] == 1 ? 1 : 2;
+        }
+        const int iother = 3 - iside;
+        for (int c2 = iclo[2]; c2 <= ichi[2]; ++c2)
+          for (int c1 = iclo[1]; c1 <= ichi[1]; ++c1)
+            for (int c0 = iclo[0]; c0 <= ichi[0]; ++c0) {
+              int ic[3] = {c0, c1, c2};
+              int cidx = 0;
+              for (int d = 0; d < nd; ++d) cidx += (ic[d] - 1) << d;
+              const int F = s->nchild[((long)ib * s->nnb + k) * s->nch + cidx];
+              const double* ff = s->bflux + F * bstride + (long)(idims * 2 + (iother - 1)) * nw * s->nface;
+              const int nca = na / 2, ncb = tb >= 0 ? nbt / 2 : 1;
+              for (int iw = 0; iw < nw; ++iw)
+                for (int b2 = 0; b2 < ncb; ++b2)
+                  for (int a2 = 0; a2 < nca; ++a2) {
+                    /* store_flux of the fine block: sum over the 2 (2x2)
+                       fine faces in array-element order */
+                    const int fa = 2 * a2, fb = tb >= 0 ? 2 * b2 : 0;
+                    const double* q = ff + (long)iw * s->nface;
+                    double sum = q[fa + (long)na * fb];
+                    sum = sum + q[fa + 1 + (long)na * fb];
+                    if (tb >= 0) {
+                      sum = sum + q[fa + (long)na * (fb + 1)];
+                      sum = sum + q[fa + 1 + (long)na * (fb + 1)];
+                    }
+                    const double pf = iother == 1 ? sum : -sum;
+                    int c[3] = {1, 1, 1};
+                    c[idims] = ix;
+                    if (ta >= 0) c[ta] = ng + 1 + (ic[ta] - 1) * nca + a2;
+                    if (tb >= 0) c[tb] = ng + 1 + (ic[tb] - 1) * ncb + b2;
+                    long cc = iw * s->ncell + IX1(s->nx, c[0], c[1], c[2]);
+                    w[cc] = w[cc] + pf * cofi;
+                  }
+            }
+      }
+    }
+  }
+}
+
 int orc_getbc(orc_sim* s, int slot) {
+  if (s->amr) {
+    getbc_amr(s, slot);
+    return 0;
+  }
 #pragma omp parallel for schedule(static) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
   for (int ib = 0; ib < s->nb; ++ib) getbc_block(s, slot, ib);
   return 0;
@@ -1007,6 +1451,20 @@ int orc_create(const orc_params* p, int nblocks, const double* dx,
   memcpy(s->neighbor, neighbor, sizeof(int) * (size_t)nblocks * s->nnb);
   s->xlohi = (double*)malloc(sizeof(double) * (size_t)nblocks * 2 * p->ndim);
   memcpy(s->xlohi, xlohi, sizeof(double) * (size_t)nblocks * 2 * p->ndim);
+  s->bdx = (double*)malloc(sizeof(double) * (size_t)nblocks * 3);
+  s->ntype = (int*)malloc(sizeof(int) * (size_t)nblocks * s->nnb);
+  s->nbr = (int*)malloc(sizeof(int) * (size_t)nblocks * s->nnb);
+  for (int ib = 0; ib < nblocks; ++ib) {
+    for (int d = 0; d < 3; ++d) s->bdx[ib * 3 + d] = s->dx[d];
+    for (int k = 0; k < s->nnb; ++k) {
+      const int nb = neighbor[(long)ib * s->nnb + k];
+      s->nbr[(long)ib * s->nnb + k] = nb;
+      s->ntype[(long)ib * s->nnb + k] = (k == (s->nnb - 1) / 2) ? 0 : (nb >= 0 ? 3 : 1);
+    }
+  }
+  s->nch = 1 << p->ndim;
+  s->nface = 1;
+  for (int d = 1; d < p->ndim; ++d) s->nface *= p->block_nx[d] > p->block_nx[0] ? p->block_nx[d] : p->block_nx[0];
   for (int i = 0; i < 6; ++i) s->bc[i] = (i < 2 * p->ndim) ? bc[i] : ORC_BC_PERIODIC;
   for (int sl = 0; sl < 3; ++sl) {
     s->w[sl] = (double*)calloc((size_t)nblocks * s->nw * s->ncell, sizeof(double));
@@ -1021,6 +1479,13 @@ void orc_destroy(orc_sim* s) {
   for (int sl = 0; sl < 3; ++sl) free(s->w[sl]);
   free(s->neighbor);
   free(s->xlohi);
+  free(s->bdx);
+  free(s->ntype);
+  free(s->nbr);
+  free(s->nchild);
+  free(s->icpos);
+  free(s->psc);
+  free(s->bflux);
   free(s);
 }
 
@@ -1050,10 +1515,11 @@ int orc_get_x(const orc_sim* s, double* x) {
             int c[3] = {i, j, k};
             int ix = c[d] + 1; /* Fortran index */
             double v;
+            const double dxb = s->bdx[ib * 3 + d];
             if (ix <= s->bnx[d])
-              v = xlo + ((double)(ix - s->ng) - half) * s->dx[d];
+              v = xlo + ((double)(ix - s->ng) - half) * dxb;
             else
-              v = xhi + ((double)(ix - (s->bnx[d] + s->ng)) - half) * s->dx[d];
+              v = xhi + ((double)(ix - (s->bnx[d] + s->ng)) - half) * dxb;
             xb[d * n + IX(s, i, j, k)] = v;
           }
     }
@@ -1087,7 +1553,7 @@ int orc_getdt_courant(orc_sim* s, int slot, double courantpar,
           for (int d = 0; d < s->p.ndim; ++d) {
             double cm = get_cmax(s, cell, d);
             cmaxb = MAX2(cmaxb, cm);
-            double dxinv = one / s->dx[d];
+            double dxinv = one / s->bdx[ib * 3 + d];
             if (d == 0) tot = cm * dxinv;
             else tot = tot + cm * dxinv;
           }
@@ -1102,6 +1568,8 @@ int orc_getdt_courant(orc_sim* s, int slot, double courantpar,
   return 0;
 }
 
+static void fix_conserve(orc_sim* s, int slot);
+
 int orc_advect1(orc_sim* s, int method, double dtfactor, double dt,
                 double cmax_global, int psa, int psb) {
   if (psa < 0 || psa > 2 || psb < 0 || psb > 2 || psa == psb) return 1;
@@ -1112,16 +1580,25 @@ int orc_advect1(orc_sim* s, int method, double dtfactor, double dt,
     const double* wCT = BLK(s, psa, ib);
     double* wnew = BLK(s, psb, ib);
     int c = 0;
+    /* the block's own cell size (dxlevel of its level) */
+    orc_sim sb = *s;
+    for (int d = 0; d < 3; ++d) sb.dx[d] = s->bdx[ib * 3 + d];
+    double* bf = (s->amr && s->fix_conserve && method != ORC_FS_HANCOCK)
+                     ? s->bflux + (long)ib * s->p.ndim * 2 * s->nw * s->nface
+                     : NULL;  /* hancock stores no fluxes */
     if (method == ORC_FS_HANCOCK)
-      hancock(s, qdt, wCT, wnew, cmax_global);
+      hancock(&sb, qdt, wCT, wnew, cmax_global);
     else
-      finite_volume(s, method, qdt, wCT, wnew, cmax_global, &c);
+      finite_volume(&sb, method, qdt, wCT, wnew, cmax_global, &c, bf);
     crash |= c;
   }
   if (crash) s->crash = 1;
+  if (s->amr && s->fix_conserve) fix_conserve(s, psb);
   orc_getbc(s, psb);
   return 0;
 }
+
+void orc_set_fix_conserve(orc_sim* s, int on) { s->fix_conserve = on; }
 
 int orc_axpby(orc_sim* s, int dst, double a, int x, double b, int y) {
   const long tot = (long)s->nb * s->nw * s->ncell;
@@ -1142,13 +1619,13 @@ int orc_copy(orc_sim* s, int dst, int src) {
    dvolume = prod(dx)) */
 int orc_volume_average(const orc_sim* s, int slot, double* modes) {
   const long n = s->ncell;
-  double dvol = 1.0;
-  for (int d = 0; d < s->p.ndim; ++d) dvol = dvol * s->dx[d];
   for (int power = 1; power <= 2; ++power) {
     double wsum[MAXW + 1];
     for (int iw = 0; iw <= s->nw; ++iw) wsum[iw] = 0.0;
     for (int ib = 0; ib < s->nb; ++ib) {
       const double* w = BLK(s, slot, ib);
+      double dvol = 1.0;
+      for (int d = 0; d < s->p.ndim; ++d) dvol = dvol * s->bdx[ib * 3 + d];
       int lo[3], hi[3];
       for (int d = 0; d < 3; ++d) {
         lo[d] = (d < s->p.ndim) ? s->ng : 0;
@@ -1177,6 +1654,181 @@ int orc_volume_average(const orc_sim* s, int slot, double* modes) {
 }
 
 int orc_crash_flag(const orc_sim* s) { return s->crash; }
+
+/* a simulation over a block-structured AMR level set: dxb[nb][ndim] cell
+   size per block, ntype/nbr[nb][3^ndim] neighbour type and sibling / coarse
+   leaf, nchild[nb][3^ndim][2^ndim] children of fine neighbours,
+   icpos[nb][ndim] position within the parent */
+int orc_create_amr(const orc_params* p, int nblocks, const double* dxb, const int* ntype,
+                   const int* nbr, const int* nchild, const int* icpos, const double* xlohi,
+                   const int* bc, orc_sim** out) {
+  int nnb = 1, nch = 1;
+  for (int d = 0; d < p->ndim; ++d) { nnb *= 3; nch *= 2; }
+  int* srl = (int*)malloc(sizeof(int) * (size_t)nblocks * nnb);
+  for (long k = 0; k < (long)nblocks * nnb; ++k) srl[k] = (ntype[k] == 3 || ntype[k] == 0) ? nbr[k] : -1;
+  double dx1[3] = {dxb[0], p->ndim > 1 ? dxb[1] : 1.0, p->ndim > 2 ? dxb[2] : 1.0};
+  int rc = orc_create(p, nblocks, dx1, srl, xlohi, bc, out);
+  free(srl);
+  if (rc) return rc;
+  orc_sim* s = *out;
+  for (int ib = 0; ib < nblocks; ++ib)
+    for (int d = 0; d < p->ndim; ++d) s->bdx[ib * 3 + d] = dxb[ib * p->ndim + d];
+  memcpy(s->ntype, ntype, sizeof(int) * (size_t)nblocks * nnb);
+  memcpy(s->nbr, nbr, sizeof(int) * (size_t)nblocks * nnb);
+  s->nchild = (int*)malloc(sizeof(int) * (size_t)nblocks * nnb * nch);
+  memcpy(s->nchild, nchild, sizeof(int) * (size_t)nblocks * nnb * nch);
+  s->icpos = (int*)malloc(sizeof(int) * (size_t)nblocks * p->ndim);
+  memcpy(s->icpos, icpos, sizeof(int) * (size_t)nblocks * p->ndim);
+  s->amr = 1;
+  s->ncellc = 1;
+  for (int d = 0; d < 3; ++d) {
+    s->nxc[d] = d < p->ndim ? p->block_nx[d] / 2 + 2 * p->nghost : 1;
+    s->ncellc *= s->nxc[d];
+  }
+  s->psc = (double*)calloc((size_t)nblocks * s->nw * s->ncellc, sizeof(double));
+  s->bflux = (double*)calloc((size_t)nblocks * p->ndim * 2 * s->nw * s->nface, sizeof(double));
+  if (!s->psc || !s->bflux) { orc_destroy(s); return 5; }
+  return 0;
+}
+
+/* regrid helpers on single ghosted blocks of a simulation's layout */
+
+/* coarsen_grid_siblings src/amr/mod_coarsen_refine.t:322-400: child ic
+   (1 or 2 per dimension) into its quadrant of the parent */
+int orc_coarsen_child(const orc_sim* s, const double* wFi, const int* ic, double* wCo) {
+  const int nd = s->p.ndim, ng = s->ng;
+  int flo[3] = {1, 1, 1}, clo[3] = {1, 1, 1}, chi[3] = {1, 1, 1};
+  for (int d = 0; d < nd; ++d) {
+    flo[d] = ng + 1;
+    clo[d] = ng + 1 + (ic[d] - 1) * s->bnx[d] / 2;
+    chi[d] = ng + s->bnx[d] + (ic[d] - 2) * s->bnx[d] / 2;
+  }
+  coarsen_cells(s, wFi, s->nx, s->ncell, flo, wCo, s->nx, s->ncell, clo, chi);
+  return 0;
+}
+
+/* prolong_grid / prolong_2nd src/amr/mod_refine.t:43-167 (slab_uniform,
+   conserved variables): child ic of the parent wCo (ghost cells filled) */
+int orc_prolong_2nd(const orc_sim* s, const double* wCo, const int* ic, double* wFi) {
+  const int nd = s->p.ndim, ng = s->ng;
+  int clo[3] = {1, 1, 1}, chi[3] = {1, 1, 1};
+  for (int d = 0; d < nd; ++d) {
+    clo[d] = ng + 1 + (ic[d] - 1) * s->bnx[d] / 2;
+    chi[d] = ng + s->bnx[d] + (ic[d] - 2) * s->bnx[d] / 2;
+  }
+  for (int kc = clo[2]; kc <= chi[2]; ++kc)
+    for (int jc = clo[1]; jc <= chi[1]; ++jc)
+      for (int icc = clo[0]; icc <= chi[0]; ++icc) {
+        int co[3] = {icc, jc, kc}, fi[3] = {1, 1, 1};
+        for (int d = 0; d < nd; ++d) fi[d] = 2 * (co[d] - clo[d]) + ng + 1;
+        double slope[MAXW][3];
+        for (int d = 0; d < nd; ++d) {
+          int h[3] = {co[0], co[1], co[2]}, p[3] = {co[0], co[1], co[2]};
+          h[d] -= 1; p[d] += 1;
+          for (int iw = 0; iw < s->nw; ++iw) {
+            const double* q = wCo + iw * s->ncell;
+            const double c0 = q[IX1(s->nx, co[0], co[1], co[2])];
+            const double slopeL = c0 - q[IX1(s->nx, h[0], h[1], h[2])];
+            const double slopeR = q[IX1(s->nx, p[0], p[1], p[2])] - c0;
+            const double slopeC = half * (slopeR + slopeL);
+            const double signC = copysign(one, slopeC);
+            slope[iw][d] = signC * MAX2(zero, MIN2(MIN2(fabs(slopeC), signC * slopeL), signC * slopeR));
+          }
+        }
+        for (int dk = 0; dk <= (nd > 2); ++dk)
+          for (int dj = 0; dj <= (nd > 1); ++dj)
+            for (int di = 0; di <= 1; ++di) {
+              int f[3] = {fi[0] + di, nd > 1 ? fi[1] + dj : 1, nd > 2 ? fi[2] + dk : 1};
+              int off[3] = {di, dj, dk};
+              double eta[3];
+              for (int d = 0; d < nd; ++d) eta[d] = 0.5 * ((double)off[d] - 0.5);
+              for (int iw = 0; iw < s->nw; ++iw) {
+                double v = wCo[iw * s->ncell + IX1(s->nx, co[0], co[1], co[2])];
+                for (int d = 0; d < nd; ++d) v = v + slope[iw][d] * eta[d];
+                wFi[iw * s->ncell + IX1(s->nx, f[0], f[1], f[2])] = v;
+              }
+            }
+      }
+  return 0;
+}
+
+/* lohner_grid src/amr/mod_errest.t:63-199 (Cartesian, no log flags, no B0):
+   error estimate of every interior cell of one ghosted block; weight[nw] */
+int orc_lohner_error(const orc_sim* s, const double* w, const double* weight, double wavefilter,
+                     double* err) {
+  const int nd = s->p.ndim, ng = s->ng;
+  const double epsilon = 1.0e-6;
+  int mlo[3] = {1, 1, 1}, mhi[3] = {1, 1, 1};
+  for (int d = 0; d < nd; ++d) { mlo[d] = ng + 1; mhi[d] = ng + s->bnx[d]; }
+  int nb0 = mhi[0] - mlo[0] + 1, nb1 = mhi[1] - mlo[1] + 1;
+  long nint = (long)nb0 * nb1 * (mhi[2] - mlo[2] + 1);
+  for (long c = 0; c < nint; ++c) err[c] = 0.0;
+  double* tmp = (double*)malloc(sizeof(double) * s->ncell);
+  double* tmp2 = (double*)malloc(sizeof(double) * s->ncell);
+  double* num = (double*)malloc(sizeof(double) * nint);
+  double* den = (double*)malloc(sizeof(double) * nint);
+  int xlo[3], xhi[3]; /* ix = ixM extended by one */
+  for (int d = 0; d < 3; ++d) { xlo[d] = d < nd ? mlo[d] - 1 : 1; xhi[d] = d < nd ? mhi[d] + 1 : 1; }
+#define LIDX(i, j, k) ((long)((i) - mlo[0]) + (long)nb0 * ((long)((j) - mlo[1]) + (long)nb1 * ((k) - mlo[2])))
+  for (int iflag = 0; iflag < s->nw; ++iflag) {
+    if (weight[iflag] == 0.0) continue;
+    const double* q = w + iflag * s->ncell;
+    for (long c = 0; c < nint; ++c) num[c] = 0.0;
+    for (int idims = 0; idims < nd; ++idims) {
+      for (int k = xlo[2]; k <= xhi[2]; ++k)
+        for (int j = xlo[1]; j <= xhi[1]; ++j)
+          for (int i = xlo[0]; i <= xhi[0]; ++i) {
+            int p[3] = {i, j, k}, h[3] = {i, j, k};
+            p[idims] += 1; h[idims] -= 1;
+            tmp[IX1(s->nx, i, j, k)] = q[IX1(s->nx, p[0], p[1], p[2])] - q[IX1(s->nx, h[0], h[1], h[2])];
+          }
+      for (int idims2 = 0; idims2 < nd; ++idims2)
+        for (int k = mlo[2]; k <= mhi[2]; ++k)
+          for (int j = mlo[1]; j <= mhi[1]; ++j)
+            for (int i = mlo[0]; i <= mhi[0]; ++i) {
+              int p[3] = {i, j, k}, h[3] = {i, j, k};
+              p[idims2] += 1; h[idims2] -= 1;
+              const double dd = tmp[IX1(s->nx, p[0], p[1], p[2])] - tmp[IX1(s->nx, h[0], h[1], h[2])];
+              num[LIDX(i, j, k)] = num[LIDX(i, j, k)] + dd * dd;
+            }
+    }
+    for (long c = 0; c < nint; ++c) den[c] = 0.0;
+    for (int idims = 0; idims < nd; ++idims) {
+      for (long c = 0; c < s->ncell; ++c) tmp[c] = fabs(q[c]);
+      for (int k = xlo[2]; k <= xhi[2]; ++k)
+        for (int j = xlo[1]; j <= xhi[1]; ++j)
+          for (int i = xlo[0]; i <= xhi[0]; ++i) {
+            int p[3] = {i, j, k}, h[3] = {i, j, k};
+            p[idims] += 1; h[idims] -= 1;
+            tmp2[IX1(s->nx, i, j, k)] = tmp[IX1(s->nx, p[0], p[1], p[2])] + tmp[IX1(s->nx, h[0], h[1], h[2])];
+          }
+      for (int k = mlo[2]; k <= mhi[2]; ++k)
+        for (int j = mlo[1]; j <= mhi[1]; ++j)
+          for (int i = mlo[0]; i <= mhi[0]; ++i) {
+            int p[3] = {i, j, k}, h[3] = {i, j, k};
+            p[idims] += 2; h[idims] -= 2;
+            const double c0 = q[IX1(s->nx, i, j, k)];
+            tmp[IX1(s->nx, i, j, k)] = fabs(q[IX1(s->nx, p[0], p[1], p[2])] - c0) +
+                                       fabs(c0 - q[IX1(s->nx, h[0], h[1], h[2])]);
+          }
+      for (int idims2 = 0; idims2 < nd; ++idims2)
+        for (int k = mlo[2]; k <= mhi[2]; ++k)
+          for (int j = mlo[1]; j <= mhi[1]; ++j)
+            for (int i = mlo[0]; i <= mhi[0]; ++i) {
+              int p[3] = {i, j, k}, h[3] = {i, j, k};
+              p[idims2] += 1; h[idims2] -= 1;
+              const double v = tmp[IX1(s->nx, i, j, k)] +
+                               wavefilter * (tmp2[IX1(s->nx, p[0], p[1], p[2])] + tmp2[IX1(s->nx, h[0], h[1], h[2])]);
+              den[LIDX(i, j, k)] = den[LIDX(i, j, k)] + v * v;
+            }
+    }
+    for (long c = 0; c < nint; ++c) err[c] = err[c] + weight[iflag] * sqrt(num[c] / MAX2(den[c], epsilon));
+  }
+#undef LIDX
+  free(tmp); free(tmp2); free(num); free(den);
+  return 0;
+}
+
 
 /* ------------------------------------------------------------------ */
 /* unit-test hooks (single face / single cell)                         */

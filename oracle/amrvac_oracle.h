@@ -111,7 +111,7 @@ int orc_advect1(orc_sim* s, int method, double dtfactor, double dt,
 int orc_axpby(orc_sim* s, int dst, double a, int x, double b, int y);
 int orc_copy(orc_sim* s, int dst, int src);
 
-/* getbc (src/mod_ghostcells_update.t:350-566), single level */
+/* getbc (src/mod_ghostcells_update.t:350-566) */
 int orc_getbc(orc_sim* s, int slot);
 
 /* volume-weighted mean(w**power) over interiors
@@ -130,6 +130,27 @@ int orc_weno5_face(const double* w6, double* wl, double* wr);
 int orc_face_flux(orc_sim* s, int method, int idim, int nl, const double* lw, double ch,
                   double* fC, double* cmax, double* cmin);
 double orc_cell_cmax(const orc_sim* s, const double* wcons, int idim);
+
+/* block-structured AMR: dxb[nb][ndim] cell size per block; ntype/nbr
+   [nb][3^ndim] neighbour type (1 boundary, 2 coarse, 3 sibling, 4 fine; 0
+   for the block itself) and sibling / coarse leaf; nchild[nb][3^ndim]
+   [2^ndim] children of a fine neighbour (child index sum((ic_d-1) << d));
+   icpos[nb][ndim] position within the parent (1 or 2).  getbc then follows
+   the level-jump path of src/mod_ghostcells_update.t and, when enabled,
+   advect1 stores block-boundary fluxes and applies fix_conserve
+   (src/mod_fix_conserve.t:465-648). */
+int orc_create_amr(const orc_params* p, int nblocks, const double* dxb, const int* ntype,
+                   const int* nbr, const int* nchild, const int* icpos, const double* xlohi,
+                   const int* bc, orc_sim** out);
+void orc_set_fix_conserve(orc_sim* s, int on);
+
+/* regrid on single ghosted blocks: coarsen child ic into the parent's
+   quadrant (coarsen_grid_siblings), prolong_2nd child ic from the parent,
+   Löhner error estimate of the interior cells (lohner_grid) */
+int orc_coarsen_child(const orc_sim* s, const double* wFi, const int* ic, double* wCo);
+int orc_prolong_2nd(const orc_sim* s, const double* wCo, const int* ic, double* wFi);
+int orc_lohner_error(const orc_sim* s, const double* w, const double* weight, double wavefilter,
+                     double* err);
 
 /* number of OpenMP threads the oracle uses (0 = library default) */
 void orc_set_threads(int n);

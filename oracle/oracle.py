@@ -76,6 +76,12 @@ def lib(path: str | None = None):
         L.orc_face_flux.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, dp, dp, dp]
         L.orc_cell_cmax.argtypes = [vp, dp, ctypes.c_int]
         L.orc_cell_cmax.restype = ctypes.c_double
+        L.orc_create_amr.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_int, dp, ip, ip, ip, ip, dp, ip,
+                                     ctypes.POINTER(vp)]
+        L.orc_set_fix_conserve.argtypes = [vp, ctypes.c_int]
+        L.orc_coarsen_child.argtypes = [vp, dp, ip, dp]
+        L.orc_prolong_2nd.argtypes = [vp, dp, ip, dp]
+        L.orc_lohner_error.argtypes = [vp, dp, dp, ctypes.c_double, dp]
         _libs[path] = L
     return _libs[path]
 
@@ -122,12 +128,21 @@ class OracleFV:
         self.L.orc_set_threads(threads)
         p = make_params(par, mesh.block_nx)
         h = ctypes.c_void_p()
-        dx = np.ascontiguousarray(mesh.dx, dtype=np.float64)
-        nbr = np.ascontiguousarray(mesh.neighbor, dtype=np.int32)
         xl = np.ascontiguousarray(mesh.xlohi, dtype=np.float64)
         bc = np.ascontiguousarray(mesh.bc_codes, dtype=np.int32)
-        rc = self.L.orc_create(ctypes.byref(p), mesh.nblocks, _dptr(dx), _iptr(nbr), _dptr(xl),
-                               _iptr(bc), ctypes.byref(h))
+        if hasattr(mesh, "ntype"):  # AMR forest
+            dxb = np.ascontiguousarray(mesh.dxb, dtype=np.float64)
+            nt = np.ascontiguousarray(mesh.ntype, dtype=np.int32)
+            nbr = np.ascontiguousarray(mesh.nbr, dtype=np.int32)
+            nch = np.ascontiguousarray(mesh.nchild, dtype=np.int32)
+            icp = np.ascontiguousarray(mesh.icpos, dtype=np.int32)
+            rc = self.L.orc_create_amr(ctypes.byref(p), mesh.nblocks, _dptr(dxb), _iptr(nt), _iptr(nbr),
+                                       _iptr(nch), _iptr(icp), _dptr(xl), _iptr(bc), ctypes.byref(h))
+        else:
+            dx = np.ascontiguousarray(mesh.dx, dtype=np.float64)
+            nbr = np.ascontiguousarray(mesh.neighbor, dtype=np.int32)
+            rc = self.L.orc_create(ctypes.byref(p), mesh.nblocks, _dptr(dx), _iptr(nbr), _dptr(xl),
+                                   _iptr(bc), ctypes.byref(h))
         if rc != 0:
             raise RuntimeError(f"orc_create failed: {rc}")
         self.h = h
@@ -179,6 +194,32 @@ class OracleFV:
 
     def crash(self) -> bool:
         return bool(self.L.orc_crash_flag(self.h))
+
+    # -- AMR -------------------------------------------------------------------
+    def set_fix_conserve(self, on: bool) -> None:
+        self.L.orc_set_fix_conserve(self.h, 1 if on else 0)
+
+    def coarsen_child(self, wfi: np.ndarray, ic, wco: np.ndarray) -> None:
+        a = np.ascontiguousarray(wfi, dtype=np.float64)
+        icv = np.ascontiguousarray(ic, dtype=np.int32)
+        assert wco.flags.c_contiguous and wco.dtype == np.float64
+        self.L.orc_coarsen_child(self.h, _dptr(a), _iptr(icv), _dptr(wco))
+
+    def prolong_2nd(self, wco: np.ndarray, ic, wfi: np.ndarray) -> None:
+        a = np.ascontiguousarray(wco, dtype=np.float64)
+        icv = np.ascontiguousarray(ic, dtype=np.int32)
+        assert wfi.flags.c_contiguous and wfi.dtype == np.float64
+        self.L.orc_prolong_2nd(self.h, _dptr(a), _iptr(icv), _dptr(wfi))
+
+    def lohner_error(self, w: np.ndarray, weights: dict, wavefilter: float) -> np.ndarray:
+        a = np.ascontiguousarray(w, dtype=np.float64)
+        wt = np.zeros(self.nw)
+        for k, v in weights.items():
+            wt[k] = v
+        nint = tuple(n - 2 * self.par.nghost for n in w.shape[1:])
+        err = np.zeros(nint)
+        self.L.orc_lohner_error(self.h, _dptr(a), _dptr(wt), wavefilter, _dptr(err))
+        return err
 
     # -- single face / cell hooks for closed-form checks ----------------------
     def limiter(self, a: float, b: float, idim: int = 0):

@@ -1,0 +1,201 @@
+"""AMR on the CPU oracle: the reference's own AMR regression logs, the regrid
+operators of the host against their C restatement, and the invariants a
+block-structured AMR update must keep (2:1 nesting, conservation with
+refluxing, exactness of prolongation on linear data)."""
+from __future__ import annotations
+
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from amrvac_amd import amr, problems
+from amrvac_amd.amr import AMRForest, AMRIntegrator, AMRPar
+from amrvac_amd.host import Par, logs_match, read_log
+from oracle.oracle import OracleFV
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+SCHEMES = {
+    "2step_tvdlf_mm": dict(time_stepper="twostep", flux_scheme="tvdlf", limiter="minmod"),
+    "3step_hll_cada": dict(time_stepper="threestep", flux_scheme="hll", limiter="cada3"),
+}
+
+
+def blast_setup(ndim: int, scheme: str, time_max: float):
+    """tests/hd/blast_wave_Cartesian_{2,3}D: bw_*d.par with the scheme file
+    (tests/schemes/*.par: cmaxmean bounds, CFL 0.9)."""
+    par = Par(physics="hd", ndim=ndim, ndir=ndim, gamma=5.0 / 3.0, typeboundspeed="cmaxmean",
+              courantpar=0.9, dtsave_log=1.0e-3, time_max=time_max, **SCHEMES[scheme])
+    if ndim == 2:
+        forest = AMRForest(2, (32, 32), (16, 16), (0.0, 0.0), (2.0, 2.0), ("cont",) * 4, max_level=3)
+        ap = AMRPar(refine_max_level=3, w_refine_weight={0: 0.5, 3: 0.5})
+    else:
+        forest = AMRForest(3, (16, 16, 16), (8, 8, 8), (0.0,) * 3, (2.0,) * 3, ("cont",) * 6, max_level=3)
+        ap = AMRPar(refine_max_level=3, w_refine_weight={0: 0.5, 4: 0.5})
+    return par, forest, ap
+
+
+def blast_init(forest, blocks):
+    return problems.blast_wave_blocks(forest, blocks)
+
+
+@pytest.mark.parametrize("ndim,scheme", [(2, "2step_tvdlf_mm"), (2, "3step_hll_cada"), (3, "2step_tvdlf_mm")])
+def test_blast_wave_amr_golden_log(ndim, scheme):
+    """The reference's AMR regression logs (restriction, prolongation,
+    refluxing, Löhner regrid every step) at the compare_logs tolerance."""
+    par, forest, ap = blast_setup(ndim, scheme, 0.05 if ndim == 2 else 0.02)
+    ti = AMRIntegrator(par, forest, ap, lambda p, f: OracleFV(p, f, threads=4), blast_init)
+    st = ti.run()
+    ref = read_log(os.path.join(GOLDEN, f"bw_{ndim}d_{scheme}.log"))
+    got = np.array(st.log)
+    assert got.shape == ref.shape
+    assert ti.regrids > 0 and forest.levmax == 3
+    assert logs_match(got, ref, rtol=1e-7, atol=1e-8)
+
+
+def test_blast_wave_3d_threestep_log_not_reproduced():
+    """bw_3d_3step_hll_cada.log is NOT reproduced: mean(m^2) is 0.4% off
+    from the first logged step while the 2-D log of the same scheme and the
+    3-D two-step log match to 1e-8.  Recorded as an open discrepancy."""
+    par, forest, ap = blast_setup(3, "3step_hll_cada", 0.003)
+    ti = AMRIntegrator(par, forest, ap, lambda p, f: OracleFV(p, f, threads=4), blast_init)
+    st = ti.run()
+    ref = read_log(os.path.join(GOLDEN, "bw_3d_3step_hll_cada.log"))[:3]
+    got = np.array(st.log)
+    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-8)
+    assert rel[:, [1, 5]].max() < 1e-7  # mean rho and mean e agree
+    assert rel[:, 7].max() > 1e-3       # mean m^2 does not
+
+
+# ---------------------------------------------------------------------------
+# regrid operators: host numpy vs the C restatement
+# ---------------------------------------------------------------------------
+def _random_block(rng, ndim, nw=4, bnx=16, ng=2):
+    return rng.normal(size=(nw,) + (bnx + 2 * ng,) * ndim)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_regrid_operators_match_restatement(ndim):
+    rng = np.random.default_rng(ndim)
+    bnx = 16 if ndim == 2 else 8
+    par = Par(physics="hd", ndim=ndim, ndir=ndim, gamma=1.4)
+    forest = AMRForest(ndim, (bnx,) * ndim, (bnx,) * ndim, (0.0,) * ndim, (1.0,) * ndim, ("cont",) * (2 * ndim))
+    orc = OracleFV(par, forest)
+    nw = par.nw
+    for ic in itertools.product(*[(1, 2)] * ndim):
+        wco = _random_block(rng, ndim, nw, bnx)
+        a = np.zeros_like(wco)
+        b = np.zeros_like(wco)
+        amr.prolong_2nd(wco, ic, a, ndim, 2)
+        orc.prolong_2nd(wco, ic, b)
+        assert np.array_equal(a, b)
+        wfi = _random_block(rng, ndim, nw, bnx)
+        a = np.zeros_like(wfi)
+        b = np.zeros_like(wfi)
+        amr.coarsen_child(wfi, ic, a, ndim, 2)
+        orc.coarsen_child(wfi, ic, b)
+        assert np.array_equal(a, b)
+    w = np.abs(_random_block(rng, ndim, nw, bnx)) + 0.5
+    weights = {0: 0.5, nw - 1: 0.5}
+    e1 = amr.lohner_error(w, ndim, 2, weights, 0.01)
+    e2 = orc.lohner_error(w, weights, 0.01)
+    assert np.allclose(e1, e2, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_prolong_2nd_closed_forms(ndim):
+    """Children of a coarse cell average back to it (conservation) and
+    linear data is reproduced exactly (the limited slope of a linear profile
+    is its slope)."""
+    bnx, ng = 8, 2
+    n = bnx + 2 * ng
+    ax = np.arange(n, dtype=np.float64)
+    grids = np.meshgrid(*([ax] * ndim), indexing="ij")  # numpy axis order (z, y, x)
+    lin = 1.0 + 0.25 * grids[-1] - 0.5 * grids[0] + (0.125 * grids[1] if ndim == 3 else 0.0)
+    wco = np.stack([lin, 2.0 * lin])
+    for ic in itertools.product(*[(1, 2)] * ndim):
+        wfi = np.zeros_like(wco)
+        amr.prolong_2nd(wco, ic, wfi, ndim, ng)
+        back = np.zeros_like(wco)
+        amr.coarsen_child(wfi, ic, back, ndim, ng)
+        sl = tuple(slice(ng + (ic[ndim - 1 - a] - 1) * bnx // 2, ng + ic[ndim - 1 - a] * bnx // 2)
+                   for a in range(ndim))
+        assert np.allclose(back[(slice(None),) + sl], wco[(slice(None),) + sl], rtol=0, atol=1e-14)
+        # fine cell centre of fine index f (interior) sits at coarse
+        # coordinate (ic-1)*bnx/2 + ng + (f - ng + 0.5)/2 - 0.5
+        fidx = [np.arange(ng, ng + bnx) for _ in range(ndim)]
+        coords = []
+        for a in range(ndim):
+            d = ndim - 1 - a
+            coords.append((ic[d] - 1) * bnx / 2 + ng + (fidx[a] - ng + 0.5) / 2 - 0.5)
+        cg = np.meshgrid(*coords, indexing="ij")
+        exact = 1.0 + 0.25 * cg[-1] - 0.5 * cg[0] + (0.125 * cg[1] if ndim == 3 else 0.0)
+        inner = tuple(slice(ng, ng + bnx) for _ in range(ndim))
+        assert np.allclose(wfi[(0,) + inner], exact, rtol=0, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# forest invariants and conservation
+# ---------------------------------------------------------------------------
+def _check_nesting(forest: AMRForest) -> None:
+    for n in forest.leaves:
+        for i in itertools.product(*[(-1, 0, 1)] * forest.ndim):
+            if all(v == 0 for v in i):
+                continue
+            k = amr.dir_index(i[::-1])
+            t = forest.ntype[n.block, k]
+            if t == amr.NB_SIBLING:
+                m = forest.nbr[n.block, k]
+                back = amr.dir_index(tuple(-v for v in i[::-1]))
+                assert forest.ntype[m, back] == amr.NB_SIBLING and forest.nbr[m, back] == n.block
+            elif t == amr.NB_COARSE:
+                assert forest.level[forest.nbr[n.block, k]] == n.level - 1
+            elif t == amr.NB_FINE:
+                kids = [c for c in forest.nchild[n.block, k] if c >= 0]
+                assert kids and all(forest.level[c] == n.level + 1 for c in kids)
+
+
+def test_periodic_amr_conserves_mass_and_energy():
+    """With refluxing, and with conservative restriction / prolongation at
+    regrid, the volume integrals of the conserved variables change only by
+    round-off on a periodic domain."""
+    par = Par(physics="hd", ndim=2, ndir=2, gamma=1.4, time_stepper="threestep", flux_scheme="hllc",
+              limiter="vanleer", courantpar=0.8)
+    forest = AMRForest(2, (64, 64), (16, 16), (0.0, 0.0), (1.0, 1.0), ("periodic",) * 4, max_level=3)
+    ap = AMRPar(refine_max_level=3, refine_threshold=0.1, w_refine_weight={0: 1.0})
+
+    def init(f, blocks):
+        # a density bump carried diagonally across coarse/fine faces
+        x = f.cell_centers(blocks)
+        r2 = (x[:, 0] - 0.5) ** 2 + (x[:, 1] - 0.5) ** 2
+        rho = 1.0 + 0.5 * np.exp(-r2 / 0.01)
+        one = np.ones_like(rho)
+        return problems.hd_to_conserved(np.stack([rho, 1.0 * one, 0.5 * one, one], axis=1), 1.4, 2)
+
+    ti = AMRIntegrator(par, forest, ap, lambda p, f: OracleFV(p, f, threads=4), init)
+    assert forest.levmin == 1 and forest.levmax == 3  # coarse/fine faces to reflux
+
+    def totals():
+        w = ti.be.get_state(0)
+        vol = forest.cell_volume()
+        inner = forest.interior(w).sum(axis=(2, 3))
+        return (inner * vol[:, None]).sum(axis=0)
+
+    t0 = totals()
+    ti.run(nsteps=12)
+    t1 = totals()
+    _check_nesting(forest)
+    assert ti.regrids > 1
+    assert abs(t1[0] - t0[0]) < 1e-13 * abs(t0[0])
+    assert abs(t1[3] - t0[3]) < 1e-13 * abs(t0[3])
+    assert np.abs(t1[1:3] - t0[1:3]).max() < 1e-13 * abs(t0[3])
+
+
+def test_nesting_after_blast_regrids():
+    par, forest, ap = blast_setup(2, "2step_tvdlf_mm", 0.01)
+    ti = AMRIntegrator(par, forest, ap, lambda p, f: OracleFV(p, f, threads=4), blast_init)
+    ti.run()
+    _check_nesting(forest)
+    assert forest.levmin >= 1 and forest.levmax == 3
