@@ -1,0 +1,212 @@
+"""Differentiable runs of the device scheme: a torch.autograd.Function whose
+forward is N ssprk3 steps of the HIP stage kernel and whose backward runs the
+hand-written adjoint stage kernels (amrvac_fv_adjoint1) with
+checkpoint-and-recompute in time.
+
+The time-step sequence is taken from the forward pass (CFL condition of
+src/mod_dt.t:160-287) and held fixed in the backward pass, like the limiter
+and Riemann branches: the gradient is that of the scheme with the forward
+run's dt sequence.  Checkpoints of the state every ``every`` steps are kept
+in device memory; in the backward pass each segment is recomputed once,
+keeping its stage states on a tape, and then reversed stage by stage (three
+adjoint launches per step, the RK combination fused into the last).
+
+PyTorch supplies device memory and the stream: the context's work is
+ordered on torch's current stream.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from .fv import DeviceFV
+from .host import FS, Par, ssprk3_coefficients
+from .mesh import UniformMesh
+
+
+class SSPRK3Run:
+    """N steps of threestep (SSPRK(3,3), src/mod_advance.t:251-267) on a
+    uniform periodic level, differentiable w.r.t. the initial state.
+
+    States live in torch device tensors in the block layout [nb, nw, (z,) y,
+    x] and are handed to the stage and adjoint kernels by pointer
+    (amrvac_fv_stage / amrvac_fv_adjoint1): no copies between steps."""
+
+    def __init__(self, par: Par, mesh: UniformMesh, nsteps: int, every: int | None = None,
+                 precision: str | None = None):
+        if par.time_stepper != "threestep":
+            raise ValueError("the differentiable run integrates with threestep")
+        if any(b != "periodic" for b in mesh.bc):
+            raise ValueError("the differentiable run needs a periodic domain")
+        self.par = par
+        self.mesh = mesh
+        self.nsteps = nsteps
+        self.every = every or max(1, int(math.sqrt(nsteps)))
+        self.be = DeviceFV(par, mesh, precision=precision)
+        self.shape = (mesh.nblocks, par.nw) + tuple(mesh.block_nx[::-1])
+        self.method = FS[par.flux_scheme]
+        self.c = ssprk3_coefficients()
+        self.dts: list[float] | None = None
+        self.checkpoints: dict[int, torch.Tensor] = {}
+
+    def _bind_stream(self) -> None:
+        self.be.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    def _step(self, u: int, u1: int, u2: int, out: int, dt: float) -> None:
+        """out = ssprk3 step of u (pointers); u1, u2 receive the stage states."""
+        c, m, be = self.c, self.method, self.be
+        be.stage(m, c["beta11"], dt, 0.0, u, u1)
+        be.stage(m, c["beta22"], dt, 0.0, u1, u2, base=u, comb_a=c["alfa21"], comb_b=c["alfa22"])
+        be.stage(m, c["beta33"], dt, 0.0, u2, out, base=u, comb_a=c["alfa31"], comb_b=c["alfa33"])
+
+    def forward(self, u0: torch.Tensor, dts: list[float] | None = None) -> torch.Tensor:
+        """Final state; keeps checkpoints for backward.  With ``dts`` the
+        given time steps are replayed instead of the CFL ones."""
+        if u0.shape != self.shape or u0.dtype != torch.float64 or not u0.is_cuda:
+            raise ValueError(f"initial state must be a float64 CUDA tensor of shape {self.shape}")
+        if dts is not None and len(dts) != self.nsteps:
+            raise ValueError("dts must hold one time step per step")
+        self._bind_stream()
+        u = u0.detach().contiguous().clone()
+        u1 = torch.empty_like(u)
+        u2 = torch.empty_like(u)
+        self.checkpoints = {0: u.clone()}
+        used = []
+        for n in range(self.nsteps):
+            dt = dts[n] if dts is not None else self.be.getdt_state(u.data_ptr(), self.par.courantpar)[0]
+            used.append(dt)
+            # the third stage overwrites u in place (its RK base, read per cell)
+            self._step(u.data_ptr(), u1.data_ptr(), u2.data_ptr(), u.data_ptr(), dt)
+            if (n + 1) % self.every == 0 and n + 1 < self.nsteps:
+                self.checkpoints[n + 1] = u.clone()
+        self.dts = used
+        return u
+
+    def backward(self, lam: torch.Tensor) -> torch.Tensor:
+        """Adjoint of the whole run: d(final)/d(initial)^T lam."""
+        self._bind_stream()
+        c, m, be = self.c, self.method, self.be
+        lam = lam.detach().to(torch.float64).contiguous().clone()
+        l1 = torch.empty_like(lam)
+        l2 = torch.empty_like(lam)
+        starts = sorted(self.checkpoints)
+        # stage states of one segment: tape[j] = (u0, u1, u2) of its step j
+        tape = torch.empty((self.every, 3) + tuple(lam.shape), dtype=torch.float64, device=lam.device)
+        for s_i in range(len(starts) - 1, -1, -1):
+            n0 = starts[s_i]
+            n1 = starts[s_i + 1] if s_i + 1 < len(starts) else self.nsteps
+            # recompute the segment from its checkpoint
+            tape[0, 0].copy_(self.checkpoints[n0])
+            for j in range(n1 - n0):
+                dt = self.dts[n0 + j]
+                u0, u1, u2 = (tape[j, q].data_ptr() for q in range(3))
+                be.stage(m, c["beta11"], dt, 0.0, u0, u1)
+                be.stage(m, c["beta22"], dt, 0.0, u1, u2, base=u0, comb_a=c["alfa21"], comb_b=c["alfa22"])
+                if j + 1 < n1 - n0:
+                    be.stage(m, c["beta33"], dt, 0.0, u2, tape[j + 1, 0].data_ptr(), base=u0,
+                             comb_a=c["alfa31"], comb_b=c["alfa33"])
+            for j in range(n1 - n0 - 1, -1, -1):
+                dt = self.dts[n0 + j]
+                u0, u1, u2 = (tape[j, q].data_ptr() for q in range(3))
+                # u3 = a31 u0 + a33 u2 + L3(u2)
+                be.adjoint1(m, c["beta33"], dt, u2, lam.data_ptr(), c["alfa33"], l2.data_ptr())
+                # u2 = a21 u0 + a22 u1 + L2(u1)
+                be.adjoint1(m, c["beta22"], dt, u1, l2.data_ptr(), c["alfa22"], l1.data_ptr())
+                # u1 = u0 + L1(u0); lam_u0 = l1 + L1^T l1 + a31 lam + a21 l2 (in place)
+                be.adjoint1(m, c["beta11"], dt, u0, l1.data_ptr(), 1.0, lam.data_ptr(),
+                            add_a=lam.data_ptr(), ca=c["alfa31"], add_b=l2.data_ptr(), cb=c["alfa21"])
+        return lam
+
+
+class _RunFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u0, run: SSPRK3Run, dts):
+        ctx.run = run
+        return run.forward(u0, dts)
+
+    @staticmethod
+    def backward(ctx, grad):
+        return ctx.run.backward(grad), None, None
+
+
+def run_differentiable(u0: torch.Tensor, run: SSPRK3Run, dts: list[float] | None = None) -> torch.Tensor:
+    """Final state of ``run`` from ``u0`` (block layout [nb, nw, (z,) y, x]),
+    differentiable w.r.t. u0 through the adjoint kernels."""
+    return _RunFunction.apply(u0, run, dts)
+
+
+# ---------------------------------------------------------------------------
+# global fields <-> block layout
+# ---------------------------------------------------------------------------
+def block_index(mesh: UniformMesh) -> np.ndarray:
+    """Flat global cell index (row-major over (z,) y, x) of every interior
+    cell in the block layout [nb, (z,) y, x]."""
+    nd = mesh.ndim
+    shp = tuple(mesh.block_nx[::-1])
+    dom = tuple(mesh.domain_nx[::-1])
+    out = np.zeros((mesh.nblocks,) + shp, dtype=np.int64)
+    for b in range(mesh.nblocks):
+        coords = []
+        for a in range(nd):
+            d = nd - 1 - a
+            g = (int(mesh.ig[b, d]) - 1) * mesh.block_nx[d] + np.arange(mesh.block_nx[d])
+            coords.append(g)
+        grids = np.meshgrid(*coords, indexing="ij")
+        out[b] = np.ravel_multi_index(tuple(grids), dom)
+    return out
+
+
+def to_blocks(u: torch.Tensor, index: torch.Tensor) -> torch.Tensor:
+    """Global fields [nw, (nz,) ny, nx] -> block layout [nb, nw, (z,) y, x]
+    (a differentiable gather)."""
+    nw = u.shape[0]
+    g = u.reshape(nw, -1)[:, index.reshape(-1)].reshape((nw,) + tuple(index.shape))
+    return g.movedim(0, 1).contiguous()
+
+
+def from_blocks(blocks: torch.Tensor, index: torch.Tensor, domain_shape) -> torch.Tensor:
+    """Block layout [nb, nw, (z,) y, x] -> global fields [nw, (nz,) ny, nx]
+    (a differentiable scatter)."""
+    nw = blocks.shape[1]
+    src = blocks.movedim(1, 0).reshape(nw, -1)
+    out = torch.zeros((nw, int(np.prod(domain_shape))), dtype=blocks.dtype, device=blocks.device)
+    out = out.index_copy(1, index.reshape(-1), src)
+    return out.reshape((nw,) + tuple(domain_shape))
+
+
+def hd_conserved(rho: torch.Tensor, v: list[torch.Tensor], p: torch.Tensor, gamma: float) -> torch.Tensor:
+    """hd_to_conserved on fields: [nw, ...]."""
+    vsq = v[0] * v[0]
+    for c in v[1:]:
+        vsq = vsq + c * c
+    e = p * (1.0 / (gamma - 1.0)) + 0.5 * vsq * rho
+    return torch.stack([rho] + [rho * c for c in v] + [e])
+
+
+def kh_fields(mesh: UniformMesh, seed: int, amp: float = 1.0e-3, device=None, rho_amp: float = 0.0):
+    """Kelvin-Helmholtz primitive fields (rho, vx, vy, p) on the global grid:
+    rho=2, vx=+0.5 for |y-0.5|<0.25, else rho=1, vx=-0.5; p=2.5;
+    vy = 0.01 sin(4 pi x + phase) with a seeded phase; seeded uniform noise
+    of amplitude ``amp`` on vx and vy and of ``rho_amp`` on rho (hash of the
+    global cell index).  With rho_amp=0 every density slope inside the bands
+    is exactly zero, the kink of the limiters, where the run has one-sided
+    derivatives only."""
+    from .problems import hashed_uniform
+    ny, nx = mesh.domain_nx[1], mesh.domain_nx[0]
+    dx = [(mesh.xprobmax[d] - mesh.xprobmin[d]) / mesh.domain_nx[d] for d in range(2)]
+    x = mesh.xprobmin[0] + (np.arange(nx) + 0.5) * dx[0]
+    y = mesh.xprobmin[1] + (np.arange(ny) + 0.5) * dx[1]
+    Y, X = np.meshgrid(y, x, indexing="ij")
+    gi = np.stack(np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")[::-1])  # [2, ny, nx] (x, y)
+    phase = 2.0 * math.pi * float(np.random.default_rng(seed).random())
+    inner = np.abs(Y - 0.5) < 0.25
+    rho = np.where(inner, 2.0, 1.0)
+    vx = np.where(inner, 0.5, -0.5) + amp * hashed_uniform(seed, 1, gi, 1)
+    vy = 0.01 * np.sin(4.0 * math.pi * X + phase) + amp * hashed_uniform(seed, 1, gi, 2)
+    if rho_amp:
+        rho = rho + rho_amp * hashed_uniform(seed, 1, gi, 0)
+    p = np.full_like(rho, 2.5)
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=device)
+    return t(rho), t(vx), t(vy), t(p)

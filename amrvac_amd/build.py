@@ -56,6 +56,8 @@ def build(verbose: bool = False, jobs: int | None = None, csrc: str = CSRC, buil
     # built without contraction so that dt and cmax_global round like the
     # CPU restatement's in every precision mode
     tasks.append((os.path.join(csrc, "fv_capi.hip"), os.path.join(build_dir, "fv_capi.o"), ["-ffp-contract=off"]))
+    # adjoint stage kernels (production precision forms only)
+    tasks.append((os.path.join(csrc, "fv_adj.hip"), os.path.join(build_dir, "fv_adj.o"), []))
     jobs = jobs or min(len(tasks), max(1, (os.cpu_count() or 4)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         results = list(ex.map(lambda t: _compile(*t), tasks))

@@ -1,0 +1,226 @@
+// fv_adj.hip -- adjoint of one finite-volume stage on a uniform periodic
+// level (the reverse of the stage kernel's flux divergence):
+//   lam_in = coef * lam_out + ca * add_a + cb * add_b
+//            + sum_faces (d fC_f / d u_in)^T mu_f,
+//   mu_f = (-qdt/dx_d) (lam_out(left of f) - lam_out(right of f)).
+//
+// adj_plane_kernel: one workgroup per (block, z plane).  The plane's
+// primitive states with a 3-cell x/y halo are staged in LDS (each cell
+// converted once); every x and y face whose 4-cell stencil reaches the tile
+// (BX+3 faces per row) is recomputed and reversed once, its four primitive
+// adjoints written to an LDS face buffer, and each cell then sums the
+// contributions of its four faces per dimension in a fixed order (no
+// atomics: the result is deterministic).  The summed primitive adjoint is
+// carried to the conserved state once per cell.  In 3-D the z faces are
+// reversed beforehand by adj_zface_kernel (one lane per face, into a global
+// face buffer) and gathered here in the same fixed order.
+#include <hip/hip_runtime.h>
+
+#include "fv_adjoint.h"
+#include "fv_launch.h"
+
+namespace fv {
+
+using fvc::AdjArgs;
+
+constexpr int AH = 3;             // primitive halo of the tile
+constexpr int AW = BX + 2 * AH;   // staged row length
+constexpr int NF = BX + 3;        // faces per row touching the tile: f in [-1, BX+1]
+constexpr int NFACE = NF * BX;    // per dimension and plane
+
+// lam_out at an arbitrary (possibly neighbouring) cell of plane z
+template <int NW, int NDIM>
+__device__ __forceinline__ void load_lam(const SweepArgs& a, const double* __restrict__ lam, int b, int x, int y,
+                                         int z, double* l) {
+  double sign;
+  const int sb = locate<NDIM>(a, b, x, y, z, sign);
+  const size_t cells = (size_t)BX * BX * a.bnz;
+  const size_t off = (size_t)sb * NW * cells + ((size_t)z * BX + y) * BX + x;
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) l[iw] = lam[off + iw * cells];
+}
+
+template <class P, class R, int RIEM, int LIM>
+__global__ void __launch_bounds__(NTHREADS) adj_zface_kernel(SweepArgs a, FVParams fp, AdjArgs g) {
+  constexpr int NW = P::NW, NDIM = P::NDIM;
+  static_assert(NDIM == 3, "z faces exist in 3-D");
+  const int b = blockIdx.x / a.bnz, k = blockIdx.x % a.bnz;
+  if (b >= a.nblocks) return;
+  const int tx = threadIdx.x & (BX - 1), ty = threadIdx.x >> 4;
+  const size_t cells = (size_t)BX * BX * a.bnz;
+  // the low z face of cell (tx, ty, k): stencil k-2 .. k+1
+  R st[4][NW];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) load_prim<P, R, NDIM>(a, fp, b, tx, ty, k - 2 + s, st[s]);
+  double ll[NW], lr[NW];
+  load_lam<NW, NDIM>(a, g.lam_out, b, tx, ty, k - 1, ll);
+  load_lam<NW, NDIM>(a, g.lam_out, b, tx, ty, k, lr);
+  R mu[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) mu[iw] = R(a.dq[2] * (ll[iw] - lr[iw]));
+  double wb[4][NW];
+  face_adj_prim<P, R, RIEM, 2, LIM>(st, fp, mu, wb);
+  const size_t ntot = (size_t)a.nblocks * cells;
+  const size_t o = (size_t)b * cells + ((size_t)k * BX + ty) * BX + tx;
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) g.gz[(size_t)(s * NW + iw) * ntot + o] = wb[s][iw];
+}
+
+template <class P, class R, int RIEM, int LIM>
+__global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVParams fp, AdjArgs g) {
+  constexpr int NW = P::NW, NDIM = P::NDIM;
+  __shared__ R Wl[NW][AW * AW];
+  __shared__ double Gl[4 * NW][NFACE];
+  const int b = blockIdx.x / a.bnz, k = blockIdx.x % a.bnz;
+  if (b >= a.nblocks) return;
+  const int t = threadIdx.x;
+  const int tx = t & (BX - 1), ty = t >> 4;
+  const size_t cells = (size_t)BX * BX * a.bnz;
+  const size_t o = (size_t)b * NW * cells + ((size_t)k * BX + ty) * BX + tx;
+
+  // ---- primitive plane with its x/y halo (corners are not read) ----------
+  for (int i = t; i < AW * AW; i += NTHREADS) {
+    const int x = i % AW - AH, y = i / AW - AH;
+    if ((x < 0 || x >= BX) && (y < 0 || y >= BX)) continue;
+    R w[NW];
+    load_prim<P, R, NDIM>(a, fp, b, x, y, k, w);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) Wl[iw][i] = w[iw];
+  }
+  double acc[NW], wsum[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    double v = g.coef * g.lam_out[o + iw * cells];
+    if (g.add_a) v = v + g.ca * g.add_a[o + iw * cells];
+    if (g.add_b) v = v + g.cb * g.add_b[o + iw * cells];
+    acc[iw] = v;
+    wsum[iw] = 0.0;
+  }
+  __syncthreads();
+
+  // ---- x faces: face j = (f, row), f = j % NF - 1 ------------------------
+  for (int j = t; j < NFACE; j += NTHREADS) {
+    const int f = j % NF - 1, row = j / NF;
+    R st[4][NW];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) st[s][iw] = Wl[iw][(row + AH) * AW + f - 2 + s + AH];
+    double ll[NW], lr[NW];
+    load_lam<NW, NDIM>(a, g.lam_out, b, f - 1, row, k, ll);
+    load_lam<NW, NDIM>(a, g.lam_out, b, f, row, k, lr);
+    R mu[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) mu[iw] = R(a.dq[0] * (ll[iw] - lr[iw]));
+    double wb[4][NW];
+    face_adj_prim<P, R, RIEM, 0, LIM>(st, fp, mu, wb);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) Gl[s * NW + iw][j] = wb[s][iw];
+  }
+  __syncthreads();
+  // cell tx is stencil position s = tx - f + 2 of faces f = tx+2 .. tx-1
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int j = ty * NF + (tx - s + 2) + 1;
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) wsum[iw] += Gl[s * NW + iw][j];
+  }
+  __syncthreads();
+
+  // ---- y faces: face j = (col, f), col = j % BX, f = j / BX - 1 ----------
+  for (int j = t; j < NFACE; j += NTHREADS) {
+    const int col = j % BX, f = j / BX - 1;
+    R st[4][NW];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) st[s][iw] = Wl[iw][(f - 2 + s + AH) * AW + col + AH];
+    double ll[NW], lr[NW];
+    load_lam<NW, NDIM>(a, g.lam_out, b, col, f - 1, k, ll);
+    load_lam<NW, NDIM>(a, g.lam_out, b, col, f, k, lr);
+    R mu[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) mu[iw] = R(a.dq[1] * (ll[iw] - lr[iw]));
+    double wb[4][NW];
+    face_adj_prim<P, R, RIEM, 1, LIM>(st, fp, mu, wb);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) Gl[s * NW + iw][j] = wb[s][iw];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int j = (ty - s + 2 + 1) * BX + tx;
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) wsum[iw] += Gl[s * NW + iw][j];
+  }
+
+  // ---- z faces (3-D): reversed by adj_zface_kernel ------------------------
+  if constexpr (NDIM == 3) {
+    const size_t ntot = (size_t)a.nblocks * cells;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      int x = tx, y = ty, z = k - s + 2;
+      double sign;
+      const int ob = locate<NDIM>(a, b, x, y, z, sign);
+      const size_t of = (size_t)ob * cells + ((size_t)z * BX + y) * BX + x;
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) wsum[iw] += g.gz[(size_t)(s * NW + iw) * ntot + of];
+    }
+  }
+
+  // ---- to the conserved state; lam_in -------------------------------------
+  double cons[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) cons[iw] = a.src[o + iw * cells];
+  prim_rev<P>(cons, wsum, fp, acc);
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) g.lam_in[o + iw * cells] = acc[iw];
+}
+
+template <class P, class R, int RIEM, int LIM>
+hipError_t launch_adj(const SweepArgs& a, const FVParams& fp, const AdjArgs& g, hipStream_t s) {
+  const dim3 grid(a.nblocks * a.bnz);
+  if constexpr (P::NDIM == 3)
+    hipLaunchKernelGGL((adj_zface_kernel<P, R, RIEM, LIM>), grid, dim3(NTHREADS), 0, s, a, fp, g);
+  hipLaunchKernelGGL((adj_plane_kernel<P, R, RIEM, LIM>), grid, dim3(NTHREADS), 0, s, a, fp, g);
+  return hipGetLastError();
+}
+
+template <class P, class R>
+hipError_t adj_riem(int riem, int lim, const SweepArgs& a, const FVParams& fp, const AdjArgs& g, hipStream_t s) {
+  if (lim != LIM_VANLEER && lim != LIM_MINMOD) return hipErrorInvalidValue;
+  const bool vl = lim == LIM_VANLEER;
+  switch (riem) {
+    case RIEM_HLL:
+      return vl ? launch_adj<P, R, RIEM_HLL, LIM_VANLEER>(a, fp, g, s) : launch_adj<P, R, RIEM_HLL, LIM_MINMOD>(a, fp, g, s);
+    case RIEM_HLLC:
+      return vl ? launch_adj<P, R, RIEM_HLLC, LIM_VANLEER>(a, fp, g, s)
+                : launch_adj<P, R, RIEM_HLLC, LIM_MINMOD>(a, fp, g, s);
+    case RIEM_TVDLF:
+      return vl ? launch_adj<P, R, RIEM_TVDLF, LIM_VANLEER>(a, fp, g, s)
+                : launch_adj<P, R, RIEM_TVDLF, LIM_MINMOD>(a, fp, g, s);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_adjoint_stage(int ndim, int mixed, int riem, int lim, const SweepArgs& a, const FVParams& fp,
+                                const AdjArgs& g, hipStream_t s) {
+  if (ndim == 2)
+    return mixed ? adj_riem<Phys<2, 2, false, DIVB_NONE>, float>(riem, lim, a, fp, g, s)
+                 : adj_riem<Phys<2, 2, false, DIVB_NONE>, double>(riem, lim, a, fp, g, s);
+  if (ndim == 3) {
+    if (!g.gz) return hipErrorInvalidValue;
+    return mixed ? adj_riem<Phys<3, 3, false, DIVB_NONE>, float>(riem, lim, a, fp, g, s)
+                 : adj_riem<Phys<3, 3, false, DIVB_NONE>, double>(riem, lim, a, fp, g, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace fv

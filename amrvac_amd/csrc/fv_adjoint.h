@@ -1,0 +1,404 @@
+// fv_adjoint.h -- reverse mode (adjoint) of the finite-volume face flux for
+// hydrodynamics: conserved stencil -> primitive (hd_to_primitive) -> TVD
+// reconstruction (minmod / vanleer, dwlimiter2) -> hd_to_conserved ->
+// hd_get_flux -> Einfeldt bounds -> TVDLF / HLL / HLLC
+// (src/mod_finite_volume.t:116-339, :348-504; src/hd/mod_hd_phys.t;
+// src/hd/mod_hd_hllc.t).  Every branch (limiter cases, Riemann patches) is
+// differentiated along the branch the forward evaluation takes.
+//
+// face_adj_prim recomputes the forward face from the primitive states of its
+// four cells and returns their primitive adjoints; prim_rev carries a cell's
+// summed primitive adjoint back to its conserved state (the reverse of the
+// conversion is linear, so it is applied once per cell rather than per face).
+#pragma once
+
+#include "fv_device.h"
+
+namespace FV_NS {
+
+// limited slope t(a, b) of dwlimiter2 for the symmetric TVD limiters and
+// its partial derivatives
+template <class R, int LIM>
+__device__ __forceinline__ R lim_grad(R a, R b, R& da, R& db) {
+  if constexpr (LIM == LIM_MINMOD) {
+    const R s = copysign(R(1), b);
+    const R sa = s * a, ab = fabs(b);
+    if (!(sa > R(0))) { da = R(0); db = R(0); return R(0); }
+    if (ab <= sa) { da = R(0); db = R(1); return s * ab; }
+    da = R(1); db = R(0); return a;
+  } else {  // vanleer: 2 max(ab, 0) / (a + b + 1e-12)
+    const R ab = a * b;
+    if (!(ab > R(0))) { da = R(0); db = R(0); return R(0); }
+    const R q = b + a + R(1.0e-12);
+    const R iq = R(1) / q;
+    const R t = R(2) * ab * iq;
+    da = R(2) * b * (q - a) * iq * iq;
+    db = R(2) * a * (q - b) * iq * iq;
+    return t;
+  }
+}
+
+// Reverse of the face flux in primitive form: from the primitive states w of
+// the face's four cells (as the sweep rounds them to R) and the flux adjoint
+// fbar, writes the primitive adjoints wb[c] = (d fC / d w_c)^T fbar.
+template <class P, class R, int RIEM, int ID, int LIM>
+__device__ void face_adj_prim(const R (*w)[P::NW], const FVParams& fp, const R* fbar, double (*wb)[P::NW]) {
+  static_assert(!P::MHD, "face_adj_prim: hydrodynamics only");
+  constexpr int NW = P::NW, ND = P::NDIR, RHO = P::RHO, E = P::E, MN = P::MOM + ID;
+  const R gam = R(fp.gamma), inv_g1 = R(fp.inv_gamma_1);
+  // ---------------- forward ----------------
+  R wL[NW], wR[NW], d1a[NW], d1b[NW], d2a[NW], d2b[NW];
+  for (int iw = 0; iw < NW; ++iw) {
+    const R dm = w[1][iw] - w[0][iw], d0 = w[2][iw] - w[1][iw], dp = w[3][iw] - w[2][iw];
+    const R t1 = lim_grad<R, LIM>(dm, d0, d1a[iw], d1b[iw]);
+    const R t2 = lim_grad<R, LIM>(d0, dp, d2a[iw], d2b[iw]);
+    wL[iw] = w[1][iw] + R(0.5) * t1;
+    wR[iw] = w[2][iw] - R(0.5) * t2;
+  }
+  // conserved face states
+  R wLC[NW], wRC[NW], vsqL = R(0), vsqR = R(0);
+  for (int k = 0; k < ND; ++k) {
+    vsqL = vsqL + wL[P::MOM + k] * wL[P::MOM + k];
+    vsqR = vsqR + wR[P::MOM + k] * wR[P::MOM + k];
+  }
+  wLC[RHO] = wL[RHO];
+  wRC[RHO] = wR[RHO];
+  wLC[E] = wL[E] * inv_g1 + R(0.5) * vsqL * wL[RHO];
+  wRC[E] = wR[E] * inv_g1 + R(0.5) * vsqR * wR[RHO];
+  for (int k = 0; k < ND; ++k) {
+    wLC[P::MOM + k] = wL[RHO] * wL[P::MOM + k];
+    wRC[P::MOM + k] = wR[RHO] * wR[P::MOM + k];
+  }
+  R fL[NW], fR[NW];
+  phys_flux<P, R, ID>(wLC, wL, R(0), fL);
+  phys_flux<P, R, ID>(wRC, wR, R(0), fR);
+  // Einfeldt bounds (hd_get_cbounds, typeboundspeed 'Einfeldt')
+  const R t1 = sqrt(wL[RHO]), t2 = sqrt(wR[RHO]);
+  const R t3 = R(1) / (t1 + t2);
+  const R um = (wL[MN] * t1 + wR[MN] * t2) * t3;
+  const R csL = gam * wL[E] / wL[RHO], csR = gam * wR[E] / wR[RHO];
+  const R dv = wR[MN] - wL[MN];
+  const R A = t1 * csL + t2 * csR;
+  const R D2 = A * t3 + R(0.5) * t1 * t2 * (t3 * t3) * (dv * dv);
+  const R dmean = sqrt(D2);
+  R cmax, cmin;
+  if constexpr (RIEM == RIEM_TVDLF) {
+    cmax = fabs(um) + dmean;
+    cmin = R(0);
+  } else {
+    cmin = um - dmean;
+    cmax = um + dmean;
+  }
+  // ---------------- reverse ----------------
+  R fLb[NW], fRb[NW], wLCb[NW], wRCb[NW];
+  for (int iw = 0; iw < NW; ++iw) { fLb[iw] = R(0); fRb[iw] = R(0); wLCb[iw] = R(0); wRCb[iw] = R(0); }
+  R cmaxb = R(0), cminb = R(0);
+  if constexpr (RIEM == RIEM_TVDLF) {
+    const R fac = R(-0.5) * R(fp.tvdlfeps) * cmax;
+    R s = R(0);
+    for (int iw = 0; iw < NW; ++iw) {
+      fLb[iw] = R(0.5) * fbar[iw];
+      fRb[iw] = R(0.5) * fbar[iw];
+      wRCb[iw] = fac * fbar[iw];
+      wLCb[iw] = -fac * fbar[iw];
+      s = s + fbar[iw] * (wRC[iw] - wLC[iw]);
+    }
+    cmaxb = R(-0.5) * R(fp.tvdlfeps) * s;
+  } else if constexpr (RIEM == RIEM_HLL) {
+    if (cmin >= R(0)) {
+      for (int iw = 0; iw < NW; ++iw) fLb[iw] = fbar[iw];
+    } else if (cmax <= R(0)) {
+      for (int iw = 0; iw < NW; ++iw) fRb[iw] = fbar[iw];
+    } else {
+      const R iD = R(1) / (cmax - cmin);
+      R Db = R(0);
+      for (int iw = 0; iw < NW; ++iw) {
+        const R dw = wRC[iw] - wLC[iw];
+        const R f = (cmax * fL[iw] - cmin * fR[iw] + cmin * cmax * dw) * iD;
+        const R Nb = fbar[iw] * iD;
+        Db = Db - fbar[iw] * f * iD;
+        fLb[iw] = Nb * cmax;
+        fRb[iw] = -Nb * cmin;
+        wRCb[iw] = Nb * cmin * cmax;
+        wLCb[iw] = -Nb * cmin * cmax;
+        cmaxb = cmaxb + Nb * (fL[iw] + cmin * dw);
+        cminb = cminb + Nb * (-fR[iw] + cmax * dw);
+      }
+      cmaxb = cmaxb + Db;
+      cminb = cminb - Db;
+    }
+  } else {  // HLLC, src/mod_finite_volume.t:439-504, hd_get_lCD / hd_get_wCD
+    int patchf = 1;
+    if (cmin >= R(0)) patchf = -2;
+    else if (cmax <= R(0)) patchf = 2;
+    R lam = R(0), iD = R(0), whr = R(0), whm = R(0);
+    if (patchf == 1) {
+      iD = R(1) / (cmax - cmin);
+      whr = (cmax * wRC[RHO] - cmin * wLC[RHO] + fL[RHO] - fR[RHO]) * iD;
+      whm = (cmax * wRC[MN] - cmin * wLC[MN] + fL[MN] - fR[MN]) * iD;
+      lam = whm / whr;
+      bool cond = true;
+      if (cmin < R(0) && lam > R(0) && lam < cmax) {
+        patchf = -1;
+      } else if (cmax > R(0) && lam < R(0) && lam > cmin) {
+        patchf = 1;
+      } else if (lam >= cmax || lam <= cmin) {
+        lam = R(0);
+        patchf = 3;
+        cond = false;
+      }
+      if (cond && lam == R(0)) patchf = (-cmin >= cmax) ? 1 : -1;
+    }
+    if (patchf == -2) {
+      for (int iw = 0; iw < NW; ++iw) fLb[iw] = fbar[iw];
+    } else if (patchf == 2) {
+      for (int iw = 0; iw < NW; ++iw) fRb[iw] = fbar[iw];
+    } else if (patchf == 3) {
+      R Db = R(0);
+      for (int iw = 0; iw < NW; ++iw) {
+        const R dw = wRC[iw] - wLC[iw];
+        const R f = (cmax * fL[iw] - cmin * fR[iw] + cmin * cmax * dw) * iD;
+        const R Nb = fbar[iw] * iD;
+        Db = Db - fbar[iw] * f * iD;
+        fLb[iw] = Nb * cmax;
+        fRb[iw] = -Nb * cmin;
+        wRCb[iw] = Nb * cmin * cmax;
+        wLCb[iw] = -Nb * cmin * cmax;
+        cmaxb = cmaxb + Nb * (fL[iw] + cmin * dw);
+        cminb = cminb + Nb * (-fR[iw] + cmax * dw);
+      }
+      cmaxb = cmaxb + Db;
+      cminb = cminb - Db;
+    } else {
+      const bool right = patchf == 1;
+      const R cs = right ? cmax : cmin;
+      const R* wS = right ? wRC : wLC;
+      const R* fS = right ? fR : fL;
+      R* wSb = right ? wRCb : wLCb;
+      R* fSb = right ? fRb : fLb;
+      const R vS = wS[MN] / wS[RHO];
+      const R csl = cs - lam;
+      const R csmls = R(1) / csl;
+      const R Z = wS[RHO] * (cs - vS);
+      R wCD[NW];
+      wCD[RHO] = Z * csmls;
+      for (int k = 0; k < ND; ++k)
+        if (k != ID) wCD[P::MOM + k] = (cs * wS[P::MOM + k] - fS[P::MOM + k]) * csmls;
+      wCD[MN] = wCD[RHO] * lam;
+      const R pCD = wS[RHO] * (cs - vS) * (lam - vS) + fS[MN] - wS[MN] * vS;
+      const R XE = cs * wS[E] - fS[E] + lam * pCD;
+      wCD[E] = XE * csmls;
+      // fCD = fS + cs (wCD - wS)
+      R csb = R(0), lamb = R(0), csmlsb = R(0), vSb = R(0), wCDb[NW];
+      for (int iw = 0; iw < NW; ++iw) {
+        fSb[iw] = fSb[iw] + fbar[iw];
+        csb = csb + fbar[iw] * (wCD[iw] - wS[iw]);
+        wCDb[iw] = cs * fbar[iw];
+        wSb[iw] = wSb[iw] - cs * fbar[iw];
+      }
+      // wCD[E]
+      {
+        const R XEb = wCDb[E] * csmls;
+        csmlsb = csmlsb + wCDb[E] * XE;
+        csb = csb + XEb * wS[E];
+        wSb[E] = wSb[E] + XEb * cs;
+        fSb[E] = fSb[E] - XEb;
+        lamb = lamb + XEb * pCD;
+        const R pCDb = XEb * lam;
+        wSb[RHO] = wSb[RHO] + pCDb * (cs - vS) * (lam - vS);
+        csb = csb + pCDb * wS[RHO] * (lam - vS);
+        lamb = lamb + pCDb * wS[RHO] * (cs - vS);
+        vSb = vSb + pCDb * (-wS[RHO] * (lam - vS) - wS[RHO] * (cs - vS) - wS[MN]);
+        fSb[MN] = fSb[MN] + pCDb;
+        wSb[MN] = wSb[MN] - pCDb * vS;
+      }
+      // wCD[MN] = wCD[RHO] * lam
+      R rhoCDb = wCDb[RHO] + wCDb[MN] * lam;
+      lamb = lamb + wCDb[MN] * wCD[RHO];
+      // transverse momenta
+      for (int k = 0; k < ND; ++k) {
+        if (k == ID) continue;
+        const int m = P::MOM + k;
+        const R Y = cs * wS[m] - fS[m];
+        const R Yb = wCDb[m] * csmls;
+        csmlsb = csmlsb + wCDb[m] * Y;
+        csb = csb + Yb * wS[m];
+        wSb[m] = wSb[m] + Yb * cs;
+        fSb[m] = fSb[m] - Yb;
+      }
+      // wCD[RHO] = Z csmls, Z = wS[RHO] (cs - vS)
+      {
+        const R Zb = rhoCDb * csmls;
+        csmlsb = csmlsb + rhoCDb * Z;
+        wSb[RHO] = wSb[RHO] + Zb * (cs - vS);
+        csb = csb + Zb * wS[RHO];
+        vSb = vSb - Zb * wS[RHO];
+      }
+      // csmls = 1/(cs - lam)
+      const R cslb = -csmlsb * csmls * csmls;
+      csb = csb + cslb;
+      lamb = lamb - cslb;
+      // vS = wS[MN] / wS[RHO]
+      wSb[MN] = wSb[MN] + vSb / wS[RHO];
+      wSb[RHO] = wSb[RHO] - vSb * vS / wS[RHO];
+      // lam = whm / whr
+      const R whmb = lamb / whr;
+      const R whrb = -lamb * lam / whr;
+      // whll = (cmax wRC - cmin wLC + fL - fR) iD for RHO and MN
+      R iDb = R(0);
+      {
+        const R Qr = whr / iD, Qm = whm / iD;  // numerators
+        (void)Qr; (void)Qm;
+        const int vars[2] = {RHO, MN};
+        const R wb[2] = {whrb, whmb};
+        const R wh[2] = {whr, whm};
+        for (int j = 0; j < 2; ++j) {
+          const int iw = vars[j];
+          const R Qb = wb[j] * iD;
+          iDb = iDb + wb[j] * (cmax * wRC[iw] - cmin * wLC[iw] + fL[iw] - fR[iw]);
+          cmaxb = cmaxb + Qb * wRC[iw];
+          cminb = cminb - Qb * wLC[iw];
+          wRCb[iw] = wRCb[iw] + Qb * cmax;
+          wLCb[iw] = wLCb[iw] - Qb * cmin;
+          fLb[iw] = fLb[iw] + Qb;
+          fRb[iw] = fRb[iw] - Qb;
+          (void)wh;
+        }
+      }
+      const R Db = -iDb * iD * iD;
+      cmaxb = cmaxb + Db;
+      cminb = cminb - Db;
+      if (right) cmaxb = cmaxb + csb;
+      else cminb = cminb + csb;
+    }
+  }
+  // bounds -> face primitive states
+  R wLb[NW], wRb[NW];
+  for (int iw = 0; iw < NW; ++iw) { wLb[iw] = R(0); wRb[iw] = R(0); }
+  {
+    R umb, dmb;
+    if constexpr (RIEM == RIEM_TVDLF) {
+      umb = copysign(R(1), um) * cmaxb;
+      dmb = cmaxb;
+    } else {
+      umb = cmaxb + cminb;
+      dmb = cmaxb - cminb;
+    }
+    const R D2b = dmb * R(0.5) / dmean;
+    const R Ab = D2b * t3;
+    R t3b = D2b * A;
+    const R Bb = D2b;
+    R t1b = Bb * R(0.5) * t2 * (t3 * t3) * (dv * dv);
+    R t2b = Bb * R(0.5) * t1 * (t3 * t3) * (dv * dv);
+    t3b = t3b + Bb * t1 * t2 * t3 * (dv * dv);
+    const R dvb = Bb * t1 * t2 * (t3 * t3) * dv;
+    t1b = t1b + Ab * csL;
+    t2b = t2b + Ab * csR;
+    const R csLb = Ab * t1, csRb = Ab * t2;
+    wLb[E] = wLb[E] + csLb * gam / wL[RHO];
+    wLb[RHO] = wLb[RHO] - csLb * csL / wL[RHO];
+    wRb[E] = wRb[E] + csRb * gam / wR[RHO];
+    wRb[RHO] = wRb[RHO] - csRb * csR / wR[RHO];
+    wRb[MN] = wRb[MN] + dvb;
+    wLb[MN] = wLb[MN] - dvb;
+    wLb[MN] = wLb[MN] + umb * t1 * t3;
+    wRb[MN] = wRb[MN] + umb * t2 * t3;
+    t1b = t1b + umb * wL[MN] * t3;
+    t2b = t2b + umb * wR[MN] * t3;
+    t3b = t3b + umb * (wL[MN] * t1 + wR[MN] * t2);
+    t1b = t1b - t3b * t3 * t3;
+    t2b = t2b - t3b * t3 * t3;
+    wLb[RHO] = wLb[RHO] + t1b * R(0.5) / t1;
+    wRb[RHO] = wRb[RHO] + t2b * R(0.5) / t2;
+  }
+  // fluxes: f_rho = vn rho, f_mk = vn m_k, f_mn += p, f_E = vn (E + p)
+  auto flux_rev = [&](const R* wX, const R* wXC, const R* fXb, R* wXb, R* wXCb) {
+    const R vn = wX[MN];
+    R vnb = fXb[RHO] * wX[RHO] + fXb[E] * (wXC[E] + wX[E]);
+    wXb[RHO] = wXb[RHO] + fXb[RHO] * vn;
+    for (int k = 0; k < ND; ++k) {
+      vnb = vnb + fXb[P::MOM + k] * wXC[P::MOM + k];
+      wXCb[P::MOM + k] = wXCb[P::MOM + k] + fXb[P::MOM + k] * vn;
+    }
+    wXb[E] = wXb[E] + fXb[MN] + fXb[E] * vn;
+    wXCb[E] = wXCb[E] + fXb[E] * vn;
+    wXb[MN] = wXb[MN] + vnb;
+  };
+  flux_rev(wL, wLC, fLb, wLb, wLCb);
+  flux_rev(wR, wRC, fRb, wRb, wRCb);
+  // to_cons: E = p inv_g1 + 0.5 vsq rho, m_k = rho v_k, rho = rho
+  auto cons_rev = [&](const R* wX, R vsq, const R* wXCb, R* wXb) {
+    wXb[RHO] = wXb[RHO] + wXCb[RHO];
+    wXb[E] = wXb[E] + inv_g1 * wXCb[E];
+    const R vsqb = R(0.5) * wX[RHO] * wXCb[E];
+    wXb[RHO] = wXb[RHO] + R(0.5) * vsq * wXCb[E];
+    for (int k = 0; k < ND; ++k) {
+      const int m = P::MOM + k;
+      wXb[RHO] = wXb[RHO] + wXCb[m] * wX[m];
+      wXb[m] = wXb[m] + R(2) * wX[m] * vsqb + wX[RHO] * wXCb[m];
+    }
+  };
+  cons_rev(wL, vsqL, wLCb, wLb);
+  cons_rev(wR, vsqR, wRCb, wRb);
+  // reconstruction: wL = w1 + t1/2, wR = w2 - t2/2
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    const R t1b = R(0.5) * wLb[iw], t2b = R(-0.5) * wRb[iw];
+    const R dmb = t1b * d1a[iw];
+    const R d0b = t1b * d1b[iw] + t2b * d2a[iw];
+    const R dpb = t2b * d2b[iw];
+    wb[0][iw] = double(-dmb);
+    wb[1][iw] = double(wLb[iw] + dmb - d0b);
+    wb[2][iw] = double(wRb[iw] + d0b - dpb);
+    wb[3][iw] = double(dpb);
+  }
+}
+
+// Reverse of hd_to_primitive (FP64) for one cell: cbar += (d w / d cons)^T wb
+// with rho = rho, v_k = m_k / rho, p = (gamma-1)(e - rho vsq / 2).
+template <class P>
+__device__ __forceinline__ void prim_rev(const double* cons, const double* wb, const FVParams& fp, double* cbar) {
+  constexpr int ND = P::NDIR, RHO = P::RHO, E = P::E;
+  const double rho = cons[RHO];
+  const double inv = 1.0 / rho;
+  double vv[ND], vsq = 0.0;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    vv[k] = cons[P::MOM + k] * inv;
+    vsq = k == 0 ? vv[k] * vv[k] : vsq + vv[k] * vv[k];
+  }
+  const double g1 = fp.gamma - 1.0;
+  const double pb = wb[E];
+  cbar[E] += g1 * pb;
+  const double tt = -0.5 * g1 * pb;
+  double rhob = wb[RHO] + tt * vsq;
+  const double vsqb = tt * rho;
+  double invb = 0.0;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const double vb = wb[P::MOM + k] + 2.0 * vv[k] * vsqb;
+    cbar[P::MOM + k] += vb * inv;
+    invb += vb * cons[P::MOM + k];
+  }
+  rhob -= invb * inv * inv;
+  cbar[RHO] += rhob;
+}
+
+// Conserved form: recomputes the forward face from the conserved values of
+// its four cells and accumulates (d fC / d cons)^T fbar into cbar.
+template <class P, class R, int RIEM, int ID, int LIM>
+__device__ void face_flux_adj(const double (*cons)[P::NW], const FVParams& fp, const R* fbar,
+                              double (*cbar)[P::NW]) {
+  constexpr int NW = P::NW;
+  R w[4][NW];
+  for (int c = 0; c < 4; ++c) {
+    double t[NW];
+    to_prim<P>(cons[c], t, fp);
+    for (int iw = 0; iw < NW; ++iw) w[c][iw] = R(t[iw]);
+  }
+  double wb[4][NW];
+  face_adj_prim<P, R, RIEM, ID, LIM>(w, fp, fbar, wb);
+  for (int c = 0; c < 4; ++c) prim_rev<P>(cons[c], wb[c], fp, cbar[c]);
+}
+
+}  // namespace FV_NS

@@ -71,6 +71,7 @@ struct amrvac_fv_ctx {
   int* d_fregoff = nullptr;  // [nb][2*ndim]
   double* d_freg = nullptr;  // flux registers
   double* d_xlo = nullptr;   // [nb][ndim] block lower corners
+  double* d_adjG = nullptr;  // 3-D adjoint z-face contributions [4*nw][nb*cells]
   int nrt = 0, npt = 0, nft = 0;
   std::vector<int> h_level;
   bool timing = false;
@@ -580,12 +581,12 @@ static void free_level(amrvac_fv_ctx* c) {
   if (c->d_stage) (void)hipFree(c->d_stage);
   c->d_nbr = nullptr; c->d_crash = nullptr; c->d_red = nullptr; c->d_red2 = nullptr; c->d_stage = nullptr;
   void* amr_bufs[] = {c->d_level, c->d_ntype, c->d_nchild, c->d_rtask, c->d_ptask, c->d_ptask2,
-                      c->d_ftask, c->d_fregoff, c->d_freg, c->d_xlo};
+                      c->d_ftask, c->d_fregoff, c->d_freg, c->d_xlo, c->d_adjG};
   for (void* p : amr_bufs)
     if (p) (void)hipFree(p);
   c->d_level = nullptr; c->d_ntype = nullptr; c->d_nchild = nullptr; c->d_rtask = nullptr;
   c->d_ptask = nullptr; c->d_ptask2 = nullptr; c->d_ftask = nullptr; c->d_fregoff = nullptr;
-  c->d_freg = nullptr; c->d_xlo = nullptr;
+  c->d_freg = nullptr; c->d_xlo = nullptr; c->d_adjG = nullptr;
   c->amr = false;
   c->nv = 0;
   c->nrt = c->npt = c->nft = 0;
@@ -784,34 +785,9 @@ static bool upload_vec(amrvac_fv_ctx* c, T** dst, const std::vector<T>& v) {
   return hipMemcpyAsync(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream) == hipSuccess;
 }
 
-extern "C" {
-
-int amrvac_fv_download(amrvac_fv_ctx* c, int slot, double* w) {
-  if (!c || !w || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
-  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
-  if (c->nb > 65535) return fail(AMRVAC_FV_EINVAL, "download supports at most 65535 blocks");
-  fvc::SweepArgs a = base_args(c);
-  a.src = c->slot[slot];
-  const int blocks_x = (int)((c->cells_g + fvc::NTHREADS - 1) / fvc::NTHREADS);
-  dim3 grid(blocks_x, c->nb);
-  if (c->cfg.ndim == 2)
-    hipLaunchKernelGGL((fv::fill_ghosted_kernel<2>), grid, dim3(fvc::NTHREADS), 0, c->stream, a, c->nw,
-                       c->cfg.nghost, c->nxg[0], c->nxg[1], c->nxg[2], c->d_stage);
-  else
-    hipLaunchKernelGGL((fv::fill_ghosted_kernel<3>), grid, dim3(fvc::NTHREADS), 0, c->stream, a, c->nw,
-                       c->cfg.nghost, c->nxg[0], c->nxg[1], c->nxg[2], c->d_stage);
-  HIPCHK(hipGetLastError());
-  const size_t n = (size_t)c->nb * c->nw * c->cells_g;
-  HIPCHK(hipMemcpyAsync(w, c->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return AMRVAC_FV_OK;
-}
-
-int amrvac_fv_getdt(amrvac_fv_ctx* c, int slot, double courantpar, double* dtmin_out, double* cmax_out) {
-  if (!c || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
-  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
-  hipError_t e = with_phys<GetdtOp>(c, (const double*)c->slot[slot], c->nb, c->bnz,
-                                    (const fvc::FVParams*)&c->fp, (const int*)c->d_level, c->d_red, c->stream);
+static int getdt_core(amrvac_fv_ctx* c, const double* u, double courantpar, double* dtmin_out, double* cmax_out) {
+  hipError_t e = with_phys<GetdtOp>(c, u, c->nb, c->bnz, (const fvc::FVParams*)&c->fp, (const int*)c->d_level,
+                                    c->d_red, c->stream);
   if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("getdt launch: ") + hipGetErrorString(e));
   hipLaunchKernelGGL(fv::reduce_max2_kernel, dim3(1), dim3(fvc::NTHREADS), 0, c->stream, c->d_red, c->nb, c->d_red2);
   HIPCHK(hipGetLastError());
@@ -826,29 +802,34 @@ int amrvac_fv_getdt(amrvac_fv_ctx* c, int slot, double courantpar, double* dtmin
   return AMRVAC_FV_OK;
 }
 
-int amrvac_fv_advect1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, double cmax_global, int psa,
-                      int psb, int comb_slot, double comb_a, double comb_b, int* crash_out) {
-  if (!c || !check_slot(psa) || !check_slot(psb) || psa == psb) return fail(AMRVAC_FV_EINVAL, "bad slots");
-  if (comb_slot >= 0 && !check_slot(comb_slot)) return fail(AMRVAC_FV_EINVAL, "bad combination slot");
-  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
-  int riem;
+static int method_to_riem(int method, int& riem) {
   switch (method) {
-    case 1: riem = fv::RIEM_HLL; break;
-    case 2: riem = fv::RIEM_HLLC; break;
-    case 5: riem = fv::RIEM_TVDLF; break;
-    case 8: riem = fv::RIEM_HANCOCK; break;
-    default: return fail(AMRVAC_FV_EINVAL, "unsupported flux scheme");
+    case 1: riem = fv::RIEM_HLL; return 0;
+    case 2: riem = fv::RIEM_HLLC; return 0;
+    case 5: riem = fv::RIEM_TVDLF; return 0;
+    case 8: riem = fv::RIEM_HANCOCK; return 0;
+    default: return -1;
   }
+}
+
+// One RK stage, dst = init + L(src) with init per comb_mode (0: dst,
+// 1: base, 2: comb_a*base + comb_b*src, 3: src); the AMR hooks
+// (refluxing, ghost cells of dst_slot) run when dst_slot >= 0.
+static int stage_core(amrvac_fv_ctx* c, int method, double dtfactor, double dt, double cmax_global,
+                      const double* src, double* dst, const double* base, int comb_mode, double comb_a,
+                      double comb_b, int dst_slot, int* crash_out) {
+  int riem;
+  if (method_to_riem(method, riem) != 0) return fail(AMRVAC_FV_EINVAL, "unsupported flux scheme");
   if (riem == fv::RIEM_HLLC && c->cfg.physics != 1) return fail(AMRVAC_FV_EINVAL, "hllc is provided for hd");
   if (riem == fv::RIEM_HANCOCK && c->cfg.limiter == 16)
     return fail(AMRVAC_FV_EINVAL, "the Hancock predictor takes a TVD limiter");
   if (c->cfg.physics == 2 && c->cfg.divb == 1 && !(cmax_global > 0.0))
     return fail(AMRVAC_FV_EINVAL, "GLM needs cmax_global > 0");
   fvc::SweepArgs a = base_args(c);
-  a.src = c->slot[psa];
-  a.dst = c->slot[psb];
-  a.base = comb_slot >= 0 ? c->slot[comb_slot] : nullptr;
-  a.comb_mode = comb_slot < 0 ? 0 : (comb_b == 0.0 ? (comb_slot == psa ? 3 : 1) : 2);
+  a.src = src;
+  a.dst = dst;
+  a.base = base;
+  a.comb_mode = comb_mode;
   a.comb_a = comb_a;
   a.comb_b = comb_b;
   const double qdt = dtfactor * dt;
@@ -890,13 +871,13 @@ int amrvac_fv_advect1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
   if (e == hipErrorInvalidValue) return fail(AMRVAC_FV_EINVAL, "no stage kernel for this configuration");
   if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("stage kernel: ") + hipGetErrorString(e));
   if (c->timing) HIPCHK(hipEventRecord(e1, c->stream));
-  if (c->amr) {
-    // refluxing, then the ghost cells of psb (advect1: fix_conserve, getbc)
+  if (c->amr && dst_slot >= 0) {
+    // refluxing, then the ghost cells of dst (advect1: fix_conserve, getbc)
     if (riem != fv::RIEM_HANCOCK) {
-      const int rc = fix_conserve_amr(c, psb);
+      const int rc = fix_conserve_amr(c, dst_slot);
       if (rc != AMRVAC_FV_OK) return rc;
     }
-    const int rc = getbc_amr(c, psb);
+    const int rc = getbc_amr(c, dst_slot);
     if (rc != AMRVAC_FV_OK) return rc;
   }
   if (crash_out) {
@@ -906,6 +887,63 @@ int amrvac_fv_advect1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
     *crash_out = h;
   }
   return AMRVAC_FV_OK;
+}
+
+extern "C" {
+
+int amrvac_fv_download(amrvac_fv_ctx* c, int slot, double* w) {
+  if (!c || !w || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  if (c->nb > 65535) return fail(AMRVAC_FV_EINVAL, "download supports at most 65535 blocks");
+  fvc::SweepArgs a = base_args(c);
+  a.src = c->slot[slot];
+  const int blocks_x = (int)((c->cells_g + fvc::NTHREADS - 1) / fvc::NTHREADS);
+  dim3 grid(blocks_x, c->nb);
+  if (c->cfg.ndim == 2)
+    hipLaunchKernelGGL((fv::fill_ghosted_kernel<2>), grid, dim3(fvc::NTHREADS), 0, c->stream, a, c->nw,
+                       c->cfg.nghost, c->nxg[0], c->nxg[1], c->nxg[2], c->d_stage);
+  else
+    hipLaunchKernelGGL((fv::fill_ghosted_kernel<3>), grid, dim3(fvc::NTHREADS), 0, c->stream, a, c->nw,
+                       c->cfg.nghost, c->nxg[0], c->nxg[1], c->nxg[2], c->d_stage);
+  HIPCHK(hipGetLastError());
+  const size_t n = (size_t)c->nb * c->nw * c->cells_g;
+  HIPCHK(hipMemcpyAsync(w, c->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_getdt(amrvac_fv_ctx* c, int slot, double courantpar, double* dtmin_out, double* cmax_out) {
+  if (!c || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  return getdt_core(c, c->slot[slot], courantpar, dtmin_out, cmax_out);
+}
+
+int amrvac_fv_getdt_state(amrvac_fv_ctx* c, const double* u, double courantpar, double* dtmin_out,
+                          double* cmax_out) {
+  if (!c || !u) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  if (c->amr) return fail(AMRVAC_FV_EINVAL, "state arrays are for a uniform level");
+  return getdt_core(c, u, courantpar, dtmin_out, cmax_out);
+}
+
+int amrvac_fv_advect1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, double cmax_global, int psa,
+                      int psb, int comb_slot, double comb_a, double comb_b, int* crash_out) {
+  if (!c || !check_slot(psa) || !check_slot(psb) || psa == psb) return fail(AMRVAC_FV_EINVAL, "bad slots");
+  if (comb_slot >= 0 && !check_slot(comb_slot)) return fail(AMRVAC_FV_EINVAL, "bad combination slot");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  const int mode = comb_slot < 0 ? 0 : (comb_b == 0.0 ? (comb_slot == psa ? 3 : 1) : 2);
+  return stage_core(c, method, dtfactor, dt, cmax_global, c->slot[psa], c->slot[psb],
+                    comb_slot >= 0 ? c->slot[comb_slot] : nullptr, mode, comb_a, comb_b, psb, crash_out);
+}
+
+int amrvac_fv_stage(amrvac_fv_ctx* c, int method, double dtfactor, double dt, double cmax_global,
+                    const double* u_in, const double* base, double comb_a, double comb_b, double* u_out) {
+  if (!c || !u_in || !u_out) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  if (c->amr) return fail(AMRVAC_FV_EINVAL, "state arrays are for a uniform level");
+  if (u_out == u_in) return fail(AMRVAC_FV_EINVAL, "u_out must not alias u_in");
+  const int mode = base ? 2 : 3;
+  return stage_core(c, method, dtfactor, dt, cmax_global, u_in, u_out, base, mode, comb_a, comb_b, -1, nullptr);
 }
 
 int amrvac_fv_copy(amrvac_fv_ctx* c, int dst, int src) {
@@ -1084,6 +1122,49 @@ int amrvac_fv_getbc(amrvac_fv_ctx* c, int slot) {
 int amrvac_fv_set_fix_conserve(amrvac_fv_ctx* c, int on) {
   if (!c) return fail(AMRVAC_FV_EINVAL, "null context");
   c->fix_conserve = on != 0;
+  return AMRVAC_FV_OK;
+}
+
+
+int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, const double* u_in,
+                       const double* lam_out, double coef, const double* add_a, double ca, const double* add_b,
+                       double cb, double* lam_in) {
+  if (!c || !u_in || !lam_out || !lam_in) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  if (lam_in == lam_out || lam_in == u_in) return fail(AMRVAC_FV_EINVAL, "lam_in must not alias lam_out or u_in");
+  if (c->cfg.physics != 1) return fail(AMRVAC_FV_EINVAL, "the adjoint stage is provided for hd");
+  if (c->amr) return fail(AMRVAC_FV_EINVAL, "the adjoint stage needs a uniform level");
+  for (int sd = 0; sd < 2 * c->cfg.ndim; ++sd)
+    if (c->bc[sd] != 5) return fail(AMRVAC_FV_EINVAL, "the adjoint stage needs periodic boundaries");
+  if (c->cfg.boundspeed != 1) return fail(AMRVAC_FV_EINVAL, "the adjoint stage needs Einfeldt bounds");
+  if (c->cfg.limiter != 1 && c->cfg.limiter != 5) return fail(AMRVAC_FV_EINVAL, "the adjoint needs minmod or vanleer");
+  int riem;
+  if (method_to_riem(method, riem) != 0 || riem == fv::RIEM_HANCOCK)
+    return fail(AMRVAC_FV_EINVAL, "adjoint: flux scheme must be hll, hllc or tvdlf");
+  if (c->cfg.ndim == 3 && !c->d_adjG) {
+    const size_t n = (size_t)4 * c->nw * c->nb * c->cells;
+    if (hipMalloc(&c->d_adjG, n * sizeof(double)) != hipSuccess) return fail(AMRVAC_FV_ENOMEM, "adjoint buffer");
+  }
+  fvc::SweepArgs a = base_args(c);
+  a.src = u_in;
+  const double qdt = dtfactor * dt;
+  a.qdt = qdt;
+  for (int d = 0; d < 3; ++d) a.dq[d] = -qdt / c->dx[d];
+  fvc::AdjArgs g{lam_out, coef, add_a, ca, add_b, cb, lam_in, c->d_adjG};
+  const int mixed = c->cfg.precision == AMRVAC_FV_PREC_MIXED ? 1 : 0;
+  hipError_t e = fv::launch_adjoint_stage(c->cfg.ndim, mixed, riem, c->cfg.limiter, a, c->fp, g, c->stream);
+  if (e == hipErrorInvalidValue) return fail(AMRVAC_FV_EINVAL, "no adjoint kernel for this configuration");
+  if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("adjoint kernels: ") + hipGetErrorString(e));
+  return AMRVAC_FV_OK;
+}
+
+
+int amrvac_fv_slot_copy(amrvac_fv_ctx* c, int slot, void* dptr, int to_slot) {
+  if (!c || !check_slot(slot) || !dptr) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  const size_t n = (size_t)c->nb * c->nw * c->cells * sizeof(double);
+  if (to_slot) HIPCHK(hipMemcpyAsync(c->slot[slot], dptr, n, hipMemcpyDeviceToDevice, c->stream));
+  else HIPCHK(hipMemcpyAsync(dptr, c->slot[slot], n, hipMemcpyDeviceToDevice, c->stream));
   return AMRVAC_FV_OK;
 }
 

@@ -16,6 +16,21 @@ struct KernelKey {
   int limiter; // runtime limiter code (TVD path)
 };
 
+// Adjoint stage operands (device arrays in the interior-only block layout
+// [nb][nw][bnz][16][16]):
+//   lam_in = coef * lam_out + ca * add_a + cb * add_b + (d stage / d src)^T lam_out
+// add_a / add_b may be null; gz is the 3-D z-face buffer, 4*nw*nb*cells doubles.
+struct AdjArgs {
+  const double* lam_out;
+  double coef;
+  const double* add_a;
+  double ca;
+  const double* add_b;
+  double cb;
+  double* lam_in;
+  double* gz;
+};
+
 // Returns hipSuccess on launch, hipErrorInvalidValue when no kernel matches.
 using SweepLauncher = hipError_t (*)(const KernelKey&, const SweepArgs&, const FVParams&, int grid,
                                      hipStream_t);
@@ -36,6 +51,12 @@ using SweepLauncher = hipError_t (*)(const KernelKey&, const SweepArgs&, const F
   }
 FV_DECLARE_LAUNCHERS(fv)
 FV_DECLARE_LAUNCHERS(fvx)
+
+namespace fv {
+// adjoint stage (hydrodynamics, 2-D / 3-D, FP64 or FP32 faces)
+hipError_t launch_adjoint_stage(int ndim, int mixed, int riem, int lim, const fvc::SweepArgs& a,
+                                const fvc::FVParams& fp, const fvc::AdjArgs& g, hipStream_t s);
+}
 
 namespace FV_NS {
 using fvc::KernelKey;

@@ -26,7 +26,8 @@ EXPORTS = ("amrvac_fv_init", "amrvac_fv_finalize", "amrvac_fv_set_stream", "amrv
            "amrvac_fv_upload", "amrvac_fv_download", "amrvac_fv_getdt", "amrvac_fv_advect1",
            "amrvac_fv_copy", "amrvac_fv_volume_average", "amrvac_fv_slot_pointer",
            "amrvac_fv_sync", "amrvac_fv_sweep_timing", "amrvac_fv_last_error",
-           "amrvac_fv_set_mesh_amr", "amrvac_fv_getbc", "amrvac_fv_set_fix_conserve")
+           "amrvac_fv_set_mesh_amr", "amrvac_fv_getbc", "amrvac_fv_set_fix_conserve", "amrvac_fv_adjoint1",
+           "amrvac_fv_slot_copy", "amrvac_fv_stage", "amrvac_fv_getdt_state")
 
 
 class FVConfig(ctypes.Structure):
@@ -72,6 +73,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.amrvac_fv_set_mesh_amr.argtypes = [vp, ctypes.c_int, dp, ip, ip, ip, ip, ip, dp, ip]
     L.amrvac_fv_getbc.argtypes = [vp, ctypes.c_int]
     L.amrvac_fv_set_fix_conserve.argtypes = [vp, ctypes.c_int]
+    dbl = ctypes.c_double
+    L.amrvac_fv_adjoint1.argtypes = [vp, ctypes.c_int, dbl, dbl, vp, vp, dbl, vp, dbl, vp, dbl, vp]
+    L.amrvac_fv_stage.argtypes = [vp, ctypes.c_int, dbl, dbl, dbl, vp, vp, dbl, dbl, vp]
+    L.amrvac_fv_getdt_state.argtypes = [vp, vp, dbl, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+    L.amrvac_fv_slot_copy.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
     _lib = L
     return L
 
@@ -173,6 +179,41 @@ class DeviceFV:
         every sweep and materialised by get_state (nothing to do); on an AMR
         mesh the ghost cells across level jumps are filled on the device."""
         self._check(self.L.amrvac_fv_getbc(self.h, slot))
+
+    # -- device arrays (zero-copy plumbing for PyTorch) ------------------------
+    def set_stream(self, stream_handle: int | None) -> None:
+        """Order all work of this context on the given hipStream_t (e.g.
+        torch.cuda.current_stream().cuda_stream); None restores its own."""
+        self._check(self.L.amrvac_fv_set_stream(self.h, ctypes.c_void_p(stream_handle) if stream_handle else None))
+
+    def slot_to(self, slot: int, dptr: int) -> None:
+        self._check(self.L.amrvac_fv_slot_copy(self.h, slot, ctypes.c_void_p(dptr), 0))
+
+    def slot_from(self, slot: int, dptr: int) -> None:
+        self._check(self.L.amrvac_fv_slot_copy(self.h, slot, ctypes.c_void_p(dptr), 1))
+
+    def adjoint1(self, method: int, dtfactor: float, dt: float, u_in: int, lam_out: int, coef: float,
+                 lam_in: int, add_a: int = 0, ca: float = 0.0, add_b: int = 0, cb: float = 0.0) -> None:
+        """lam_in = coef*lam_out + ca*add_a + cb*add_b + (d stage/d u_in)^T lam_out
+        (device pointers; 0 for an absent add term)."""
+        v = ctypes.c_void_p
+        self._check(self.L.amrvac_fv_adjoint1(self.h, method, dtfactor, dt, v(u_in), v(lam_out), coef,
+                                              v(add_a) if add_a else None, ca, v(add_b) if add_b else None, cb,
+                                              v(lam_in)))
+
+    def stage(self, method: int, dtfactor: float, dt: float, cmax_global: float, u_in: int, u_out: int,
+              base: int = 0, comb_a: float = 0.0, comb_b: float = 0.0) -> None:
+        """u_out = (comb_a*base + comb_b*u_in, or u_in) + qdt L(u_in) on device pointers."""
+        v = ctypes.c_void_p
+        self._check(self.L.amrvac_fv_stage(self.h, method, dtfactor, dt, cmax_global, v(u_in),
+                                           v(base) if base else None, comb_a, comb_b, v(u_out)))
+
+    def getdt_state(self, u: int, courantpar: float):
+        dt = ctypes.c_double()
+        cm = ctypes.c_double()
+        self._check(self.L.amrvac_fv_getdt_state(self.h, ctypes.c_void_p(u), courantpar, ctypes.byref(dt),
+                                                 ctypes.byref(cm)))
+        return dt.value, cm.value
 
     def set_fix_conserve(self, on: bool) -> None:
         self._check(self.L.amrvac_fv_set_fix_conserve(self.h, 1 if on else 0))

@@ -145,6 +145,34 @@ int amrvac_fv_advect1(amrvac_fv_ctx* ctx, int method, double dtfactor,
                       int comb_slot, double comb_a, double comb_b,
                       int* crash_out);
 
+/* State-array forms for a uniform level, for drivers that keep their states
+   in their own device memory (arrays in the slot layout
+   [nb][nw][bnz][16][16], ordered on the context's stream):
+     amrvac_fv_stage: u_out = init + qdt L(u_in) as advect1 with
+       init = comb_a*base + comb_b*u_in (base != NULL) or u_in (base == NULL);
+       u_out must not alias u_in (base may alias u_out).
+     amrvac_fv_getdt_state: amrvac_fv_getdt of the state u. */
+int amrvac_fv_stage(amrvac_fv_ctx* ctx, int method, double dtfactor, double dt,
+                    double cmax_global, const double* u_in, const double* base,
+                    double comb_a, double comb_b, double* u_out);
+int amrvac_fv_getdt_state(amrvac_fv_ctx* ctx, const double* u, double courantpar,
+                          double* dtmin_out, double* cmax_out);
+
+/* Adjoint of one stage (no counterpart in the reference, which is not
+   differentiable; the reverse of finite_volume's flux divergence,
+   src/mod_finite_volume.t:116-339, for hd on a uniform periodic level with
+   Einfeldt bounds, minmod / vanleer and hll / hllc / tvdlf):
+     lam_in = coef*lam_out + ca*add_a + cb*add_b
+              + (d (qdt L(u)) / d u)^T lam_out  at u = u_in,
+   qdt = dtfactor*dt; add_a / add_b may be NULL.  All arrays are device
+   arrays in the slot layout; lam_in must not alias lam_out or u_in (it may
+   alias add_a or add_b).  Limiter and Riemann branches are differentiated
+   along the branch the forward state takes; the result is deterministic. */
+int amrvac_fv_adjoint1(amrvac_fv_ctx* ctx, int method, double dtfactor, double dt,
+                       const double* u_in, const double* lam_out, double coef,
+                       const double* add_a, double ca, const double* add_b,
+                       double cb, double* lam_in);
+
 /* ps[dst] = ps[src] (interior cells) */
 int amrvac_fv_copy(amrvac_fv_ctx* ctx, int dst, int src);
 
@@ -157,6 +185,12 @@ int amrvac_fv_volume_average(amrvac_fv_ctx* ctx, int slot, double* modes);
    stream. */
 int amrvac_fv_slot_pointer(amrvac_fv_ctx* ctx, int slot, void** dptr,
                            long long* nbytes);
+
+/* Device-to-device copy between a slot (interior-only layout) and a device
+   array of the same size: to_slot != 0 copies dptr into the slot, else the
+   slot into dptr; ordered on the context's stream.  For checkpoints and
+   adjoint drivers that keep states in their own device memory. */
+int amrvac_fv_slot_copy(amrvac_fv_ctx* ctx, int slot, void* dptr, int to_slot);
 
 /* Wait for the context's stream. */
 int amrvac_fv_sync(amrvac_fv_ctx* ctx);
