@@ -210,3 +210,55 @@ def kh_fields(mesh: UniformMesh, seed: int, amp: float = 1.0e-3, device=None, rh
     p = np.full_like(rho, 2.5)
     t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=device)
     return t(rho), t(vx), t(vy), t(p)
+
+
+def mhd_conserved(rho: torch.Tensor, v: list[torch.Tensor], p: torch.Tensor, b: list[torch.Tensor],
+                  gamma: float) -> torch.Tensor:
+    """mhd_to_conserved on fields (no B0): [rho, m_1..3, e, b_1..3]."""
+    vsq = v[0] * v[0]
+    for c in v[1:]:
+        vsq = vsq + c * c
+    bsq = b[0] * b[0]
+    for c in b[1:]:
+        bsq = bsq + c * c
+    e = p * (1.0 / (gamma - 1.0)) + 0.5 * (vsq * rho + bsq)
+    return torch.stack([rho] + [rho * c for c in v] + [e] + list(b))
+
+
+def orszag_tang_fields(mesh: UniformMesh, seed: int, rho_amp: float = 1.0e-2, vz_amp: float = 1.0e-3, device=None,
+                       field_amp: float = 0.0):
+    """3-D Orszag-Tang fields on the global grid [nz, ny, nx]: rho =
+    25/(36 pi) (1 + rho_amp r), p = 5/(12 pi), v = (-sin 2 pi y, sin 2 pi x,
+    vz_amp r'), and the unit-amplitude field shape (-sin 2 pi y, sin 4 pi x,
+    0)/sqrt(4 pi) that the scalar B amplitude multiplies; r, r' seeded
+    uniform noise in [-1, 1) (hash of the global cell index).  field_amp > 0
+    adds seeded noise of that amplitude to vx, vy, p (relative) and to the
+    field shape (in units of 1/sqrt(4 pi)): the plain fields are constant
+    along some axes, where every limiter slope sits on its kink."""
+    from .problems import hashed_uniform
+    nx, ny, nz = mesh.domain_nx
+    dx = [(mesh.xprobmax[d] - mesh.xprobmin[d]) / mesh.domain_nx[d] for d in range(3)]
+    x = mesh.xprobmin[0] + (np.arange(nx) + 0.5) * dx[0]
+    y = mesh.xprobmin[1] + (np.arange(ny) + 0.5) * dx[1]
+    Z, Y, X = np.meshgrid(np.arange(nz), y, x, indexing="ij")
+    gz, gy, gx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    gi = np.stack([gx, gy, gz])
+    rho = 25.0 / (36.0 * math.pi) * (1.0 + rho_amp * hashed_uniform(seed, 1, gi, 0))
+    vx = -np.sin(2.0 * math.pi * Y)
+    vy = np.sin(2.0 * math.pi * X)
+    vz = vz_amp * hashed_uniform(seed, 1, gi, 3)
+    p = np.full_like(rho, 5.0 / (12.0 * math.pi))
+    s4 = 1.0 / math.sqrt(4.0 * math.pi)
+    bx = -s4 * np.sin(2.0 * math.pi * Y)
+    by = s4 * np.sin(4.0 * math.pi * X)
+    bz = np.zeros_like(rho)
+    if field_amp:
+        vx = vx + field_amp * hashed_uniform(seed, 1, gi, 1)
+        vy = vy + field_amp * hashed_uniform(seed, 1, gi, 2)
+        p = p * (1.0 + field_amp * hashed_uniform(seed, 1, gi, 4))
+        bx = bx + s4 * field_amp * hashed_uniform(seed, 1, gi, 5)
+        by = by + s4 * field_amp * hashed_uniform(seed, 1, gi, 6)
+        bz = bz + s4 * field_amp * hashed_uniform(seed, 1, gi, 7)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=device)
+    return t(rho), [t(vx), t(vy), t(vz)], t(p), [t(bx), t(by), t(bz)]
+

@@ -1,5 +1,6 @@
 // fv_adj.hip -- adjoint of one finite-volume stage on a uniform periodic
-// level (the reverse of the stage kernel's flux divergence):
+// level, hydrodynamics or ideal MHD without divergence cleaning (the reverse
+// of the stage kernel's flux divergence):
 //   lam_in = coef * lam_out + ca * add_a + cb * add_b
 //            + sum_faces (d fC_f / d u_in)^T mu_f,
 //   mu_f = (-qdt/dx_d) (lam_out(left of f) - lam_out(right of f)).
@@ -72,7 +73,8 @@ template <class P, class R, int RIEM, int LIM>
 __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVParams fp, AdjArgs g) {
   constexpr int NW = P::NW, NDIM = P::NDIM;
   __shared__ R Wl[NW][AW * AW];
-  __shared__ double Gl[4 * NW][NFACE];
+  // face contributions: computed in R, so stored in R without rounding
+  __shared__ R Gl[4 * NW][NFACE];
   const int b = blockIdx.x / a.bnz, k = blockIdx.x % a.bnz;
   if (b >= a.nblocks) return;
   const int t = threadIdx.x;
@@ -119,7 +121,7 @@ __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVPara
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) Gl[s * NW + iw][j] = wb[s][iw];
+      for (int iw = 0; iw < NW; ++iw) Gl[s * NW + iw][j] = R(wb[s][iw]);
   }
   __syncthreads();
   // cell tx is stencil position s = tx - f + 2 of faces f = tx+2 .. tx-1
@@ -127,7 +129,7 @@ __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVPara
   for (int s = 0; s < 4; ++s) {
     const int j = ty * NF + (tx - s + 2) + 1;
 #pragma unroll
-    for (int iw = 0; iw < NW; ++iw) wsum[iw] += Gl[s * NW + iw][j];
+    for (int iw = 0; iw < NW; ++iw) wsum[iw] += double(Gl[s * NW + iw][j]);
   }
   __syncthreads();
 
@@ -150,14 +152,14 @@ __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVPara
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) Gl[s * NW + iw][j] = wb[s][iw];
+      for (int iw = 0; iw < NW; ++iw) Gl[s * NW + iw][j] = R(wb[s][iw]);
   }
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int j = (ty - s + 2 + 1) * BX + tx;
 #pragma unroll
-    for (int iw = 0; iw < NW; ++iw) wsum[iw] += Gl[s * NW + iw][j];
+    for (int iw = 0; iw < NW; ++iw) wsum[iw] += double(Gl[s * NW + iw][j]);
   }
 
   // ---- z faces (3-D): reversed by adj_zface_kernel ------------------------
@@ -200,8 +202,10 @@ hipError_t adj_riem(int riem, int lim, const SweepArgs& a, const FVParams& fp, c
     case RIEM_HLL:
       return vl ? launch_adj<P, R, RIEM_HLL, LIM_VANLEER>(a, fp, g, s) : launch_adj<P, R, RIEM_HLL, LIM_MINMOD>(a, fp, g, s);
     case RIEM_HLLC:
-      return vl ? launch_adj<P, R, RIEM_HLLC, LIM_VANLEER>(a, fp, g, s)
-                : launch_adj<P, R, RIEM_HLLC, LIM_MINMOD>(a, fp, g, s);
+      if constexpr (P::MHD) return hipErrorInvalidValue;
+      else
+        return vl ? launch_adj<P, R, RIEM_HLLC, LIM_VANLEER>(a, fp, g, s)
+                  : launch_adj<P, R, RIEM_HLLC, LIM_MINMOD>(a, fp, g, s);
     case RIEM_TVDLF:
       return vl ? launch_adj<P, R, RIEM_TVDLF, LIM_VANLEER>(a, fp, g, s)
                 : launch_adj<P, R, RIEM_TVDLF, LIM_MINMOD>(a, fp, g, s);
@@ -210,15 +214,23 @@ hipError_t adj_riem(int riem, int lim, const SweepArgs& a, const FVParams& fp, c
   }
 }
 
-hipError_t launch_adjoint_stage(int ndim, int mixed, int riem, int lim, const SweepArgs& a, const FVParams& fp,
-                                const AdjArgs& g, hipStream_t s) {
-  if (ndim == 2)
-    return mixed ? adj_riem<Phys<2, 2, false, DIVB_NONE>, float>(riem, lim, a, fp, g, s)
-                 : adj_riem<Phys<2, 2, false, DIVB_NONE>, double>(riem, lim, a, fp, g, s);
-  if (ndim == 3) {
-    if (!g.gz) return hipErrorInvalidValue;
-    return mixed ? adj_riem<Phys<3, 3, false, DIVB_NONE>, float>(riem, lim, a, fp, g, s)
-                 : adj_riem<Phys<3, 3, false, DIVB_NONE>, double>(riem, lim, a, fp, g, s);
+template <class P>
+hipError_t adj_prec(int mixed, int riem, int lim, const SweepArgs& a, const FVParams& fp, const AdjArgs& g,
+                    hipStream_t s) {
+  if (P::NDIM == 3 && !g.gz) return hipErrorInvalidValue;
+  return mixed ? adj_riem<P, float>(riem, lim, a, fp, g, s) : adj_riem<P, double>(riem, lim, a, fp, g, s);
+}
+
+// hydrodynamics (physics 1) and ideal MHD without divergence cleaning
+// (physics 2, ndir 3), 2-D / 3-D
+hipError_t launch_adjoint_stage(int ndim, int physics, int mixed, int riem, int lim, const SweepArgs& a,
+                                const FVParams& fp, const AdjArgs& g, hipStream_t s) {
+  if (physics == 1) {
+    if (ndim == 2) return adj_prec<Phys<2, 2, false, DIVB_NONE>>(mixed, riem, lim, a, fp, g, s);
+    if (ndim == 3) return adj_prec<Phys<3, 3, false, DIVB_NONE>>(mixed, riem, lim, a, fp, g, s);
+  } else if (physics == 2) {
+    if (ndim == 2) return adj_prec<Phys<2, 3, true, DIVB_NONE>>(mixed, riem, lim, a, fp, g, s);
+    if (ndim == 3) return adj_prec<Phys<3, 3, true, DIVB_NONE>>(mixed, riem, lim, a, fp, g, s);
   }
   return hipErrorInvalidValue;
 }

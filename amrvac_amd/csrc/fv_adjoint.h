@@ -42,8 +42,8 @@ __device__ __forceinline__ R lim_grad(R a, R b, R& da, R& db) {
 // the face's four cells (as the sweep rounds them to R) and the flux adjoint
 // fbar, writes the primitive adjoints wb[c] = (d fC / d w_c)^T fbar.
 template <class P, class R, int RIEM, int ID, int LIM>
-__device__ void face_adj_prim(const R (*w)[P::NW], const FVParams& fp, const R* fbar, double (*wb)[P::NW]) {
-  static_assert(!P::MHD, "face_adj_prim: hydrodynamics only");
+__device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const R* fbar, double (*wb)[P::NW]) {
+  static_assert(!P::MHD, "hydrodynamics");
   constexpr int NW = P::NW, ND = P::NDIR, RHO = P::RHO, E = P::E, MN = P::MOM + ID;
   const R gam = R(fp.gamma), inv_g1 = R(fp.inv_gamma_1);
   // ---------------- forward ----------------
@@ -354,6 +354,253 @@ __device__ void face_adj_prim(const R (*w)[P::NW], const FVParams& fp, const R* 
   }
 }
 
+// Ideal MHD without divergence cleaning sources in the face (typedivbfix
+// 'none' or 'linde'): TVD reconstruction -> mhd_to_conserved ->
+// mhd_get_flux (src/mhd/mod_mhd_phys.t:3810-3889) -> Einfeldt bounds with
+// the fast speed (mhd_get_cbounds :3037-3127, mhd_get_csound_prim
+// :3329-3379) -> HLL or TVDLF; the normal field takes Lax-Friedrichs
+// dissipation under HLL (flux_type, :580-588).
+template <class P, class R, int RIEM, int ID, int LIM>
+__device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const R* fbar, double (*wb)[P::NW]) {
+  static_assert(P::MHD && !P::GLM, "ideal MHD without GLM");
+  static_assert(RIEM == RIEM_HLL || RIEM == RIEM_TVDLF, "MHD faces take hll or tvdlf");
+  constexpr int NW = P::NW, ND = P::NDIR, RHO = P::RHO, E = P::E, MN = P::MOM + ID, BN = P::MAG + ID;
+  constexpr int BTV = (RIEM == RIEM_HLL && P::NDIM > 1) ? BN : -1;
+  const R gam = R(fp.gamma), inv_g1 = R(fp.inv_gamma_1), eps = R(fp.tvdlfeps);
+  // ---------------- forward ----------------
+  R wL[NW], wR[NW], d1a[NW], d1b[NW], d2a[NW], d2b[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    const R dm = w[1][iw] - w[0][iw], d0 = w[2][iw] - w[1][iw], dp = w[3][iw] - w[2][iw];
+    const R t1 = lim_grad<R, LIM>(dm, d0, d1a[iw], d1b[iw]);
+    const R t2 = lim_grad<R, LIM>(d0, dp, d2a[iw], d2b[iw]);
+    wL[iw] = w[1][iw] + R(0.5) * t1;
+    wR[iw] = w[2][iw] - R(0.5) * t2;
+  }
+  R wLC[NW], wRC[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) { wLC[iw] = wL[iw]; wRC[iw] = wR[iw]; }
+  to_cons<P, R>(wLC, inv_g1);
+  to_cons<P, R>(wRC, inv_g1);
+  R fL[NW], fR[NW];
+  phys_flux<P, R, ID>(wLC, wL, R(0), fL);
+  phys_flux<P, R, ID>(wRC, wR, R(0), fR);
+  // fast speeds, kept with their intermediates for the reverse pass
+  struct Fast { R ir, cs, b2, cf2, av, sq, c; };
+  auto fast = [&](const R* x) {
+    Fast q;
+    q.ir = R(1) / x[RHO];
+    q.cs = gam * x[E] * q.ir;
+    R b2 = x[P::MAG] * x[P::MAG];
+#pragma unroll
+    for (int k = 1; k < ND; ++k) b2 = b2 + x[P::MAG + k] * x[P::MAG + k];
+    q.b2 = b2;
+    q.cf2 = b2 * q.ir + q.cs;
+    q.av = q.cf2 * q.cf2 - R(4) * q.cs * (x[BN] * x[BN]) * q.ir;
+    q.sq = q.av > R(0) ? sqrt(q.av) : R(0);
+    q.c = sqrt(R(0.5) * (q.cf2 + q.sq));
+    return q;
+  };
+  const Fast qL = fast(wL), qR = fast(wR);
+  const R t1 = sqrt(wL[RHO]), t2 = sqrt(wR[RHO]);
+  const R t3 = R(1) / (t1 + t2);
+  const R um = (wL[MN] * t1 + wR[MN] * t2) * t3;
+  const R dv = wR[MN] - wL[MN];
+  const R A = t1 * (qL.c * qL.c) + t2 * (qR.c * qR.c);
+  const R D2 = A * t3 + R(0.5) * t1 * t2 * (t3 * t3) * (dv * dv);
+  const R dmean = sqrt(D2);
+  R cmax, cmin;
+  if constexpr (RIEM == RIEM_TVDLF) {
+    cmax = fabs(um) + dmean;
+    cmin = R(0);
+  } else {
+    cmin = um - dmean;
+    cmax = um + dmean;
+  }
+  // ---------------- reverse ----------------
+  R fLb[NW], fRb[NW], wLCb[NW], wRCb[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) { fLb[iw] = R(0); fRb[iw] = R(0); wLCb[iw] = R(0); wRCb[iw] = R(0); }
+  R cmaxb = R(0), cminb = R(0);
+  if constexpr (RIEM == RIEM_TVDLF) {
+    const R fac = R(-0.5) * eps * cmax;
+    R s = R(0);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      fLb[iw] = R(0.5) * fbar[iw];
+      fRb[iw] = R(0.5) * fbar[iw];
+      wRCb[iw] = fac * fbar[iw];
+      wLCb[iw] = -fac * fbar[iw];
+      s = s + fbar[iw] * (wRC[iw] - wLC[iw]);
+    }
+    cmaxb = R(-0.5) * eps * s;
+  } else {
+    const bool mid = !(cmin >= R(0)) && !(cmax <= R(0));
+    const R iD = mid ? R(1) / (cmax - cmin) : R(0);
+    R Db = R(0);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      if (iw == BTV) {
+        // fC = -eps/2 max(cmax, |cmin|) (wRC - wLC)
+        const bool use_max = cmax >= fabs(cmin);
+        const R M = use_max ? cmax : fabs(cmin);
+        const R dw = wRC[iw] - wLC[iw];
+        wRCb[iw] = R(-0.5) * eps * M * fbar[iw];
+        wLCb[iw] = R(0.5) * eps * M * fbar[iw];
+        const R Mb = R(-0.5) * eps * dw * fbar[iw];
+        if (use_max) cmaxb = cmaxb + Mb;
+        else cminb = cminb + copysign(R(1), cmin) * Mb;
+      } else if (cmin >= R(0)) {
+        fLb[iw] = fbar[iw];
+      } else if (cmax <= R(0)) {
+        fRb[iw] = fbar[iw];
+      } else {
+        const R dw = wRC[iw] - wLC[iw];
+        const R f = (cmax * fL[iw] - cmin * fR[iw] + cmin * cmax * dw) * iD;
+        const R Nb = fbar[iw] * iD;
+        Db = Db - fbar[iw] * f * iD;
+        fLb[iw] = Nb * cmax;
+        fRb[iw] = -Nb * cmin;
+        wRCb[iw] = Nb * cmin * cmax;
+        wLCb[iw] = -Nb * cmin * cmax;
+        cmaxb = cmaxb + Nb * (fL[iw] + cmin * dw);
+        cminb = cminb + Nb * (-fR[iw] + cmax * dw);
+      }
+    }
+    cmaxb = cmaxb + Db;
+    cminb = cminb - Db;
+  }
+  R wLb[NW], wRb[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) { wLb[iw] = R(0); wRb[iw] = R(0); }
+  // bounds -> face primitive states
+  {
+    R umb, dmb;
+    if constexpr (RIEM == RIEM_TVDLF) {
+      umb = copysign(R(1), um) * cmaxb;
+      dmb = cmaxb;
+    } else {
+      umb = cmaxb + cminb;
+      dmb = cmaxb - cminb;
+    }
+    const R D2b = dmb * R(0.5) / dmean;
+    const R Ab = D2b * t3;
+    R t3b = D2b * A;
+    R t1b = D2b * R(0.5) * t2 * (t3 * t3) * (dv * dv);
+    R t2b = D2b * R(0.5) * t1 * (t3 * t3) * (dv * dv);
+    t3b = t3b + D2b * t1 * t2 * t3 * (dv * dv);
+    const R dvb = D2b * t1 * t2 * (t3 * t3) * dv;
+    t1b = t1b + Ab * (qL.c * qL.c);
+    t2b = t2b + Ab * (qR.c * qR.c);
+    const R cLb = R(2) * qL.c * Ab * t1, cRb = R(2) * qR.c * Ab * t2;
+    wRb[MN] = wRb[MN] + dvb;
+    wLb[MN] = wLb[MN] - dvb;
+    wLb[MN] = wLb[MN] + umb * t1 * t3;
+    wRb[MN] = wRb[MN] + umb * t2 * t3;
+    t1b = t1b + umb * wL[MN] * t3;
+    t2b = t2b + umb * wR[MN] * t3;
+    t3b = t3b + umb * (wL[MN] * t1 + wR[MN] * t2);
+    t1b = t1b - t3b * t3 * t3;
+    t2b = t2b - t3b * t3 * t3;
+    wLb[RHO] = wLb[RHO] + t1b * R(0.5) / t1;
+    wRb[RHO] = wRb[RHO] + t2b * R(0.5) / t2;
+    // fast speed reverse: c = sqrt((cf2 + sqrt(av)) / 2)
+    auto fast_rev = [&](const R* x, const Fast& q, R cb, R* xb) {
+      const R hb = cb * R(0.5) / q.c;
+      R cf2b = R(0.5) * hb;
+      const R sqb = R(0.5) * hb;
+      const R avb = q.av > R(0) ? sqb * R(0.5) / q.sq : R(0);
+      const R bn = x[BN];
+      cf2b = cf2b + R(2) * q.cf2 * avb;
+      R csb = R(-4) * (bn * bn) * q.ir * avb;
+      R irb = R(-4) * q.cs * (bn * bn) * avb;
+      const R bnb = R(-8) * q.cs * bn * q.ir * avb;
+      const R b2b = cf2b * q.ir;
+      irb = irb + cf2b * q.b2;
+      csb = csb + cf2b;
+      xb[E] = xb[E] + csb * gam * q.ir;
+      irb = irb + csb * gam * x[E];
+#pragma unroll
+      for (int k = 0; k < ND; ++k) xb[P::MAG + k] = xb[P::MAG + k] + R(2) * x[P::MAG + k] * b2b;
+      xb[BN] = xb[BN] + bnb;
+      xb[RHO] = xb[RHO] - irb * q.ir * q.ir;
+    };
+    fast_rev(wL, qL, cLb, wLb);
+    fast_rev(wR, qR, cRb, wRb);
+  }
+  // fluxes (mhd_get_flux without B0 / GLM)
+  auto flux_rev = [&](const R* x, const R* xC, const R* fb, R* xb, R* xCb) {
+    const R vn = x[MN], bn = x[BN];
+    R bsq = R(0), bv = R(0);
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+      bsq = bsq + x[P::MAG + k] * x[P::MAG + k];
+      bv = bv + x[P::MAG + k] * x[P::MOM + k];
+    }
+    const R ptot = x[E] + R(0.5) * bsq;
+    R vnb = fb[RHO] * x[RHO] + fb[E] * (xC[E] + ptot);
+    R bnb = -fb[E] * bv;
+    R ptotb = fb[MN] + fb[E] * vn;
+    const R bvb = -fb[E] * bn;
+    xb[RHO] = xb[RHO] + fb[RHO] * vn;
+    xCb[E] = xCb[E] + fb[E] * vn;
+    R cmnb = R(0);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const int m = P::MOM + d, b = P::MAG + d;
+      // f[m] = xC[MN] v_d - bn B_d
+      cmnb = cmnb + fb[m] * x[m];
+      xb[m] = xb[m] + fb[m] * xC[MN] + bvb * x[b] - fb[b] * bn;
+      bnb = bnb - fb[m] * x[b] - fb[b] * x[m];
+      // f[b] = vn B_d - bn v_d; bv; ptot
+      vnb = vnb + fb[b] * x[b];
+      xb[b] = xb[b] - fb[m] * bn + bvb * x[m] + fb[b] * vn + ptotb * x[b];
+    }
+    xCb[MN] = xCb[MN] + cmnb;
+    xb[E] = xb[E] + ptotb;
+    xb[MN] = xb[MN] + vnb;
+    xb[BN] = xb[BN] + bnb;
+  };
+  flux_rev(wL, wLC, fLb, wLb, wLCb);
+  flux_rev(wR, wRC, fRb, wRb, wRCb);
+  // to_cons: E = p inv_g1 + (vsq rho + bsq)/2, m_k = rho v_k
+  auto cons_rev = [&](const R* x, const R* xCb, R* xb) {
+    R vsq = R(0);
+#pragma unroll
+    for (int k = 0; k < ND; ++k) vsq = vsq + x[P::MOM + k] * x[P::MOM + k];
+    xb[RHO] = xb[RHO] + xCb[RHO] + R(0.5) * vsq * xCb[E];
+    xb[E] = xb[E] + inv_g1 * xCb[E];
+    const R vsqb = R(0.5) * x[RHO] * xCb[E];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+      const int m = P::MOM + k, b = P::MAG + k;
+      xb[RHO] = xb[RHO] + xCb[m] * x[m];
+      xb[m] = xb[m] + R(2) * x[m] * vsqb + x[RHO] * xCb[m];
+      xb[b] = xb[b] + xCb[b] + x[b] * xCb[E];
+    }
+  };
+  cons_rev(wL, wLCb, wLb);
+  cons_rev(wR, wRCb, wRb);
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    const R t1b = R(0.5) * wLb[iw], t2b = R(-0.5) * wRb[iw];
+    const R dmb = t1b * d1a[iw];
+    const R d0b = t1b * d1b[iw] + t2b * d2a[iw];
+    const R dpb = t2b * d2b[iw];
+    wb[0][iw] = double(-dmb);
+    wb[1][iw] = double(wLb[iw] + dmb - d0b);
+    wb[2][iw] = double(wRb[iw] + d0b - dpb);
+    wb[3][iw] = double(dpb);
+  }
+}
+
+template <class P, class R, int RIEM, int ID, int LIM>
+__device__ __forceinline__ void face_adj_prim(const R (*w)[P::NW], const FVParams& fp, const R* fbar,
+                                              double (*wb)[P::NW]) {
+  if constexpr (P::MHD) face_adj_prim_mhd<P, R, RIEM, ID, LIM>(w, fp, fbar, wb);
+  else face_adj_prim_hd<P, R, RIEM, ID, LIM>(w, fp, fbar, wb);
+}
+
 // Reverse of hd_to_primitive (FP64) for one cell: cbar += (d w / d cons)^T wb
 // with rho = rho, v_k = m_k / rho, p = (gamma-1)(e - rho vsq / 2).
 template <class P>
@@ -371,6 +618,10 @@ __device__ __forceinline__ void prim_rev(const double* cons, const double* wb, c
   const double pb = wb[E];
   cbar[E] += g1 * pb;
   const double tt = -0.5 * g1 * pb;
+  if constexpr (P::MHD) {  // p = (gamma-1)(e - (rho vsq + bsq)/2)
+#pragma unroll
+    for (int k = 0; k < ND; ++k) cbar[P::MAG + k] += wb[P::MAG + k] + 2.0 * tt * cons[P::MAG + k];
+  }
   double rhob = wb[RHO] + tt * vsq;
   const double vsqb = tt * rho;
   double invb = 0.0;

@@ -1132,7 +1132,9 @@ int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt,
   if (!c || !u_in || !lam_out || !lam_in) return fail(AMRVAC_FV_EINVAL, "bad argument");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
   if (lam_in == lam_out || lam_in == u_in) return fail(AMRVAC_FV_EINVAL, "lam_in must not alias lam_out or u_in");
-  if (c->cfg.physics != 1) return fail(AMRVAC_FV_EINVAL, "the adjoint stage is provided for hd");
+  if (c->cfg.physics == 2 && (c->cfg.divb != 0 || c->cfg.ndir != 3))
+    return fail(AMRVAC_FV_EINVAL, "the MHD adjoint stage is provided for ndir 3 without divergence cleaning");
+  if (c->cfg.physics != 1 && c->cfg.physics != 2) return fail(AMRVAC_FV_EINVAL, "unknown physics");
   if (c->amr) return fail(AMRVAC_FV_EINVAL, "the adjoint stage needs a uniform level");
   for (int sd = 0; sd < 2 * c->cfg.ndim; ++sd)
     if (c->bc[sd] != 5) return fail(AMRVAC_FV_EINVAL, "the adjoint stage needs periodic boundaries");
@@ -1141,6 +1143,7 @@ int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt,
   int riem;
   if (method_to_riem(method, riem) != 0 || riem == fv::RIEM_HANCOCK)
     return fail(AMRVAC_FV_EINVAL, "adjoint: flux scheme must be hll, hllc or tvdlf");
+  if (riem == fv::RIEM_HLLC && c->cfg.physics != 1) return fail(AMRVAC_FV_EINVAL, "hllc is provided for hd");
   if (c->cfg.ndim == 3 && !c->d_adjG) {
     const size_t n = (size_t)4 * c->nw * c->nb * c->cells;
     if (hipMalloc(&c->d_adjG, n * sizeof(double)) != hipSuccess) return fail(AMRVAC_FV_ENOMEM, "adjoint buffer");
@@ -1152,7 +1155,8 @@ int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt,
   for (int d = 0; d < 3; ++d) a.dq[d] = -qdt / c->dx[d];
   fvc::AdjArgs g{lam_out, coef, add_a, ca, add_b, cb, lam_in, c->d_adjG};
   const int mixed = c->cfg.precision == AMRVAC_FV_PREC_MIXED ? 1 : 0;
-  hipError_t e = fv::launch_adjoint_stage(c->cfg.ndim, mixed, riem, c->cfg.limiter, a, c->fp, g, c->stream);
+  hipError_t e =
+      fv::launch_adjoint_stage(c->cfg.ndim, c->cfg.physics, mixed, riem, c->cfg.limiter, a, c->fp, g, c->stream);
   if (e == hipErrorInvalidValue) return fail(AMRVAC_FV_EINVAL, "no adjoint kernel for this configuration");
   if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("adjoint kernels: ") + hipGetErrorString(e));
   return AMRVAC_FV_OK;

@@ -53,8 +53,9 @@ FV_DECLARE_LAUNCHERS(fv)
 FV_DECLARE_LAUNCHERS(fvx)
 
 namespace fv {
-// adjoint stage (hydrodynamics, 2-D / 3-D, FP64 or FP32 faces)
-hipError_t launch_adjoint_stage(int ndim, int mixed, int riem, int lim, const fvc::SweepArgs& a,
+// adjoint stage (hydrodynamics or ideal MHD without divergence cleaning,
+// 2-D / 3-D, FP64 or FP32 faces)
+hipError_t launch_adjoint_stage(int ndim, int physics, int mixed, int riem, int lim, const fvc::SweepArgs& a,
                                 const fvc::FVParams& fp, const fvc::AdjArgs& g, hipStream_t s);
 }
 

@@ -52,6 +52,14 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         return _lib
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: run `python -m amrvac_amd.build` first")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64 and
+    # libhsa-runtime64.  Loaded first, it also serves this library (same
+    # sonames); loaded after /opt/rocm's, both runtimes would claim the
+    # device and PyTorch's fails to initialise.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
     L.amrvac_fv_init.argtypes = [ctypes.POINTER(FVConfig), ctypes.POINTER(vp)]

@@ -83,6 +83,72 @@ def cpu_baseline(seconds: float, n: int = 64):
                       f"FP64 CPU restatement, OpenMP over {threads} threads"}
 
 
+def adjoint_cost(kind: str, reps: int = 2):
+    """Cost of the gradient against the forward run it differentiates, on the
+    differentiable workloads: ``kh2d`` = 2-D HD Kelvin-Helmholtz 128^2,
+    threestep / HLLC / vanleer, 200 steps, checkpoints every 20, loss on the
+    final density; ``ot3d`` = 3-D ideal MHD Orszag-Tang 128^3, threestep /
+    HLL / vanleer, 100 steps, sqrt(N) checkpoints, loss = final magnetic
+    energy.  Both mixed precision (FP32 faces, FP64 state).  Returns forward
+    time (CFL dt per step, as a plain run), forward+backward time and
+    backward / forward."""
+    import torch
+    from amrvac_amd import adjoint
+    from amrvac_amd.host import Par
+    from amrvac_amd.mesh import UniformMesh
+    dev = torch.device("cuda")
+    if kind == "kh2d":
+        n, nsteps, every = 128, 200, 20
+        mesh = UniformMesh(2, (n, n), (16, 16), (0.0, 0.0), (1.0, 1.0), ("periodic",) * 4)
+        par = Par(physics="hd", ndim=2, ndir=2, gamma=1.4, time_stepper="threestep", flux_scheme="hllc",
+                  limiter="vanleer", courantpar=0.8)
+        idx = torch.as_tensor(adjoint.block_index(mesh), device=dev)
+        run = adjoint.SSPRK3Run(par, mesh, nsteps, every=every, precision="mixed")
+        rho, vx, vy, p = adjoint.kh_fields(mesh, 20261202, device=dev, rho_amp=1.0e-3)
+        rt, vxt, vyt, pt = adjoint.kh_fields(mesh, 20261203, device=dev, rho_amp=1.0e-3)
+        with torch.no_grad():
+            target = run.forward(adjoint.to_blocks(adjoint.hd_conserved(rt, [vxt, vyt], pt, 1.4), idx))[:, 0].clone()
+        xs = (rho, vx, vy)
+
+        def loss(xs):
+            u0 = adjoint.to_blocks(adjoint.hd_conserved(xs[0], [xs[1], xs[2]], p, 1.4), idx)
+            return ((adjoint.run_differentiable(u0, run)[:, 0] - target) ** 2).mean()
+        workload = "2-D HD Kelvin-Helmholtz 128^2 (16^2 blocks), threestep/HLLC/vanleer, 200 steps, checkpoints every 20"
+    else:
+        n, nsteps = 128, 100
+        mesh = UniformMesh(3, (n, n, n), (16, 16, 16), (0.0,) * 3, (1.0,) * 3, ("periodic",) * 6)
+        par = Par(physics="mhd", ndim=3, ndir=3, gamma=5.0 / 3.0, typedivbfix="none", time_stepper="threestep",
+                  flux_scheme="hll", limiter="vanleer", courantpar=0.6)
+        idx = torch.as_tensor(adjoint.block_index(mesh), device=dev)
+        run = adjoint.SSPRK3Run(par, mesh, nsteps, precision="mixed")
+        rho, v, p, bs = adjoint.orszag_tang_fields(mesh, 20261202, device=dev)
+        xs = (torch.tensor(1.0, dtype=torch.float64, device=dev), rho)
+
+        def loss(xs):
+            u0 = adjoint.to_blocks(adjoint.mhd_conserved(xs[1], v, p, [xs[0] * c for c in bs], 5.0 / 3.0), idx)
+            bf = adjoint.run_differentiable(u0, run)[:, 5:8]
+            return 0.5 * (bf * bf).sum() / n ** 3
+        workload = (f"3-D ideal MHD Orszag-Tang 128^3 (16^3 blocks), threestep/HLL/vanleer, 100 steps, "
+                    f"checkpoints every {run.every}")
+    tf = tg = float("inf")
+    for _ in range(reps + 1):  # the first pass warms up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            loss(xs)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        xg = [x.clone().requires_grad_(True) for x in xs]
+        torch.autograd.grad(loss(xg), xg)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tf, tg = min(tf, t1 - t0), min(tg, t2 - t1)
+    print(f"[bench] adjoint {kind}: forward {tf * 1e3:.1f} ms, forward+backward {tg * 1e3:.1f} ms",
+          file=sys.stderr, flush=True)
+    return {"workload": workload, "precision": "mixed", "forward_ms": round(tf * 1e3, 2),
+            "forward_backward_ms": round(tg * 1e3, 2), "backward_over_forward": round((tg - tf) / tf, 3)}
+
+
 def load_traffic(name: str):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -102,6 +168,7 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256, help="cells per side (multiple of 16)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline budget (0: skip)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy one state slot device-to-device (a plain copy of "
                          "known size, the reference point for HBM byte counters)")
@@ -170,6 +237,8 @@ def main() -> int:
                            "roofline_frac": round(achmx / HBM_PEAK_GBS, 4), "avg_launch_ms": round(msmx, 4),
                            "traffic": load_traffic("mixed")}
         del w64, wmx
+    if not args.no_adjoint:
+        result["adjoint_forward_cost"] = {k: adjoint_cost(k) for k in ("kh2d", "ot3d")}
     if args.cpu_seconds > 0:
         try:
             result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -160,8 +160,9 @@ int amrvac_fv_getdt_state(amrvac_fv_ctx* ctx, const double* u, double courantpar
 
 /* Adjoint of one stage (no counterpart in the reference, which is not
    differentiable; the reverse of finite_volume's flux divergence,
-   src/mod_finite_volume.t:116-339, for hd on a uniform periodic level with
-   Einfeldt bounds, minmod / vanleer and hll / hllc / tvdlf):
+   src/mod_finite_volume.t:116-339, on a uniform periodic level with
+   Einfeldt bounds and minmod / vanleer: hd with hll / hllc / tvdlf, or
+   ideal mhd (ndir 3, typedivbfix 'none') with hll / tvdlf):
      lam_in = coef*lam_out + ca*add_a + cb*add_b
               + (d (qdt L(u)) / d u)^T lam_out  at u = u_in,
    qdt = dtfactor*dt; add_a / add_b may be NULL.  All arrays are device

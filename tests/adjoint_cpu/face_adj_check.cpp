@@ -38,12 +38,10 @@ double check(std::mt19937_64& rng, int trials) {
       prim[0] = rho;
       for (int d = 0; d < P::NDIR; ++d) prim[1 + d] = (d == ID ? u0 : 0.0) + 0.3 * U(rng);
       prim[P::E] = 1.0 + 0.5 * U(rng);
-      // conserved
-      double vsq = 0.0;
-      for (int d = 0; d < P::NDIR; ++d) vsq += prim[1 + d] * prim[1 + d];
-      c[k][0] = rho;
-      for (int d = 0; d < P::NDIR; ++d) c[k][1 + d] = rho * prim[1 + d];
-      c[k][P::E] = prim[P::E] / 0.4 + 0.5 * rho * vsq;
+      if constexpr (P::MHD)
+        for (int d = 0; d < P::NDIR; ++d) prim[P::MAG + d] = 0.8 * U(rng);
+      for (int iw = 0; iw < NW; ++iw) c[k][iw] = prim[iw];
+      to_cons<P, double>(c[k], 1.0 / 0.4);
     }
     double fbar[NW], cbar[4][NW] = {};
     for (int iw = 0; iw < NW; ++iw) fbar[iw] = U(rng);
@@ -82,6 +80,8 @@ int main() {
   std::mt19937_64 rng(20261202);
   using P2 = Phys<2, 2, false, DIVB_NONE>;
   using P3 = Phys<3, 3, false, DIVB_NONE>;
+  using M2 = Phys<2, 3, true, DIVB_NONE>;
+  using M3 = Phys<3, 3, true, DIVB_NONE>;
   struct R { const char* name; double err; };
   R res[] = {
       {"2d hllc vanleer x", check<P2, RIEM_HLLC, 0, LIM_VANLEER>(rng, 400)},
@@ -91,6 +91,10 @@ int main() {
       {"2d hllc minmod x", check<P2, RIEM_HLLC, 0, LIM_MINMOD>(rng, 400)},
       {"3d hllc vanleer z", check<P3, RIEM_HLLC, 2, LIM_VANLEER>(rng, 400)},
       {"3d hll vanleer y", check<P3, RIEM_HLL, 1, LIM_VANLEER>(rng, 400)},
+      {"3d mhd hll vanleer x", check<M3, RIEM_HLL, 0, LIM_VANLEER>(rng, 400)},
+      {"3d mhd hll minmod z", check<M3, RIEM_HLL, 2, LIM_MINMOD>(rng, 400)},
+      {"3d mhd tvdlf vanleer y", check<M3, RIEM_TVDLF, 1, LIM_VANLEER>(rng, 400)},
+      {"2d mhd hll vanleer y", check<M2, RIEM_HLL, 1, LIM_VANLEER>(rng, 400)},
   };
   int bad = 0;
   for (auto& r : res) {

@@ -1,0 +1,154 @@
+"""Adjoint of 3-D ideal MHD on the device (threestep / HLL / vanleer,
+Einfeldt bounds, no divergence cleaning, periodic): gradients of the final
+magnetic energy w.r.t. the Orszag-Tang field amplitude b0 and the initial
+density field, against FP64 central finite differences."""
+from __future__ import annotations
+
+import time
+
+import pytest
+import torch
+
+import torch_fv_ref
+
+from amrvac_amd import adjoint
+from amrvac_amd.host import Par
+from amrvac_amd.mesh import UniformMesh
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 5.0 / 3.0
+SEED = 20261202
+# seeded noise on v, p and B: the Orszag-Tang fields are constant along
+# some axes, where every limiter slope sits on its kink (the run then has
+# one-sided derivatives only); the noise moves the evaluation point off them
+FIELD_AMP = 1.0e-3
+
+
+class _MHDProblem:
+    """loss(b0, rho0) = sum over cells of |B_final|^2 / 2 * dV"""
+
+    def __init__(self, n, nsteps, precision="fp64", every=None, field_amp=FIELD_AMP):
+        self.mesh = UniformMesh(3, (n, n, n), (16, 16, 16), (0.0,) * 3, (1.0,) * 3, ("periodic",) * 6)
+        self.par = Par(physics="mhd", ndim=3, ndir=3, gamma=GAMMA, typedivbfix="none", time_stepper="threestep",
+                       flux_scheme="hll", limiter="vanleer", courantpar=0.6)
+        self.dev = torch.device("cuda")
+        self.idx = torch.as_tensor(adjoint.block_index(self.mesh), device=self.dev)
+        self.run = adjoint.SSPRK3Run(self.par, self.mesh, nsteps, every=every, precision=precision)
+        self.rho, self.v, self.p, self.bshape = adjoint.orszag_tang_fields(self.mesh, SEED, device=self.dev,
+                                                                           field_amp=field_amp)
+        self.b0 = torch.tensor(1.0, dtype=torch.float64, device=self.dev)
+        self.dV = 1.0 / n ** 3
+        self.dts = None
+
+    def loss(self, b0, rho, dts=None):
+        u0 = adjoint.to_blocks(adjoint.mhd_conserved(rho, self.v, self.p, [b0 * c for c in self.bshape], GAMMA),
+                               self.idx)
+        uf = adjoint.run_differentiable(u0, self.run, dts)
+        bf = uf[:, 5:8]
+        return 0.5 * (bf * bf).sum() * self.dV
+
+    def grad(self):
+        b0 = self.b0.clone().requires_grad_(True)
+        rho = self.rho.clone().requires_grad_(True)
+        L = self.loss(b0, rho)
+        self.dts = list(self.run.dts)
+        gb, gr = torch.autograd.grad(L, (b0, rho))
+        return L.item(), gb.item(), gr.detach()
+
+
+@pytest.mark.parametrize("physics,riemann,lim", [("mhd", "hll", "vanleer"), ("mhd", "tvdlf", "minmod"),
+                                                 ("hd", "hllc", "vanleer")])
+def test_adjoint_3d_matches_torch_autograd(physics, riemann, lim):
+    """3-D gradients of sum(u_final * r) (r seeded) w.r.t. the initial
+    conserved state against torch autograd through the FP64 PyTorch
+    restatement, on the CPU, for the dt sequence of the device run."""
+    print(f"[adjoint-3d] {physics} {riemann}/{lim} 32^3 vs torch autograd", flush=True)
+    mesh = UniformMesh(3, (32, 32, 32), (16, 16, 16), (0.0,) * 3, (1.0,) * 3, ("periodic",) * 6)
+    if physics == "hd":
+        par = Par(physics="hd", ndim=3, ndir=3, gamma=1.4, time_stepper="threestep", flux_scheme=riemann,
+                  limiter=lim, courantpar=0.8)
+    else:
+        par = Par(physics="mhd", ndim=3, ndir=3, gamma=GAMMA, typedivbfix="none", time_stepper="threestep",
+                  flux_scheme=riemann, limiter=lim, courantpar=0.6)
+    dev = torch.device("cuda")
+    idx = torch.as_tensor(adjoint.block_index(mesh), device=dev)
+    rho, v, p, b = adjoint.orszag_tang_fields(mesh, SEED, device=dev)
+    gen = torch.Generator(device="cpu").manual_seed(SEED)
+    # a generic point: the Orszag-Tang fields are constant along some axes
+    # (every slope there sits on a limiter kink), so all fields get noise
+    noise = lambda q: q + 1.0e-3 * torch.randn(q.shape, generator=gen, dtype=torch.float64).to(dev)
+    v, b, p = [noise(c) for c in v], [noise(c) for c in b], noise(p)
+    u0 = (adjoint.hd_conserved(rho, v, p, par.gamma) if physics == "hd"
+          else adjoint.mhd_conserved(rho, v, p, b, par.gamma))
+    r = torch.randn(u0.shape, generator=gen, dtype=torch.float64)
+    run = adjoint.SSPRK3Run(par, mesh, 4, every=2)
+    x = u0.clone().requires_grad_(True)
+    uf = adjoint.from_blocks(adjoint.run_differentiable(adjoint.to_blocks(x, idx), run), idx, (32, 32, 32))
+    g_dev, = torch.autograd.grad((uf * r.to(dev)).sum(), x)
+    sch = torch_fv_ref.Scheme(3, physics, par.gamma, lim, riemann)
+    xc = u0.detach().cpu().clone().requires_grad_(True)
+    ufc = sch.run(xc, run.dts, mesh.dx)
+    assert torch.allclose(ufc.detach(), uf.detach().cpu(), rtol=1e-10, atol=1e-12)
+    g_ref, = torch.autograd.grad((ufc * r).sum(), xc)
+    rel = ((g_dev.cpu() - g_ref).norm() / g_ref.norm()).item()
+    worst = ((g_dev.cpu() - g_ref).abs().max() / g_ref.abs().max()).item()
+    print(f"[adjoint-3d] rel L2 {rel:.3e}, max {worst:.3e}", flush=True)
+    assert rel < 1e-9
+
+
+def _fd(pb: _MHDProblem, gb: float, gr: torch.Tensor, ndir: int, eps: float = 1.0e-6):
+    """Adjoint against FP64 central differences: the relative error along
+    b0, and the normwise relative error over ``ndir`` seeded directions of
+    the density field, relative perturbations d = rho0 * r with r standard
+    normal per cell (unit-norm directions change the loss by less than its
+    rounding noise)."""
+    with torch.no_grad():
+        lp = pb.loss(pb.b0 + eps, pb.rho, dts=pb.dts).item()
+        lm = pb.loss(pb.b0 - eps, pb.rho, dts=pb.dts).item()
+        fdb = (lp - lm) / (2.0 * eps)
+        relb = abs(fdb - gb) / abs(fdb)
+        print(f"[adjoint-mhd] b0: fd {fdb:.10e} adjoint {gb:.10e} rel {relb:.2e}", flush=True)
+        gen = torch.Generator(device="cpu").manual_seed(SEED)
+        fd, an = [], []
+        for k in range(ndir):
+            d = pb.rho * torch.randn(pb.rho.shape, generator=gen, dtype=torch.float64).to(pb.dev)
+            lp = pb.loss(pb.b0, pb.rho + eps * d, dts=pb.dts).item()
+            lm = pb.loss(pb.b0, pb.rho - eps * d, dts=pb.dts).item()
+            fd.append((lp - lm) / (2.0 * eps))
+            an.append((gr * d).sum().item())
+            print(f"[adjoint-mhd] rho dir {k}: fd {fd[-1]:.10e} adjoint {an[-1]:.10e}", flush=True)
+    fd, an = torch.tensor(fd), torch.tensor(an)
+    relr = ((fd - an).norm() / fd.norm()).item()
+    print(f"[adjoint-mhd] density directions, normwise {relr:.2e}", flush=True)
+    return relb, relr
+
+
+def test_mhd_adjoint_fp64_finite_differences_32():
+    print("[adjoint-mhd] 32^3, 10 steps, FP64", flush=True)
+    pb = _MHDProblem(32, 10, "fp64", every=3)
+    _, gb, gr = pb.grad()
+    relb, relr = _fd(pb, gb, gr, 4)
+    assert relb < 1.0e-6 and relr < 1.0e-4
+
+
+def test_mhd_adjoint_config_128():
+    """128^3, 100 steps, sqrt(N) checkpoints: FP64 adjoint against FP64
+    central differences (1e-4), and the mixed-precision adjoint (FP32
+    faces) against the same differences along its own dt sequence."""
+    print("[adjoint-mhd] 128^3, 100 steps, FP64 and mixed", flush=True)
+    pb = _MHDProblem(128, 100, "fp64")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, gb, gr = pb.grad()
+    torch.cuda.synchronize()
+    print(f"[adjoint-mhd] fp64 forward+backward {time.perf_counter() - t0:.3f} s", flush=True)
+    relb, relr = _fd(pb, gb, gr, 4)
+    assert relb < 1.0e-6 and relr < 1.0e-4
+    pm = _MHDProblem(128, 100, "mixed")
+    _, gbm, grm = pm.grad()
+    pb.dts = pm.dts
+    relbm, relrm = _fd(pb, gbm, grm, 4)
+    print(f"[adjoint-mhd] mixed vs fp64 density gradient rel L2 {((grm - gr).norm() / gr.norm()).item():.3e}",
+          flush=True)
+    assert relbm < 1.0e-6 and relrm < 1.0e-2

@@ -11,7 +11,7 @@ from amrvac_amd.host import Par, TimeIntegrator
 from amrvac_amd.mesh import UniformMesh
 from oracle.oracle import OracleFV
 
-import torch_hd_ref
+import torch_fv_ref
 
 
 @pytest.mark.parametrize("riemann,lim", [("hllc", "vanleer"), ("hll", "minmod"), ("tvdlf", "vanleer")])
@@ -31,5 +31,33 @@ def test_torch_reference_matches_oracle(riemann, lim):
     ti = TimeIntegrator(par, mesh, be)
     ti.run(nsteps=8)
     got_orc = torch.as_tensor(mesh.to_global(be.get_state(0)))
-    got_t = torch_hd_ref.run(u0, ti.state.dts, mesh.dx, par.gamma, lim, riemann)
+    got_t = torch_fv_ref.run(u0, ti.state.dts, mesh.dx, par.gamma, lim, riemann)
+    assert torch.allclose(got_t, got_orc, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("physics,riemann,lim", [("hd", "hllc", "vanleer"), ("mhd", "hll", "vanleer"),
+                                                 ("mhd", "tvdlf", "minmod")])
+def test_torch_reference_3d_matches_oracle(physics, riemann, lim):
+    mesh = UniformMesh(3, (16, 16, 32), (16, 16, 16), (0.0,) * 3, (1.0,) * 3, ("periodic",) * 6)
+    if physics == "hd":
+        par = Par(physics="hd", ndim=3, ndir=3, gamma=1.4, time_stepper="threestep", flux_scheme=riemann,
+                  limiter=lim, courantpar=0.8)
+        rho, v, p, b = adjoint.orszag_tang_fields(mesh, 7)
+        u0 = adjoint.hd_conserved(rho, v, p, par.gamma)
+    else:
+        par = Par(physics="mhd", ndim=3, ndir=3, gamma=5.0 / 3.0, typedivbfix="none", time_stepper="threestep",
+                  flux_scheme=riemann, limiter=lim, courantpar=0.6)
+        rho, v, p, b = adjoint.orszag_tang_fields(mesh, 7)
+        u0 = adjoint.mhd_conserved(rho, v, p, b, par.gamma)
+    idx = torch.as_tensor(adjoint.block_index(mesh))
+    w = np.zeros((mesh.nblocks, par.nw) + mesh.nxg[::-1])
+    mesh.interior(w)[:] = adjoint.to_blocks(u0, idx).numpy()
+    be = OracleFV(par, mesh, threads=2)
+    be.set_state(0, w)
+    be.getbc(0)
+    ti = TimeIntegrator(par, mesh, be)
+    ti.run(nsteps=3)
+    got_orc = torch.as_tensor(mesh.to_global(be.get_state(0)))
+    sch = torch_fv_ref.Scheme(3, physics, par.gamma, lim, riemann)
+    got_t = sch.run(u0, ti.state.dts, mesh.dx)
     assert torch.allclose(got_t, got_orc, rtol=1e-12, atol=1e-13)
