@@ -65,7 +65,10 @@ using fvc::KernelKey;
 
 template <class P, class R, int RIEM, int LIM, int NG>
 hipError_t launch_one(const SweepArgs& a, const FVParams& fp, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((sweep_kernel<P, R, RIEM, LIM, NG>), dim3(grid), dim3(NTHREADS), 0, s, a, fp);
+  if (a.blevel)
+    hipLaunchKernelGGL((sweep_kernel<P, R, RIEM, LIM, NG, true>), dim3(grid), dim3(NTHREADS), 0, s, a, fp);
+  else
+    hipLaunchKernelGGL((sweep_kernel<P, R, RIEM, LIM, NG, false>), dim3(grid), dim3(NTHREADS), 0, s, a, fp);
   return hipGetLastError();
 }
 

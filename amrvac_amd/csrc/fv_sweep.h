@@ -217,7 +217,10 @@ __device__ __forceinline__ void level_params(const SweepArgs& a, int b, FVParams
   glm_decay = a.glm_decay_lev[lev];
 }
 
-template <class P, class R, int RIEM, int LIM, int NG>
+// AMR: level-dependent scheme constants and the refluxing registers; the
+// uniform-level kernels are compiled without them (their registers would
+// spill in the 3-D MHD kernels).
+template <class P, class R, int RIEM, int LIM, int NG, bool AMR>
 __global__ void __launch_bounds__(NTHREADS, kMinWaves<R>)
 sweep_kernel(SweepArgs a, FVParams fp0) {
   constexpr int NDIM = P::NDIM;
@@ -244,11 +247,16 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
   if (b >= a.nblocks) return;
   FVParams fp = fp0;
   double dq[3], glm_decay;
-  level_params(a, b, fp, dq, glm_decay);
   // refluxing registers of the block's faces (x-, x+, y-, y+, z-, z+)
   int freg[2 * NDIM];
+  if constexpr (AMR) {
+    level_params(a, b, fp, dq, glm_decay);
 #pragma unroll
-  for (int f = 0; f < 2 * NDIM; ++f) freg[f] = a.fregoff ? a.fregoff[b * 2 * NDIM + f] : -1;
+    for (int f = 0; f < 2 * NDIM; ++f) freg[f] = a.fregoff ? a.fregoff[b * 2 * NDIM + f] : -1;
+  } else {
+    dq[0] = a.dq[0]; dq[1] = a.dq[1]; dq[2] = a.dq[2];
+    glm_decay = a.glm_decay;
+  }
   const int tid = threadIdx.x;
   const int tx = tid & (BX - 1);
   const int ty = tid >> 4;
@@ -433,7 +441,7 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
       __syncthreads();
       // ---- store_flux (src/mod_fix_conserve.t:597-648): the scaled fluxes
       //      of the block-boundary faces, for refluxing --------------------
-      if (a.freg) {
+      if (AMR && a.freg) {
         const size_t nf = (size_t)a.nface;
 #pragma unroll
         for (int iw = 0; iw < NW; ++iw) {
