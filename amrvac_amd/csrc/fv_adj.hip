@@ -13,8 +13,9 @@
 // contributions of its four faces per dimension in a fixed order (no
 // atomics: the result is deterministic).  The summed primitive adjoint is
 // carried to the conserved state once per cell.  In 3-D the z faces are
-// reversed beforehand by adj_zface_kernel (one lane per face, into a global
-// face buffer) and gathered here in the same fixed order.
+// reversed beforehand by adj_zface_kernel (each lane a run of ZC faces of
+// one column, into a global face buffer in the face precision) and gathered
+// here in the same fixed order.
 #include <hip/hip_runtime.h>
 
 #include "fv_adjoint.h"
@@ -26,8 +27,11 @@ using fvc::AdjArgs;
 
 constexpr int AH = 3;             // primitive halo of the tile
 constexpr int AW = BX + 2 * AH;   // staged row length
+constexpr int LH = 2;             // lam_out halo of the tile
+constexpr int LW = BX + 2 * LH;
 constexpr int NF = BX + 3;        // faces per row touching the tile: f in [-1, BX+1]
 constexpr int NFACE = NF * BX;    // per dimension and plane
+constexpr int ZC = 8;             // z faces per lane in adj_zface_kernel
 
 // lam_out at an arbitrary (possibly neighbouring) cell of plane z
 template <int NW, int NDIM>
@@ -41,38 +45,55 @@ __device__ __forceinline__ void load_lam(const SweepArgs& a, const double* __res
   for (int iw = 0; iw < NW; ++iw) l[iw] = lam[off + iw * cells];
 }
 
+// Low z faces of ZC consecutive cells of one column per lane: the stencil
+// window (4 primitive states) and the two lam_out values slide along z, so
+// each cell is loaded and converted once per chunk.  Face contributions go
+// to gz[(s*NW + iw) * ncells + cell], in the face precision R.
 template <class P, class R, int RIEM, int LIM>
 __global__ void __launch_bounds__(NTHREADS) adj_zface_kernel(SweepArgs a, FVParams fp, AdjArgs g) {
   constexpr int NW = P::NW, NDIM = P::NDIM;
   static_assert(NDIM == 3, "z faces exist in 3-D");
-  const int b = blockIdx.x / a.bnz, k = blockIdx.x % a.bnz;
+  const int nch = a.bnz / ZC;
+  const int b = blockIdx.x / nch, k0 = (blockIdx.x % nch) * ZC;
   if (b >= a.nblocks) return;
   const int tx = threadIdx.x & (BX - 1), ty = threadIdx.x >> 4;
   const size_t cells = (size_t)BX * BX * a.bnz;
-  // the low z face of cell (tx, ty, k): stencil k-2 .. k+1
-  R st[4][NW];
-#pragma unroll
-  for (int s = 0; s < 4; ++s) load_prim<P, R, NDIM>(a, fp, b, tx, ty, k - 2 + s, st[s]);
-  double ll[NW], lr[NW];
-  load_lam<NW, NDIM>(a, g.lam_out, b, tx, ty, k - 1, ll);
-  load_lam<NW, NDIM>(a, g.lam_out, b, tx, ty, k, lr);
-  R mu[NW];
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) mu[iw] = R(a.dq[2] * (ll[iw] - lr[iw]));
-  double wb[4][NW];
-  face_adj_prim<P, R, RIEM, 2, LIM>(st, fp, mu, wb);
   const size_t ntot = (size_t)a.nblocks * cells;
-  const size_t o = (size_t)b * cells + ((size_t)k * BX + ty) * BX + tx;
+  R* gz = static_cast<R*>(g.gz);
+  // window: st[s] = cell k-2+s of the face at k; ll/lr = lam_out(k-1), (k)
+  R st[4][NW];
+  double ll[NW], lr[NW];
 #pragma unroll
-  for (int s = 0; s < 4; ++s)
+  for (int s = 0; s < 3; ++s) load_prim<P, R, NDIM>(a, fp, b, tx, ty, k0 - 2 + s, st[s + 1]);
+  load_lam<NW, NDIM>(a, g.lam_out, b, tx, ty, k0 - 1, lr);
+  for (int k = k0; k < k0 + ZC; ++k) {
 #pragma unroll
-    for (int iw = 0; iw < NW; ++iw) g.gz[(size_t)(s * NW + iw) * ntot + o] = wb[s][iw];
+    for (int iw = 0; iw < NW; ++iw) {
+      st[0][iw] = st[1][iw];
+      st[1][iw] = st[2][iw];
+      st[2][iw] = st[3][iw];
+      ll[iw] = lr[iw];
+    }
+    load_prim<P, R, NDIM>(a, fp, b, tx, ty, k + 1, st[3]);
+    load_lam<NW, NDIM>(a, g.lam_out, b, tx, ty, k, lr);
+    R mu[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) mu[iw] = R(a.dq[2] * (ll[iw] - lr[iw]));
+    double wb[4][NW];
+    face_adj_prim<P, R, RIEM, 2, LIM>(st, fp, mu, wb);
+    const size_t o = (size_t)b * cells + ((size_t)k * BX + ty) * BX + tx;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) gz[(size_t)(s * NW + iw) * ntot + o] = R(wb[s][iw]);
+  }
 }
 
 template <class P, class R, int RIEM, int LIM>
 __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVParams fp, AdjArgs g) {
   constexpr int NW = P::NW, NDIM = P::NDIM;
   __shared__ R Wl[NW][AW * AW];
+  __shared__ double Ll[NW][LW * LW];
   // face contributions: computed in R, so stored in R without rounding
   __shared__ R Gl[4 * NW][NFACE];
   const int b = blockIdx.x / a.bnz, k = blockIdx.x % a.bnz;
@@ -90,6 +111,15 @@ __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVPara
     load_prim<P, R, NDIM>(a, fp, b, x, y, k, w);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) Wl[iw][i] = w[iw];
+  }
+  // lam_out of the plane with a 2-cell x/y halo (corners are not read)
+  for (int i = t; i < LW * LW; i += NTHREADS) {
+    const int x = i % LW - LH, y = i / LW - LH;
+    if ((x < 0 || x >= BX) && (y < 0 || y >= BX)) continue;
+    double l[NW];
+    load_lam<NW, NDIM>(a, g.lam_out, b, x, y, k, l);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) Ll[iw][i] = l[iw];
   }
   double acc[NW], wsum[NW];
 #pragma unroll
@@ -110,12 +140,10 @@ __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVPara
     for (int s = 0; s < 4; ++s)
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) st[s][iw] = Wl[iw][(row + AH) * AW + f - 2 + s + AH];
-    double ll[NW], lr[NW];
-    load_lam<NW, NDIM>(a, g.lam_out, b, f - 1, row, k, ll);
-    load_lam<NW, NDIM>(a, g.lam_out, b, f, row, k, lr);
     R mu[NW];
 #pragma unroll
-    for (int iw = 0; iw < NW; ++iw) mu[iw] = R(a.dq[0] * (ll[iw] - lr[iw]));
+    for (int iw = 0; iw < NW; ++iw)
+      mu[iw] = R(a.dq[0] * (Ll[iw][(row + LH) * LW + f - 1 + LH] - Ll[iw][(row + LH) * LW + f + LH]));
     double wb[4][NW];
     face_adj_prim<P, R, RIEM, 0, LIM>(st, fp, mu, wb);
 #pragma unroll
@@ -141,12 +169,10 @@ __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVPara
     for (int s = 0; s < 4; ++s)
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) st[s][iw] = Wl[iw][(f - 2 + s + AH) * AW + col + AH];
-    double ll[NW], lr[NW];
-    load_lam<NW, NDIM>(a, g.lam_out, b, col, f - 1, k, ll);
-    load_lam<NW, NDIM>(a, g.lam_out, b, col, f, k, lr);
     R mu[NW];
 #pragma unroll
-    for (int iw = 0; iw < NW; ++iw) mu[iw] = R(a.dq[1] * (ll[iw] - lr[iw]));
+    for (int iw = 0; iw < NW; ++iw)
+      mu[iw] = R(a.dq[1] * (Ll[iw][(f - 1 + LH) * LW + col + LH] - Ll[iw][(f + LH) * LW + col + LH]));
     double wb[4][NW];
     face_adj_prim<P, R, RIEM, 1, LIM>(st, fp, mu, wb);
 #pragma unroll
@@ -165,6 +191,7 @@ __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVPara
   // ---- z faces (3-D): reversed by adj_zface_kernel ------------------------
   if constexpr (NDIM == 3) {
     const size_t ntot = (size_t)a.nblocks * cells;
+    const R* gz = static_cast<const R*>(g.gz);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       int x = tx, y = ty, z = k - s + 2;
@@ -172,7 +199,7 @@ __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVPara
       const int ob = locate<NDIM>(a, b, x, y, z, sign);
       const size_t of = (size_t)ob * cells + ((size_t)z * BX + y) * BX + x;
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) wsum[iw] += g.gz[(size_t)(s * NW + iw) * ntot + of];
+      for (int iw = 0; iw < NW; ++iw) wsum[iw] += double(gz[(size_t)(s * NW + iw) * ntot + of]);
     }
   }
 
@@ -188,8 +215,11 @@ __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVPara
 template <class P, class R, int RIEM, int LIM>
 hipError_t launch_adj(const SweepArgs& a, const FVParams& fp, const AdjArgs& g, hipStream_t s) {
   const dim3 grid(a.nblocks * a.bnz);
-  if constexpr (P::NDIM == 3)
-    hipLaunchKernelGGL((adj_zface_kernel<P, R, RIEM, LIM>), grid, dim3(NTHREADS), 0, s, a, fp, g);
+  if constexpr (P::NDIM == 3) {
+    if (a.bnz % ZC != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((adj_zface_kernel<P, R, RIEM, LIM>), dim3(a.nblocks * (a.bnz / ZC)), dim3(NTHREADS), 0, s, a,
+                       fp, g);
+  }
   hipLaunchKernelGGL((adj_plane_kernel<P, R, RIEM, LIM>), grid, dim3(NTHREADS), 0, s, a, fp, g);
   return hipGetLastError();
 }
