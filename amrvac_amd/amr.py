@@ -556,13 +556,27 @@ class AMRIntegrator(TimeIntegrator):
         pass
 
     def regrid(self, time_advance: bool = True) -> None:
-        """errest + amr_coarsen_refine (+ getbc) on ps."""
+        """errest + amr_coarsen_refine (+ getbc) on ps.  A backend with a
+        device regrid (amr_device_regrid) estimates and moves the data
+        without a round trip through the host; the new leaves of the
+        initial tree (time_advance False) take the initial condition."""
         forest, par = self.forest, self.par
-        w = self.be.get_state(0)
-        refine, coarsen = errest_flags(forest, w, self.ap)
+        device = getattr(self.be, "amr_device_regrid", False)
+        if device:
+            refine, coarsen = self.be.errest_flags(forest, self.ap)
+        else:
+            w = self.be.get_state(0)
+            refine, coarsen = errest_flags(forest, w, self.ap)
         proper_nesting(forest, self.ap, refine, coarsen)
         if not refine.any() and not coarsen.any():
             return
+        if device and time_advance:
+            plan = apply_regrid(forest, refine, coarsen)
+            self.be.regrid(plan, forest)
+            self.regrids += 1
+            return
+        if device:
+            w = self.be.get_state(0)
         plan = apply_regrid(forest, refine, coarsen)
         nd, ng = forest.ndim, forest.nghost
         new = np.zeros((forest.nblocks,) + w.shape[1:])

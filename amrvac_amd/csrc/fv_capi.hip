@@ -72,6 +72,16 @@ struct amrvac_fv_ctx {
   double* d_freg = nullptr;  // flux registers
   double* d_xlo = nullptr;   // [nb][ndim] block lower corners
   double* d_adjG = nullptr;  // 3-D adjoint z-face contributions [4*nw][nb*cells]
+  // device regrid: new-leaf state, plan, Löhner flags and weights (kept across
+  // regrids, grown on demand)
+  double* d_remap = nullptr;
+  size_t remap_cap = 0;
+  int* d_plan = nullptr;
+  size_t plan_cap = 0;
+  unsigned char* d_flags = nullptr;
+  size_t flags_cap = 0;
+  int* d_iws = nullptr;
+  double* d_wts = nullptr;
   int nrt = 0, npt = 0, nft = 0;
   std::vector<int> h_level;
   bool timing = false;
@@ -467,6 +477,151 @@ hipError_t launch_getdt(const double* w, int nb, int bnz, const FVParams& fp, co
   return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------------------
+// regrid on the device: Löhner flags and the data of the new leaves
+// ---------------------------------------------------------------------------
+// variable iw of cell (x, y, z) of block b, ghost cells through the neighbour
+// table (the values download would write into the ghosted layout)
+template <int NDIM>
+__device__ __forceinline__ double cell_value(const SweepArgs& a, int nw, int b, int x, int y, int z, int iw) {
+  double sign;
+  const int sb = locate<NDIM>(a, b, x, y, z, sign);
+  const size_t cells = (size_t)BX * BX * a.bnz;
+  return sign * a.src[(size_t)sb * nw * cells + iw * cells + ((size_t)z * BX + y) * BX + x];
+}
+
+// lohner_grid (src/amr/mod_errest.t:63-199, Cartesian, no B0, no log
+// variables): per cell
+//   error = sum_w weight_w sqrt(num_w / max(den_w, 1e-6)),
+//   num = sum_{d,d2} (dq_d(x+e_d2) - dq_d(x-e_d2))^2,  dq_d(p) = q(p+e_d) - q(p-e_d),
+//   den = sum_{d,d2} (|q(x+2e_d)-q(x)| + |q(x)-q(x-2e_d)|
+//                     + wavefilter (aq_d(x+e_d2) + aq_d(x-e_d2)))^2,  aq_d(p) = |q(p+e_d)| + |q(p-e_d)|,
+// evaluated in that order without contraction.  flags[b]: bit 0 when some
+// cell has error >= thr, bit 1 when every cell has error < thr and
+// error <= cthr (the coarsening test of errest, :10-61).
+template <int NDIM>
+__global__ void __launch_bounds__(NTHREADS) lohner_kernel(SweepArgs a, int nw, int nwt, const int* __restrict__ iws,
+                                                          const double* __restrict__ wts, double wavefilter, double thr,
+                                                          double cthr, unsigned char* __restrict__ flags) {
+  const int b = blockIdx.x;
+  const int cells = BX * BX * a.bnz;
+  int any_r = 0, all_c = 1;
+  for (int c = threadIdx.x; c < cells; c += NTHREADS) {
+    const int X[3] = {c % BX, (c / BX) % BX, c / (BX * BX)};
+    auto q = [&](int iw, int d1, int s1, int d2, int s2) {
+      int p[3] = {X[0], X[1], X[2]};
+      p[d1] += s1;
+      p[d2] += s2;
+      return cell_value<NDIM>(a, nw, b, p[0], p[1], p[2], iw);
+    };
+    double error = 0.0;
+    for (int t = 0; t < nwt; ++t) {
+      const int iw = iws[t];
+      double num = 0.0, den = 0.0;
+      for (int d = 0; d < NDIM; ++d)
+        for (int d2 = 0; d2 < NDIM; ++d2) {
+          const double tj = q(iw, d2, 1, d, 1) - q(iw, d2, 1, d, -1);
+          const double th = q(iw, d2, -1, d, 1) - q(iw, d2, -1, d, -1);
+          const double diff = tj - th;
+          num = num + diff * diff;
+        }
+      for (int d = 0; d < NDIM; ++d) {
+        const double q0 = q(iw, d, 0, d, 0);
+        const double tt = fabs(q(iw, d, 2, d, 0) - q0) + fabs(q0 - q(iw, d, -2, d, 0));
+        for (int d2 = 0; d2 < NDIM; ++d2) {
+          const double aj = fabs(q(iw, d2, 1, d, 1)) + fabs(q(iw, d2, 1, d, -1));
+          const double ah = fabs(q(iw, d2, -1, d, 1)) + fabs(q(iw, d2, -1, d, -1));
+          const double v = tt + wavefilter * (aj + ah);
+          den = den + v * v;
+        }
+      }
+      error = error + wts[t] * sqrt(num / (den > 1.0e-6 ? den : 1.0e-6));
+    }
+    if (error >= thr) any_r = 1;
+    if (!(error < thr && error <= cthr)) all_c = 0;
+  }
+  any_r = __syncthreads_or(any_r);
+  all_c = __syncthreads_and(all_c);
+  if (threadIdx.x == 0) flags[b] = (unsigned char)((any_r ? 1 : 0) | (all_c ? 2 : 0));
+}
+
+// Data of the new leaves after amr_coarsen_refine (src/amr/mod_coarsen_refine.t:
+// 15-173), from the old leaves (a.src, old neighbour table).  plan[b*S ..]
+// with S = 2 + 2^ndim: kind (0 copy, 1 coarsen, 2 refine), then the old
+// block (copy, refine: the parent) or the 2^ndim old children in child order
+// (coarsen), and last the child position sum_d (ic_d - 1) 2^d (refine).
+//   coarsen: coarsen_grid (src/amr/mod_coarsen.t:12-79), the mean of the
+//            2^ndim fine cells summed in array-element order;
+//   refine:  prolong_2nd (src/amr/mod_refine.t:43-167) on conserved
+//            variables: the coarse value plus limited slopes, the slope
+//            of dimension d = sign(c) max(0, min(|c|, sign(c) l, sign(c) r))
+//            with c = (l + r)/2, times eta = (offset - 1/2)/2.
+template <int NDIM>
+__global__ void __launch_bounds__(NTHREADS) remap_kernel(SweepArgs a, int nw, const int* __restrict__ plan,
+                                                         double* __restrict__ out) {
+  constexpr int S = 2 + (1 << NDIM);
+  const int b = blockIdx.x;
+  const int* p = plan + (size_t)b * S;
+  const int kind = p[0];
+  const int n[3] = {BX, BX, a.bnz};
+  const size_t cells = (size_t)BX * BX * a.bnz;
+  double* o = out + (size_t)b * nw * cells;
+  for (int c = threadIdx.x; c < (int)cells; c += NTHREADS) {
+    const int X[3] = {c % BX, (c / BX) % BX, c / (BX * BX)};
+    if (kind == 0) {
+      const double* src = a.src + (size_t)p[1] * nw * cells;
+      for (int iw = 0; iw < nw; ++iw) o[iw * cells + c] = src[iw * cells + c];
+    } else if (kind == 1) {
+      int f[3] = {0, 0, 0}, cidx = 0;
+      for (int d = 0; d < NDIM; ++d) {
+        const int ic = X[d] >= n[d] / 2 ? 2 : 1;
+        cidx += (ic - 1) << d;
+        f[d] = 2 * (X[d] - (ic - 1) * (n[d] / 2));
+      }
+      const double* wf = a.src + (size_t)p[1 + cidx] * nw * cells;
+      for (int iw = 0; iw < nw; ++iw) {
+        double sum = 0.0;
+        bool first = true;
+        for (int dk = 0; dk <= (NDIM > 2 ? 1 : 0); ++dk)
+          for (int dj = 0; dj <= 1; ++dj)
+            for (int di = 0; di <= 1; ++di) {
+              const double v = wf[iw * cells + ((size_t)(f[2] + dk) * BX + (f[1] + dj)) * BX + f[0] + di];
+              sum = first ? v : sum + v;
+              first = false;
+            }
+        o[iw * cells + c] = sum * (1.0 / double(1 << NDIM));
+      }
+    } else {
+      const int P = p[1], icode = p[S - 1];
+      int cc[3] = {0, 0, 0}, off[3] = {0, 0, 0};
+      for (int d = 0; d < NDIM; ++d) {
+        const int ic = 1 + ((icode >> d) & 1);
+        cc[d] = (ic - 1) * (n[d] / 2) + X[d] / 2;
+        off[d] = X[d] % 2;
+      }
+      for (int iw = 0; iw < nw; ++iw) {
+        const double c0 = cell_value<NDIM>(a, nw, P, cc[0], cc[1], cc[2], iw);
+        double v = c0;
+        for (int d = 0; d < NDIM; ++d) {
+          int m[3] = {cc[0], cc[1], cc[2]}, pl[3] = {cc[0], cc[1], cc[2]};
+          m[d] -= 1;
+          pl[d] += 1;
+          const double sl = c0 - cell_value<NDIM>(a, nw, P, m[0], m[1], m[2], iw);
+          const double sr = cell_value<NDIM>(a, nw, P, pl[0], pl[1], pl[2], iw) - c0;
+          const double sc = 0.5 * (sr + sl);
+          const double sg = copysign(1.0, sc);
+          double lim = fabs(sc);
+          if (sg * sl < lim) lim = sg * sl;
+          if (sg * sr < lim) lim = sg * sr;
+          const double slope = sg * (lim > 0.0 ? lim : 0.0);
+          v = v + slope * (0.5 * (double(off[d]) - 0.5));
+        }
+        o[iw * cells + c] = v;
+      }
+    }
+  }
+}
 }  // namespace fv
 
 namespace {
@@ -597,6 +752,9 @@ int amrvac_fv_finalize(amrvac_fv_ctx* c) {
   if (!c) return AMRVAC_FV_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_level(c);
+  void* regrid_bufs[] = {c->d_remap, c->d_plan, c->d_flags, c->d_iws, c->d_wts};
+  for (void* p : regrid_bufs)
+    if (p) (void)hipFree(p);
   for (auto& ev : c->events) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -783,6 +941,18 @@ static bool upload_vec(amrvac_fv_ctx* c, T** dst, const std::vector<T>& v) {
   if (v.empty()) return true;
   if (hipMalloc(dst, sizeof(T) * v.size()) != hipSuccess) return false;
   return hipMemcpyAsync(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream) == hipSuccess;
+}
+
+// grow a device buffer of the context to hold n elements
+template <class T>
+static bool ensure_buf(T** buf, size_t* cap, size_t n) {
+  if (*buf && *cap >= n) return true;
+  if (*buf) (void)hipFree(*buf);
+  *buf = nullptr;
+  *cap = 0;
+  if (hipMalloc(buf, n * sizeof(T)) != hipSuccess) return false;
+  *cap = n;
+  return true;
 }
 
 static int getdt_core(amrvac_fv_ctx* c, const double* u, double courantpar, double* dtmin_out, double* cmax_out) {
@@ -1111,6 +1281,78 @@ int amrvac_fv_set_mesh_amr(amrvac_fv_ctx* c, int nblocks, const double* dx1, con
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_errest(amrvac_fv_ctx* c, int slot, int nwt, const int* iws, const double* weights, double wavefilter,
+                     double threshold, double coarsen_threshold, unsigned char* flags) {
+  if (!c || !check_slot(slot) || nwt < 1 || nwt > 64 || !iws || !weights || !flags)
+    return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  for (int t = 0; t < nwt; ++t)
+    if (iws[t] < 0 || iws[t] >= c->nw) return fail(AMRVAC_FV_EINVAL, "errest variable out of range");
+  if (c->cfg.nghost < 2) return fail(AMRVAC_FV_EINVAL, "the Löhner estimate reads two ghost layers");
+  if (!c->d_iws) {
+    if (hipMalloc(&c->d_iws, 64 * sizeof(int)) != hipSuccess || hipMalloc(&c->d_wts, 64 * sizeof(double)) != hipSuccess)
+      return fail(AMRVAC_FV_ENOMEM, "errest buffers");
+  }
+  if (!ensure_buf(&c->d_flags, &c->flags_cap, (size_t)c->nb)) return fail(AMRVAC_FV_ENOMEM, "errest flags");
+  HIPCHK(hipMemcpyAsync(c->d_iws, iws, nwt * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_wts, weights, nwt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  fvc::SweepArgs a = base_args(c);
+  a.src = c->slot[slot];
+  if (c->cfg.ndim == 2)
+    hipLaunchKernelGGL((fv::lohner_kernel<2>), dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, nwt,
+                       c->d_iws, c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags);
+  else
+    hipLaunchKernelGGL((fv::lohner_kernel<3>), dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, nwt,
+                       c->d_iws, c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(flags, c->d_flags, (size_t)c->nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_regrid(amrvac_fv_ctx* c, int nblocks, const int* plan, const double* dx1, const int* level,
+                     const int* ntype, const int* nbr, const int* nchild, const int* icpos, const double* xlohi,
+                     const int* bc) {
+  if (!c || !plan || nblocks < 1) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0] || !c->amr) return fail(AMRVAC_FV_ESTATE, "regrid needs an AMR mesh");
+  const int nd = c->cfg.ndim, S = 2 + (1 << nd), nold = c->nb;
+  for (int b = 0; b < nblocks; ++b) {
+    const int* p = plan + (size_t)b * S;
+    if (p[0] == 0 || p[0] == 2) {
+      if (p[1] < 0 || p[1] >= nold) return fail(AMRVAC_FV_EINVAL, "regrid plan: old block out of range");
+      if (p[0] == 2 && (p[S - 1] < 0 || p[S - 1] >= (1 << nd)))
+        return fail(AMRVAC_FV_EINVAL, "regrid plan: child position out of range");
+    } else if (p[0] == 1) {
+      for (int j = 0; j < (1 << nd); ++j)
+        if (p[1 + j] < 0 || p[1 + j] >= nold) return fail(AMRVAC_FV_EINVAL, "regrid plan: old child out of range");
+    } else {
+      return fail(AMRVAC_FV_EINVAL, "regrid plan: kind must be 0 (copy), 1 (coarsen) or 2 (refine)");
+    }
+  }
+  const size_t cells = (size_t)c->nw * c->cells;
+  if (!ensure_buf(&c->d_remap, &c->remap_cap, (size_t)nblocks * cells) ||
+      !ensure_buf(&c->d_plan, &c->plan_cap, (size_t)nblocks * S))
+    return fail(AMRVAC_FV_ENOMEM, "regrid buffers");
+  HIPCHK(hipMemcpyAsync(c->d_plan, plan, (size_t)nblocks * S * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  fvc::SweepArgs a = base_args(c);
+  a.src = c->slot[0];
+  if (nd == 2)
+    hipLaunchKernelGGL((fv::remap_kernel<2>), dim3(nblocks), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, c->d_plan,
+                       c->d_remap);
+  else
+    hipLaunchKernelGGL((fv::remap_kernel<3>), dim3(nblocks), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, c->d_plan,
+                       c->d_remap);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // the new connectivity (frees the old level), then the new leaves' data and
+  // their ghost cells
+  const int rc = amrvac_fv_set_mesh_amr(c, nblocks, dx1, level, ntype, nbr, nchild, icpos, xlohi, bc);
+  if (rc != AMRVAC_FV_OK) return rc;
+  HIPCHK(hipMemcpyAsync(c->slot[0], c->d_remap, (size_t)nblocks * cells * sizeof(double), hipMemcpyDeviceToDevice,
+                        c->stream));
+  return getbc_amr(c, 0);
 }
 
 int amrvac_fv_getbc(amrvac_fv_ctx* c, int slot) {

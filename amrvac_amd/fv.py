@@ -27,7 +27,8 @@ EXPORTS = ("amrvac_fv_init", "amrvac_fv_finalize", "amrvac_fv_set_stream", "amrv
            "amrvac_fv_copy", "amrvac_fv_volume_average", "amrvac_fv_slot_pointer",
            "amrvac_fv_sync", "amrvac_fv_sweep_timing", "amrvac_fv_last_error",
            "amrvac_fv_set_mesh_amr", "amrvac_fv_getbc", "amrvac_fv_set_fix_conserve", "amrvac_fv_adjoint1",
-           "amrvac_fv_slot_copy", "amrvac_fv_stage", "amrvac_fv_getdt_state")
+           "amrvac_fv_slot_copy", "amrvac_fv_stage", "amrvac_fv_getdt_state", "amrvac_fv_errest",
+           "amrvac_fv_regrid")
 
 
 class FVConfig(ctypes.Structure):
@@ -85,6 +86,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.amrvac_fv_adjoint1.argtypes = [vp, ctypes.c_int, dbl, dbl, vp, vp, dbl, vp, dbl, vp, dbl, vp]
     L.amrvac_fv_stage.argtypes = [vp, ctypes.c_int, dbl, dbl, dbl, vp, vp, dbl, dbl, vp]
     L.amrvac_fv_getdt_state.argtypes = [vp, vp, dbl, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+    L.amrvac_fv_errest.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, dp, dbl, dbl, dbl,
+                                   ctypes.POINTER(ctypes.c_ubyte)]
+    L.amrvac_fv_regrid.argtypes = [vp, ctypes.c_int, ip, dp, ip, ip, ip, ip, ip, dp, ip]
     L.amrvac_fv_slot_copy.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
     _lib = L
     return L
@@ -222,6 +226,57 @@ class DeviceFV:
         self._check(self.L.amrvac_fv_getdt_state(self.h, ctypes.c_void_p(u), courantpar, ctypes.byref(dt),
                                                  ctypes.byref(cm)))
         return dt.value, cm.value
+
+    # -- regrid on the device (AMR forests) ------------------------------------
+    @property
+    def amr_device_regrid(self) -> bool:
+        return self.amr
+
+    def errest_flags(self, forest, ap):
+        """errest (Löhner) of slot 0 on the device: (refine, coarsen) flags
+        per leaf with the level limits of errest applied; proper nesting
+        is the caller's."""
+        weights = sorted((iw, w) for iw, w in ap.w_refine_weight.items() if w != 0.0)
+        iws = np.ascontiguousarray([iw for iw, _ in weights], dtype=np.int32)
+        wts = np.ascontiguousarray([w for _, w in weights], dtype=np.float64)
+        thr = ap.refine_threshold
+        flags = np.zeros(forest.nblocks, dtype=np.uint8)
+        self._check(self.L.amrvac_fv_errest(self.h, 0, len(iws), _ip(iws), _dp(wts), ap.amr_wavefilter, thr,
+                                            ap.derefine_ratio * thr,
+                                            flags.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))))
+        lev = np.asarray(forest.level)
+        refine = ((flags & 1) != 0) & (lev < ap.refine_max_level)
+        coarsen = ((flags & 2) != 0) & (lev > 1)
+        return refine, coarsen
+
+    def regrid(self, plan, forest) -> None:
+        """Move slot 0 to the forest's new leaves on the device (plan from
+        amr.apply_regrid), then install the new connectivity and ghost
+        cells."""
+        nd = forest.ndim
+        S = 2 + (1 << nd)
+        P = np.zeros((forest.nblocks, S), dtype=np.int32)
+        for b, org in enumerate(plan):
+            if org[0] == "copy":
+                P[b, 0], P[b, 1] = 0, org[1]
+            elif org[0] == "coarsen":
+                P[b, 0] = 1
+                P[b, 1:1 + (1 << nd)] = org[1]
+            else:
+                P[b, 0], P[b, 1] = 2, org[1]
+                P[b, S - 1] = sum((org[2][d] - 1) << d for d in range(nd))
+        xl = np.ascontiguousarray(forest.xlohi, dtype=np.float64)
+        bc = np.ascontiguousarray(forest.bc_codes, dtype=np.int32)
+        dx1 = np.ascontiguousarray(forest.dx[1], dtype=np.float64)
+        lev = np.ascontiguousarray(forest.level, dtype=np.int32)
+        nt = np.ascontiguousarray(forest.ntype, dtype=np.int32)
+        nbr = np.ascontiguousarray(forest.nbr, dtype=np.int32)
+        nch = np.ascontiguousarray(forest.nchild, dtype=np.int32)
+        icp = np.ascontiguousarray(forest.icpos, dtype=np.int32)
+        self._check(self.L.amrvac_fv_regrid(self.h, forest.nblocks, _ip(np.ascontiguousarray(P)), _dp(dx1),
+                                            _ip(lev), _ip(nt), _ip(nbr), _ip(nch), _ip(icp), _dp(xl), _ip(bc)))
+        self.mesh = forest
+        self.shape = (forest.nblocks, self.nw) + forest.nxg[::-1]
 
     def set_fix_conserve(self, on: bool) -> None:
         self._check(self.L.amrvac_fv_set_fix_conserve(self.h, 1 if on else 0))

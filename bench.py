@@ -149,6 +149,46 @@ def adjoint_cost(kind: str, reps: int = 2):
             "forward_backward_ms": round(tg * 1e3, 2), "backward_over_forward": round((tg - tf) / tf, 3)}
 
 
+def amr_blast(precision: str, steps: int, noise: float = 0.01):
+    """The block-structured AMR workload: 2-D HD blast wave (rho = 1 with
+    seeded 1% noise, p = 1e-2, p = 10 for r < 0.05 at the centre), base
+    grid 64^2 in 16^2 blocks, 4 levels (effective 512^2), Löhner on rho and
+    e (weights 0.5/0.5, tol 0.1) with a regrid every step, twostep / TVDLF
+    / minmod / cmaxmean, CFL 0.9.  Estimation and data movement of the
+    regrid run on the device; the tree stays on the host.  Cell-updates
+    count the leaf cells of each step times its two stages.  The 1% noise
+    puts every block at the finest level (the estimator sees it as
+    structure); noise=0 gives a tree that follows the blast."""
+    from amrvac_amd import problems
+    from amrvac_amd.amr import AMRForest, AMRIntegrator, AMRPar
+    from amrvac_amd.fv import DeviceFV
+    from amrvac_amd.host import Par
+    par = Par(physics="hd", ndim=2, ndir=2, gamma=5.0 / 3.0, time_stepper="twostep", flux_scheme="tvdlf",
+              limiter="minmod", typeboundspeed="cmaxmean", courantpar=0.9, precision=precision)
+    forest = AMRForest(2, (64, 64), (16, 16), (0.0, 0.0), (1.0, 1.0), ("cont",) * 4, max_level=4)
+    ap = AMRPar(refine_max_level=4, w_refine_weight={0: 0.5, 3: 0.5})
+
+    def init(f, blocks):
+        return problems.blast_wave_blocks(f, blocks, rho0=1.0, p_out=1.0e-2, p_in=10.0, radius=0.05, noise=noise)
+    ti = AMRIntegrator(par, forest, ap, lambda p, f: DeviceFV(p, f), init)
+    ti.run(nsteps=2)
+    ti.be.sync()
+    regrids0 = ti.regrids
+    updates = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        updates += forest.nblocks * 256 * 2
+        ti.run(nsteps=1)
+    ti.be.sync()
+    dt = time.perf_counter() - t0
+    print(f"[bench] amr blast {precision}: {steps} steps in {dt:.3f} s, {forest.nblocks} leaves, "
+          f"levels {forest.levmin}-{forest.levmax}, {ti.regrids - regrids0} regrids", file=sys.stderr, flush=True)
+    return {"value": round(updates / dt / 1e6, 2), "unit": "Mcell-updates/s", "ms_per_step": round(dt / steps * 1e3, 3),
+            "noise": noise,
+            "steps": steps, "leaves": forest.nblocks, "levmax": forest.levmax,
+            "regrids": ti.regrids - regrids0}
+
+
 def load_traffic(name: str):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -169,6 +209,8 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost legs")
+    ap.add_argument("--no-amr", action="store_true", help="skip the AMR blast-wave leg")
+    ap.add_argument("--amr-steps", type=int, default=200)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy one state slot device-to-device (a plain copy of "
                          "known size, the reference point for HBM byte counters)")
@@ -239,6 +281,11 @@ def main() -> int:
         del w64, wmx
     if not args.no_adjoint:
         result["adjoint_forward_cost"] = {k: adjoint_cost(k) for k in ("kh2d", "ot3d")}
+    if not args.no_amr:
+        result["amr_blast"] = {"workload": "2-D HD blast wave, 64^2 base in 16^2 blocks, 4 levels (512^2 effective), "
+                                           "Loehner regrid every step, twostep/TVDLF/minmod",
+                               **{p: amr_blast(p, args.amr_steps) for p in ("fp64", "mixed")},
+                               "fp64_noise_free": amr_blast("fp64", args.amr_steps, noise=0.0)}
     if args.cpu_seconds > 0:
         try:
             result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

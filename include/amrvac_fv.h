@@ -107,6 +107,31 @@ int amrvac_fv_set_mesh_amr(amrvac_fv_ctx* ctx, int nblocks, const double* dx1, c
    psb itself. */
 int amrvac_fv_getbc(amrvac_fv_ctx* ctx, int slot);
 
+/* Replaces: errest with the Löhner estimator (src/amr/mod_errest.t:10-199,
+   lohner_grid; Cartesian, no B0, no log variables) on every leaf of slot
+   `slot` (ghost cells as getbc leaves them).  nwt variables iws[] with
+   weights[] (w_refine_weight, zero weights left out, ascending iw).
+   flags[nb]: bit 0 = some cell has error >= threshold (refine candidate),
+   bit 1 = every cell has error < threshold and <= coarsen_threshold
+   (derefine_ratio*threshold; coarsen candidate).  The level limits and
+   proper nesting stay with the host. */
+int amrvac_fv_errest(amrvac_fv_ctx* ctx, int slot, int nwt, const int* iws,
+                     const double* weights, double wavefilter, double threshold,
+                     double coarsen_threshold, unsigned char* flags);
+
+/* Replaces: the data movement of amr_coarsen_refine
+   (src/amr/mod_coarsen_refine.t:15-173): coarsen_grid
+   (src/amr/mod_coarsen.t:12-79) and prolong_2nd (src/amr/mod_refine.t:
+   43-167), then build_connectivity and getbc of the new leaves.
+   plan[nblocks*(2+2^ndim)] per new leaf: kind 0 copy (old block), 1
+   coarsen (the 2^ndim old children in child order), 2 refine (old parent,
+   and last the child position sum_d (ic_d-1) 2^d).  The remaining
+   arguments describe the new leaves as for amrvac_fv_set_mesh_amr.  Slot 0
+   is remapped; slots 1 and 2 are reallocated without data. */
+int amrvac_fv_regrid(amrvac_fv_ctx* ctx, int nblocks, const int* plan, const double* dx1,
+                     const int* level, const int* ntype, const int* nbr, const int* nchild,
+                     const int* icpos, const double* xlohi, const int* bc);
+
 /* Replaces: fix_conserve_at_step (src/mod_advance.t:63-111): when on,
    advect1 on an AMR mesh stores the block-boundary fluxes (store_flux,
    src/mod_fix_conserve.t:597-648) and corrects the coarse cells next to

@@ -135,3 +135,54 @@ def test_amr_3d_exact_matches_oracle_bitwise():
     ti_orc.run(nsteps=4)
     assert ti_dev.state.dts == ti_orc.state.dts
     assert np.array_equal(f_dev.interior(ti_dev.be.get_state(0)), f_orc.interior(ti_orc.be.get_state(0)))
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_device_errest_and_regrid_match_host(ndim):
+    """The device Löhner flags and remap (coarsen_grid, prolong_2nd) against
+    the host's on the same ghosted state: flags identical, new leaves bit
+    for bit."""
+    from amrvac_amd.amr import apply_regrid, coarsen_child, errest_flags, prolong_2nd, proper_nesting
+    from amrvac_amd.fv import DeviceFV
+    print(f"[amr] device errest/regrid vs host, {ndim}-D", flush=True)
+    if ndim == 2:
+        par, forest, ap = _blast("3step_hll_cada", 0.02)
+    else:
+        par = Par(physics="hd", ndim=3, ndir=3, gamma=5.0 / 3.0, typeboundspeed="cmaxmean", courantpar=0.9,
+                  time_stepper="twostep", flux_scheme="tvdlf", limiter="minmod")
+        forest = AMRForest(3, (32, 32, 32), (16, 16, 16), (0.0,) * 3, (2.0,) * 3, ("cont",) * 6, max_level=3)
+        ap = AMRPar(refine_max_level=3, w_refine_weight={0: 0.5, 4: 0.5})
+    init = lambda f, b: problems.blast_wave_blocks(f, b, radius=0.3)
+    ti = AMRIntegrator(par, forest, ap, lambda p, f: DeviceFV(p, f, precision="fp64_exact"), init)
+    checked = 0
+    for _ in range(80):
+        if checked >= 2:
+            break
+        ti.setdt()
+        ti.advect()
+        ti.state.it += 1
+        ti.state.global_time += ti.state.dt
+        be = ti.be
+        w = be.get_state(0)
+        r_dev, c_dev = be.errest_flags(forest, ap)
+        r_host, c_host = errest_flags(forest, w, ap)
+        assert np.array_equal(r_dev, r_host) and np.array_equal(c_dev, c_host)
+        proper_nesting(forest, ap, r_dev, c_dev)
+        if not r_dev.any() and not c_dev.any():
+            continue
+        plan = apply_regrid(forest, r_dev, c_dev)
+        nd, ng = forest.ndim, forest.nghost
+        new = np.zeros((forest.nblocks,) + w.shape[1:])
+        for b, org in enumerate(plan):
+            if org[0] == "copy":
+                new[b] = w[org[1]]
+            elif org[0] == "coarsen":
+                for j, ob in enumerate(org[1]):
+                    coarsen_child(w[ob], tuple(1 + ((j >> d) & 1) for d in range(nd)), new[b], nd, ng)
+            else:
+                prolong_2nd(w[org[1]], org[2], new[b], nd, ng)
+        be.regrid(plan, forest)
+        got = be.get_state(0)
+        assert np.array_equal(forest.interior(got), forest.interior(new))
+        checked += 1
+    assert checked > 0
