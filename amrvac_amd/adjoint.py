@@ -50,6 +50,8 @@ class SSPRK3Run:
         self.c = ssprk3_coefficients()
         self.dts: list[float] | None = None
         self.checkpoints: dict[int, torch.Tensor] = {}
+        self._tape_buf: torch.Tensor | None = None
+        self._tape_segment = -1
 
     def _bind_stream(self) -> None:
         self.be.set_stream(torch.cuda.current_stream().cuda_stream)
@@ -61,9 +63,19 @@ class SSPRK3Run:
         be.stage(m, c["beta22"], dt, 0.0, u1, u2, base=u, comb_a=c["alfa21"], comb_b=c["alfa22"])
         be.stage(m, c["beta33"], dt, 0.0, u2, out, base=u, comb_a=c["alfa31"], comb_b=c["alfa33"])
 
+    def _tape(self, like: torch.Tensor) -> torch.Tensor:
+        """Stage states of one segment: tape[j] = (u0, u1, u2) of its step j
+        (every+1 entries, the last holding the segment's end state)."""
+        if getattr(self, "_tape_buf", None) is None or self._tape_buf.device != like.device:
+            self._tape_buf = torch.empty((self.every + 1, 3) + tuple(like.shape), dtype=torch.float64,
+                                         device=like.device)
+        return self._tape_buf
+
     def forward(self, u0: torch.Tensor, dts: list[float] | None = None) -> torch.Tensor:
         """Final state; keeps checkpoints for backward.  With ``dts`` the
-        given time steps are replayed instead of the CFL ones."""
+        given time steps are replayed instead of the CFL ones.  The last
+        segment runs on the stage tape, so the backward pass starts
+        without recomputing it."""
         if u0.shape != self.shape or u0.dtype != torch.float64 or not u0.is_cuda:
             raise ValueError(f"initial state must be a float64 CUDA tensor of shape {self.shape}")
         if dts is not None and len(dts) != self.nsteps:
@@ -73,16 +85,31 @@ class SSPRK3Run:
         u1 = torch.empty_like(u)
         u2 = torch.empty_like(u)
         self.checkpoints = {0: u.clone()}
+        n_last = ((self.nsteps - 1) // self.every) * self.every
+        tape = self._tape(u)
+        out = torch.empty_like(u)
         used = []
         for n in range(self.nsteps):
-            dt = dts[n] if dts is not None else self.be.getdt_state(u.data_ptr(), self.par.courantpar)[0]
+            if n < n_last:
+                cur = u.data_ptr()
+            else:
+                if n == n_last:
+                    tape[0, 0].copy_(u)
+                cur = tape[n - n_last, 0].data_ptr()
+            dt = dts[n] if dts is not None else self.be.getdt_state(cur, self.par.courantpar)[0]
             used.append(dt)
-            # the third stage overwrites u in place (its RK base, read per cell)
-            self._step(u.data_ptr(), u1.data_ptr(), u2.data_ptr(), u.data_ptr(), dt)
-            if (n + 1) % self.every == 0 and n + 1 < self.nsteps:
-                self.checkpoints[n + 1] = u.clone()
+            if n < n_last:
+                # the third stage overwrites u in place (its RK base, read per cell)
+                self._step(cur, u1.data_ptr(), u2.data_ptr(), cur, dt)
+                if (n + 1) % self.every == 0:
+                    self.checkpoints[n + 1] = u.clone()
+            else:
+                j = n - n_last
+                nxt = out.data_ptr() if n + 1 == self.nsteps else tape[j + 1, 0].data_ptr()
+                self._step(cur, tape[j, 1].data_ptr(), tape[j, 2].data_ptr(), nxt, dt)
         self.dts = used
-        return u
+        self._tape_segment = n_last
+        return out
 
     def backward(self, lam: torch.Tensor) -> torch.Tensor:
         """Adjoint of the whole run: d(final)/d(initial)^T lam."""
@@ -92,21 +119,22 @@ class SSPRK3Run:
         l1 = torch.empty_like(lam)
         l2 = torch.empty_like(lam)
         starts = sorted(self.checkpoints)
-        # stage states of one segment: tape[j] = (u0, u1, u2) of its step j
-        tape = torch.empty((self.every, 3) + tuple(lam.shape), dtype=torch.float64, device=lam.device)
+        tape = self._tape(lam)
         for s_i in range(len(starts) - 1, -1, -1):
             n0 = starts[s_i]
             n1 = starts[s_i + 1] if s_i + 1 < len(starts) else self.nsteps
-            # recompute the segment from its checkpoint
-            tape[0, 0].copy_(self.checkpoints[n0])
-            for j in range(n1 - n0):
-                dt = self.dts[n0 + j]
-                u0, u1, u2 = (tape[j, q].data_ptr() for q in range(3))
-                be.stage(m, c["beta11"], dt, 0.0, u0, u1)
-                be.stage(m, c["beta22"], dt, 0.0, u1, u2, base=u0, comb_a=c["alfa21"], comb_b=c["alfa22"])
-                if j + 1 < n1 - n0:
-                    be.stage(m, c["beta33"], dt, 0.0, u2, tape[j + 1, 0].data_ptr(), base=u0,
-                             comb_a=c["alfa31"], comb_b=c["alfa33"])
+            if n0 != self._tape_segment:
+                # recompute the segment from its checkpoint
+                tape[0, 0].copy_(self.checkpoints[n0])
+                for j in range(n1 - n0):
+                    dt = self.dts[n0 + j]
+                    u0, u1, u2 = (tape[j, q].data_ptr() for q in range(3))
+                    be.stage(m, c["beta11"], dt, 0.0, u0, u1)
+                    be.stage(m, c["beta22"], dt, 0.0, u1, u2, base=u0, comb_a=c["alfa21"], comb_b=c["alfa22"])
+                    if j + 1 < n1 - n0:
+                        be.stage(m, c["beta33"], dt, 0.0, u2, tape[j + 1, 0].data_ptr(), base=u0,
+                                 comb_a=c["alfa31"], comb_b=c["alfa33"])
+            self._tape_segment = -1  # the tape is reused from here on
             for j in range(n1 - n0 - 1, -1, -1):
                 dt = self.dts[n0 + j]
                 u0, u1, u2 = (tape[j, q].data_ptr() for q in range(3))

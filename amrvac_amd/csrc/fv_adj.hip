@@ -6,17 +6,20 @@
 //   mu_f = (-qdt/dx_d) (lam_out(left of f) - lam_out(right of f)).
 //
 // adj_plane_kernel: one workgroup per (block, z plane).  The plane's
-// primitive states with a 3-cell x/y halo are staged in LDS (each cell
-// converted once); every x and y face whose 4-cell stencil reaches the tile
-// (BX+3 faces per row) is recomputed and reversed once, its four primitive
-// adjoints written to an LDS face buffer, and each cell then sums the
-// contributions of its four faces per dimension in a fixed order (no
-// atomics: the result is deterministic).  The summed primitive adjoint is
-// carried to the conserved state once per cell.  In 3-D the z faces are
-// reversed beforehand by adj_zface_kernel (each lane a run of ZC faces of
-// one column, into a global face buffer in the face precision) and gathered
-// here in the same fixed order.
+// primitive states with a 3-cell x/y halo and lam_out with a 2-cell halo
+// are staged in LDS (each cell converted once); every x and y face whose
+// 4-cell stencil reaches the tile (BX+3 faces per row) is recomputed and
+// reversed once, and its four primitive adjoints are added to per-cell LDS
+// sums in four passes (one per stencil position; within a pass the faces
+// write distinct cells), so the summation order is fixed and the result
+// deterministic without atomics.  The summed primitive adjoint is carried to
+// the conserved state once per cell.  In 3-D the z faces are reversed
+// beforehand by adj_zface_kernel (each lane a run of ZC faces of one
+// column, into a global face buffer in the face precision) and gathered
+// here in a fixed order.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "fv_adjoint.h"
 #include "fv_launch.h"
@@ -89,13 +92,20 @@ __global__ void __launch_bounds__(NTHREADS) adj_zface_kernel(SweepArgs a, FVPara
   }
 }
 
+// occupancy: the FP32-face kernels keep to 3 workgroups per CU
+template <class P, class R>
+constexpr int kAdjWaves = std::is_same<R, float>::value ? 3 : (P::NW > 5 ? 1 : 2);
+
 template <class P, class R, int RIEM, int LIM>
-__global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVParams fp, AdjArgs g) {
+__global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R>)) adj_plane_kernel(SweepArgs a, FVParams fp, AdjArgs g) {
   constexpr int NW = P::NW, NDIM = P::NDIM;
+  // lam_out is kept in the face precision: mu is rounded to R anyway
   __shared__ R Wl[NW][AW * AW];
-  __shared__ double Ll[NW][LW * LW];
-  // face contributions: computed in R, so stored in R without rounding
-  __shared__ R Gl[4 * NW][NFACE];
+  __shared__ R Ll[NW][LW * LW];
+  // per-cell sums of the faces' primitive adjoints, filled in four passes
+  // per face round (stencil position s = 0..3): within a pass the faces of
+  // a round write distinct cells, so the order of the sums is fixed
+  __shared__ double Acc[NW][BX * BX];
   const int b = blockIdx.x / a.bnz, k = blockIdx.x % a.bnz;
   if (b >= a.nblocks) return;
   const int t = threadIdx.x;
@@ -103,7 +113,6 @@ __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVPara
   const size_t cells = (size_t)BX * BX * a.bnz;
   const size_t o = (size_t)b * NW * cells + ((size_t)k * BX + ty) * BX + tx;
 
-  // ---- primitive plane with its x/y halo (corners are not read) ----------
   for (int i = t; i < AW * AW; i += NTHREADS) {
     const int x = i % AW - AH, y = i / AW - AH;
     if ((x < 0 || x >= BX) && (y < 0 || y >= BX)) continue;
@@ -112,81 +121,83 @@ __global__ void __launch_bounds__(NTHREADS) adj_plane_kernel(SweepArgs a, FVPara
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) Wl[iw][i] = w[iw];
   }
-  // lam_out of the plane with a 2-cell x/y halo (corners are not read)
   for (int i = t; i < LW * LW; i += NTHREADS) {
     const int x = i % LW - LH, y = i / LW - LH;
     if ((x < 0 || x >= BX) && (y < 0 || y >= BX)) continue;
     double l[NW];
     load_lam<NW, NDIM>(a, g.lam_out, b, x, y, k, l);
 #pragma unroll
-    for (int iw = 0; iw < NW; ++iw) Ll[iw][i] = l[iw];
+    for (int iw = 0; iw < NW; ++iw) Ll[iw][i] = R(l[iw]);
   }
-  double acc[NW], wsum[NW];
+  double acc[NW];
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
     double v = g.coef * g.lam_out[o + iw * cells];
     if (g.add_a) v = v + g.ca * g.add_a[o + iw * cells];
     if (g.add_b) v = v + g.cb * g.add_b[o + iw * cells];
     acc[iw] = v;
-    wsum[iw] = 0.0;
+    Acc[iw][t] = 0.0;
   }
   __syncthreads();
 
   // ---- x faces: face j = (f, row), f = j % NF - 1 ------------------------
-  for (int j = t; j < NFACE; j += NTHREADS) {
+  for (int j0 = 0; j0 < NFACE; j0 += NTHREADS) {
+    const int j = j0 + t;
+    const bool on = j < NFACE;
     const int f = j % NF - 1, row = j / NF;
-    R st[4][NW];
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) st[s][iw] = Wl[iw][(row + AH) * AW + f - 2 + s + AH];
-    R mu[NW];
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw)
-      mu[iw] = R(a.dq[0] * (Ll[iw][(row + LH) * LW + f - 1 + LH] - Ll[iw][(row + LH) * LW + f + LH]));
     double wb[4][NW];
-    face_adj_prim<P, R, RIEM, 0, LIM>(st, fp, mu, wb);
+    if (on) {
+      R st[4][NW];
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+      for (int s = 0; s < 4; ++s)
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) Gl[s * NW + iw][j] = R(wb[s][iw]);
+        for (int iw = 0; iw < NW; ++iw) st[s][iw] = Wl[iw][(row + AH) * AW + f - 2 + s + AH];
+      R mu[NW];
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw)
+        mu[iw] = R(a.dq[0] * (double(Ll[iw][(row + LH) * LW + f - 1 + LH]) - double(Ll[iw][(row + LH) * LW + f + LH])));
+      face_adj_prim<P, R, RIEM, 0, LIM>(st, fp, mu, wb);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      __syncthreads();
+      const int x = f - 2 + s;
+      if (on && x >= 0 && x < BX)
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) Acc[iw][row * BX + x] += wb[s][iw];
+    }
   }
-  __syncthreads();
-  // cell tx is stencil position s = tx - f + 2 of faces f = tx+2 .. tx-1
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int j = ty * NF + (tx - s + 2) + 1;
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) wsum[iw] += double(Gl[s * NW + iw][j]);
-  }
-  __syncthreads();
-
   // ---- y faces: face j = (col, f), col = j % BX, f = j / BX - 1 ----------
-  for (int j = t; j < NFACE; j += NTHREADS) {
+  for (int j0 = 0; j0 < NFACE; j0 += NTHREADS) {
+    const int j = j0 + t;
+    const bool on = j < NFACE;
     const int col = j % BX, f = j / BX - 1;
-    R st[4][NW];
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) st[s][iw] = Wl[iw][(f - 2 + s + AH) * AW + col + AH];
-    R mu[NW];
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw)
-      mu[iw] = R(a.dq[1] * (Ll[iw][(f - 1 + LH) * LW + col + LH] - Ll[iw][(f + LH) * LW + col + LH]));
     double wb[4][NW];
-    face_adj_prim<P, R, RIEM, 1, LIM>(st, fp, mu, wb);
+    if (on) {
+      R st[4][NW];
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+      for (int s = 0; s < 4; ++s)
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) Gl[s * NW + iw][j] = R(wb[s][iw]);
+        for (int iw = 0; iw < NW; ++iw) st[s][iw] = Wl[iw][(f - 2 + s + AH) * AW + col + AH];
+      R mu[NW];
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw)
+        mu[iw] = R(a.dq[1] * (double(Ll[iw][(f - 1 + LH) * LW + col + LH]) - double(Ll[iw][(f + LH) * LW + col + LH])));
+      face_adj_prim<P, R, RIEM, 1, LIM>(st, fp, mu, wb);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      __syncthreads();
+      const int y = f - 2 + s;
+      if (on && y >= 0 && y < BX)
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) Acc[iw][y * BX + col] += wb[s][iw];
+    }
   }
   __syncthreads();
+  double wsum[NW];
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int j = (ty - s + 2 + 1) * BX + tx;
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) wsum[iw] += double(Gl[s * NW + iw][j]);
-  }
+  for (int iw = 0; iw < NW; ++iw) wsum[iw] = Acc[iw][t];
 
   // ---- z faces (3-D): reversed by adj_zface_kernel ------------------------
   if constexpr (NDIM == 3) {
