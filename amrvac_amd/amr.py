@@ -567,7 +567,8 @@ class AMRIntegrator(TimeIntegrator):
         else:
             w = self.be.get_state(0)
             refine, coarsen = errest_flags(forest, w, self.ap)
-        proper_nesting(forest, self.ap, refine, coarsen)
+        if refine.any() or coarsen.any():
+            proper_nesting(forest, self.ap, refine, coarsen)
         if not refine.any() and not coarsen.any():
             return
         if device and time_advance:
