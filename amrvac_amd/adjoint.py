@@ -35,7 +35,7 @@ class SSPRK3Run:
     (amrvac_fv_stage / amrvac_fv_adjoint1): no copies between steps."""
 
     def __init__(self, par: Par, mesh: UniformMesh, nsteps: int, every: int | None = None,
-                 precision: str | None = None):
+                 precision: str | None = None, backend=None):
         if par.time_stepper != "threestep":
             raise ValueError("the differentiable run integrates with threestep")
         if any(b != "periodic" for b in mesh.bc):
@@ -44,7 +44,11 @@ class SSPRK3Run:
         self.mesh = mesh
         self.nsteps = nsteps
         self.every = every or max(1, int(math.sqrt(nsteps)))
-        self.be = DeviceFV(par, mesh, precision=precision)
+        # the backend: DeviceFV (the HIP kernels), or an object with the same
+        # stage / adjoint1 / getdt_state / set_stream methods on the
+        # tensors' memory (the driver tests use a CPU one)
+        self.be = backend if backend is not None else DeviceFV(par, mesh, precision=precision)
+        self.device_backend = backend is None
         self.shape = (mesh.nblocks, par.nw) + tuple(mesh.block_nx[::-1])
         self.method = FS[par.flux_scheme]
         self.c = ssprk3_coefficients()
@@ -54,7 +58,8 @@ class SSPRK3Run:
         self._tape_segment = -1
 
     def _bind_stream(self) -> None:
-        self.be.set_stream(torch.cuda.current_stream().cuda_stream)
+        if self.device_backend:
+            self.be.set_stream(torch.cuda.current_stream().cuda_stream)
 
     def _step(self, u: int, u1: int, u2: int, out: int, dt: float) -> None:
         """out = ssprk3 step of u (pointers); u1, u2 receive the stage states."""
@@ -76,8 +81,9 @@ class SSPRK3Run:
         given time steps are replayed instead of the CFL ones.  The last
         segment runs on the stage tape, so the backward pass starts
         without recomputing it."""
-        if u0.shape != self.shape or u0.dtype != torch.float64 or not u0.is_cuda:
-            raise ValueError(f"initial state must be a float64 CUDA tensor of shape {self.shape}")
+        if u0.shape != self.shape or u0.dtype != torch.float64 or u0.is_cuda != self.device_backend:
+            raise ValueError(f"initial state must be a float64 {'CUDA ' if self.device_backend else ''}tensor "
+                             f"of shape {self.shape}")
         if dts is not None and len(dts) != self.nsteps:
             raise ValueError("dts must hold one time step per step")
         self._bind_stream()
