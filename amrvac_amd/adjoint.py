@@ -49,6 +49,7 @@ class SSPRK3Run:
         # tensors' memory (the driver tests use a CPU one)
         self.be = backend if backend is not None else DeviceFV(par, mesh, precision=precision)
         self.device_backend = backend is None
+        self.precision = precision
         self.shape = (mesh.nblocks, par.nw) + tuple(mesh.block_nx[::-1])
         self.method = FS[par.flux_scheme]
         self.c = ssprk3_coefficients()
@@ -117,41 +118,98 @@ class SSPRK3Run:
         self._tape_segment = n_last
         return out
 
-    def backward(self, lam: torch.Tensor) -> torch.Tensor:
-        """Adjoint of the whole run: d(final)/d(initial)^T lam."""
-        self._bind_stream()
+    def _recompute(self, be, tape: torch.Tensor, n0: int, n1: int) -> None:
+        """Stage states of steps n0..n1-1 into tape, from the checkpoint."""
+        c, m = self.c, self.method
+        tape[0, 0].copy_(self.checkpoints[n0])
+        for j in range(n1 - n0):
+            dt = self.dts[n0 + j]
+            u0, u1, u2 = (tape[j, q].data_ptr() for q in range(3))
+            be.stage(m, c["beta11"], dt, 0.0, u0, u1)
+            be.stage(m, c["beta22"], dt, 0.0, u1, u2, base=u0, comb_a=c["alfa21"], comb_b=c["alfa22"])
+            if j + 1 < n1 - n0:
+                be.stage(m, c["beta33"], dt, 0.0, u2, tape[j + 1, 0].data_ptr(), base=u0,
+                         comb_a=c["alfa31"], comb_b=c["alfa33"])
+
+    def _reverse(self, tape: torch.Tensor, n0: int, n1: int, lam, l1, l2) -> None:
+        """The adjoint stages of steps n1-1..n0 from the stage tape."""
         c, m, be = self.c, self.method, self.be
+        for j in range(n1 - n0 - 1, -1, -1):
+            dt = self.dts[n0 + j]
+            u0, u1, u2 = (tape[j, q].data_ptr() for q in range(3))
+            # u3 = a31 u0 + a33 u2 + L3(u2)
+            be.adjoint1(m, c["beta33"], dt, u2, lam.data_ptr(), c["alfa33"], l2.data_ptr())
+            # u2 = a21 u0 + a22 u1 + L2(u1)
+            be.adjoint1(m, c["beta22"], dt, u1, l2.data_ptr(), c["alfa22"], l1.data_ptr())
+            # u1 = u0 + L1(u0); lam_u0 = l1 + L1^T l1 + a31 lam + a21 l2 (in place)
+            be.adjoint1(m, c["beta11"], dt, u0, l1.data_ptr(), 1.0, lam.data_ptr(),
+                        add_a=lam.data_ptr(), ca=c["alfa31"], add_b=l2.data_ptr(), cb=c["alfa21"])
+
+    def backward(self, lam: torch.Tensor) -> torch.Tensor:
+        """Adjoint of the whole run: d(final)/d(initial)^T lam.
+
+        On the device the recompute of the next segment (on a second stream
+        and context, into a second tape) overlaps the reverse sweep of the
+        current one; events order each tape's reuse."""
+        self._bind_stream()
         lam = lam.detach().to(torch.float64).contiguous().clone()
         l1 = torch.empty_like(lam)
         l2 = torch.empty_like(lam)
         starts = sorted(self.checkpoints)
-        tape = self._tape(lam)
-        for s_i in range(len(starts) - 1, -1, -1):
-            n0 = starts[s_i]
-            n1 = starts[s_i + 1] if s_i + 1 < len(starts) else self.nsteps
-            if n0 != self._tape_segment:
-                # recompute the segment from its checkpoint
-                tape[0, 0].copy_(self.checkpoints[n0])
-                for j in range(n1 - n0):
-                    dt = self.dts[n0 + j]
-                    u0, u1, u2 = (tape[j, q].data_ptr() for q in range(3))
-                    be.stage(m, c["beta11"], dt, 0.0, u0, u1)
-                    be.stage(m, c["beta22"], dt, 0.0, u1, u2, base=u0, comb_a=c["alfa21"], comb_b=c["alfa22"])
-                    if j + 1 < n1 - n0:
-                        be.stage(m, c["beta33"], dt, 0.0, u2, tape[j + 1, 0].data_ptr(), base=u0,
-                                 comb_a=c["alfa31"], comb_b=c["alfa33"])
-            self._tape_segment = -1  # the tape is reused from here on
-            for j in range(n1 - n0 - 1, -1, -1):
-                dt = self.dts[n0 + j]
-                u0, u1, u2 = (tape[j, q].data_ptr() for q in range(3))
-                # u3 = a31 u0 + a33 u2 + L3(u2)
-                be.adjoint1(m, c["beta33"], dt, u2, lam.data_ptr(), c["alfa33"], l2.data_ptr())
-                # u2 = a21 u0 + a22 u1 + L2(u1)
-                be.adjoint1(m, c["beta22"], dt, u1, l2.data_ptr(), c["alfa22"], l1.data_ptr())
-                # u1 = u0 + L1(u0); lam_u0 = l1 + L1^T l1 + a31 lam + a21 l2 (in place)
-                be.adjoint1(m, c["beta11"], dt, u0, l1.data_ptr(), 1.0, lam.data_ptr(),
-                            add_a=lam.data_ptr(), ca=c["alfa31"], add_b=l2.data_ptr(), cb=c["alfa21"])
+        segs = [(n0, starts[i + 1] if i + 1 < len(starts) else self.nsteps) for i, n0 in enumerate(starts)][::-1]
+        have_last = segs[0][0] == self._tape_segment
+        self._tape_segment = -1  # the tapes are reused from here on
+        if not self.device_backend:
+            tape = self._tape(lam)
+            for i, (n0, n1) in enumerate(segs):
+                if i > 0 or not have_last:
+                    self._recompute(self.be, tape, n0, n1)
+                self._reverse(tape, n0, n1, lam, l1, l2)
+            return lam
+        tapes = [self._tape(lam), self._tape2(lam)]
+        main = torch.cuda.current_stream()
+        side = self._side_stream()
+        side.wait_stream(main)
+        ready: list = [None] * len(segs)
+        free: list = [None, None]
+
+        def launch_recompute(i: int) -> None:
+            k = i % 2
+            with torch.cuda.stream(side):
+                if free[k] is not None:
+                    side.wait_event(free[k])
+                self._recompute(self._side_be, tapes[k], *segs[i])
+                ev = torch.cuda.Event()
+                ev.record(side)
+                ready[i] = ev
+
+        if not have_last:
+            launch_recompute(0)
+        for i, (n0, n1) in enumerate(segs):
+            if i + 1 < len(segs):
+                launch_recompute(i + 1)
+            if ready[i] is not None:
+                main.wait_event(ready[i])
+            self._reverse(tapes[i % 2], n0, n1, lam, l1, l2)
+            ev = torch.cuda.Event()
+            ev.record(main)
+            free[i % 2] = ev
+        main.wait_stream(side)
         return lam
+
+    def _tape2(self, like: torch.Tensor) -> torch.Tensor:
+        if getattr(self, "_tape_buf2", None) is None or self._tape_buf2.device != like.device:
+            self._tape_buf2 = torch.empty_like(self._tape(like))
+        return self._tape_buf2
+
+    def _side_stream(self):
+        """A second stream (default priority) and a second context on it for
+        the overlapped recompute."""
+        if getattr(self, "_side", None) is None:
+            self._side = torch.cuda.Stream()
+            self._side_be = DeviceFV(self.par, self.mesh, precision=self.precision)
+            self._side_be.set_stream(self._side.cuda_stream)
+        return self._side
 
 
 class _RunFunction(torch.autograd.Function):
