@@ -702,7 +702,7 @@ int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
     return fail(AMRVAC_FV_EINVAL, "block_nx(1) and block_nx(2) must be 16");
   if (c.ndim == 3 && c.block_nx[2] < 4) return fail(AMRVAC_FV_EINVAL, "block_nx(3) must be at least 4");
   const bool weno = c.limiter == 16;
-  if (!weno && !(c.limiter >= 1 && c.limiter <= 7) && c.limiter != 9)
+  if (!weno && !(c.limiter >= 1 && c.limiter <= 9))
     return fail(AMRVAC_FV_EINVAL, "unsupported limiter code");
   if (c.nghost != (weno ? 3 : 2))
     return fail(AMRVAC_FV_EINVAL, weno ? "weno5 needs nghostcells = 3" : "nghostcells must be 2");

@@ -119,7 +119,7 @@ enum { DIVB_NONE = 0, DIVB_GLM = 1, DIVB_LINDE = 4 };
 enum { RIEM_HLL = 1, RIEM_HLLC = 2, RIEM_TVDLF = 5, RIEM_HANCOCK = 8 };
 enum {
   LIM_MINMOD = 1, LIM_WOODWARD = 2, LIM_MCBETA = 3, LIM_SUPERBEE = 4,
-  LIM_VANLEER = 5, LIM_ALBADA = 6, LIM_KOREN = 7, LIM_CADA3 = 9, LIM_WENO5 = 16
+  LIM_VANLEER = 5, LIM_ALBADA = 6, LIM_KOREN = 7, LIM_CADA = 8, LIM_CADA3 = 9, LIM_WENO5 = 16
 };
 
 template <int NDIM_, int NDIR_, bool MHD_, int DIVB_>
@@ -326,11 +326,19 @@ __device__ __forceinline__ void cbounds(const R* wLC, const R* wRC, const R* wLp
 // slope limiters, dwlimiter2 src/limiter/mod_limiter.t:128-334 for one cell:
 // a = dwC(i-1), b = dwC(i); gives ldw(i) and rdw(i)
 // ---------------------------------------------------------------------------
+// the limited third-order slope factor of cada (src/limiter/mod_limiter.t:
+// 231-252) for theta = th
 template <class R>
-__device__ __forceinline__ void cada3_side(R th, R eta, R& out) {
+__device__ __forceinline__ R cada_limited(R th) {
   const R cadalfa = R(0.5), cadbeta = R(2), cadgamma = R(1.6);
   const R A = (R(2) + th) * R(1.0 / 3.0);
-  const R lim = mx2(R(0), mn2(A, mx2(-cadalfa * th, mn2(mn2(cadbeta * th, A), cadgamma))));
+  return mx2(R(0), mn2(A, mx2(-cadalfa * th, mn2(mn2(cadbeta * th, A), cadgamma))));
+}
+
+template <class R>
+__device__ __forceinline__ void cada3_side(R th, R eta, R& out) {
+  const R A = (R(2) + th) * R(1.0 / 3.0);
+  const R lim = cada_limited(th);
   if (eta <= R(1) - R(1.0e-14)) {
     out = A;
   } else if (eta >= R(1) + R(1.0e-14)) {
@@ -359,6 +367,10 @@ __device__ __forceinline__ void dwlimiter(int lim_rt, R a, R b, R rdelinv, R& ld
     cada3_side(fdiv(b, a + copysign(eps, a)), eta, r);
     ldw = l * b;
     rdw = r * a;
+  } else if (lim == LIM_CADA) {
+    const R eps = R(1.4901161193847656e-08);  // sqrt(epsilon(1d0))
+    ldw = cada_limited(fdiv(a, b + copysign(eps, b))) * b;
+    rdw = cada_limited(fdiv(b, a + copysign(eps, a))) * a;
   } else if (lim == LIM_KOREN) {
     const R t = copysign(R(1), b);
     const R t2 = mn2(R(2) * fabs(b), R(2) * t * a);

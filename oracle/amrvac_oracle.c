@@ -374,6 +374,16 @@ static void dwlimiter_cell(const orc_sim* s, int idim, double a, double b,
       *ldw = tmp * MAX2(zero, MIN2(tmp2, (a * tmp + 2 * fabs(b)) * third));
       *rdw = tmp * MAX2(zero, MIN2(tmp2, (2 * a * tmp + fabs(b)) * third));
       return;
+    case ORC_LIM_CADA: { /* :231-252 */
+      double A;
+      tmp = a / (b + copysign(eps, b));
+      A = (two + tmp) * third;
+      *ldw = MAX2(zero, MIN2(A, MAX2(-cadalfa * tmp, MIN2(MIN2(cadbeta * tmp, A), cadgamma)))) * b;
+      tmp = b / (a + copysign(eps, a));
+      A = (two + tmp) * third;
+      *rdw = MAX2(zero, MIN2(A, MAX2(-cadalfa * tmp, MIN2(MIN2(cadbeta * tmp, A), cadgamma)))) * a;
+      return;
+    }
     case ORC_LIM_CADA3: { /* :253-290 */
       double r = s->p.cada3_radius * s->dx[idim];
       double rdelinv = one / (r * r);

@@ -16,7 +16,7 @@ from oracle.oracle import OracleFV
 
 pytestmark = pytest.mark.gpu
 
-LIMS = ["minmod", "woodward", "mcbeta", "superbee", "vanleer", "albada", "koren", "cada3", "weno5"]
+LIMS = ["minmod", "woodward", "mcbeta", "superbee", "vanleer", "albada", "koren", "cada", "cada3", "weno5"]
 
 
 def _cases():

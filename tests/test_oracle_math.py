@@ -282,13 +282,13 @@ def check_limiters(lib_path=None) -> str:
     msgs = []
     base = dict(physics="hd", ndim=2, ndir=2, gamma=1.4)
     sims = {lim: _sim(Par(limiter=lim, **base), lib_path=lib_path)
-            for lim in ("minmod", "vanleer", "koren", "cada3", "woodward", "superbee", "mcbeta")}
+            for lim in ("minmod", "vanleer", "koren", "cada", "cada3", "woodward", "superbee", "mcbeta")}
     rng = np.random.default_rng(11)
     for _ in range(200):
         a, b = rng.normal(size=2)
         for lim, s in sims.items():
             l, r = s.limiter(a, b)
-            if a * b < 0 and lim != "cada3" and (l != 0.0 or r != 0.0):
+            if a * b < 0 and lim not in ("cada", "cada3") and (l != 0.0 or r != 0.0):
                 msgs.append(f"{lim} not zero for opposite signs")
         l, r = sims["minmod"].limiter(a, b)
         if a * b > 0 and l != math.copysign(min(abs(a), abs(b)), b):
@@ -317,6 +317,20 @@ def check_limiters(lib_path=None) -> str:
     # theta=10 in the limited region: min(A, max(.., min(2 theta, A, 1.6))) = 1.6
     if abs(l - 1.6 * 0.05) > 1e-9:
         msgs.append(f"cada3 limited {l}")
+    # cada is cada3's limited branch everywhere: the same value as cada3 where
+    # eta > 1, and its slope ratio stays within [0, 1.6] (cadgamma)
+    c, c3 = sims["cada"], sims["cada3"]
+    for _ in range(200):
+        a, b = rng.normal(size=2)
+        l, r = c.limiter(a, b)
+        l3, r3 = c3.limiter(a, b)
+        if (a * a + b * b) > 1.0 and (l != l3 or r != r3):  # eta >> 1 at dx = 1/16
+            msgs.append(f"cada vs cada3 limited {a},{b}")
+        if a * b > 0 and not (0.0 <= l / b <= 1.6 * (1 + 1e-12) and 0.0 <= r / a <= 1.6 * (1 + 1e-12)):
+            msgs.append(f"cada slope ratio outside [0, 1.6] at {a},{b}")
+    l, r = c.limiter(0.5, 0.5)
+    if abs(l - 0.5) > 1e-7 or abs(r - 0.5) > 1e-7:
+        msgs.append(f"cada equal slopes {l},{r}")
     return "; ".join(msgs[:4])
 
 
