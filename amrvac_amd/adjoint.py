@@ -36,8 +36,8 @@ class SSPRK3Run:
 
     def __init__(self, par: Par, mesh: UniformMesh, nsteps: int, every: int | None = None,
                  precision: str | None = None, backend=None):
-        if par.time_stepper != "threestep":
-            raise ValueError("the differentiable run integrates with threestep")
+        if par.time_stepper != "threestep" or par.integrator != "ssprk3" or par.ssprk_order != 3:
+            raise ValueError("the differentiable run integrates with threestep SSPRK(3,3)")
         if any(b != "periodic" for b in mesh.bc):
             raise ValueError("the differentiable run needs a periodic domain")
         self.par = par

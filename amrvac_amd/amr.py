@@ -603,7 +603,7 @@ class AMRIntegrator(TimeIntegrator):
     def advect(self) -> None:
         fix = self.forest.levmax > self.forest.levmin
         par, st = self.par, self.state
-        if par.time_stepper == "twostep":
+        if par.integrator == "Predictor_Corrector":
             from .host import FS
             self.be.set_fix_conserve(False)
             self.be.advect1(FS[par.typepred1], 0.5, st.dt, st.cmax_global, 0, 1, ("copy", 0))

@@ -140,9 +140,15 @@ __global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R>)) adj_plane_kernel(
   }
   __syncthreads();
 
+  // Face j of a round goes to lane (j - j0 + 64 r) mod 256: the last,
+  // partial round of each dimension (NFACE - NTHREADS faces) falls to wave
+  // r, with r different for x and y and varying over the planes, so that no
+  // wave reverses the partial rounds of both dimensions.  Which lane reverses
+  // a face does not change the order of the per-cell sums.
+  const int rx = k & 3, ry = (k + 2) & 3;
   // ---- x faces: face j = (f, row), f = j % NF - 1 ------------------------
   for (int j0 = 0; j0 < NFACE; j0 += NTHREADS) {
-    const int j = j0 + t;
+    const int j = j0 + ((t + (NTHREADS - 64 * rx)) & (NTHREADS - 1));
     const bool on = j < NFACE;
     const int f = j % NF - 1, row = j / NF;
     double wb[4][NW];
@@ -169,7 +175,7 @@ __global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R>)) adj_plane_kernel(
   }
   // ---- y faces: face j = (col, f), col = j % BX, f = j / BX - 1 ----------
   for (int j0 = 0; j0 < NFACE; j0 += NTHREADS) {
-    const int j = j0 + t;
+    const int j = j0 + ((t + (NTHREADS - 64 * ry)) & (NTHREADS - 1));
     const bool on = j < NFACE;
     const int col = j % BX, f = j / BX - 1;
     double wb[4][NW];

@@ -3,6 +3,7 @@
 // reduction, volume averages, ghost-cell fill for download).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -44,7 +45,7 @@ struct amrvac_fv_ctx {
   int nxg[3] = {1, 1, 1};
   double dx[3] = {1, 1, 1};
   int bc[6] = {5, 5, 5, 5, 5, 5};
-  double* slot[3] = {nullptr, nullptr, nullptr};
+  double* slot[4] = {nullptr, nullptr, nullptr, nullptr};  // ps, ps1, ps2, ps3
   int* d_nbr = nullptr;
   int* d_crash = nullptr;
   double* d_red = nullptr;    // per-block reduction results
@@ -165,6 +166,44 @@ __global__ void __launch_bounds__(NTHREADS) reduce_max2_kernel(const double* __r
 }
 
 // per-block sums of w and w^2 over the interior, [nb][2][nw]
+// State combination between stages (the update loops of advect,
+// src/mod_advance.t:113-133, 269-291, 485-507, 515-557): dst = a postfix
+// program over the slots, one rounding per operation, elementwise over the
+// leaves' interior cells.  The program is uniform over the grid and checked
+// on the host (depth <= COMB_DEPTH, constants in range).
+constexpr int COMB_OPS = 32, COMB_CONSTS = 16, COMB_DEPTH = 8;
+enum { COP_CONST = 8, COP_ADD = 9, COP_SUB = 10, COP_MUL = 11, COP_DIV = 12, COP_NEG = 13 };
+struct CombProg {
+  int nops;
+  int ops[COMB_OPS];
+  double k[COMB_CONSTS];
+};
+
+__global__ void __launch_bounds__(NTHREADS) combine_kernel(double* dst, const double* s0, const double* s1,
+                                                           const double* s2, const double* s3, size_t n,
+                                                           CombProg p) {
+  const size_t stride = (size_t)gridDim.x * NTHREADS;
+  for (size_t e = (size_t)blockIdx.x * NTHREADS + threadIdx.x; e < n; e += stride) {
+    double st[COMB_DEPTH];
+    int sp = 0, kc = 0;
+    for (int q = 0; q < p.nops; ++q) {
+      const int op = p.ops[q];
+      if (op <= 3) {
+        st[sp++] = op == 0 ? s0[e] : op == 1 ? s1[e] : op == 2 ? s2[e] : s3[e];
+      } else if (op == COP_CONST) {
+        st[sp++] = p.k[kc++];
+      } else if (op == COP_NEG) {
+        st[sp - 1] = -st[sp - 1];
+      } else {
+        const double a = st[sp - 2], b = st[sp - 1];
+        --sp;
+        st[sp - 1] = op == COP_ADD ? a + b : op == COP_SUB ? a - b : op == COP_MUL ? a * b : a / b;
+      }
+    }
+    dst[e] = st[0];
+  }
+}
+
 __global__ void __launch_bounds__(NTHREADS) volsum_kernel(const double* __restrict__ w, int nblocks, int nw,
                                                           size_t cells, double* out) {
   const int b = blockIdx.x;
@@ -678,7 +717,7 @@ struct GetdtOp {
   }
 };
 
-bool check_slot(int s) { return s >= 0 && s <= 2; }
+bool check_slot(int s) { return s >= 0 && s <= 3; }
 
 }  // namespace
 
@@ -728,7 +767,7 @@ int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
 }
 
 static void free_level(amrvac_fv_ctx* c) {
-  for (int s = 0; s < 3; ++s) { if (c->slot[s]) (void)hipFree(c->slot[s]); c->slot[s] = nullptr; }
+  for (int s = 0; s < 4; ++s) { if (c->slot[s]) (void)hipFree(c->slot[s]); c->slot[s] = nullptr; }
   if (c->d_nbr) (void)hipFree(c->d_nbr);
   if (c->d_crash) (void)hipFree(c->d_crash);
   if (c->d_red) (void)hipFree(c->d_red);
@@ -809,7 +848,7 @@ static int alloc_level(amrvac_fv_ctx* c, int nblocks, int capacity, const double
   for (int d = 0; d < 3; ++d) c->dx[d] = d < nd ? dx[d] : 1.0;
   for (int s = 0; s < 6; ++s) c->bc[s] = s < 2 * nd ? bc[s] : 5;
   const size_t n = (size_t)capacity * c->nw * c->cells;
-  for (int s = 0; s < 3; ++s) {
+  for (int s = 0; s < 4; ++s) {
     if (hipMalloc(&c->slot[s], n * sizeof(double)) != hipSuccess) {
       free_level(c);
       return fail(AMRVAC_FV_ENOMEM, "state slot allocation failed");
@@ -1114,6 +1153,40 @@ int amrvac_fv_stage(amrvac_fv_ctx* c, int method, double dtfactor, double dt, do
   if (u_out == u_in) return fail(AMRVAC_FV_EINVAL, "u_out must not alias u_in");
   const int mode = base ? 2 : 3;
   return stage_core(c, method, dtfactor, dt, cmax_global, u_in, u_out, base, mode, comb_a, comb_b, -1, nullptr);
+}
+
+int amrvac_fv_combine(amrvac_fv_ctx* c, int dst_slot, int nops, const int* ops, int nconst, const double* consts) {
+  if (!c || !check_slot(dst_slot) || nops < 1 || nops > fv::COMB_OPS || !ops || nconst < 0 ||
+      nconst > fv::COMB_CONSTS || (nconst > 0 && !consts))
+    return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  fv::CombProg p{};
+  p.nops = nops;
+  int depth = 0, nk = 0;
+  for (int q = 0; q < nops; ++q) {
+    const int op = ops[q];
+    if ((op >= 0 && op <= 3) || op == fv::COP_CONST) {
+      if (op == fv::COP_CONST && nk >= nconst) return fail(AMRVAC_FV_EINVAL, "combination: too few constants");
+      nk += op == fv::COP_CONST;
+      if (++depth > fv::COMB_DEPTH) return fail(AMRVAC_FV_EINVAL, "combination: stack deeper than 8");
+    } else if (op == fv::COP_NEG) {
+      if (depth < 1) return fail(AMRVAC_FV_EINVAL, "combination: stack underflow");
+    } else if (op >= fv::COP_ADD && op <= fv::COP_DIV) {
+      if (depth < 2) return fail(AMRVAC_FV_EINVAL, "combination: stack underflow");
+      --depth;
+    } else {
+      return fail(AMRVAC_FV_EINVAL, "combination: unknown operation");
+    }
+    p.ops[q] = op;
+  }
+  if (depth != 1 || nk != nconst) return fail(AMRVAC_FV_EINVAL, "combination: malformed program");
+  for (int i = 0; i < nconst; ++i) p.k[i] = consts[i];
+  const size_t n = (size_t)c->nb * c->nw * c->cells;
+  const int grid = (int)std::min<size_t>((n + fvc::NTHREADS - 1) / fvc::NTHREADS, 8192);
+  hipLaunchKernelGGL(fv::combine_kernel, dim3(grid), dim3(fvc::NTHREADS), 0, c->stream, c->slot[dst_slot],
+                     c->slot[0], c->slot[1], c->slot[2], c->slot[3], n, p);
+  HIPCHK(hipGetLastError());
+  return AMRVAC_FV_OK;
 }
 
 int amrvac_fv_copy(amrvac_fv_ctx* c, int dst, int src) {

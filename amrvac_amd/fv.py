@@ -24,7 +24,7 @@ PREC = {"fp64": 0, "mixed": 1, "fp64_exact": 2}
 # every symbol include/amrvac_fv.h declares
 EXPORTS = ("amrvac_fv_init", "amrvac_fv_finalize", "amrvac_fv_set_stream", "amrvac_fv_set_mesh",
            "amrvac_fv_upload", "amrvac_fv_download", "amrvac_fv_getdt", "amrvac_fv_advect1",
-           "amrvac_fv_copy", "amrvac_fv_volume_average", "amrvac_fv_slot_pointer",
+           "amrvac_fv_copy", "amrvac_fv_combine", "amrvac_fv_volume_average", "amrvac_fv_slot_pointer",
            "amrvac_fv_sync", "amrvac_fv_sweep_timing", "amrvac_fv_last_error",
            "amrvac_fv_set_mesh_amr", "amrvac_fv_getbc", "amrvac_fv_set_fix_conserve", "amrvac_fv_adjoint1",
            "amrvac_fv_slot_copy", "amrvac_fv_stage", "amrvac_fv_getdt_state", "amrvac_fv_errest",
@@ -74,6 +74,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, ip]
     L.amrvac_fv_copy.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.amrvac_fv_combine.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_double)]
     L.amrvac_fv_volume_average.argtypes = [vp, ctypes.c_int, dp]
     L.amrvac_fv_slot_pointer.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_longlong)]
     L.amrvac_fv_sync.argtypes = [vp]
@@ -289,6 +291,13 @@ class DeviceFV:
 
     def copy(self, dst: int, src: int) -> None:
         self._check(self.L.amrvac_fv_copy(self.h, dst, src))
+
+    def combine(self, dst: int, expr) -> None:
+        """ps[dst] = expr (an amrvac_amd.host.Expr over the slots)."""
+        expr.check()
+        ops = np.ascontiguousarray(expr.ops, dtype=np.int32)
+        k = np.ascontiguousarray(expr.consts + [0.0], dtype=np.float64)
+        self._check(self.L.amrvac_fv_combine(self.h, dst, len(expr.ops), _ip(ops), len(expr.consts), _dp(k)))
 
     def advect1(self, method, dtfactor, dt, cmax_global, psa, psb, comb, check_crash: bool = False) -> None:
         if comb is None:

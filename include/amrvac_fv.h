@@ -6,7 +6,7 @@
  * these entry points through ISO_C_BINDING; every
  * argument is a plain C scalar, pointer or size.  All blocks of one level
  * live in HBM as one batched array per state slot (0 = ps, 1 = ps1,
- * 2 = ps2, src/mod_physicaldata.t:111-122).
+ * 2 = ps2, 3 = ps3, src/mod_physicaldata.t:111-122).
  *
  * Conventions
  *  - Return value 0 on success, a negative AMRVAC_FV_E* code otherwise; the
@@ -79,7 +79,7 @@ int amrvac_fv_set_stream(amrvac_fv_ctx* ctx, void* stream);
    dx[ndim]: cell size.  neighbor[nb*3^ndim]: same-level neighbour block
    for direction n = (i1+1)+3(i2+1)+9(i3+1), negative = physical boundary.
    xlohi[nb*ndim*2]: block extents.  bc[2*ndim]: boundary type per side.
-   Allocates the three state slots (interior cells only, FP64). */
+   Allocates the four state slots (interior cells only, FP64). */
 int amrvac_fv_set_mesh(amrvac_fv_ctx* ctx, int nblocks, const double* dx,
                        const int* neighbor, const double* xlohi, const int* bc);
 
@@ -127,7 +127,7 @@ int amrvac_fv_errest(amrvac_fv_ctx* ctx, int slot, int nwt, const int* iws,
    coarsen (the 2^ndim old children in child order), 2 refine (old parent,
    and last the child position sum_d (ic_d-1) 2^d).  The remaining
    arguments describe the new leaves as for amrvac_fv_set_mesh_amr.  Slot 0
-   is remapped; slots 1 and 2 are reallocated without data. */
+   is remapped; slots 1 to 3 are reallocated without data. */
 int amrvac_fv_regrid(amrvac_fv_ctx* ctx, int nblocks, const int* plan, const double* dx1,
                      const int* level, const int* ntype, const int* nbr, const int* nchild,
                      const int* icpos, const double* xlohi, const int* bc);
@@ -201,6 +201,17 @@ int amrvac_fv_adjoint1(amrvac_fv_ctx* ctx, int method, double dtfactor, double d
 
 /* ps[dst] = ps[src] (interior cells) */
 int amrvac_fv_copy(amrvac_fv_ctx* ctx, int dst, int src);
+
+/* Replaces: the state updates of advect between two advect1 calls
+   (src/mod_advance.t:113-133 RK2_alfa / ssprk2, 269-291 RK3_BT, 485-507
+   rk4, 515-557 ssprk5), e.g. ps = (1/3)*(-ps+ps1+2*ps2+ps3).  ps[dst] is
+   set, per interior cell and variable of every leaf, to the postfix program
+   ops[0..nops) over the slots: 0..3 push ps, ps1, ps2, ps3; 8 pushes the
+   next of consts[0..nconst); 9 add, 10 subtract, 11 multiply, 12 divide,
+   13 negate (one IEEE rounding each, no contraction).  At most 32 ops, 16
+   constants, 8 deep; dst may be one of the operands. */
+int amrvac_fv_combine(amrvac_fv_ctx* ctx, int dst_slot, int nops, const int* ops,
+                      int nconst, const double* consts);
 
 /* Replaces: get_volume_average for power 1 and 2
    (src/io/mod_input_output.t:2694-2729); modes[2*nw] */

@@ -69,7 +69,7 @@ struct orc_sim {
   int nface;       /* transverse cells of one block face */
   double* bflux;   /* [nb][ndim][2][nw][nface] scaled boundary face fluxes */
   int bc[6];
-  double* w[3];
+  double* w[4];  /* ps, ps1, ps2, ps3 */
   int crash;
   int ftype[3][MAXW];
   double inv_gamma_1, gamma_1;
@@ -1476,7 +1476,7 @@ int orc_create(const orc_params* p, int nblocks, const double* dx,
   s->nface = 1;
   for (int d = 1; d < p->ndim; ++d) s->nface *= p->block_nx[d] > p->block_nx[0] ? p->block_nx[d] : p->block_nx[0];
   for (int i = 0; i < 6; ++i) s->bc[i] = (i < 2 * p->ndim) ? bc[i] : ORC_BC_PERIODIC;
-  for (int sl = 0; sl < 3; ++sl) {
+  for (int sl = 0; sl < 4; ++sl) {
     s->w[sl] = (double*)calloc((size_t)nblocks * s->nw * s->ncell, sizeof(double));
     if (!s->w[sl]) { orc_destroy(s); return 5; }
   }
@@ -1486,7 +1486,7 @@ int orc_create(const orc_params* p, int nblocks, const double* dx,
 
 void orc_destroy(orc_sim* s) {
   if (!s) return;
-  for (int sl = 0; sl < 3; ++sl) free(s->w[sl]);
+  for (int sl = 0; sl < 4; ++sl) free(s->w[sl]);
   free(s->neighbor);
   free(s->xlohi);
   free(s->bdx);
@@ -1500,13 +1500,13 @@ void orc_destroy(orc_sim* s) {
 }
 
 int orc_set_w(orc_sim* s, int slot, const double* w) {
-  if (slot < 0 || slot > 2) return 1;
+  if (slot < 0 || slot > 3) return 1;
   memcpy(s->w[slot], w, sizeof(double) * (size_t)s->nb * s->nw * s->ncell);
   return 0;
 }
 
 int orc_get_w(const orc_sim* s, int slot, double* w) {
-  if (slot < 0 || slot > 2) return 1;
+  if (slot < 0 || slot > 3) return 1;
   memcpy(w, s->w[slot], sizeof(double) * (size_t)s->nb * s->nw * s->ncell);
   return 0;
 }
@@ -1582,7 +1582,7 @@ static void fix_conserve(orc_sim* s, int slot);
 
 int orc_advect1(orc_sim* s, int method, double dtfactor, double dt,
                 double cmax_global, int psa, int psb) {
-  if (psa < 0 || psa > 2 || psb < 0 || psb > 2 || psa == psb) return 1;
+  if (psa < 0 || psa > 3 || psb < 0 || psb > 3 || psa == psb) return 1;
   const double qdt = dtfactor * dt;
   int crash = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : crash) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
@@ -1617,6 +1617,38 @@ int orc_axpby(orc_sim* s, int dst, double a, int x, double b, int y) {
   const double* yv = s->w[y];
 #pragma omp parallel for schedule(static) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
   for (long i = 0; i < tot; ++i) d[i] = a * xv[i] + b * yv[i];
+  return 0;
+}
+
+/* ps(dst) = a state combination given as a postfix program, over every
+   cell of every block, one rounding per operation: the update loops of
+   advect between the stages (src/mod_advance.t:113-133, 269-291, 485-507,
+   515-557).  ops: 0..3 push slot ps..ps3, 8 push the next constant, 9 add,
+   10 subtract, 11 multiply, 12 divide, 13 negate; the program is checked by
+   the caller (at most 8 deep). */
+int orc_combine(orc_sim* s, int dst, int nops, const int* ops, const double* k) {
+  const long tot = (long)s->nb * s->nw * s->ncell;
+  double* d = s->w[dst];
+#pragma omp parallel for schedule(static) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
+  for (long i = 0; i < tot; ++i) {
+    double st[8];
+    int sp = 0, kc = 0;
+    for (int q = 0; q < nops; ++q) {
+      const int op = ops[q];
+      if (op >= 0 && op <= 3) {
+        st[sp++] = s->w[op][i];
+      } else if (op == 8) {
+        st[sp++] = k[kc++];
+      } else if (op == 13) {
+        st[sp - 1] = -st[sp - 1];
+      } else {
+        const double a = st[sp - 2], b = st[sp - 1];
+        --sp;
+        st[sp - 1] = op == 9 ? a + b : op == 10 ? a - b : op == 11 ? a * b : a / b;
+      }
+    }
+    d[i] = st[0];
+  }
   return 0;
 }
 

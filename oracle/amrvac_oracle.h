@@ -110,6 +110,7 @@ int orc_advect1(orc_sim* s, int method, double dtfactor, double dt,
    of src/mod_advance.t:251-267 */
 int orc_axpby(orc_sim* s, int dst, double a, int x, double b, int y);
 int orc_copy(orc_sim* s, int dst, int src);
+int orc_combine(orc_sim* s, int dst, int nops, const int* ops, const double* k);
 
 /* getbc (src/mod_ghostcells_update.t:350-566) */
 int orc_getbc(orc_sim* s, int slot);

@@ -67,6 +67,8 @@ def lib(path: str | None = None):
         L.orc_axpby.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_double, ctypes.c_int]
         L.orc_copy.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.orc_combine.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_double)]
         L.orc_getbc.argtypes = [vp, ctypes.c_int]
         L.orc_volume_average.argtypes = [vp, ctypes.c_int, dp]
         L.orc_crash_flag.argtypes = [vp]
@@ -175,6 +177,12 @@ class OracleFV:
 
     def copy(self, dst: int, src: int) -> None:
         self.L.orc_copy(self.h, dst, src)
+
+    def combine(self, dst: int, expr) -> None:
+        expr.check()
+        ops = np.ascontiguousarray(expr.ops, dtype=np.int32)
+        k = np.ascontiguousarray(expr.consts + [0.0], dtype=np.float64)
+        self.L.orc_combine(self.h, dst, len(expr.ops), _iptr(ops), _dptr(k))
 
     def advect1(self, method, dtfactor, dt, cmax_global, psa, psb, comb) -> None:
         if comb is not None:

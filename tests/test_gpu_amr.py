@@ -22,6 +22,9 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 SCHEMES = {
     "2step_tvdlf_mm": dict(time_stepper="twostep", flux_scheme="tvdlf", limiter="minmod"),
     "3step_hll_cada": dict(time_stepper="threestep", flux_scheme="hll", limiter="cada3"),
+    "4step_hll_mc": dict(time_stepper="fourstep", time_integrator="rk4", flux_scheme="hll", limiter="woodward"),
+    "4step_hllc_ko": dict(time_stepper="fourstep", time_integrator="ssprk4", flux_scheme="hllc", limiter="koren"),
+    "rk4_tvdlf_cada": dict(time_stepper="fourstep", time_integrator="rk4", flux_scheme="tvdlf", limiter="cada3"),
 }
 
 

@@ -1,4 +1,4 @@
-"""Every limiter, Riemann solver, bound-speed rule and time stepper of the
+"""Every limiter, Riemann solver, bound-speed rule and time integrator of the
 device path against the CPU oracle, on small meshes: the contraction-free
 FP64 kernels must reproduce the oracle bit for bit (state and dt sequence),
 covering the kernel instantiations the benchmark setups do not reach
@@ -34,26 +34,39 @@ def _cases():
     for lim, fs in (("vanleer", "hll"), ("minmod", "tvdlf"), ("cada3", "hll")):  # 3-D ideal MHD, no cleaning
         out.append(("mhd3", lim, fs, "Einfeldt", "threestep"))
     out.append(("hd3", "mcbeta", "hllc", "cmaxleftright", "threestep"))
+    # every explicit integrator of advect (stepper:integrator:ssprk_order:rk3_switch)
+    for ts in ("twostep:RK2_alfa:3:3", "twostep:ssprk2:3:3", "threestep:ssprk3:2:3", "threestep:RK3_BT:3:1",
+               "threestep:RK3_BT:3:3", "fourstep:ssprk4:3:3", "fourstep:ssprk4:2:3", "fourstep:rk4:3:3",
+               "fivestep:ssprk5:3:3", "fivestep:ssprk5:2:3"):
+        out.append(("hd2", "vanleer", "hll", "Einfeldt", ts))
+    out.append(("mhd3glm", "cada3", "hll", "Einfeldt", "fourstep:rk4:3:3"))
     return out
+
+
+def _stepper(ts):
+    if ":" not in ts:
+        return dict(time_stepper=ts)
+    stepper, integ, order, sw = ts.split(":")
+    return dict(time_stepper=stepper, time_integrator=integ, ssprk_order=int(order), rk3_switch=int(sw))
 
 
 def _setup(phys, lim, fs, bs, ts):
     if phys == "hd2":
-        par = Par(physics="hd", ndim=2, ndir=2, gamma=1.4, time_stepper=ts, flux_scheme=fs, limiter=lim,
-                  typeboundspeed=bs, courantpar=0.6)
+        par = Par(physics="hd", ndim=2, ndir=2, gamma=1.4, flux_scheme=fs, limiter=lim,
+                  typeboundspeed=bs, courantpar=0.6, **_stepper(ts))
         mesh = UniformMesh(2, (32, 32), (16, 16), (0.0, 0.0), (1.0, 1.0),
                            ("periodic", "periodic", "symm", "cont"), nghost=par.nghost)
         w0 = problems.kelvin_helmholtz_2d(mesh, gamma=par.gamma)
     elif phys == "hd3":
-        par = Par(physics="hd", ndim=3, ndir=3, gamma=5.0 / 3.0, time_stepper=ts, flux_scheme=fs, limiter=lim,
-                  typeboundspeed=bs, courantpar=0.6)
+        par = Par(physics="hd", ndim=3, ndir=3, gamma=5.0 / 3.0, flux_scheme=fs, limiter=lim,
+                  typeboundspeed=bs, courantpar=0.6, **_stepper(ts))
         mesh = UniformMesh(3, (16, 16, 32), (16, 16, 16), (0.0,) * 3, (1.0, 1.0, 2.0), ("periodic",) * 6,
                            nghost=par.nghost)
         w0 = problems.blast_wave(mesh, gamma=par.gamma, p_in=1.0, radius=0.2)
     else:
         divb = "glm" if phys == "mhd3glm" else "none"
-        par = Par(physics="mhd", ndim=3, ndir=3, gamma=5.0 / 3.0, typedivbfix=divb, time_stepper=ts,
-                  flux_scheme=fs, limiter=lim, typeboundspeed=bs, courantpar=0.6)
+        par = Par(physics="mhd", ndim=3, ndir=3, gamma=5.0 / 3.0, typedivbfix=divb, flux_scheme=fs,
+                  limiter=lim, typeboundspeed=bs, courantpar=0.6, **_stepper(ts))
         mesh = UniformMesh(3, (16, 16, 32), (16, 16, 16), (0.0,) * 3, (1.0, 1.0, 2.0), ("periodic",) * 6,
                            nghost=par.nghost)
         w0 = problems.orszag_tang_3d(mesh, gamma=par.gamma)
