@@ -32,7 +32,7 @@ SMALLDOUBLE = 1.0e-12  # src/mod_constants.t:8
 BIGDOUBLE = 1.0e99     # src/mod_constants.t:11
 
 # flux scheme codes, src/mod_global_parameters.t:457-469
-FS = {"hll": 1, "hllc": 2, "tvdlf": 5, "hancock": 8}
+FS = {"hll": 1, "hllc": 2, "hlld": 3, "tvdlf": 5, "hancock": 8}
 # limiter codes, src/limiter/mod_limiter.t:15-40
 LIMITERS = {"minmod": 1, "woodward": 2, "mcbeta": 3, "superbee": 4,
             "vanleer": 5, "albada": 6, "koren": 7, "cada": 8, "cada3": 9,

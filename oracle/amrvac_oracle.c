@@ -524,6 +524,116 @@ static void riemann_hll(const orc_sim* s, int idim, const double* wLC,
   }
 }
 
+/* get_Riemann_flux_hlld src/mod_finite_volume.t:505-718 (Miyoshi & Kusano
+   2005; no B0, no stagger grid, no equilibrium variables, ndir = 3): the
+   four star states w1L, w1R, w2L, w2R of one face, then the region of the
+   fan containing the face picks the flux per variable.  wLp/wRp are the
+   primitive states after modify_wLR, wLC/wRC the conserved ones. */
+static void riemann_hlld(const orc_sim* s, int idim, const double* wLC, const double* wRC,
+                         const double* wLp, const double* wRp, const double* fL,
+                         const double* fR, double cmax, double cmin, double* fC) {
+  const double sR = cmax, sL = cmin;
+  const int rho_ = s->rho_, e_ = s->e_, p_ = s->e_;
+  const int ip1 = idim, ip2 = (idim + 1) % 3, ip3 = (idim + 2) % 3;
+  const int m1 = s->mom[ip1], m2 = s->mom[ip2], m3 = s->mom[ip3];
+  const int b1 = s->mag[ip1], b2 = s->mag[ip2], b3 = s->mag[ip3];
+  double w1R[MAXW], w1L[MAXW], w2R[MAXW], w2L[MAXW];
+  for (int iw = 0; iw < s->nw; ++iw) { w1R[iw] = w1L[iw] = w2R[iw] = w2L[iw] = 0.0; }
+  const double* BR = wRC;  /* magnetic field of the reconstructions, by index */
+  const double* BL = wLC;
+  /* HLL estimate of the normal field, Li 2005 eq. (33) */
+  const double Bx = (sR * BR[b1] - sL * BL[b1]) / (sR - sL);
+  const double ptR = wRp[p_] + 0.5 * (BR[s->mag[0]] * BR[s->mag[0]] + BR[s->mag[1]] * BR[s->mag[1]] +
+                                      BR[s->mag[2]] * BR[s->mag[2]]);
+  const double ptL = wLp[p_] + 0.5 * (BL[s->mag[0]] * BL[s->mag[0]] + BL[s->mag[1]] * BL[s->mag[1]] +
+                                      BL[s->mag[2]] * BL[s->mag[2]]);
+  const double suR = (sR - wRp[m1]) * wRC[rho_];
+  const double suL = (sL - wLp[m1]) * wLC[rho_];
+  const double sm = (suR * wRp[m1] - suL * wLp[m1] - ptR + ptL) / (suR - suL);
+  w1R[m1] = sm; w1L[m1] = sm; w2R[m1] = sm; w2L[m1] = sm;
+  w1R[b1] = Bx; w1L[b1] = Bx;
+  w1R[rho_] = suR / (sR - sm);
+  w1L[rho_] = suL / (sL - sm);
+  double r1R = suR * (sR - sm) - Bx * Bx;
+  if (r1R != 0.0) r1R = 1.0 / r1R;
+  double r1L = suL * (sL - sm) - Bx * Bx;
+  if (r1L != 0.0) r1L = 1.0 / r1L;
+  w1R[m2] = wRp[m2] - Bx * BR[b2] * (sm - wRp[m1]) * r1R;
+  w1L[m2] = wLp[m2] - Bx * BL[b2] * (sm - wLp[m1]) * r1L;
+  w1R[b2] = (suR * (sR - wRp[m1]) - Bx * Bx) * r1R;
+  w1L[b2] = (suL * (sL - wLp[m1]) - Bx * Bx) * r1L;
+  w1R[m3] = wRp[m3] - Bx * BR[b3] * (sm - wRp[m1]) * r1R;
+  w1L[m3] = wLp[m3] - Bx * BL[b3] * (sm - wLp[m1]) * r1L;
+  w1R[b3] = BR[b3] * w1R[b2];
+  w1L[b3] = BL[b3] * w1L[b2];
+  w1R[b2] = BR[b2] * w1R[b2];
+  w1L[b2] = BL[b2] * w1L[b2];
+  /* total pressure of the star region (Guo eq. 25), kept in the energy slot
+     until the energy is formed (p_ = e_) */
+  w1R[p_] = suR * (sm - wRp[m1]) + ptR;
+  w1L[p_] = w1R[p_];
+  {
+    const int* mo = s->mom; const int* ma = s->mag;
+    const double vbR = wRp[mo[0]] * wRC[ma[0]] + wRp[mo[1]] * wRC[ma[1]] + wRp[mo[2]] * wRC[ma[2]];
+    const double vb1R = w1R[mo[0]] * w1R[ma[0]] + w1R[mo[1]] * w1R[ma[1]] + w1R[mo[2]] * w1R[ma[2]];
+    w1R[e_] = ((sR - wRp[m1]) * wRC[e_] - ptR * wRp[m1] + w1R[p_] * sm + Bx * (vbR - vb1R)) / (sR - sm);
+    const double vbL = wLp[mo[0]] * wLC[ma[0]] + wLp[mo[1]] * wLC[ma[1]] + wLp[mo[2]] * wLC[ma[2]];
+    const double vb1L = w1L[mo[0]] * w1L[ma[0]] + w1L[mo[1]] * w1L[ma[1]] + w1L[mo[2]] * w1L[ma[2]];
+    w1L[e_] = ((sL - wLp[m1]) * wLC[e_] - ptL * wLp[m1] + w1L[p_] * sm + Bx * (vbL - vb1L)) / (sL - sm);
+  }
+  /* the Alfven-wave (double star) states */
+  w2R[rho_] = w1R[rho_]; w2L[rho_] = w1L[rho_];
+  w2R[b1] = w1R[b1]; w2L[b1] = w1L[b1];
+  const double sR1 = sqrt(w1R[rho_]);
+  const double sL1 = sqrt(w1L[rho_]);
+  const double tmp = 1.0 / (sR1 + sL1);
+  const double signBx = copysign(1.0, Bx);
+  const double s1R = sm + fabs(Bx) / sR1;
+  const double s1L = sm - fabs(Bx) / sL1;
+  w2R[m2] = (sL1 * w1L[m2] + sR1 * w1R[m2] + (w1R[b2] - w1L[b2]) * signBx) * tmp;
+  w2L[m2] = w2R[m2];
+  w2R[b2] = (sL1 * w1R[b2] + sR1 * w1L[b2] + sL1 * sR1 * (w1R[m2] - w1L[m2]) * signBx) * tmp;
+  w2L[b2] = w2R[b2];
+  w2R[m3] = (sL1 * w1L[m3] + sR1 * w1R[m3] + (w1R[b3] - w1L[b3]) * signBx) * tmp;
+  w2L[m3] = w2R[m3];
+  w2R[b3] = (sL1 * w1R[b3] + sR1 * w1L[b3] + sL1 * sR1 * (w1R[m3] - w1L[m3]) * signBx) * tmp;
+  w2L[b3] = w2R[b3];
+  {
+    const int* mo = s->mom; const int* ma = s->mag;
+    const double d1R = w1R[mo[0]] * w1R[ma[0]] + w1R[mo[1]] * w1R[ma[1]] + w1R[mo[2]] * w1R[ma[2]];
+    const double d2R = w2R[mo[0]] * w2R[ma[0]] + w2R[mo[1]] * w2R[ma[1]] + w2R[mo[2]] * w2R[ma[2]];
+    w2R[e_] = w1R[e_] + sR1 * (d1R - d2R) * signBx;
+    const double d1L = w1L[mo[0]] * w1L[ma[0]] + w1L[mo[1]] * w1L[ma[1]] + w1L[mo[2]] * w1L[ma[2]];
+    const double d2L = w2L[mo[0]] * w2L[ma[0]] + w2L[mo[1]] * w2L[ma[1]] + w2L[mo[2]] * w2L[ma[2]];
+    w2L[e_] = w1L[e_] - sL1 * (d1L - d2L) * signBx;
+  }
+  /* velocities to momenta */
+  for (int c = 0; c < 3; ++c) {
+    const int m = s->mom[c];
+    w1R[m] = w1R[m] * w1R[rho_];
+    w1L[m] = w1L[m] * w1L[rho_];
+    w2R[m] = w2R[m] * w2R[rho_];
+    w2L[m] = w2L[m] * w2L[rho_];
+  }
+  for (int iw = 0; iw < s->nw; ++iw) {
+    if (s->ftype[idim][iw] == FT_SPECIAL) {
+      fC[iw] = fL[iw];
+    } else if (sL > 0.0) {
+      fC[iw] = fL[iw];
+    } else if (s1L >= 0.0) {
+      fC[iw] = fL[iw] + sL * (w1L[iw] - wLC[iw]);
+    } else if (sm >= 0.0) {
+      fC[iw] = fL[iw] + sL * (w1L[iw] - wLC[iw]) + s1L * (w2L[iw] - w1L[iw]);
+    } else if (s1R >= 0.0) {
+      fC[iw] = fR[iw] + sR * (w1R[iw] - wRC[iw]) + s1R * (w2R[iw] - w1R[iw]);
+    } else if (sR >= 0.0) {
+      fC[iw] = fR[iw] + sR * (w1R[iw] - wRC[iw]);
+    } else {
+      fC[iw] = fR[iw];
+    }
+  }
+}
+
 /* get_Riemann_flux_hllc src/mod_finite_volume.t:439-504 with hd_get_lCD
    src/hd/mod_hd_hllc.t:56-152 and hd_get_wCD :154-257 */
 static void riemann_hllc(const orc_sim* s, int idim, const double* wLC,
@@ -819,6 +929,8 @@ static void finite_volume(const orc_sim* s, int method, double qdt,
             get_cbounds(s, wLC, wRC, wLp[i], wRp[i], idim, 1, &cmax, &cmin, crash);
             if (method == ORC_FS_HLLC)
               riemann_hllc(s, idim, wLC, wRC, fL, fR, cmax, cmin, fC[i]);
+            else if (method == ORC_FS_HLLD)
+              riemann_hlld(s, idim, wLC, wRC, wLp[i], wRp[i], fL, fR, cmax, cmin, fC[i]);
             else
               riemann_hll(s, idim, wLC, wRC, fL, fR, cmax, cmin, fC[i]);
           }
@@ -1913,6 +2025,7 @@ int orc_face_flux(orc_sim* s, int method, int idim, int nl, const double* lw_in,
   } else {
     get_cbounds(s, wLC, wRC, wLp[i], wRp[i], idim, 1, &cmax, &cmin, &crash);
     if (method == ORC_FS_HLLC) riemann_hllc(s, idim, wLC, wRC, fL, fR, cmax, cmin, fC);
+    else if (method == ORC_FS_HLLD) riemann_hlld(s, idim, wLC, wRC, wLp[i], wRp[i], fL, fR, cmax, cmin, fC);
     else riemann_hll(s, idim, wLC, wRC, fL, fR, cmax, cmin, fC);
   }
   if (cmax_out) *cmax_out = cmax;

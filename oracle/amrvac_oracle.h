@@ -31,6 +31,7 @@ extern "C" {
 /* flux schemes (src/mod_global_parameters.t:457-469) */
 #define ORC_FS_HLL 1
 #define ORC_FS_HLLC 2
+#define ORC_FS_HLLD 3
 #define ORC_FS_TVDLF 5
 #define ORC_FS_HANCOCK 8
 /* limiters (src/limiter/mod_limiter.t:15-40) */

@@ -199,7 +199,7 @@ def check_flux_consistency(lib_path=None) -> str:
                 if not np.allclose(f, ref, rtol=1e-12, atol=1e-12):
                     msgs.append(f"hd {fs} dim {d}: {f} vs {ref}")
         pm = np.concatenate([[0.5 + rng.random()], rng.normal(size=3), [0.2 + rng.random()], rng.normal(size=3)])
-        for fs in ("tvdlf", "hll"):
+        for fs in ("tvdlf", "hll", "hlld"):
             for d in range(3):
                 f, _, _ = mhd.face_flux(FS[fs], d, _prims_const(pm, pm))
                 ref = mhd_flux(pm, 5 / 3, d)
@@ -454,6 +454,159 @@ def test_conservation_and_uniform_states():
     assert check_conservation_and_uniform_states() == ""
 
 
+def check_hlld_discontinuities(lib_path=None) -> str:
+    """HLLD (Miyoshi & Kusano 2005) resolves isolated discontinuities
+    exactly: across a stationary contact (only rho jumps, B_n != 0) and a
+    stationary rotational discontinuity (v_n = -s B_n/sqrt(rho), B_t
+    rotated at constant |B_t|, [v_t] = B_n [B_t]/(rho v_n)) the face flux
+    is f(u_L) = f(u_R); HLL is not.  With B = 0 it is the hydrodynamic HLLC
+    flux (same contact speed and star states) for subsonic states."""
+    g = 5.0 / 3.0
+    rng = np.random.default_rng(11)
+    mhd = _sim(Par(physics="mhd", ndim=3, ndir=3, gamma=g, typedivbfix="none", limiter="minmod"), lib_path=lib_path)
+    hd = _sim(Par(physics="hd", ndim=3, ndir=3, gamma=g, limiter="minmod"), lib_path=lib_path)
+    msgs = []
+
+    def rel(a, b):
+        return float(np.abs(a - b).max() / np.abs(b).max())
+
+    for _ in range(10):
+        for d in range(3):
+            v = np.zeros(3)
+            v[[(d + 1) % 3, (d + 2) % 3]] = rng.normal(size=2)
+            B = rng.normal(size=3)
+            B[d] = 0.5 + rng.random()
+            pl = np.concatenate([[0.5 + rng.random()], v, [0.2 + rng.random()], B])
+            pr = pl.copy()
+            pr[0] = 0.5 + 2.0 * rng.random()
+            f, _, _ = mhd.face_flux(FS["hlld"], d, _prims_const(pl, pr))
+            fh, _, _ = mhd.face_flux(FS["hll"], d, _prims_const(pl, pr))
+            ref = mhd_flux(pl, g, d)
+            if rel(f, ref) > 1e-13 or rel(fh, ref) < 1e-3:
+                msgs.append(f"contact dim {d}: hlld {rel(f, ref):.2e} hll {rel(fh, ref):.2e}")
+            rho, p, bn, sgn = 0.5 + rng.random(), 0.2 + rng.random(), 0.5 + rng.random(), rng.choice([-1.0, 1.0])
+            vn = -sgn * bn / math.sqrt(rho)
+            c = bn / (rho * vn)
+            bt, th = 0.3 + rng.random(), rng.random(2) * 2.0 * math.pi
+            w0 = rng.normal(size=2)
+            prims = []
+            for t in th:
+                btv = bt * np.array([math.cos(t), math.sin(t)])
+                vv, BB = np.zeros(3), np.zeros(3)
+                vv[d], BB[d] = vn, bn
+                vv[[(d + 1) % 3, (d + 2) % 3]] = c * btv + w0
+                BB[[(d + 1) % 3, (d + 2) % 3]] = btv
+                prims.append(np.concatenate([[rho], vv, [p], BB]))
+            f, _, _ = mhd.face_flux(FS["hlld"], d, _prims_const(prims[0], prims[1]))
+            fh, _, _ = mhd.face_flux(FS["hll"], d, _prims_const(prims[0], prims[1]))
+            ref = mhd_flux(prims[0], g, d)
+            if rel(f, ref) > 1e-13 or rel(fh, ref) < 1e-3:
+                msgs.append(f"rotational dim {d}: hlld {rel(f, ref):.2e} hll {rel(fh, ref):.2e}")
+            pl = np.concatenate([[0.5 + rng.random()], 0.3 * rng.normal(size=3), [0.5 + rng.random()]])
+            pr = np.concatenate([[0.5 + rng.random()], 0.3 * rng.normal(size=3), [0.5 + rng.random()]])
+            fc, _, _ = hd.face_flux(FS["hllc"], d, _prims_const(pl, pr))
+            fm, _, _ = mhd.face_flux(FS["hlld"], d, _prims_const(np.concatenate([pl, np.zeros(3)]),
+                                                                 np.concatenate([pr, np.zeros(3)])))
+            if rel(fm[:5], fc) > 1e-13 or np.abs(fm[5:]).max() != 0.0:
+                msgs.append(f"B=0 dim {d}: hlld vs hllc {rel(fm[:5], fc):.2e}")
+    return "; ".join(msgs[:3])
+
+
+def mk_hlld_flux(pl, pr, sL, sR, gamma, d):
+    """HLLD flux of Miyoshi & Kusano (2005, J. Comput. Phys. 208, 315),
+    written from the paper in its own variables for the face normal d:
+    S_M by eq. (38), the total pressure of the star states by eq. (41)
+    (the paper's closed form, not the p_T* = rho(S-u)(S_M-u) + p_T
+    shortcut), rho*, v*, B*, e* by eqs. (43)-(48), the double-star states
+    by eqs. (59)-(63) and the fluxes by eq. (66).  pl, pr: primitive
+    (rho, v1..3, p, B1..3); sL, sR: the outer wave speeds."""
+    t = [(d + 1) % 3, (d + 2) % 3]
+
+    def unpack(pw):
+        rho, v, p, B = pw[0], pw[1:4].copy(), pw[4], pw[5:8].copy()
+        e = p / (gamma - 1.0) + 0.5 * rho * (v @ v) + 0.5 * (B @ B)
+        return rho, v, p, B, e, p + 0.5 * (B @ B)
+
+    rL, vL, _, BL, eL, ptL = unpack(pl)
+    rR, vR, _, BR, eR, ptR = unpack(pr)
+    Bx = (sR * BR[d] - sL * BL[d]) / (sR - sL)
+    uL, uR = vL[d], vR[d]
+    sM = ((sR - uR) * rR * uR - (sL - uL) * rL * uL - ptR + ptL) / ((sR - uR) * rR - (sL - uL) * rL)
+    pts = ((sR - uR) * rR * ptL - (sL - uL) * rL * ptR + rL * rR * (sR - uR) * (sL - uL) * (uR - uL)) / (
+        (sR - uR) * rR - (sL - uL) * rL)
+
+    def star(r, v, B, e, pt, S, u):
+        rs = r * (S - u) / (S - sM)
+        den = r * (S - u) * (S - sM) - Bx * Bx
+        vs, Bs = v.copy(), B.copy()
+        vs[d], Bs[d] = sM, Bx
+        if den != 0.0:
+            vs[t] = v[t] - Bx * B[t] * (sM - u) / den
+            Bs[t] = B[t] * (r * (S - u) ** 2 - Bx * Bx) / den
+        else:
+            vs[t], Bs[t] = v[t], B[t]
+        es = ((S - u) * e - pt * u + pts * sM + Bx * (v @ B - vs @ Bs)) / (S - sM)
+        return rs, vs, Bs, es
+
+    rsL, vsL, BsL, esL = star(rL, vL, BL, eL, ptL, sL, uL)
+    rsR, vsR, BsR, esR = star(rR, vR, BR, eR, ptR, sR, uR)
+    qL, qR = math.sqrt(rsL), math.sqrt(rsR)
+    sgn = math.copysign(1.0, Bx)
+    vss, Bss = vsL.copy(), BsL.copy()
+    vss[t] = (qL * vsL[t] + qR * vsR[t] + (BsR[t] - BsL[t]) * sgn) / (qL + qR)
+    Bss[t] = (qL * BsR[t] + qR * BsL[t] + qL * qR * (vsR[t] - vsL[t]) * sgn) / (qL + qR)
+    essL = esL - qL * (vsL @ BsL - vss @ Bss) * sgn
+    essR = esR + qR * (vsR @ BsR - vss @ Bss) * sgn
+    s1L, s1R = sM - abs(Bx) / qL, sM + abs(Bx) / qR
+
+    def cons(r, v, e, B):
+        return np.concatenate([[r], r * v, [e], B])
+
+    UL, UR = cons(rL, vL, eL, BL), cons(rR, vR, eR, BR)
+    FL, FR = mhd_flux(pl, gamma, d), mhd_flux(pr, gamma, d)
+    U1L, U1R = cons(rsL, vsL, esL, BsL), cons(rsR, vsR, esR, BsR)
+    U2L, U2R = cons(rsL, vss, essL, Bss), cons(rsR, vss, essR, Bss)
+    if sL > 0.0:
+        return FL
+    if s1L >= 0.0:
+        return FL + sL * (U1L - UL)
+    if sM >= 0.0:
+        return FL + sL * (U1L - UL) + s1L * (U2L - U1L)
+    if s1R >= 0.0:
+        return FR + sR * (U1R - UR) + s1R * (U2R - U1R)
+    if sR >= 0.0:
+        return FR + sR * (U1R - UR)
+    return FR
+
+
+def check_hlld_against_paper(lib_path=None) -> str:
+    """The oracle's HLLD flux (the reference's transcription, with Guo's
+    total-pressure shortcut) against the paper's formulation above, for
+    random states in every fan region (normal B of both signs)."""
+    g = 5.0 / 3.0
+    rng = np.random.default_rng(5)
+    mhd = _sim(Par(physics="mhd", ndim=3, ndir=3, gamma=g, typedivbfix="none", limiter="minmod"), lib_path=lib_path)
+    worst = 0.0
+    for _ in range(200):
+        pl = np.concatenate([[0.3 + rng.random()], rng.normal(size=3), [0.1 + rng.random()], rng.normal(size=3)])
+        pr = np.concatenate([[0.3 + rng.random()], rng.normal(size=3), [0.1 + rng.random()], rng.normal(size=3)])
+        d = int(rng.integers(3))
+        f, cmax, cmin = mhd.face_flux(FS["hlld"], d, _prims_const(pl, pr))
+        ref = mk_hlld_flux(pl, pr, cmin, cmax, g, d)
+        worst = max(worst, float(np.abs(f - ref).max() / np.abs(ref).max()))
+    if worst > 1e-11:
+        return f"hlld vs the paper's formulation: {worst:.2e}"
+    return ""
+
+
+def test_hlld_matches_the_published_formulation():
+    assert check_hlld_against_paper() == ""
+
+
+def test_hlld_resolves_isolated_discontinuities():
+    assert check_hlld_discontinuities() == ""
+
+
 ALL_CHECKS = {
     "sod_hllc_vanleer": lambda p: check_sod("3step_hllc_vanleer", p),
     "sod_hll_cada3": lambda p: check_sod("3step_hll_cada3", p),
@@ -464,6 +617,8 @@ ALL_CHECKS = {
     "smooth_hll_cada3": lambda p: check_smooth_order("threestep_hll_cada3", p),
     "smooth_tvdlf_weno5": lambda p: check_smooth_order("threestep_tvdlf_weno5", p),
     "hllc_contact": check_hllc_stationary_contact,
+    "hlld_discontinuities": check_hlld_discontinuities,
+    "hlld_paper": check_hlld_against_paper,
     "flux_consistency": check_flux_consistency,
     "signal_speeds": check_signal_speeds,
     "glm_interface": check_glm_interface_state,

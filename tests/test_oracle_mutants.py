@@ -29,6 +29,15 @@ MUTANTS = {
                                    "flux_consistency"),
     "hd_momentum_flux_no_pressure": ("f[s->mom[idim]] = f[s->mom[idim]] + w[s->e_];",
                                      "f[s->mom[idim]] = f[s->mom[idim]];", 0, "flux_consistency"),
+    "hlld_contact_speed_pressure_sign": ("- ptR + ptL) / (suR - suL);", "+ ptR - ptL) / (suR - suL);", 0,
+                                         "hlld_discontinuities"),
+    "hlld_alfven_state_sign": ("(w1R[b2] - w1L[b2]) * signBx) * tmp;", "(w1R[b2] - w1L[b2]) * Bx) * tmp;", 0,
+                               "hlld_paper"),
+    "hlld_tangential_momentum_term": ("w1R[m2] = wRp[m2] - Bx * BR[b2] * (sm - wRp[m1]) * r1R;",
+                                      "w1R[m2] = wRp[m2] - Bx * BR[b2] * (sm - wRp[m1]);", 0,
+                                      "hlld_paper"),
+    "hlld_star_energy_work": ("+ w1R[p_] * sm + Bx * (vbR - vb1R)) / (sR - sm);",
+                              "+ w1R[p_] * sm - Bx * (vbR - vb1R)) / (sR - sm);", 0, "hlld_paper"),
     "mhd_energy_flux_sign": ("- bn * bv;", "+ bn * bv;", 0, "flux_consistency"),
     "mhd_induction_sign": ("vn * w[s->mag[c]] - bn * w[s->mom[c]]", "vn * w[s->mag[c]] + bn * w[s->mom[c]]", 0,
                            "flux_consistency"),
