@@ -1015,6 +1015,7 @@ static int method_to_riem(int method, int& riem) {
   switch (method) {
     case 1: riem = fv::RIEM_HLL; return 0;
     case 2: riem = fv::RIEM_HLLC; return 0;
+    case 3: riem = fv::RIEM_HLLD; return 0;
     case 5: riem = fv::RIEM_TVDLF; return 0;
     case 8: riem = fv::RIEM_HANCOCK; return 0;
     default: return -1;
@@ -1030,6 +1031,8 @@ static int stage_core(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
   int riem;
   if (method_to_riem(method, riem) != 0) return fail(AMRVAC_FV_EINVAL, "unsupported flux scheme");
   if (riem == fv::RIEM_HLLC && c->cfg.physics != 1) return fail(AMRVAC_FV_EINVAL, "hllc is provided for hd");
+  if (riem == fv::RIEM_HLLD && (c->cfg.physics != 2 || c->cfg.ndir != 3))
+    return fail(AMRVAC_FV_EINVAL, "hlld is provided for mhd with ndir = 3");
   if (riem == fv::RIEM_HANCOCK && c->cfg.limiter == 16)
     return fail(AMRVAC_FV_EINVAL, "the Hancock predictor takes a TVD limiter");
   if (c->cfg.physics == 2 && c->cfg.divb == 1 && !(cmax_global > 0.0))
@@ -1456,7 +1459,7 @@ int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt,
   if (c->cfg.boundspeed != 1) return fail(AMRVAC_FV_EINVAL, "the adjoint stage needs Einfeldt bounds");
   if (c->cfg.limiter != 1 && c->cfg.limiter != 5) return fail(AMRVAC_FV_EINVAL, "the adjoint needs minmod or vanleer");
   int riem;
-  if (method_to_riem(method, riem) != 0 || riem == fv::RIEM_HANCOCK)
+  if (method_to_riem(method, riem) != 0 || riem == fv::RIEM_HANCOCK || riem == fv::RIEM_HLLD)
     return fail(AMRVAC_FV_EINVAL, "adjoint: flux scheme must be hll, hllc or tvdlf");
   if (riem == fv::RIEM_HLLC && c->cfg.physics != 1) return fail(AMRVAC_FV_EINVAL, "hllc is provided for hd");
   if (c->cfg.ndim == 3 && !c->d_adjG) {

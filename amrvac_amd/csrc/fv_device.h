@@ -116,7 +116,7 @@ __device__ __forceinline__ float fsqrt(float a) {
 // src/mhd/mod_mhd_phys.t:478-503)
 // ---------------------------------------------------------------------------
 enum { DIVB_NONE = 0, DIVB_GLM = 1, DIVB_LINDE = 4 };
-enum { RIEM_HLL = 1, RIEM_HLLC = 2, RIEM_TVDLF = 5, RIEM_HANCOCK = 8 };
+enum { RIEM_HLL = 1, RIEM_HLLC = 2, RIEM_HLLD = 3, RIEM_TVDLF = 5, RIEM_HANCOCK = 8 };
 enum {
   LIM_MINMOD = 1, LIM_WOODWARD = 2, LIM_MCBETA = 3, LIM_SUPERBEE = 4,
   LIM_VANLEER = 5, LIM_ALBADA = 6, LIM_KOREN = 7, LIM_CADA = 8, LIM_CADA3 = 9, LIM_WENO5 = 16
@@ -461,6 +461,110 @@ __device__ __forceinline__ void weno5(R wmm, R wm, R w0, R wp, R wpp, R wppp, R&
   }
 }
 
+// get_Riemann_flux_hlld src/mod_finite_volume.t:505-718 (Miyoshi & Kusano
+// 2005 with Guo 2016's star pressure; no B0, no stagger grid, ndir = 3):
+// the single-star states w1 across the fast waves, the double-star states w2
+// across the Alfven waves, then the region of the fan at the face.  wLp/wRp
+// are the primitive states after modify_wLR, wLC/wRC the conserved ones.
+template <class P, class R, int ID>
+__device__ __forceinline__ void hlld_flux(const R* wLC, const R* wRC, const R* wLp, const R* wRp, const R* fL,
+                                          const R* fR, R sR, R sL, R* fC) {
+  static_assert(P::MHD && P::NDIR == 3, "HLLD is provided for MHD with three vector components");
+  constexpr int NW = P::NW, RHO = P::RHO, E = P::E, MG = P::MAG, MO = P::MOM;
+  constexpr int I2 = (ID + 1) % 3, I3 = (ID + 2) % 3;
+  constexpr int M1 = MO + ID, M2 = MO + I2, M3 = MO + I3;
+  constexpr int B1 = MG + ID, B2 = MG + I2, B3 = MG + I3;
+  const R Bx = fdiv(sR * wRC[B1] - sL * wLC[B1], sR - sL);
+  const R ptR = wRp[E] + R(0.5) * (wRC[MG] * wRC[MG] + wRC[MG + 1] * wRC[MG + 1] + wRC[MG + 2] * wRC[MG + 2]);
+  const R ptL = wLp[E] + R(0.5) * (wLC[MG] * wLC[MG] + wLC[MG + 1] * wLC[MG + 1] + wLC[MG + 2] * wLC[MG + 2]);
+  const R vR = wRp[M1], vL = wLp[M1];
+  const R suR = (sR - vR) * wRC[RHO];
+  const R suL = (sL - vL) * wLC[RHO];
+  const R sm = fdiv(suR * vR - suL * vL - ptR + ptL, suR - suL);
+  // single-star states (velocities in the momentum slots until the end)
+  R w1R[NW], w1L[NW], w2R[NW], w2L[NW];
+  w1R[M1] = sm; w1L[M1] = sm; w2R[M1] = sm; w2L[M1] = sm;
+  w1R[B1] = Bx; w1L[B1] = Bx;
+  w1R[RHO] = fdiv(suR, sR - sm);
+  w1L[RHO] = fdiv(suL, sL - sm);
+  R r1R = suR * (sR - sm) - Bx * Bx;
+  if (r1R != R(0)) r1R = frcp(r1R);
+  R r1L = suL * (sL - sm) - Bx * Bx;
+  if (r1L != R(0)) r1L = frcp(r1L);
+  w1R[M2] = wRp[M2] - Bx * wRC[B2] * (sm - vR) * r1R;
+  w1L[M2] = wLp[M2] - Bx * wLC[B2] * (sm - vL) * r1L;
+  const R qR = (suR * (sR - vR) - Bx * Bx) * r1R;
+  const R qL = (suL * (sL - vL) - Bx * Bx) * r1L;
+  w1R[M3] = wRp[M3] - Bx * wRC[B3] * (sm - vR) * r1R;
+  w1L[M3] = wLp[M3] - Bx * wLC[B3] * (sm - vL) * r1L;
+  w1R[B3] = wRC[B3] * qR;
+  w1L[B3] = wLC[B3] * qL;
+  w1R[B2] = wRC[B2] * qR;
+  w1L[B2] = wLC[B2] * qL;
+  // total pressure of the star region (Guo eq. 25), then the energies
+  const R pts = suR * (sm - vR) + ptR;
+  {
+    const R vbR = wRp[MO] * wRC[MG] + wRp[MO + 1] * wRC[MG + 1] + wRp[MO + 2] * wRC[MG + 2];
+    const R vb1R = w1R[MO] * w1R[MG] + w1R[MO + 1] * w1R[MG + 1] + w1R[MO + 2] * w1R[MG + 2];
+    w1R[E] = fdiv((sR - vR) * wRC[E] - ptR * vR + pts * sm + Bx * (vbR - vb1R), sR - sm);
+    const R vbL = wLp[MO] * wLC[MG] + wLp[MO + 1] * wLC[MG + 1] + wLp[MO + 2] * wLC[MG + 2];
+    const R vb1L = w1L[MO] * w1L[MG] + w1L[MO + 1] * w1L[MG + 1] + w1L[MO + 2] * w1L[MG + 2];
+    w1L[E] = fdiv((sL - vL) * wLC[E] - ptL * vL + pts * sm + Bx * (vbL - vb1L), sL - sm);
+  }
+  // double-star states
+  w2R[RHO] = w1R[RHO]; w2L[RHO] = w1L[RHO];
+  w2R[B1] = Bx; w2L[B1] = Bx;
+  const R sR1 = fsqrt(w1R[RHO]);
+  const R sL1 = fsqrt(w1L[RHO]);
+  const R tmp = frcp(sR1 + sL1);
+  const R sgn = copysign(R(1), Bx);
+  const R s1R = sm + fdiv(fabs(Bx), sR1);
+  const R s1L = sm - fdiv(fabs(Bx), sL1);
+  w2R[M2] = (sL1 * w1L[M2] + sR1 * w1R[M2] + (w1R[B2] - w1L[B2]) * sgn) * tmp;
+  w2L[M2] = w2R[M2];
+  w2R[B2] = (sL1 * w1R[B2] + sR1 * w1L[B2] + sL1 * sR1 * (w1R[M2] - w1L[M2]) * sgn) * tmp;
+  w2L[B2] = w2R[B2];
+  w2R[M3] = (sL1 * w1L[M3] + sR1 * w1R[M3] + (w1R[B3] - w1L[B3]) * sgn) * tmp;
+  w2L[M3] = w2R[M3];
+  w2R[B3] = (sL1 * w1R[B3] + sR1 * w1L[B3] + sL1 * sR1 * (w1R[M3] - w1L[M3]) * sgn) * tmp;
+  w2L[B3] = w2R[B3];
+  {
+    const R d1R = w1R[MO] * w1R[MG] + w1R[MO + 1] * w1R[MG + 1] + w1R[MO + 2] * w1R[MG + 2];
+    const R d2R = w2R[MO] * w2R[MG] + w2R[MO + 1] * w2R[MG + 1] + w2R[MO + 2] * w2R[MG + 2];
+    w2R[E] = w1R[E] + sR1 * (d1R - d2R) * sgn;
+    const R d1L = w1L[MO] * w1L[MG] + w1L[MO + 1] * w1L[MG + 1] + w1L[MO + 2] * w1L[MG + 2];
+    const R d2L = w2L[MO] * w2L[MG] + w2L[MO + 1] * w2L[MG + 1] + w2L[MO + 2] * w2L[MG + 2];
+    w2L[E] = w1L[E] - sL1 * (d1L - d2L) * sgn;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    w1R[MO + c] = w1R[MO + c] * w1R[RHO];
+    w1L[MO + c] = w1L[MO + c] * w1L[RHO];
+    w2R[MO + c] = w2R[MO + c] * w2R[RHO];
+    w2L[MO + c] = w2L[MO + c] * w2L[RHO];
+  }
+  // flux_type: GLM's psi and normal B take the known flux
+  // (src/mhd/mod_mhd_phys.t:580-586)
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    if (P::GLM && (iw == P::PSI || iw == B1)) {
+      fC[iw] = fL[iw];
+    } else if (sL > R(0)) {
+      fC[iw] = fL[iw];
+    } else if (s1L >= R(0)) {
+      fC[iw] = fL[iw] + sL * (w1L[iw] - wLC[iw]);
+    } else if (sm >= R(0)) {
+      fC[iw] = fL[iw] + sL * (w1L[iw] - wLC[iw]) + s1L * (w2L[iw] - w1L[iw]);
+    } else if (s1R >= R(0)) {
+      fC[iw] = fR[iw] + sR * (w1R[iw] - wRC[iw]) + s1R * (w2R[iw] - w1R[iw]);
+    } else if (sR >= R(0)) {
+      fC[iw] = fR[iw] + sR * (w1R[iw] - wRC[iw]);
+    } else {
+      fC[iw] = fR[iw];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // one face: reconstruct_LR (src/mod_finite_volume.t:1110-1228), modify_wLR
 // (GLM, src/mhd/mod_mhd_phys.t:6326-6377), fluxes, bounds and the Riemann
@@ -541,7 +645,9 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
     // flux_type: normal B gets Lax-Friedrichs dissipation only without GLM
     // (src/mhd/mod_mhd_phys.t:580-588)
     constexpr int BTV = (P::MHD && !P::GLM && P::NDIM > 1) ? (P::MAG + ID) : -1;
-    if constexpr (RIEM == RIEM_HLL) {
+    if constexpr (RIEM == RIEM_HLLD) {
+      hlld_flux<P, R, ID>(wLC, wRC, wLp, wRp, fL, fR, cmax, cmin, fC);
+    } else if constexpr (RIEM == RIEM_HLL) {
       const R inv = frcp(cmax - cmin);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) {

@@ -99,6 +99,9 @@ hipError_t launch_riem(const KernelKey& k, const SweepArgs& a, const FVParams& f
     case RIEM_HLLC:
       if constexpr (!P::MHD) return launch_lim<P, R, RIEM_HLLC>(k, a, fp, grid, s);
       else return hipErrorInvalidValue;
+    case RIEM_HLLD:
+      if constexpr (P::MHD && P::NDIR == 3) return launch_lim<P, R, RIEM_HLLD>(k, a, fp, grid, s);
+      else return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
 }

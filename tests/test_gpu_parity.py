@@ -178,12 +178,20 @@ def test_alfven_golden_on_device():
     assert logs_match(log, gold)
 
 
-def test_field_loop_golden_on_device():
+@pytest.mark.parametrize("scheme", ["3step_hll_cada", "3step_hlld_cada"])
+def test_field_loop_golden_on_device(scheme):
+    """tests/mhd/field_loop_2.5D: HLL with Linde cleaning, HLLD with GLM;
+    the contraction-free kernels equal the oracle's log exactly, the
+    production kernels match the reference's golden log."""
     from test_oracle_golden import run_field_loop
     from amrvac_amd.fv import DeviceFV
-    log = run_field_loop(lambda p, m: DeviceFV(p, m, precision="fp64"))
-    gold = read_log(os.path.join(GOLD, "fl_2d_3step_hll_cada.log"))
+    print(f"[parity] field loop {scheme}", flush=True)
+    log = run_field_loop(lambda p, m: DeviceFV(p, m, precision="fp64"), scheme=scheme)
+    gold = read_log(os.path.join(GOLD, f"fl_2d_{scheme}.log"))
     assert logs_match(log, gold)
+    exact = run_field_loop(lambda p, m: DeviceFV(p, m, precision="fp64_exact"), scheme=scheme)
+    orc = run_field_loop(lambda p, m: OracleFV(p, m, threads=4), scheme=scheme)
+    assert np.allclose(exact, orc, rtol=1e-12, atol=1e-15)
 
 
 def test_download_ghosts_match_oracle_getbc():

@@ -40,6 +40,12 @@ def _cases():
                "fivestep:ssprk5:3:3", "fivestep:ssprk5:2:3"):
         out.append(("hd2", "vanleer", "hll", "Einfeldt", ts))
     out.append(("mhd3glm", "cada3", "hll", "Einfeldt", "fourstep:rk4:3:3"))
+    # HLLD (3-D MHD, with and without GLM; every bound-speed rule)
+    for lim, bs in (("cada3", "Einfeldt"), ("vanleer", "cmaxmean"), ("koren", "cmaxleftright")):
+        out.append(("mhd3glm", lim, "hlld", bs, "threestep"))
+    out.append(("mhd3", "minmod", "hlld", "Einfeldt", "threestep"))
+    out.append(("mhd3glm", "weno5", "hlld", "Einfeldt", "threestep"))
+    out.append(("mhd3", "vanleer", "hlld", "Einfeldt", "twostep"))
     return out
 
 
