@@ -155,7 +155,7 @@ int amrvac_fv_getdt(amrvac_fv_ctx* ctx, int slot, double courantpar,
 
 /* Replaces: advect1(method, dtfactor, idim^LIM, qtC, psa, qt, psb)
    (src/mod_advance.t:611-666) for idimmin=1, idimmax=ndim on one level:
-   finite_volume (fs_hll/fs_hllc/fs_tvdlf) or hancock (fs_hancock) for every
+   finite_volume (fs_hll/fs_hllc/fs_hlld/fs_tvdlf) or hancock (fs_hancock) for every
    block from slot psa into slot psb with qdt = dtfactor*dt, followed by the
    ghost-cell update of psb (on a uniform level ghost cells are read from
    the neighbours by the next sweep, so nothing is stored).
