@@ -14,9 +14,9 @@
 // write distinct cells), so the summation order is fixed and the result
 // deterministic without atomics.  The summed primitive adjoint is carried to
 // the conserved state once per cell.  In 3-D the z faces are reversed
-// beforehand by adj_zface_kernel (each lane a run of ZC faces of one
-// column, into a global face buffer in the face precision) and gathered
-// here in a fixed order.
+// beforehand by adj_zface_kernel (each lane one column of a ZC-cell chunk),
+// which leaves each cell's summed z-face contribution in a global buffer in
+// the face precision; the per-cell LDS sums start from it.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -34,7 +34,7 @@ constexpr int LH = 2;             // lam_out halo of the tile
 constexpr int LW = BX + 2 * LH;
 constexpr int NF = BX + 3;        // faces per row touching the tile: f in [-1, BX+1]
 constexpr int NFACE = NF * BX;    // per dimension and plane
-constexpr int ZC = 8;             // z faces per lane in adj_zface_kernel
+constexpr int ZC = 16;            // cells per lane and chunk in adj_zface_kernel
 
 // lam_out at an arbitrary (possibly neighbouring) cell of plane z
 template <int NW, int NDIM>
@@ -48,10 +48,13 @@ __device__ __forceinline__ void load_lam(const SweepArgs& a, const double* __res
   for (int iw = 0; iw < NW; ++iw) l[iw] = lam[off + iw * cells];
 }
 
-// Low z faces of ZC consecutive cells of one column per lane: the stencil
-// window (4 primitive states) and the two lam_out values slide along z, so
-// each cell is loaded and converted once per chunk.  Face contributions go
-// to gz[(s*NW + iw) * ncells + cell], in the face precision R.
+// z faces of one column per lane, for a chunk of ZC cells: every face whose
+// 4-cell stencil reaches a cell of the chunk (faces k0-1 .. k0+ZC+1, the
+// face k lying between cells k-1 and k) is reversed with a sliding window of
+// four primitive states and two lam_out values, and its four contributions
+// are added to a window of four running per-cell sums.  A cell's sum is
+// complete after its fourth face (faces in increasing z, so the order is
+// fixed) and is written once: gz[iw * ncells + cell], in the face precision.
 template <class P, class R, int RIEM, int LIM>
 __global__ void __launch_bounds__(NTHREADS) adj_zface_kernel(SweepArgs a, FVParams fp, AdjArgs g) {
   constexpr int NW = P::NW, NDIM = P::NDIM;
@@ -63,13 +66,18 @@ __global__ void __launch_bounds__(NTHREADS) adj_zface_kernel(SweepArgs a, FVPara
   const size_t cells = (size_t)BX * BX * a.bnz;
   const size_t ntot = (size_t)a.nblocks * cells;
   R* gz = static_cast<R*>(g.gz);
-  // window: st[s] = cell k-2+s of the face at k; ll/lr = lam_out(k-1), (k)
+  // st[s] = cell k-2+s of the face k; ll/lr = lam_out(k-1), (k);
+  // sum[m] = running sum of cell k-2+m
   R st[4][NW];
-  double ll[NW], lr[NW];
+  double ll[NW], lr[NW], sum[4][NW];
 #pragma unroll
-  for (int s = 0; s < 3; ++s) load_prim<P, R, NDIM>(a, fp, b, tx, ty, k0 - 2 + s, st[s + 1]);
-  load_lam<NW, NDIM>(a, g.lam_out, b, tx, ty, k0 - 1, lr);
-  for (int k = k0; k < k0 + ZC; ++k) {
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) sum[m][iw] = 0.0;
+#pragma unroll
+  for (int s = 0; s < 3; ++s) load_prim<P, R, NDIM>(a, fp, b, tx, ty, k0 - 3 + s, st[s + 1]);
+  load_lam<NW, NDIM>(a, g.lam_out, b, tx, ty, k0 - 2, lr);
+  for (int k = k0 - 1; k <= k0 + ZC + 1; ++k) {
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
       st[0][iw] = st[1][iw];
@@ -82,13 +90,20 @@ __global__ void __launch_bounds__(NTHREADS) adj_zface_kernel(SweepArgs a, FVPara
     R mu[NW];
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) mu[iw] = R(a.dq[2] * (ll[iw] - lr[iw]));
-    double wb[4][NW];
-    face_adj_prim<P, R, RIEM, 2, LIM>(st, fp, mu, wb);
-    const size_t o = (size_t)b * cells + ((size_t)k * BX + ty) * BX + tx;
+    face_adj_prim<P, R, RIEM, 2, LIM, true>(st, fp, mu, sum);
+    // cell k-2 has all four of its faces
+    if (k - 2 >= k0) {
+      const size_t o = (size_t)b * cells + ((size_t)(k - 2) * BX + ty) * BX + tx;
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+      for (int iw = 0; iw < NW; ++iw) gz[(size_t)iw * ntot + o] = R(sum[0][iw]);
+    }
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) gz[(size_t)(s * NW + iw) * ntot + o] = R(wb[s][iw]);
+    for (int iw = 0; iw < NW; ++iw) {
+      sum[0][iw] = sum[1][iw];
+      sum[1][iw] = sum[2][iw];
+      sum[2][iw] = sum[3][iw];
+      sum[3][iw] = 0.0;
+    }
   }
 }
 
@@ -129,6 +144,7 @@ __global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R>)) adj_plane_kernel(
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) Ll[iw][i] = R(l[iw]);
   }
+  // the per-cell sums start from the cell's z-face contributions (3-D)
   double acc[NW];
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
@@ -136,7 +152,12 @@ __global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R>)) adj_plane_kernel(
     if (g.add_a) v = v + g.ca * g.add_a[o + iw * cells];
     if (g.add_b) v = v + g.cb * g.add_b[o + iw * cells];
     acc[iw] = v;
-    Acc[iw][t] = 0.0;
+    if constexpr (NDIM == 3) {
+      const size_t ntot = (size_t)a.nblocks * cells;
+      Acc[iw][t] = double(static_cast<const R*>(g.gz)[(size_t)iw * ntot + (size_t)b * cells + (size_t)k * BX * BX + t]);
+    } else {
+      Acc[iw][t] = 0.0;
+    }
   }
   __syncthreads();
 
@@ -204,21 +225,6 @@ __global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R>)) adj_plane_kernel(
   double wsum[NW];
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) wsum[iw] = Acc[iw][t];
-
-  // ---- z faces (3-D): reversed by adj_zface_kernel ------------------------
-  if constexpr (NDIM == 3) {
-    const size_t ntot = (size_t)a.nblocks * cells;
-    const R* gz = static_cast<const R*>(g.gz);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      int x = tx, y = ty, z = k - s + 2;
-      double sign;
-      const int ob = locate<NDIM>(a, b, x, y, z, sign);
-      const size_t of = (size_t)ob * cells + ((size_t)z * BX + y) * BX + x;
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) wsum[iw] += double(gz[(size_t)(s * NW + iw) * ntot + of]);
-    }
-  }
 
   // ---- to the conserved state; lam_in -------------------------------------
   double cons[NW];

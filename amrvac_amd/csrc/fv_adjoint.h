@@ -30,7 +30,7 @@ __device__ __forceinline__ R lim_grad(R a, R b, R& da, R& db) {
     const R ab = a * b;
     if (!(ab > R(0))) { da = R(0); db = R(0); return R(0); }
     const R q = b + a + R(1.0e-12);
-    const R iq = R(1) / q;
+    const R iq = frcp(q);
     const R t = R(2) * ab * iq;
     da = R(2) * b * (q - a) * iq * iq;
     db = R(2) * a * (q - b) * iq * iq;
@@ -40,8 +40,9 @@ __device__ __forceinline__ R lim_grad(R a, R b, R& da, R& db) {
 
 // Reverse of the face flux in primitive form: from the primitive states w of
 // the face's four cells (as the sweep rounds them to R) and the flux adjoint
-// fbar, writes the primitive adjoints wb[c] = (d fC / d w_c)^T fbar.
-template <class P, class R, int RIEM, int ID, int LIM>
+// fbar, writes the primitive adjoints wb[c] = (d fC / d w_c)^T fbar (or adds
+// them to wb[c] when ACC).
+template <class P, class R, int RIEM, int ID, int LIM, bool ACC = false>
 __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const R* fbar, double (*wb)[P::NW]) {
   static_assert(!P::MHD, "hydrodynamics");
   constexpr int NW = P::NW, ND = P::NDIR, RHO = P::RHO, E = P::E, MN = P::MOM + ID;
@@ -73,14 +74,14 @@ __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const 
   phys_flux<P, R, ID>(wLC, wL, R(0), fL);
   phys_flux<P, R, ID>(wRC, wR, R(0), fR);
   // Einfeldt bounds (hd_get_cbounds, typeboundspeed 'Einfeldt')
-  const R t1 = sqrt(wL[RHO]), t2 = sqrt(wR[RHO]);
-  const R t3 = R(1) / (t1 + t2);
+  const R t1 = fsqrt(wL[RHO]), t2 = fsqrt(wR[RHO]);
+  const R t3 = frcp(t1 + t2);
   const R um = (wL[MN] * t1 + wR[MN] * t2) * t3;
-  const R csL = gam * wL[E] / wL[RHO], csR = gam * wR[E] / wR[RHO];
+  const R csL = fdiv(gam * wL[E], wL[RHO]), csR = fdiv(gam * wR[E], wR[RHO]);
   const R dv = wR[MN] - wL[MN];
   const R A = t1 * csL + t2 * csR;
   const R D2 = A * t3 + R(0.5) * t1 * t2 * (t3 * t3) * (dv * dv);
-  const R dmean = sqrt(D2);
+  const R dmean = fsqrt(D2);
   R cmax, cmin;
   if constexpr (RIEM == RIEM_TVDLF) {
     cmax = fabs(um) + dmean;
@@ -110,7 +111,7 @@ __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const 
     } else if (cmax <= R(0)) {
       for (int iw = 0; iw < NW; ++iw) fRb[iw] = fbar[iw];
     } else {
-      const R iD = R(1) / (cmax - cmin);
+      const R iD = frcp(cmax - cmin);
       R Db = R(0);
       for (int iw = 0; iw < NW; ++iw) {
         const R dw = wRC[iw] - wLC[iw];
@@ -133,10 +134,10 @@ __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const 
     else if (cmax <= R(0)) patchf = 2;
     R lam = R(0), iD = R(0), whr = R(0), whm = R(0);
     if (patchf == 1) {
-      iD = R(1) / (cmax - cmin);
+      iD = frcp(cmax - cmin);
       whr = (cmax * wRC[RHO] - cmin * wLC[RHO] + fL[RHO] - fR[RHO]) * iD;
       whm = (cmax * wRC[MN] - cmin * wLC[MN] + fL[MN] - fR[MN]) * iD;
-      lam = whm / whr;
+      lam = fdiv(whm, whr);
       bool cond = true;
       if (cmin < R(0) && lam > R(0) && lam < cmax) {
         patchf = -1;
@@ -176,9 +177,9 @@ __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const 
       const R* fS = right ? fR : fL;
       R* wSb = right ? wRCb : wLCb;
       R* fSb = right ? fRb : fLb;
-      const R vS = wS[MN] / wS[RHO];
+      const R vS = fdiv(wS[MN], wS[RHO]);
       const R csl = cs - lam;
-      const R csmls = R(1) / csl;
+      const R csmls = frcp(csl);
       const R Z = wS[RHO] * (cs - vS);
       R wCD[NW];
       wCD[RHO] = Z * csmls;
@@ -239,15 +240,17 @@ __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const 
       csb = csb + cslb;
       lamb = lamb - cslb;
       // vS = wS[MN] / wS[RHO]
-      wSb[MN] = wSb[MN] + vSb / wS[RHO];
-      wSb[RHO] = wSb[RHO] - vSb * vS / wS[RHO];
+      const R irS = frcp(wS[RHO]);
+      wSb[MN] = wSb[MN] + vSb * irS;
+      wSb[RHO] = wSb[RHO] - vSb * vS * irS;
       // lam = whm / whr
-      const R whmb = lamb / whr;
-      const R whrb = -lamb * lam / whr;
+      const R irh = frcp(whr);
+      const R whmb = lamb * irh;
+      const R whrb = -lamb * lam * irh;
       // whll = (cmax wRC - cmin wLC + fL - fR) iD for RHO and MN
       R iDb = R(0);
       {
-        const R Qr = whr / iD, Qm = whm / iD;  // numerators
+        const R Qr = fdiv(whr, iD), Qm = fdiv(whm, iD);  // numerators
         (void)Qr; (void)Qm;
         const int vars[2] = {RHO, MN};
         const R wb[2] = {whrb, whmb};
@@ -284,7 +287,7 @@ __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const 
       umb = cmaxb + cminb;
       dmb = cmaxb - cminb;
     }
-    const R D2b = dmb * R(0.5) / dmean;
+    const R D2b = fdiv(dmb * R(0.5), dmean);
     const R Ab = D2b * t3;
     R t3b = D2b * A;
     const R Bb = D2b;
@@ -295,10 +298,11 @@ __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const 
     t1b = t1b + Ab * csL;
     t2b = t2b + Ab * csR;
     const R csLb = Ab * t1, csRb = Ab * t2;
-    wLb[E] = wLb[E] + csLb * gam / wL[RHO];
-    wLb[RHO] = wLb[RHO] - csLb * csL / wL[RHO];
-    wRb[E] = wRb[E] + csRb * gam / wR[RHO];
-    wRb[RHO] = wRb[RHO] - csRb * csR / wR[RHO];
+    const R irL = frcp(wL[RHO]), irR = frcp(wR[RHO]);
+    wLb[E] = wLb[E] + csLb * gam * irL;
+    wLb[RHO] = wLb[RHO] - csLb * csL * irL;
+    wRb[E] = wRb[E] + csRb * gam * irR;
+    wRb[RHO] = wRb[RHO] - csRb * csR * irR;
     wRb[MN] = wRb[MN] + dvb;
     wLb[MN] = wLb[MN] - dvb;
     wLb[MN] = wLb[MN] + umb * t1 * t3;
@@ -308,8 +312,8 @@ __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const 
     t3b = t3b + umb * (wL[MN] * t1 + wR[MN] * t2);
     t1b = t1b - t3b * t3 * t3;
     t2b = t2b - t3b * t3 * t3;
-    wLb[RHO] = wLb[RHO] + t1b * R(0.5) / t1;
-    wRb[RHO] = wRb[RHO] + t2b * R(0.5) / t2;
+    wLb[RHO] = wLb[RHO] + fdiv(t1b * R(0.5), t1);
+    wRb[RHO] = wRb[RHO] + fdiv(t2b * R(0.5), t2);
   }
   // fluxes: f_rho = vn rho, f_mk = vn m_k, f_mn += p, f_E = vn (E + p)
   auto flux_rev = [&](const R* wX, const R* wXC, const R* fXb, R* wXb, R* wXCb) {
@@ -347,10 +351,17 @@ __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const 
     const R dmb = t1b * d1a[iw];
     const R d0b = t1b * d1b[iw] + t2b * d2a[iw];
     const R dpb = t2b * d2b[iw];
-    wb[0][iw] = double(-dmb);
-    wb[1][iw] = double(wLb[iw] + dmb - d0b);
-    wb[2][iw] = double(wRb[iw] + d0b - dpb);
-    wb[3][iw] = double(dpb);
+    if constexpr (ACC) {
+      wb[0][iw] += double(-dmb);
+      wb[1][iw] += double(wLb[iw] + dmb - d0b);
+      wb[2][iw] += double(wRb[iw] + d0b - dpb);
+      wb[3][iw] += double(dpb);
+    } else {
+      wb[0][iw] = double(-dmb);
+      wb[1][iw] = double(wLb[iw] + dmb - d0b);
+      wb[2][iw] = double(wRb[iw] + d0b - dpb);
+      wb[3][iw] = double(dpb);
+    }
   }
 }
 
@@ -360,7 +371,7 @@ __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const 
 // the fast speed (mhd_get_cbounds :3037-3127, mhd_get_csound_prim
 // :3329-3379) -> HLL or TVDLF; the normal field takes Lax-Friedrichs
 // dissipation under HLL (flux_type, :580-588).
-template <class P, class R, int RIEM, int ID, int LIM>
+template <class P, class R, int RIEM, int ID, int LIM, bool ACC = false>
 __device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const R* fbar, double (*wb)[P::NW]) {
   static_assert(P::MHD && !P::GLM, "ideal MHD without GLM");
   static_assert(RIEM == RIEM_HLL || RIEM == RIEM_TVDLF, "MHD faces take hll or tvdlf");
@@ -389,7 +400,7 @@ __device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const
   struct Fast { R ir, cs, b2, cf2, av, sq, c; };
   auto fast = [&](const R* x) {
     Fast q;
-    q.ir = R(1) / x[RHO];
+    q.ir = frcp(x[RHO]);
     q.cs = gam * x[E] * q.ir;
     R b2 = x[P::MAG] * x[P::MAG];
 #pragma unroll
@@ -397,18 +408,18 @@ __device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const
     q.b2 = b2;
     q.cf2 = b2 * q.ir + q.cs;
     q.av = q.cf2 * q.cf2 - R(4) * q.cs * (x[BN] * x[BN]) * q.ir;
-    q.sq = q.av > R(0) ? sqrt(q.av) : R(0);
-    q.c = sqrt(R(0.5) * (q.cf2 + q.sq));
+    q.sq = q.av > R(0) ? fsqrt(q.av) : R(0);
+    q.c = fsqrt(R(0.5) * (q.cf2 + q.sq));
     return q;
   };
   const Fast qL = fast(wL), qR = fast(wR);
-  const R t1 = sqrt(wL[RHO]), t2 = sqrt(wR[RHO]);
-  const R t3 = R(1) / (t1 + t2);
+  const R t1 = fsqrt(wL[RHO]), t2 = fsqrt(wR[RHO]);
+  const R t3 = frcp(t1 + t2);
   const R um = (wL[MN] * t1 + wR[MN] * t2) * t3;
   const R dv = wR[MN] - wL[MN];
   const R A = t1 * (qL.c * qL.c) + t2 * (qR.c * qR.c);
   const R D2 = A * t3 + R(0.5) * t1 * t2 * (t3 * t3) * (dv * dv);
-  const R dmean = sqrt(D2);
+  const R dmean = fsqrt(D2);
   R cmax, cmin;
   if constexpr (RIEM == RIEM_TVDLF) {
     cmax = fabs(um) + dmean;
@@ -436,7 +447,7 @@ __device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const
     cmaxb = R(-0.5) * eps * s;
   } else {
     const bool mid = !(cmin >= R(0)) && !(cmax <= R(0));
-    const R iD = mid ? R(1) / (cmax - cmin) : R(0);
+    const R iD = mid ? frcp(cmax - cmin) : R(0);
     R Db = R(0);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
@@ -483,7 +494,7 @@ __device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const
       umb = cmaxb + cminb;
       dmb = cmaxb - cminb;
     }
-    const R D2b = dmb * R(0.5) / dmean;
+    const R D2b = fdiv(dmb * R(0.5), dmean);
     const R Ab = D2b * t3;
     R t3b = D2b * A;
     R t1b = D2b * R(0.5) * t2 * (t3 * t3) * (dv * dv);
@@ -502,14 +513,14 @@ __device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const
     t3b = t3b + umb * (wL[MN] * t1 + wR[MN] * t2);
     t1b = t1b - t3b * t3 * t3;
     t2b = t2b - t3b * t3 * t3;
-    wLb[RHO] = wLb[RHO] + t1b * R(0.5) / t1;
-    wRb[RHO] = wRb[RHO] + t2b * R(0.5) / t2;
+    wLb[RHO] = wLb[RHO] + fdiv(t1b * R(0.5), t1);
+    wRb[RHO] = wRb[RHO] + fdiv(t2b * R(0.5), t2);
     // fast speed reverse: c = sqrt((cf2 + sqrt(av)) / 2)
     auto fast_rev = [&](const R* x, const Fast& q, R cb, R* xb) {
-      const R hb = cb * R(0.5) / q.c;
+      const R hb = fdiv(cb * R(0.5), q.c);
       R cf2b = R(0.5) * hb;
       const R sqb = R(0.5) * hb;
-      const R avb = q.av > R(0) ? sqb * R(0.5) / q.sq : R(0);
+      const R avb = q.av > R(0) ? fdiv(sqb * R(0.5), q.sq) : R(0);
       const R bn = x[BN];
       cf2b = cf2b + R(2) * q.cf2 * avb;
       R csb = R(-4) * (bn * bn) * q.ir * avb;
@@ -587,18 +598,25 @@ __device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const
     const R dmb = t1b * d1a[iw];
     const R d0b = t1b * d1b[iw] + t2b * d2a[iw];
     const R dpb = t2b * d2b[iw];
-    wb[0][iw] = double(-dmb);
-    wb[1][iw] = double(wLb[iw] + dmb - d0b);
-    wb[2][iw] = double(wRb[iw] + d0b - dpb);
-    wb[3][iw] = double(dpb);
+    if constexpr (ACC) {
+      wb[0][iw] += double(-dmb);
+      wb[1][iw] += double(wLb[iw] + dmb - d0b);
+      wb[2][iw] += double(wRb[iw] + d0b - dpb);
+      wb[3][iw] += double(dpb);
+    } else {
+      wb[0][iw] = double(-dmb);
+      wb[1][iw] = double(wLb[iw] + dmb - d0b);
+      wb[2][iw] = double(wRb[iw] + d0b - dpb);
+      wb[3][iw] = double(dpb);
+    }
   }
 }
 
-template <class P, class R, int RIEM, int ID, int LIM>
+template <class P, class R, int RIEM, int ID, int LIM, bool ACC = false>
 __device__ __forceinline__ void face_adj_prim(const R (*w)[P::NW], const FVParams& fp, const R* fbar,
                                               double (*wb)[P::NW]) {
-  if constexpr (P::MHD) face_adj_prim_mhd<P, R, RIEM, ID, LIM>(w, fp, fbar, wb);
-  else face_adj_prim_hd<P, R, RIEM, ID, LIM>(w, fp, fbar, wb);
+  if constexpr (P::MHD) face_adj_prim_mhd<P, R, RIEM, ID, LIM, ACC>(w, fp, fbar, wb);
+  else face_adj_prim_hd<P, R, RIEM, ID, LIM, ACC>(w, fp, fbar, wb);
 }
 
 // Reverse of hd_to_primitive (FP64) for one cell: cbar += (d w / d cons)^T wb
@@ -607,7 +625,7 @@ template <class P>
 __device__ __forceinline__ void prim_rev(const double* cons, const double* wb, const FVParams& fp, double* cbar) {
   constexpr int ND = P::NDIR, RHO = P::RHO, E = P::E;
   const double rho = cons[RHO];
-  const double inv = 1.0 / rho;
+  const double inv = frcp(rho);
   double vv[ND], vsq = 0.0;
 #pragma unroll
   for (int k = 0; k < ND; ++k) {

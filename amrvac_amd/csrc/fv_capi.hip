@@ -72,7 +72,7 @@ struct amrvac_fv_ctx {
   int* d_fregoff = nullptr;  // [nb][2*ndim]
   double* d_freg = nullptr;  // flux registers
   double* d_xlo = nullptr;   // [nb][ndim] block lower corners
-  double* d_adjG = nullptr;  // 3-D adjoint z-face contributions [4*nw][nb*cells]
+  double* d_adjG = nullptr;  // 3-D adjoint: summed z-face contributions per cell [nw][nb*cells]
   // device regrid: new-leaf state, plan, Löhner flags and weights (kept across
   // regrids, grown on demand)
   double* d_remap = nullptr;
@@ -1463,7 +1463,7 @@ int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt,
     return fail(AMRVAC_FV_EINVAL, "adjoint: flux scheme must be hll, hllc or tvdlf");
   if (riem == fv::RIEM_HLLC && c->cfg.physics != 1) return fail(AMRVAC_FV_EINVAL, "hllc is provided for hd");
   if (c->cfg.ndim == 3 && !c->d_adjG) {
-    const size_t n = (size_t)4 * c->nw * c->nb * c->cells;
+    const size_t n = (size_t)c->nw * c->nb * c->cells;
     if (hipMalloc(&c->d_adjG, n * sizeof(double)) != hipSuccess) return fail(AMRVAC_FV_ENOMEM, "adjoint buffer");
   }
   fvc::SweepArgs a = base_args(c);

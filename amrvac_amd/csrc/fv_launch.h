@@ -19,7 +19,7 @@ struct KernelKey {
 // Adjoint stage operands (device arrays in the interior-only block layout
 // [nb][nw][bnz][16][16]):
 //   lam_in = coef * lam_out + ca * add_a + cb * add_b + (d stage / d src)^T lam_out
-// add_a / add_b may be null; gz is the 3-D z-face buffer, 4*nw*nb*cells
+// add_a / add_b may be null; gz is the 3-D z-face buffer, nw*nb*cells
 // values of the face precision (double or float).
 struct AdjArgs {
   const double* lam_out;
