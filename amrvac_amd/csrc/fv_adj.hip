@@ -128,6 +128,21 @@ __global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R>)) adj_plane_kernel(
   const size_t cells = (size_t)BX * BX * a.bnz;
   const size_t o = (size_t)b * NW * cells + ((size_t)k * BX + ty) * BX + tx;
 
+  // lam_out and the z-face sum of the own cell are loaded first, so that
+  // their latency overlaps the staging; the per-cell sums start from the
+  // z-face contributions (3-D)
+  double lo[NW];
+  R zs[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    lo[iw] = g.lam_out[o + iw * cells];
+    if constexpr (NDIM == 3) {
+      const size_t ntot = (size_t)a.nblocks * cells;
+      zs[iw] = static_cast<const R*>(g.gz)[(size_t)iw * ntot + (size_t)b * cells + (size_t)k * BX * BX + t];
+    } else {
+      zs[iw] = R(0);
+    }
+  }
   for (int i = t; i < AW * AW; i += NTHREADS) {
     const int x = i % AW - AH, y = i / AW - AH;
     if ((x < 0 || x >= BX) && (y < 0 || y >= BX)) continue;
@@ -144,20 +159,14 @@ __global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R>)) adj_plane_kernel(
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) Ll[iw][i] = R(l[iw]);
   }
-  // the per-cell sums start from the cell's z-face contributions (3-D)
   double acc[NW];
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
-    double v = g.coef * g.lam_out[o + iw * cells];
+    double v = g.coef * lo[iw];
     if (g.add_a) v = v + g.ca * g.add_a[o + iw * cells];
     if (g.add_b) v = v + g.cb * g.add_b[o + iw * cells];
     acc[iw] = v;
-    if constexpr (NDIM == 3) {
-      const size_t ntot = (size_t)a.nblocks * cells;
-      Acc[iw][t] = double(static_cast<const R*>(g.gz)[(size_t)iw * ntot + (size_t)b * cells + (size_t)k * BX * BX + t]);
-    } else {
-      Acc[iw][t] = 0.0;
-    }
+    Acc[iw][t] = double(zs[iw]);
   }
   __syncthreads();
 
