@@ -529,9 +529,12 @@ class AMRIntegrator(TimeIntegrator):
     """The reference's time loop over an AMR forest.  ``make_backend(par,
     forest)`` returns a backend (the host protocol plus
     ``set_fix_conserve``) for the current leaves; ``init_fn(forest, blocks)``
-    returns the ghosted conserved initial state of those leaves."""
+    returns the ghosted conserved initial state of those leaves.  With
+    ``restart`` the forest and init_fn come from a snapshot and the initial
+    refinement is skipped (the reference's restart reads the tree instead of
+    running settree unless reset_grid, src/amrvac.t:47-95)."""
 
-    def __init__(self, par: Par, forest: AMRForest, ap: AMRPar, make_backend, init_fn):
+    def __init__(self, par: Par, forest: AMRForest, ap: AMRPar, make_backend, init_fn, restart: bool = False):
         self.forest = forest
         self.ap = ap
         self.make_backend = make_backend
@@ -542,6 +545,8 @@ class AMRIntegrator(TimeIntegrator):
         # initlevelone + settree: refine on the initial condition
         be.set_state(0, init_fn(forest, list(range(forest.nblocks))))
         be.getbc(0)
+        if restart:
+            return
         for levnew in range(2, ap.refine_max_level + 1):
             self.regrid(time_advance=False)
             if forest.levmax != levnew:
