@@ -1030,7 +1030,6 @@ static int stage_core(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
                       double comb_b, int dst_slot, int* crash_out) {
   int riem;
   if (method_to_riem(method, riem) != 0) return fail(AMRVAC_FV_EINVAL, "unsupported flux scheme");
-  if (riem == fv::RIEM_HLLC && c->cfg.physics != 1) return fail(AMRVAC_FV_EINVAL, "hllc is provided for hd");
   if (riem == fv::RIEM_HLLD && (c->cfg.physics != 2 || c->cfg.ndir != 3))
     return fail(AMRVAC_FV_EINVAL, "hlld is provided for mhd with ndir = 3");
   if (riem == fv::RIEM_HANCOCK && c->cfg.limiter == 16)
@@ -1461,7 +1460,8 @@ int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt,
   int riem;
   if (method_to_riem(method, riem) != 0 || riem == fv::RIEM_HANCOCK || riem == fv::RIEM_HLLD)
     return fail(AMRVAC_FV_EINVAL, "adjoint: flux scheme must be hll, hllc or tvdlf");
-  if (riem == fv::RIEM_HLLC && c->cfg.physics != 1) return fail(AMRVAC_FV_EINVAL, "hllc is provided for hd");
+  if (riem == fv::RIEM_HLLC && c->cfg.physics != 1)
+    return fail(AMRVAC_FV_EINVAL, "adjoint: hllc is provided for hd");
   if (c->cfg.ndim == 3 && !c->d_adjG) {
     const size_t n = (size_t)c->nw * c->nb * c->cells;
     if (hipMalloc(&c->d_adjG, n * sizeof(double)) != hipSuccess) return fail(AMRVAC_FV_ENOMEM, "adjoint buffer");

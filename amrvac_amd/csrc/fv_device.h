@@ -663,13 +663,15 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
           else fC[iw] = num * inv;
         }
       }
-    } else {  // HLLC (hydrodynamics)
+    } else {  // HLLC: hd_get_lCD / hd_get_wCD or mhd_get_lCD / mhd_get_wCD
       constexpr int RHO = P::RHO, MN = P::MOM + ID, E = P::E;
       int patchf = 1;
       if (cmin >= R(0)) patchf = -2;
       else if (cmax <= R(0)) patchf = 2;
       R lambdaCD = R(0);
       R Fhll[NW];
+      // the HLL state of the magnetic field and momentum (MHD contact state)
+      R whll[P::MHD ? NW : 1];
       if (patchf == 1) {
         R whll_m = R(0), whll_r = R(0);
         const R inv = frcp(cmax - cmin);
@@ -687,6 +689,7 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
           }
           if (iw == MN) whll_m = wh;
           if (iw == RHO) whll_r = wh;
+          if constexpr (P::MHD) whll[iw] = wh;
         }
         lambdaCD = fdiv(whll_m, whll_r);
         bool cond = true;
@@ -714,17 +717,44 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
           fSub[iw] = right ? fR[iw] : fL[iw];
         }
         const R vSub = fdiv(wSub[MN], wSub[RHO]);
-        const R csmls = frcp(cspeed - lambdaCD);
-        wCD[RHO] = wSub[RHO] * (cspeed - vSub) * csmls;
+        if constexpr (P::MHD) {
+          // mhd_get_wCD src/mhd/mod_mhd_hllc.t:147-240 (Li 2005)
+          constexpr int BN = P::MAG + ID;
+          const R den = cspeed - lambdaCD;
+          wCD[RHO] = fdiv(wSub[RHO] * (cspeed - vSub), den);
 #pragma unroll
-        for (int d = 0; d < P::NDIR; ++d) {
-          if (d != ID) wCD[P::MOM + d] = (cspeed * wSub[P::MOM + d] - fSub[P::MOM + d]) * csmls;
-          else wCD[P::MOM + d] = wCD[RHO] * lambdaCD;
+          for (int d = 0; d < P::NDIR; ++d) wCD[P::MAG + d] = whll[P::MAG + d];
+#pragma unroll
+          for (int d = 0; d < P::NDIR; ++d) {
+            if (d != ID)
+              wCD[P::MOM + d] = fdiv(cspeed * wSub[P::MOM + d] - fSub[P::MOM + d] - wCD[BN] * wCD[P::MAG + d], den);
+            else
+              wCD[P::MOM + d] = wCD[RHO] * lambdaCD;
+          }
+          R vb = whll[P::MOM] * whll[P::MAG];
+#pragma unroll
+          for (int d = 1; d < P::NDIR; ++d) vb = vb + whll[P::MOM + d] * whll[P::MAG + d];
+          const R vdotb = fdiv(vb, whll[RHO]);
+          const R pCD = wSub[RHO] * (cspeed - vSub) * (lambdaCD - vSub) + fSub[MN] - wSub[MN] * vSub + wCD[BN] * wCD[BN];
+          wCD[E] = fdiv(cspeed * wSub[E] - fSub[E] + lambdaCD * pCD - vdotb * wCD[BN], den);
+#pragma unroll
+          for (int iw = 0; iw < NW; ++iw) {
+            if (iw == BN || (P::GLM && iw == P::PSI)) fC[iw] = R(0);
+            else fC[iw] = fSub[iw] + cspeed * (wCD[iw] - wSub[iw]);
+          }
+        } else {
+          const R csmls = frcp(cspeed - lambdaCD);
+          wCD[RHO] = wSub[RHO] * (cspeed - vSub) * csmls;
+#pragma unroll
+          for (int d = 0; d < P::NDIR; ++d) {
+            if (d != ID) wCD[P::MOM + d] = (cspeed * wSub[P::MOM + d] - fSub[P::MOM + d]) * csmls;
+            else wCD[P::MOM + d] = wCD[RHO] * lambdaCD;
+          }
+          const R pCD = wSub[RHO] * (cspeed - vSub) * (lambdaCD - vSub) + fSub[MN] - wSub[MN] * vSub;
+          wCD[E] = (cspeed * wSub[E] - fSub[E] + lambdaCD * pCD) * csmls;
+#pragma unroll
+          for (int iw = 0; iw < NW; ++iw) fC[iw] = fSub[iw] + cspeed * (wCD[iw] - wSub[iw]);
         }
-        const R pCD = wSub[RHO] * (cspeed - vSub) * (lambdaCD - vSub) + fSub[MN] - wSub[MN] * vSub;
-        wCD[E] = (cspeed * wSub[E] - fSub[E] + lambdaCD * pCD) * csmls;
-#pragma unroll
-        for (int iw = 0; iw < NW; ++iw) fC[iw] = fSub[iw] + cspeed * (wCD[iw] - wSub[iw]);
       } else if (patchf == -2) {
 #pragma unroll
         for (int iw = 0; iw < NW; ++iw) fC[iw] = fL[iw];
@@ -735,6 +765,10 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
 #pragma unroll
         for (int iw = 0; iw < NW; ++iw) fC[iw] = Fhll[iw];
       }
+      // flux_type: the normal field takes Lax-Friedrichs dissipation only
+      // without GLM (src/mod_finite_volume.t:477-479)
+      if constexpr (BTV >= 0)
+        fC[BTV] = -R(fp.tvdlfeps) * R(0.5) * mx2(cmax, R(fabs(cmin))) * (wRC[BTV] - wLC[BTV]);
     }
   }
 }

@@ -96,9 +96,7 @@ hipError_t launch_riem(const KernelKey& k, const SweepArgs& a, const FVParams& f
     case RIEM_HLL: return launch_lim<P, R, RIEM_HLL>(k, a, fp, grid, s);
     case RIEM_TVDLF: return launch_lim<P, R, RIEM_TVDLF>(k, a, fp, grid, s);
     case RIEM_HANCOCK: return launch_lim<P, R, RIEM_HANCOCK>(k, a, fp, grid, s);
-    case RIEM_HLLC:
-      if constexpr (!P::MHD) return launch_lim<P, R, RIEM_HLLC>(k, a, fp, grid, s);
-      else return hipErrorInvalidValue;
+    case RIEM_HLLC: return launch_lim<P, R, RIEM_HLLC>(k, a, fp, grid, s);
     case RIEM_HLLD:
       if constexpr (P::MHD && P::NDIR == 3) return launch_lim<P, R, RIEM_HLLD>(k, a, fp, grid, s);
       else return hipErrorInvalidValue;

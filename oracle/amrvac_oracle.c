@@ -635,7 +635,9 @@ static void riemann_hlld(const orc_sim* s, int idim, const double* wLC, const do
 }
 
 /* get_Riemann_flux_hllc src/mod_finite_volume.t:439-504 with hd_get_lCD
-   src/hd/mod_hd_hllc.t:56-152 and hd_get_wCD :154-257 */
+   src/hd/mod_hd_hllc.t:56-152 and hd_get_wCD :154-257, or mhd_get_lCD
+   src/mhd/mod_mhd_hllc.t:55-145 (the same contact speed and patch logic,
+   flathllc off) and mhd_get_wCD :147-240 */
 static void riemann_hllc(const orc_sim* s, int idim, const double* wLC,
                          const double* wRC, const double* fL, const double* fR,
                          double cmax, double cmin, double* fC) {
@@ -680,19 +682,44 @@ static void riemann_hllc(const orc_sim* s, int idim, const double* wLC,
       fSub = fL;
     }
     double wCD[MAXW];
-    double csmls = one / (cspeed - lambdaCD);
-    wCD[rho] = wSub[rho] * (cspeed - vSub) * csmls;
-    for (int c = 0; c < s->p.ndir; ++c) {
-      if (c != idim)
-        wCD[s->mom[c]] = (cspeed * wSub[s->mom[c]] - fSub[s->mom[c]]) * csmls;
-      else
-        wCD[s->mom[c]] = wCD[rho] * lambdaCD;
+    if (s->p.phys == ORC_PHYS_MHD) {
+      /* mhd_get_wCD src/mhd/mod_mhd_hllc.t:147-240 (Li 2005; no tracers):
+         the HLL magnetic field, eqs. 17, 20-22 and 31; the normal field
+         and psi take no flux in the contact region */
+      const double den = cspeed - lambdaCD;
+      const int bn = s->mag[idim];
+      wCD[rho] = wSub[rho] * (cspeed - vSub) / den;
+      for (int c = 0; c < s->p.ndir; ++c) wCD[s->mag[c]] = whll[s->mag[c]];
+      for (int c = 0; c < s->p.ndir; ++c) {
+        const int m = s->mom[c];
+        if (c != idim)
+          wCD[m] = (cspeed * wSub[m] - fSub[m] - wCD[bn] * wCD[s->mag[c]]) / den;
+        else
+          wCD[m] = wCD[rho] * lambdaCD;
+      }
+      double vb = whll[s->mom[0]] * whll[s->mag[0]];
+      for (int c = 1; c < s->p.ndir; ++c) vb = vb + whll[s->mom[c]] * whll[s->mag[c]];
+      const double VdotBCD = vb / whll[rho];
+      const double pCD = wSub[rho] * (cspeed - vSub) * (lambdaCD - vSub) + fSub[mn] - wSub[mn] * vSub +
+                         wCD[bn] * wCD[bn];
+      wCD[e] = (cspeed * wSub[e] - fSub[e] + lambdaCD * pCD - VdotBCD * wCD[bn]) / den;
+      for (int iw = 0; iw < nw; ++iw)
+        fCD[iw] = (iw == bn || iw == s->psi_) ? zero : fSub[iw] + cspeed * (wCD[iw] - wSub[iw]);
+    } else {
+      double csmls = one / (cspeed - lambdaCD);
+      wCD[rho] = wSub[rho] * (cspeed - vSub) * csmls;
+      for (int c = 0; c < s->p.ndir; ++c) {
+        if (c != idim)
+          wCD[s->mom[c]] = (cspeed * wSub[s->mom[c]] - fSub[s->mom[c]]) * csmls;
+        else
+          wCD[s->mom[c]] = wCD[rho] * lambdaCD;
+      }
+      double pCD = wSub[rho] * (cspeed - vSub) * (lambdaCD - vSub) + fSub[mn] -
+                   wSub[mn] * vSub;
+      wCD[e] = (cspeed * wSub[e] - fSub[e] + lambdaCD * pCD) * csmls;
+      for (int iw = 0; iw < nw; ++iw)
+        fCD[iw] = fSub[iw] + cspeed * (wCD[iw] - wSub[iw]);
     }
-    double pCD = wSub[rho] * (cspeed - vSub) * (lambdaCD - vSub) + fSub[mn] -
-                 wSub[mn] * vSub;
-    wCD[e] = (cspeed * wSub[e] - fSub[e] + lambdaCD * pCD) * csmls;
-    for (int iw = 0; iw < nw; ++iw)
-      fCD[iw] = fSub[iw] + cspeed * (wCD[iw] - wSub[iw]);
   }
   for (int iw = 0; iw < nw; ++iw) {
     if (s->ftype[idim][iw] == FT_TVDLF) {

@@ -46,6 +46,11 @@ def _cases():
     out.append(("mhd3", "minmod", "hlld", "Einfeldt", "threestep"))
     out.append(("mhd3glm", "weno5", "hlld", "Einfeldt", "threestep"))
     out.append(("mhd3", "vanleer", "hlld", "Einfeldt", "twostep"))
+    # MHD HLLC (Li 2005 contact state), with and without GLM
+    out.append(("mhd3glm", "koren", "hllc", "cmaxmean", "fourstep:ssprk4:3:3"))
+    out.append(("mhd3glm", "cada3", "hllc", "Einfeldt", "threestep"))
+    out.append(("mhd3", "vanleer", "hllc", "Einfeldt", "threestep"))
+    out.append(("mhd3", "minmod", "hllc", "cmaxleftright", "twostep"))
     return out
 
 

@@ -199,7 +199,7 @@ def check_flux_consistency(lib_path=None) -> str:
                 if not np.allclose(f, ref, rtol=1e-12, atol=1e-12):
                     msgs.append(f"hd {fs} dim {d}: {f} vs {ref}")
         pm = np.concatenate([[0.5 + rng.random()], rng.normal(size=3), [0.2 + rng.random()], rng.normal(size=3)])
-        for fs in ("tvdlf", "hll", "hlld"):
+        for fs in ("tvdlf", "hll", "hllc", "hlld"):
             for d in range(3):
                 f, _, _ = mhd.face_flux(FS[fs], d, _prims_const(pm, pm))
                 ref = mhd_flux(pm, 5 / 3, d)
@@ -603,6 +603,43 @@ def test_hlld_matches_the_published_formulation():
     assert check_hlld_against_paper() == ""
 
 
+def check_mhd_hllc_contact(lib_path=None) -> str:
+    """MHD HLLC (Li 2005, mhd_get_wCD) keeps an isolated stationary contact
+    exactly, with and without a normal field (HLL does not), and reduces to
+    the hydrodynamic HLLC flux at B = 0 for subsonic states."""
+    g = 5.0 / 3.0
+    rng = np.random.default_rng(13)
+    mhd = _sim(Par(physics="mhd", ndim=3, ndir=3, gamma=g, typedivbfix="none", limiter="minmod"), lib_path=lib_path)
+    hd = _sim(Par(physics="hd", ndim=3, ndir=3, gamma=g, limiter="minmod"), lib_path=lib_path)
+    msgs = []
+    for _ in range(10):
+        for d in range(3):
+            for bn in (0.0, 0.5 + rng.random()):
+                v = np.zeros(3)
+                v[[(d + 1) % 3, (d + 2) % 3]] = rng.normal(size=2)
+                B = rng.normal(size=3)
+                B[d] = bn
+                pl = np.concatenate([[0.5 + rng.random()], v, [0.2 + rng.random()], B])
+                pr = pl.copy()
+                pr[0] = 0.5 + 2.0 * rng.random()
+                f, _, _ = mhd.face_flux(FS["hllc"], d, _prims_const(pl, pr))
+                ref = mhd_flux(pl, g, d)
+                if np.abs(f - ref).max() > 1e-13 * np.abs(ref).max():
+                    msgs.append(f"contact dim {d} bn {bn:.2f}: {np.abs(f - ref).max():.2e}")
+            pl = np.concatenate([[0.5 + rng.random()], 0.3 * rng.normal(size=3), [0.5 + rng.random()]])
+            pr = np.concatenate([[0.5 + rng.random()], 0.3 * rng.normal(size=3), [0.5 + rng.random()]])
+            fc, _, _ = hd.face_flux(FS["hllc"], d, _prims_const(pl, pr))
+            fm, _, _ = mhd.face_flux(FS["hllc"], d, _prims_const(np.concatenate([pl, np.zeros(3)]),
+                                                                 np.concatenate([pr, np.zeros(3)])))
+            if np.abs(fm[:5] - fc).max() > 1e-13 * np.abs(fc).max():
+                msgs.append(f"B=0 dim {d}: {np.abs(fm[:5] - fc).max():.2e}")
+    return "; ".join(msgs[:3])
+
+
+def test_mhd_hllc_keeps_contacts():
+    assert check_mhd_hllc_contact() == ""
+
+
 def test_hlld_resolves_isolated_discontinuities():
     assert check_hlld_discontinuities() == ""
 
@@ -619,6 +656,7 @@ ALL_CHECKS = {
     "hllc_contact": check_hllc_stationary_contact,
     "hlld_discontinuities": check_hlld_discontinuities,
     "hlld_paper": check_hlld_against_paper,
+    "mhd_hllc_contact": check_mhd_hllc_contact,
     "flux_consistency": check_flux_consistency,
     "signal_speeds": check_signal_speeds,
     "glm_interface": check_glm_interface_state,

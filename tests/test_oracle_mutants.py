@@ -38,6 +38,12 @@ MUTANTS = {
                                       "hlld_paper"),
     "hlld_star_energy_work": ("+ w1R[p_] * sm + Bx * (vbR - vb1R)) / (sR - sm);",
                               "+ w1R[p_] * sm - Bx * (vbR - vb1R)) / (sR - sm);", 0, "hlld_paper"),
+    "mhd_hllc_pressure_bn_term": ("- wSub[mn] * vSub +\n                         wCD[bn] * wCD[bn];",
+                                  "- wSub[mn] * vSub -\n                         wCD[bn] * wCD[bn];", 0, "flux_consistency"),
+    "mhd_hllc_tangential_momentum_field": ("fSub[m] - wCD[bn] * wCD[s->mag[c]]) / den;",
+                                           "fSub[m] + wCD[bn] * wCD[s->mag[c]]) / den;", 0, "flux_consistency"),
+    "mhd_hllc_energy_vdotb": ("lambdaCD * pCD - VdotBCD * wCD[bn]) / den;", "lambdaCD * pCD) / den;", 0,
+                              "flux_consistency"),
     "mhd_energy_flux_sign": ("- bn * bv;", "+ bn * bv;", 0, "flux_consistency"),
     "mhd_induction_sign": ("vn * w[s->mag[c]] - bn * w[s->mom[c]]", "vn * w[s->mag[c]] + bn * w[s->mom[c]]", 0,
                            "flux_consistency"),
