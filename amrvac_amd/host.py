@@ -38,7 +38,8 @@ LIMITERS = {"minmod": 1, "woodward": 2, "mcbeta": 3, "superbee": 4,
             "vanleer": 5, "albada": 6, "koren": 7, "cada": 8, "cada3": 9,
             "weno5": 16}
 BOUNDSPEED = {"Einfeldt": 1, "cmaxmean": 2, "cmaxleftright": 3}
-DIVB = {"none": 0, "glm": 1, "linde": 4}
+DIVB = {"none": 0, "glm": 1, "linde": 4, "ct": 8}
+TYPE_CT = {"average": 1, "uct_contact": 2, "uct_hll": 3}
 
 
 @dataclass
@@ -55,6 +56,7 @@ class Par:
     mhd_glm_alpha: float = 0.5
     mhd_glm_extended: bool = True
     divbdiff: float = 0.8
+    type_ct: str = "uct_contact"   # mhd_list, constrained transport with typedivbfix='ct'
     time_stepper: str = "twostep"  # 'onestep' | 'twostep' | 'threestep' | 'fourstep' | 'fivestep'
     time_integrator: str = "default"
     ssprk_order: int = 3
@@ -87,6 +89,22 @@ class Par:
         if self.physics == "hd":
             return self.ndir + 2
         return 2 * self.ndir + 2 + (1 if self.typedivbfix == "glm" else 0)
+
+    @property
+    def stagger(self) -> bool:
+        """stagger_grid: constrained transport keeps B on cell faces."""
+        return self.physics == "mhd" and self.typedivbfix == "ct"
+
+    @property
+    def nws(self) -> int:
+        """Face-field components (nws = ndim, src/mhd/mod_mhd_phys.t:558)."""
+        return self.ndim if self.stagger else 0
+
+    @property
+    def nwt(self) -> int:
+        """Variables per block of a backend slot: nw cell variables, then
+        the nws face-field components."""
+        return self.nw + self.nws
 
     @property
     def need_global_cmax(self) -> bool:

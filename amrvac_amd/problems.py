@@ -223,6 +223,79 @@ def field_loop_output_vars(mesh: UniformMesh, w: np.ndarray, t: float) -> np.nda
     return np.stack([d2, d4, berr], axis=1)
 
 
+def field_loop_faces(mesh: UniformMesh) -> np.ndarray:
+    """The face field of the field loop from its vector potential
+    (b_from_vector_potentialA src/mod_constrained_transport.t:103-173 with
+    initvecpot_usr): A_z = A0 (R0 - r) inside r <= R0 at the cell corners
+    x + dx/2, y + dy/2, then B = curl A as circulation / face area.
+    Returns ws [nb, 2, ny, nx]: component d at the index of the cell whose
+    upper d face it sits on, set on the faces of the interior cells and
+    their lower neighbours (ghost faces are left zero for getbc)."""
+    a0, r0 = 1.0e-3, 0.3
+    x = mesh.cell_centers()
+    dx, dy = mesh.dx
+    xc = x[:, 0] + 0.5 * dx
+    yc = x[:, 1] + 0.5 * dy
+    rx = np.sqrt(xc ** 2 + yc ** 2)
+    az = np.where(rx <= r0, a0 * (r0 - rx), 0.0) * 1.0  # times dsC = 1 (ignorable z)
+    ng = mesh.nghost
+    ny, nx = az.shape[-2:]
+    ws = np.zeros((az.shape[0], 2, ny, nx))
+    J, I = slice(ng, ny - ng), slice(ng, nx - ng)
+    I1, J1 = slice(ng - 1, nx - ng), slice(ng - 1, ny - ng)
+    # x faces: lvc(1,2,3) = +1, area dy
+    ws[:, 0, J, I1] = (0.0 + (az[:, J, I1] - az[:, ng - 1:ny - ng - 1, I1])) / dy
+    # y faces: lvc(2,1,3) = -1, area dx
+    ws[:, 1, J1, I] = (0.0 + (-1.0) * (az[:, J1, I] - az[:, J1, ng - 1:nx - ng - 1])) / dx
+    return ws
+
+
+def field_loop_ct(mesh: UniformMesh, gamma: float = 5.0 / 3.0) -> np.ndarray:
+    """The field loop on a staggered mesh (stagger_grid branch of
+    initonegrid_usr): face field from the vector potential, cell field by
+    mhd_face_to_center (src/mhd/mod_mhd_phys.t:7586-7648).  Returns the
+    backend state [nb, 8 + 2, ny, nx]: conserved cell variables, then the
+    face field."""
+    ws = field_loop_faces(mesh)
+    dx, dy = mesh.dx
+    ng = mesh.nghost
+    ny, nx = ws.shape[-2:]
+    J, I = slice(ng, ny - ng), slice(ng, nx - ng)
+    b1 = np.zeros_like(ws[:, 0])
+    b2 = np.zeros_like(ws[:, 0])
+    b1[:, J, I] = 0.5 / dy * (ws[:, 0, J, I] * dy + ws[:, 0, J, ng - 1:nx - ng - 1] * dy)
+    b2[:, J, I] = 0.5 / dx * (ws[:, 1, J, I] * dx + ws[:, 1, ng - 1:ny - ng - 1, I] * dx)
+    one = np.ones_like(b1)
+    prim = np.stack([one, 2.0 * one, 1.0 * one, 0.0 * one, 1.0 * one, b1, b2, 0.0 * one], axis=1)
+    return np.concatenate([mhd_to_conserved(prim, gamma, 3), ws], axis=1)
+
+
+def field_loop_output_vars_ct(mesh: UniformMesh, w: np.ndarray) -> np.ndarray:
+    """set_output_vars of the field-loop module on a staggered mesh: div B
+    of the face field (get_divb src/physics/mod_functions_bfield.t:30-37,
+    both orders) and |B - B(t=0)| against the face-averaged initial field;
+    w is the backend state with the face field after the 8 cell variables.
+    Returns [nb, 3, ny, nx]."""
+    ng = mesh.nghost
+    dx, dy = mesh.dx
+    bx, by = w[:, 5], w[:, 6]
+    ws = w[:, 8:10]
+    ny, nx = bx.shape[-2:]
+    J, I = slice(ng, ny - ng), slice(ng, nx - ng)
+    div = np.zeros_like(bx[:, J, I])
+    div = div + ws[:, 0, J, I] * dy - ws[:, 0, J, ng - 1:nx - ng - 1] * dy
+    div = div + ws[:, 1, J, I] * dx - ws[:, 1, ng - 1:ny - ng - 1, I] * dx
+    div = div / (dx * dy)
+    d2 = np.zeros_like(bx)
+    d2[:, J, I] = div
+    w0 = field_loop_faces(mesh)
+    e1 = 0.5 * (w0[:, 0, J, I] + w0[:, 0, J, ng - 1:nx - ng - 1])
+    e2 = 0.5 * (w0[:, 1, J, I] + w0[:, 1, ng - 1:ny - ng - 1, I])
+    berr = np.zeros_like(bx)
+    berr[:, J, I] = np.sqrt((bx[:, J, I] - e1) ** 2 + (by[:, J, I] - e2) ** 2)
+    return np.stack([d2, d2.copy(), berr], axis=1)
+
+
 def blast_wave(mesh: UniformMesh, gamma: float = 5.0 / 3.0, seed: int = 20261202,
                rho0: float = 1.0, p_out: float = 1.0e-2, p_in: float = 10.0,
                radius: float = 0.05, noise: float = 0.01) -> np.ndarray:

@@ -47,6 +47,8 @@ enum { FT_DEFAULT = 0, FT_TVDLF = 1, FT_SPECIAL = 2 };
 struct orc_sim {
   orc_params p;
   int nw;
+  int nws, nwt;    /* face-field components (constrained transport) and nw + nws */
+  int ct;          /* type_ct, 0 without constrained transport */
   int rho_, mom[3], e_, mag[3], psi_;
   int ng, bnx[3], nx[3];
   long ncell;
@@ -87,7 +89,7 @@ static inline long IX(const orc_sim* s, int i, int j, int k) {
   return (long)i + (long)s->nx[0] * ((long)j + (long)s->nx[1] * (long)k);
 }
 static inline double* BLK(const orc_sim* s, int slot, int ib) {
-  return s->w[slot] + (long)ib * s->nw * s->ncell;
+  return s->w[slot] + (long)ib * s->nwt * s->ncell;
 }
 
 /* ------------------------------------------------------------------ */
@@ -512,7 +514,8 @@ static void riemann_hll(const orc_sim* s, int idim, const double* wLC,
                         double cmax, double cmin, double* fC) {
   for (int iw = 0; iw < s->nw; ++iw) {
     if (s->ftype[idim][iw] == FT_TVDLF) {
-      fC[iw] = -s->p.tvdlfeps * half * MAX2(cmax, fabs(cmin)) * (wRC[iw] - wLC[iw]);
+      /* constrained transport: no normal-field flux (:389-392) */
+      fC[iw] = s->ct ? zero : -s->p.tvdlfeps * half * MAX2(cmax, fabs(cmin)) * (wRC[iw] - wLC[iw]);
     } else if (cmin >= zero) {
       fC[iw] = fL[iw];
     } else if (cmax <= zero) {
@@ -531,7 +534,8 @@ static void riemann_hll(const orc_sim* s, int idim, const double* wLC,
    primitive states after modify_wLR, wLC/wRC the conserved ones. */
 static void riemann_hlld(const orc_sim* s, int idim, const double* wLC, const double* wRC,
                          const double* wLp, const double* wRp, const double* fL,
-                         const double* fR, double cmax, double cmin, double* fC) {
+                         const double* fR, double cmax, double cmin, const double* bx_ct,
+                         double* fC) {
   const double sR = cmax, sL = cmin;
   const int rho_ = s->rho_, e_ = s->e_, p_ = s->e_;
   const int ip1 = idim, ip2 = (idim + 1) % 3, ip3 = (idim + 2) % 3;
@@ -541,8 +545,10 @@ static void riemann_hlld(const orc_sim* s, int idim, const double* wLC, const do
   for (int iw = 0; iw < s->nw; ++iw) { w1R[iw] = w1L[iw] = w2R[iw] = w2L[iw] = 0.0; }
   const double* BR = wRC;  /* magnetic field of the reconstructions, by index */
   const double* BL = wLC;
-  /* HLL estimate of the normal field, Li 2005 eq. (33) */
-  const double Bx = (sR * BR[b1] - sL * BL[b1]) / (sR - sL);
+  /* HLL estimate of the normal field, Li 2005 eq. (33); with constrained
+     transport the face field of block (ps, whatever state the stage
+     reads, :536-537) */
+  const double Bx = bx_ct ? *bx_ct : (sR * BR[b1] - sL * BL[b1]) / (sR - sL);
   const double ptR = wRp[p_] + 0.5 * (BR[s->mag[0]] * BR[s->mag[0]] + BR[s->mag[1]] * BR[s->mag[1]] +
                                       BR[s->mag[2]] * BR[s->mag[2]]);
   const double ptL = wLp[p_] + 0.5 * (BL[s->mag[0]] * BL[s->mag[0]] + BL[s->mag[1]] * BL[s->mag[1]] +
@@ -918,22 +924,320 @@ static void add_sources(const orc_sim* s, double qdt, const double* wCT,
   else if (s->p.divb == ORC_DIVB_LINDE) add_source_linde(s, qdt, wCT, w);
 }
 
+/* ------------------------------------------------------------------ */
+/* constrained transport (stagger_grid, uniform Cartesian blocks)        */
+/* ------------------------------------------------------------------ */
+
+/* Levi-Civita symbol lvc of src/mod_global_parameters.t, 0-based */
+static int lvc0(int a, int b, int c) {
+  if (a == b || b == c || a == c) return 0;
+  return ((a + 1) % 3 == b) ? 1 : -1;
+}
+
+/* per-block face data of one stage for the edge electric fields */
+typedef struct {
+  double* fm;     /* [ndim][ndir][ncell] unscaled flux of mag(c) through the upper idim face */
+  double* vnorm;  /* [ndim][ncell] uct_contact: mean normal velocity of the face */
+  double* cbmin;  /* [ndim][ncell] uct_hll: max(-cmin, 0) */
+  double* cbmax;  /* [ndim][ncell] uct_hll: max(cmax, 0) */
+  double* vbar;   /* [ndim][2][ncell] uct_hll: vbarC */
+} ct_faces;
+
+static double surfaceC(const orc_sim* s, int idim) {
+  /* get_surface_area src/mod_geometry.t:196-211 (Cartesian) */
+  if (s->p.ndim == 2) return idim == 0 ? s->dx[1] : s->dx[0];
+  if (idim == 0) return s->dx[1] * s->dx[2];
+  if (idim == 1) return s->dx[0] * s->dx[2];
+  return s->dx[0] * s->dx[1];
+}
+
+static double dsC(const orc_sim* s, int idir) {
+  /* src/amr/mod_amr_solution_node.t:462-467: dx along the edge, 1 for an
+     ignorable direction */
+  return idir < s->p.ndim ? s->dx[idir] : one;
+}
+
+static inline long SH(const orc_sim* s, long c, int d, int by) {
+  return c + (d == 0 ? by : d == 1 ? (long)by * s->nx[0] : (long)by * s->nx[0] * s->nx[1]);
+}
+
+/* the box lo..hi of cell (or face, edge) indices: interior shifted by
+   lo_off/hi_off per dimension */
+static void ct_box(const orc_sim* s, const int* lo_off, const int* hi_off, int* lo, int* hi) {
+  for (int d = 0; d < 3; ++d) {
+    if (d < s->p.ndim) {
+      lo[d] = s->ng + lo_off[d];
+      hi[d] = s->ng + s->bnx[d] - 1 + hi_off[d];
+    } else {
+      lo[d] = 0;
+      hi[d] = 0;
+    }
+  }
+}
+
+/* reconstruct src/mod_constrained_transport.t:177-237 (TVD limiters): the
+   value of face quantity q on both sides of the edges of ixC along idir.
+   qL/qR are indexed like q. */
+static void ct_reconstruct(const orc_sim* s, const int* clo, const int* chi, int idir, const double* q,
+                           double* qL, double* qR) {
+  int glo[3], ghi[3];
+  for (int d = 0; d < 3; ++d) { glo[d] = clo[d]; ghi[d] = chi[d]; }
+  glo[idir] -= 1;
+  ghi[idir] += 1;  /* gxC = [ixCmin - 1, jxCmax] */
+  const long n = s->ncell;
+  double* dq = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int k = glo[2]; k <= ghi[2]; ++k)
+    for (int j = glo[1]; j <= ghi[1]; ++j)
+      for (int i = glo[0]; i <= ghi[0]; ++i) {
+        const long c = IX(s, i, j, k);
+        qR[c] = q[SH(s, c, idir, 1)];
+        qL[c] = q[c];
+        dq[c] = qR[c] - qL[c];
+      }
+  for (int k = clo[2]; k <= chi[2]; ++k)
+    for (int j = clo[1]; j <= chi[1]; ++j)
+      for (int i = clo[0]; i <= chi[0]; ++i) {
+        const long c = IX(s, i, j, k);
+        double ldq, rdq_dummy, ldq_dummy, rdq;
+        /* ldq(ix) from dq(ix-1), dq(ix); rdq(ix+1) from dq(ix), dq(ix+1) */
+        dwlimiter_cell(s, idir, dq[SH(s, c, idir, -1)], dq[c], &ldq, &rdq_dummy);
+        dwlimiter_cell(s, idir, dq[c], dq[SH(s, c, idir, 1)], &ldq_dummy, &rdq);
+        qL[c] = qL[c] + half * ldq;
+        qR[c] = qR[c] - half * rdq;
+      }
+  free(dq);
+}
+
+/* mhd_update_faces_average / _contact / _hll src/mhd/mod_mhd_phys.t:
+   6951-7479 (no resistivity, no ambipolar term, no user electric field):
+   edge electric fields times qdt and edge length, then the circulation
+   around every face updates the face field bfaces of the new state.
+   wp: primitive stage input; bfacesCT: face field of the stage input. */
+static void ct_update_faces(const orc_sim* s, double qdt, const double* wp, const ct_faces* F,
+                            const double* bfacesCT, double* bfaces) {
+  const int nd = s->p.ndim, ndir = s->p.ndir;
+  const long n = s->ncell;
+  const int sdim = nd == 3 ? 0 : 2;
+  double* fE = (double*)calloc((size_t)3 * n, sizeof(double));
+  const double* fm = F->fm;
+#define FM(c, iw_c, face_d) fm[((long)(face_d) * ndir + (iw_c)) * n + (c)]
+  if (s->ct == ORC_CT_AVERAGE || s->ct == ORC_CT_CONTACT) {
+    double* ECC = NULL;
+    double* EL = NULL;
+    double* ER = NULL;
+    if (s->ct == ORC_CT_CONTACT) {
+      /* ECC = b x v at cell centres over the whole block */
+      ECC = (double*)calloc((size_t)3 * n, sizeof(double));
+      EL = (double*)calloc((size_t)n, sizeof(double));
+      ER = (double*)calloc((size_t)n, sizeof(double));
+      const int b1 = s->mag[0], b2 = s->mag[1], m1 = s->mom[0], m2 = s->mom[1];
+      for (long c = 0; c < n; ++c) {
+        if (nd == 3) {
+          const int b3 = s->mag[2], m3 = s->mom[2];
+          ECC[0 * n + c] = wp[b2 * n + c] * wp[m3 * n + c] - wp[b3 * n + c] * wp[m2 * n + c];
+          ECC[1 * n + c] = wp[b3 * n + c] * wp[m1 * n + c] - wp[b1 * n + c] * wp[m3 * n + c];
+        }
+        ECC[2 * n + c] = wp[b1 * n + c] * wp[m2 * n + c] - wp[b2 * n + c] * wp[m1 * n + c];
+      }
+    }
+    for (int idim1 = 0; idim1 < nd; ++idim1)
+      for (int idim2 = 0; idim2 < nd; ++idim2)
+        for (int idir = sdim; idir < 3; ++idir) {
+          if (lvc0(idim1, idim2, idir) != 1) continue;
+          int lo_off[3], hi_off[3], lo[3], hi[3];
+          for (int d = 0; d < 3; ++d) { lo_off[d] = (d == idir) ? 0 : -1; hi_off[d] = 0; }
+          ct_box(s, lo_off, hi_off, lo, hi);
+          const int c1 = idim1, c2 = idim2;  /* mag(idim1), mag(idim2) as component index */
+          for (int k = lo[2]; k <= hi[2]; ++k)
+            for (int j = lo[1]; j <= hi[1]; ++j)
+              for (int i = lo[0]; i <= hi[0]; ++i) {
+                const long c = IX(s, i, j, k);
+                fE[idir * n + c] = quarter * (FM(c, c1, idim2) + FM(SH(s, c, idim1, 1), c1, idim2) -
+                                              FM(c, c2, idim1) - FM(SH(s, c, idim2, 1), c2, idim1));
+                if (s->ct == ORC_CT_AVERAGE) fE[idir * n + c] = fE[idir * n + c] * qdt * dsC(s, idir);
+              }
+          if (s->ct != ORC_CT_CONTACT) continue;
+          const double* E = ECC + (long)idir * n;
+          const double* vn1 = F->vnorm + (long)idim1 * n;
+          const double* vn2 = F->vnorm + (long)idim2 * n;
+          /* first pair: ixA = [ixCmin, ixCmax + e(idim1)] */
+          for (int k = lo[2]; k <= hi[2] + (idim1 == 2); ++k)
+            for (int j = lo[1]; j <= hi[1] + (idim1 == 1); ++j)
+              for (int i = lo[0]; i <= hi[0] + (idim1 == 0); ++i) {
+                const long c = IX(s, i, j, k);
+                EL[c] = FM(c, c1, idim2) - E[c];
+                ER[c] = FM(c, c1, idim2) - E[SH(s, c, idim2, 1)];
+              }
+          for (int k = lo[2]; k <= hi[2]; ++k)
+            for (int j = lo[1]; j <= hi[1]; ++j)
+              for (int i = lo[0]; i <= hi[0]; ++i) {
+                const long c = IX(s, i, j, k), c1p = SH(s, c, idim1, 1), c2p = SH(s, c, idim2, 1);
+                double ELC, ERC;
+                if (vn1[c] > 0.0) ELC = EL[c];
+                else if (vn1[c] < 0.0) ELC = EL[c1p];
+                else ELC = 0.5 * (EL[c] + EL[c1p]);
+                if (vn1[c2p] > 0.0) ERC = ER[c];
+                else if (vn1[c2p] < 0.0) ERC = ER[c1p];
+                else ERC = 0.5 * (ER[c] + ER[c1p]);
+                fE[idir * n + c] = fE[idir * n + c] + 0.25 * (ELC + ERC);
+              }
+          /* second pair: ixA = [ixCmin, ixCmax + e(idim2)] */
+          for (int k = lo[2]; k <= hi[2] + (idim2 == 2); ++k)
+            for (int j = lo[1]; j <= hi[1] + (idim2 == 1); ++j)
+              for (int i = lo[0]; i <= hi[0] + (idim2 == 0); ++i) {
+                const long c = IX(s, i, j, k);
+                EL[c] = -FM(c, c2, idim1) - E[c];
+                ER[c] = -FM(c, c2, idim1) - E[SH(s, c, idim1, 1)];
+              }
+          for (int k = lo[2]; k <= hi[2]; ++k)
+            for (int j = lo[1]; j <= hi[1]; ++j)
+              for (int i = lo[0]; i <= hi[0]; ++i) {
+                const long c = IX(s, i, j, k), c1p = SH(s, c, idim1, 1), c2p = SH(s, c, idim2, 1);
+                double ELC, ERC;
+                if (vn2[c] > 0.0) ELC = EL[c];
+                else if (vn2[c] < 0.0) ELC = EL[c2p];
+                else ELC = 0.5 * (EL[c] + EL[c2p]);
+                if (vn2[c1p] > 0.0) ERC = ER[c];
+                else if (vn2[c1p] < 0.0) ERC = ER[c2p];
+                else ERC = 0.5 * (ER[c] + ER[c2p]);
+                fE[idir * n + c] = fE[idir * n + c] + 0.25 * (ELC + ERC);
+                fE[idir * n + c] = fE[idir * n + c] * qdt * dsC(s, idir);
+              }
+        }
+    free(ECC);
+    free(EL);
+    free(ER);
+  } else { /* uct_hll */
+    double* vtilL[2];
+    double* vtilR[2];
+    double* btilL[3] = {NULL, NULL, NULL};
+    double* btilR[3] = {NULL, NULL, NULL};
+    for (int q = 0; q < 2; ++q) {
+      vtilL[q] = (double*)calloc((size_t)n, sizeof(double));
+      vtilR[q] = (double*)calloc((size_t)n, sizeof(double));
+    }
+    for (int idir = sdim; idir < 3; ++idir) {
+      int lo_off[3], hi_off[3], lo[3], hi[3];
+      for (int d = 0; d < 3; ++d) { lo_off[d] = (d == idir) ? 0 : -1; hi_off[d] = 0; }
+      ct_box(s, lo_off, hi_off, lo, hi);
+      const int idim1 = (idir + 1) % 3, idim2 = (idir + 2) % 3;
+      for (int q = 0; q < 3; ++q) {
+        if (q != idim1 && q != idim2) continue;
+        if (!btilL[q]) {
+          btilL[q] = (double*)calloc((size_t)n, sizeof(double));
+          btilR[q] = (double*)calloc((size_t)n, sizeof(double));
+        }
+      }
+      ct_reconstruct(s, lo, hi, idim2, F->vbar + ((long)idim1 * 2 + 0) * n, vtilL[1], vtilR[1]);
+      ct_reconstruct(s, lo, hi, idim1, F->vbar + ((long)idim2 * 2 + 1) * n, vtilL[0], vtilR[0]);
+      ct_reconstruct(s, lo, hi, idim2, bfacesCT + (long)idim1 * n, btilL[idim1], btilR[idim1]);
+      ct_reconstruct(s, lo, hi, idim1, bfacesCT + (long)idim2 * n, btilL[idim2], btilR[idim2]);
+      const double* cmn1 = F->cbmin + (long)idim1 * n;
+      const double* cmx1 = F->cbmax + (long)idim1 * n;
+      const double* cmn2 = F->cbmin + (long)idim2 * n;
+      const double* cmx2 = F->cbmax + (long)idim2 * n;
+      for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+          for (int i = lo[0]; i <= hi[0]; ++i) {
+            const long c = IX(s, i, j, k), cp = SH(s, c, idim2, 1), jc = SH(s, c, idim1, 1);
+            const double cm1 = MAX2(cmn1[cp], cmn1[c]);
+            const double cp1 = MAX2(cmx1[cp], cmx1[c]);
+            const double cm2 = MAX2(cmn2[jc], cmn2[c]);
+            const double cp2 = MAX2(cmx2[jc], cmx2[c]);
+            double e = -(cp1 * vtilL[0][c] * btilL[idim2][c] + cm1 * vtilR[0][c] * btilR[idim2][c] -
+                         cp1 * cm1 * (btilR[idim2][c] - btilL[idim2][c])) /
+                           (cp1 + cm1) +
+                       (cp2 * vtilL[1][c] * btilL[idim1][c] + cm2 * vtilR[1][c] * btilR[idim1][c] -
+                        cp2 * cm2 * (btilR[idim1][c] - btilL[idim1][c])) /
+                           (cp2 + cm2);
+            fE[idir * n + c] = qdt * dsC(s, idir) * e;
+          }
+    }
+    for (int q = 0; q < 2; ++q) { free(vtilL[q]); free(vtilR[q]); }
+    for (int q = 0; q < 3; ++q) { free(btilL[q]); free(btilR[q]); }
+  }
+#undef FM
+  /* circulation around each face, then B_face -= circ / surfaceC */
+  for (int idim1 = 0; idim1 < nd; ++idim1) {
+    int lo_off[3], hi_off[3], lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) { lo_off[d] = (d == idim1) ? -1 : 0; hi_off[d] = 0; }
+    ct_box(s, lo_off, hi_off, lo, hi);
+    const double sC = surfaceC(s, idim1);
+    for (int k = lo[2]; k <= hi[2]; ++k)
+      for (int j = lo[1]; j <= hi[1]; ++j)
+        for (int i = lo[0]; i <= hi[0]; ++i) {
+          const long c = IX(s, i, j, k);
+          double circ = zero;
+          for (int idim2 = 0; idim2 < nd; ++idim2)
+            for (int idir = sdim; idir < 3; ++idir) {
+              const int l = lvc0(idim1, idim2, idir);
+              if (l == 0) continue;
+              const double dE = fE[idir * n + c] - fE[idir * n + SH(s, c, idim2, -1)];
+              if (l == 1) circ = circ + dE;
+              else circ = circ - dE;
+            }
+          if (sC > smalldouble) bfaces[idim1 * n + c] = bfaces[idim1 * n + c] - circ / sC;
+        }
+  }
+  free(fE);
+}
+
+/* mhd_face_to_center src/mhd/mod_mhd_phys.t:7586-7648 (arithmetic mean of
+   the two faces, area weighted) on the interior cells */
+static void ct_face_to_center(const orc_sim* s, double* w) {
+  const long n = s->ncell;
+  const double* ws = w + (long)s->nw * n;
+  int lo_off[3] = {0, 0, 0}, hi_off[3] = {0, 0, 0}, lo[3], hi[3];
+  ct_box(s, lo_off, hi_off, lo, hi);
+  for (int d = 0; d < s->p.ndim; ++d) {
+    const double sC = surfaceC(s, d), sf = sC; /* surface = surfaceC on Cartesian blocks */
+    for (int k = lo[2]; k <= hi[2]; ++k)
+      for (int j = lo[1]; j <= hi[1]; ++j)
+        for (int i = lo[0]; i <= hi[0]; ++i) {
+          const long c = IX(s, i, j, k);
+          w[s->mag[d] * n + c] = half / sf * (ws[d * n + c] * sC + ws[d * n + SH(s, c, d, -1)] * sC);
+        }
+  }
+}
+
 /* finite_volume src/mod_finite_volume.t:116-339 (slab_uniform, no local
-   timestep, no stagger grid, fix_small_values off) */
+   timestep, fix_small_values off).  With constrained transport the fluxes
+   are also computed on transverse_ghost_cells extra lines (:183-190), the
+   normal field of the reconstructions is the face field of the stage input
+   (mhd_modify_wLR src/mhd/mod_mhd_phys.t:6338-6345), and the face field of
+   wnew is advanced by the edge electric fields before the cell field is
+   reset from it (:249, :323).  ws_block: the face field of ps (block, which
+   the HLLD solver reads, :536-537). */
 static void finite_volume(const orc_sim* s, int method, double qdt,
                           const double* wCT, double* wnew, double ch, int* crash,
-                          double* bf) {
+                          double* bf, const double* ws_block) {
   const int nw = s->nw;
   const long n = s->ncell;
   double* wprim = (double*)malloc(sizeof(double) * (size_t)(nw * n));
   block_to_primitive(s, wCT, wprim);
+  const int ct = s->ct;
+  const int tgc = ct == ORC_CT_HLL ? 2 : ct ? 1 : 0;
+  const double* wsCT = wCT + (long)nw * n;
+  ct_faces F = {NULL, NULL, NULL, NULL, NULL};
+  if (ct) {
+    F.fm = (double*)calloc((size_t)s->p.ndim * s->p.ndir * n, sizeof(double));
+    if (ct == ORC_CT_CONTACT) F.vnorm = (double*)calloc((size_t)s->p.ndim * n, sizeof(double));
+    if (ct == ORC_CT_HLL) {
+      F.cbmin = (double*)calloc((size_t)s->p.ndim * n, sizeof(double));
+      F.cbmax = (double*)calloc((size_t)s->p.ndim * n, sizeof(double));
+      F.vbar = (double*)calloc((size_t)s->p.ndim * 2 * n, sizeof(double));
+    }
+  }
   for (int idim = 0; idim < s->p.ndim; ++idim) {
-    int alo, ahi, blo, bhi;
+    int alo, ahi, blo, bhi, ta, tb;
     transverse_ranges(s, idim, &alo, &ahi, &blo, &bhi);
+    trans_dims(idim, &ta, &tb);
+    const int ea = ta < s->p.ndim ? tgc : 0, eb = tb < s->p.ndim ? tgc : 0;
     const int ilo = s->ng - 1, ihi = s->ng + s->bnx[idim] - 1; /* ixC */
     const double dxinv = -qdt / s->dx[idim];
-    for (int b = blo; b <= bhi; ++b)
-      for (int a = alo; a <= ahi; ++a) {
+    for (int b = blo - eb; b <= bhi + eb; ++b)
+      for (int a = alo - ea; a <= ahi + ea; ++a) {
+        const int inner = a >= alo && a <= ahi && b >= blo && b <= bhi;
         double lw[MAXW][MAXLINE];
         double wLp[MAXLINE][MAXW], wRp[MAXLINE][MAXW];
         double fC[MAXLINE][MAXW];
@@ -942,11 +1246,17 @@ static void finite_volume(const orc_sim* s, int method, double qdt,
         for (int i = ilo; i <= ihi; ++i) {
           double wLC[MAXW], wRC[MAXW], fL[MAXW], fR[MAXW];
           double cmax = 0.0, cmin = 0.0;
+          const long cf = line_cell(s, idim, a, b, i);
           for (int iw = 0; iw < nw; ++iw) { wLC[iw] = wLp[i][iw]; wRC[iw] = wRp[i][iw]; }
           to_conserved_cell(s, wLC);
           to_conserved_cell(s, wRC);
           if (s->p.phys == ORC_PHYS_MHD && s->p.divb == ORC_DIVB_GLM)
             modify_wLR_glm(s, wLC, wRC, wLp[i], wRp[i], idim, ch);
+          if (ct) {
+            const double bn = wsCT[(long)idim * n + cf];
+            const int bi = s->mag[idim];
+            wLC[bi] = bn; wRC[bi] = bn; wLp[i][bi] = bn; wRp[i][bi] = bn;
+          }
           get_flux(s, wLC, wLp[i], idim, ch, fL);
           get_flux(s, wRC, wRp[i], idim, ch, fR);
           if (method == ORC_FS_TVDLF) {
@@ -957,11 +1267,33 @@ static void finite_volume(const orc_sim* s, int method, double qdt,
             if (method == ORC_FS_HLLC)
               riemann_hllc(s, idim, wLC, wRC, fL, fR, cmax, cmin, fC[i]);
             else if (method == ORC_FS_HLLD)
-              riemann_hlld(s, idim, wLC, wRC, wLp[i], wRp[i], fL, fR, cmax, cmin, fC[i]);
+              riemann_hlld(s, idim, wLC, wRC, wLp[i], wRp[i], fL, fR, cmax, cmin,
+                           ct ? ws_block + (long)idim * n + cf : NULL, fC[i]);
             else
               riemann_hll(s, idim, wLC, wRC, fL, fR, cmax, cmin, fC[i]);
           }
+          if (ct) {
+            /* face data of the edge electric fields: unscaled fluxes of the
+               field components and the ct velocities
+               (mhd_get_ct_velocity_* src/mhd/mod_mhd_phys.t:3261-3326) */
+            for (int m = 0; m < s->p.ndir; ++m)
+              F.fm[((long)idim * s->p.ndir + m) * n + cf] = fC[i][s->mag[m]];
+            if (ct == ORC_CT_CONTACT)
+              F.vnorm[(long)idim * n + cf] = 0.5 * (wLp[i][s->mom[idim]] + wRp[i][s->mom[idim]]);
+            if (ct == ORC_CT_HLL) {
+              double cbmax = MAX2(cmax, zero), cbmin;
+              cbmin = (method == ORC_FS_TVDLF) ? cbmax : MAX2(-cmin, zero);
+              F.cbmin[(long)idim * n + cf] = cbmin;
+              F.cbmax[(long)idim * n + cf] = cbmax;
+              const int iN = (idim + 1) % s->p.ndir, iE = (idim + 2) % s->p.ndir;
+              const double l1 = wLp[i][s->mom[iN]], r1 = wRp[i][s->mom[iN]], l2 = wLp[i][s->mom[iE]];
+              F.vbar[((long)idim * 2 + 0) * n + cf] = (cbmax * l1 + cbmin * r1) / (cbmax + cbmin);
+              /* the second average takes vbarRC(:,idim,1), as the reference does */
+              F.vbar[((long)idim * 2 + 1) * n + cf] = (cbmax * l2 + cbmin * r1) / (cbmax + cbmin);
+            }
+          }
         }
+        if (!inner) continue;
         /* flux-divergence update :269-284 */
         for (int iw = 0; iw < nw; ++iw) {
           for (int i = ilo; i <= ihi; ++i) fC[i][iw] = dxinv * fC[i][iw];
@@ -976,6 +1308,11 @@ static void finite_volume(const orc_sim* s, int method, double qdt,
           }
         }
       }
+  }
+  if (ct) {
+    ct_update_faces(s, qdt, wprim, &F, wsCT, wnew + (long)nw * n);
+    ct_face_to_center(s, wnew);
+    free(F.fm); free(F.vnorm); free(F.cbmin); free(F.cbmax); free(F.vbar);
   }
   add_sources(s, qdt, wCT, wnew, ch);
   free(wprim);
@@ -1061,7 +1398,7 @@ static void getbc_block_srl(orc_sim* s, int slot, int ib) {
       else if (dirs[d] == 0) { lo[d] = ng; hi[d] = ng + s->bnx[d] - 1; sh[d] = 0; }
       else { lo[d] = ng + s->bnx[d]; hi[d] = s->nx[d] - 1; sh[d] = -s->bnx[d]; }
     }
-    for (int iw = 0; iw < s->nw; ++iw)
+    for (int iw = 0; iw < s->nwt; ++iw)
       for (int k = lo[2]; k <= hi[2]; ++k)
         for (int j = lo[1]; j <= hi[1]; ++j)
           for (int i = lo[0]; i <= hi[0]; ++i)
@@ -1556,6 +1893,17 @@ int orc_create(const orc_params* p, int nblocks, const double* dx,
   if (p->ndim < 1 || p->ndim > 3 || p->ndir < p->ndim || p->ndir > 3) { free(s); return 2; }
   s->nw = orc_nw(p);
   if (s->nw > MAXW) { free(s); return 3; }
+  s->nws = 0;
+  s->ct = 0;
+  if (p->phys == ORC_PHYS_MHD && p->divb == ORC_DIVB_CT) {
+    /* constrained transport: uniform periodic meshes, ndim 2 or 3 */
+    if (p->ndim < 2 || p->type_ct < ORC_CT_AVERAGE || p->type_ct > ORC_CT_HLL) { free(s); return 6; }
+    for (int i = 0; i < 2 * p->ndim; ++i)
+      if (bc[i] != ORC_BC_PERIODIC) { free(s); return 6; }
+    s->nws = p->ndim;
+    s->ct = p->type_ct;
+  }
+  s->nwt = s->nw + s->nws;
   s->rho_ = 0;
   for (int c = 0; c < p->ndir; ++c) s->mom[c] = 1 + c;
   s->e_ = p->ndir + 1;
@@ -1616,7 +1964,7 @@ int orc_create(const orc_params* p, int nblocks, const double* dx,
   for (int d = 1; d < p->ndim; ++d) s->nface *= p->block_nx[d] > p->block_nx[0] ? p->block_nx[d] : p->block_nx[0];
   for (int i = 0; i < 6; ++i) s->bc[i] = (i < 2 * p->ndim) ? bc[i] : ORC_BC_PERIODIC;
   for (int sl = 0; sl < 4; ++sl) {
-    s->w[sl] = (double*)calloc((size_t)nblocks * s->nw * s->ncell, sizeof(double));
+    s->w[sl] = (double*)calloc((size_t)nblocks * s->nwt * s->ncell, sizeof(double));
     if (!s->w[sl]) { orc_destroy(s); return 5; }
   }
   *out = s;
@@ -1640,13 +1988,13 @@ void orc_destroy(orc_sim* s) {
 
 int orc_set_w(orc_sim* s, int slot, const double* w) {
   if (slot < 0 || slot > 3) return 1;
-  memcpy(s->w[slot], w, sizeof(double) * (size_t)s->nb * s->nw * s->ncell);
+  memcpy(s->w[slot], w, sizeof(double) * (size_t)s->nb * s->nwt * s->ncell);
   return 0;
 }
 
 int orc_get_w(const orc_sim* s, int slot, double* w) {
   if (slot < 0 || slot > 3) return 1;
-  memcpy(w, s->w[slot], sizeof(double) * (size_t)s->nb * s->nw * s->ncell);
+  memcpy(w, s->w[slot], sizeof(double) * (size_t)s->nb * s->nwt * s->ncell);
   return 0;
 }
 
@@ -1738,7 +2086,8 @@ int orc_advect1(orc_sim* s, int method, double dtfactor, double dt,
     if (method == ORC_FS_HANCOCK)
       hancock(&sb, qdt, wCT, wnew, cmax_global);
     else
-      finite_volume(&sb, method, qdt, wCT, wnew, cmax_global, &c, bf);
+      finite_volume(&sb, method, qdt, wCT, wnew, cmax_global, &c, bf,
+                    s->ct ? BLK(s, 0, ib) + (long)s->nw * s->ncell : NULL);
     crash |= c;
   }
   if (crash) s->crash = 1;
@@ -1750,7 +2099,7 @@ int orc_advect1(orc_sim* s, int method, double dtfactor, double dt,
 void orc_set_fix_conserve(orc_sim* s, int on) { s->fix_conserve = on; }
 
 int orc_axpby(orc_sim* s, int dst, double a, int x, double b, int y) {
-  const long tot = (long)s->nb * s->nw * s->ncell;
+  const long tot = (long)s->nb * s->nwt * s->ncell;
   double* d = s->w[dst];
   const double* xv = s->w[x];
   const double* yv = s->w[y];
@@ -1766,7 +2115,7 @@ int orc_axpby(orc_sim* s, int dst, double a, int x, double b, int y) {
    10 subtract, 11 multiply, 12 divide, 13 negate; the program is checked by
    the caller (at most 8 deep). */
 int orc_combine(orc_sim* s, int dst, int nops, const int* ops, const double* k) {
-  const long tot = (long)s->nb * s->nw * s->ncell;
+  const long tot = (long)s->nb * s->nwt * s->ncell;
   double* d = s->w[dst];
 #pragma omp parallel for schedule(static) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
   for (long i = 0; i < tot; ++i) {
@@ -1792,7 +2141,7 @@ int orc_combine(orc_sim* s, int dst, int nops, const int* ops, const double* k) 
 }
 
 int orc_copy(orc_sim* s, int dst, int src) {
-  memcpy(s->w[dst], s->w[src], sizeof(double) * (size_t)s->nb * s->nw * s->ncell);
+  memcpy(s->w[dst], s->w[src], sizeof(double) * (size_t)s->nb * s->nwt * s->ncell);
   return 0;
 }
 
@@ -1843,6 +2192,7 @@ int orc_crash_flag(const orc_sim* s) { return s->crash; }
 int orc_create_amr(const orc_params* p, int nblocks, const double* dxb, const int* ntype,
                    const int* nbr, const int* nchild, const int* icpos, const double* xlohi,
                    const int* bc, orc_sim** out) {
+  if (p->phys == ORC_PHYS_MHD && p->divb == ORC_DIVB_CT) return 6; /* no staggered AMR */
   int nnb = 1, nch = 1;
   for (int d = 0; d < p->ndim; ++d) { nnb *= 3; nch *= 2; }
   int* srl = (int*)malloc(sizeof(int) * (size_t)nblocks * nnb);
@@ -2052,7 +2402,7 @@ int orc_face_flux(orc_sim* s, int method, int idim, int nl, const double* lw_in,
   } else {
     get_cbounds(s, wLC, wRC, wLp[i], wRp[i], idim, 1, &cmax, &cmin, &crash);
     if (method == ORC_FS_HLLC) riemann_hllc(s, idim, wLC, wRC, fL, fR, cmax, cmin, fC);
-    else if (method == ORC_FS_HLLD) riemann_hlld(s, idim, wLC, wRC, wLp[i], wRp[i], fL, fR, cmax, cmin, fC);
+    else if (method == ORC_FS_HLLD) riemann_hlld(s, idim, wLC, wRC, wLp[i], wRp[i], fL, fR, cmax, cmin, NULL, fC);
     else riemann_hll(s, idim, wLC, wRC, fL, fR, cmax, cmin, fC);
   }
   if (cmax_out) *cmax_out = cmax;

@@ -28,6 +28,11 @@ extern "C" {
 #define ORC_DIVB_NONE 0
 #define ORC_DIVB_GLM 1
 #define ORC_DIVB_LINDE 4
+#define ORC_DIVB_CT 8
+/* type_ct of constrained transport (src/mhd/mod_mhd_phys.t:766-781) */
+#define ORC_CT_AVERAGE 1
+#define ORC_CT_CONTACT 2
+#define ORC_CT_HLL 3
 /* flux schemes (src/mod_global_parameters.t:457-469) */
 #define ORC_FS_HLL 1
 #define ORC_FS_HLLC 2
@@ -67,6 +72,7 @@ typedef struct orc_params {
   int check_small_values;  /* default 1 */
   int nghost;              /* 2 (3 for weno5) */
   int block_nx[3];         /* interior cells per block */
+  int type_ct;             /* ORC_CT_* when divb = ORC_DIVB_CT */
 } orc_params;
 
 typedef struct orc_sim orc_sim;
@@ -88,7 +94,12 @@ int orc_create(const orc_params* p, int nblocks, const double* dx,
                orc_sim** out);
 void orc_destroy(orc_sim* s);
 
-/* whole-slot copies in the reference layout (3 slots: ps, ps1, ps2) */
+/* whole-slot copies in the reference layout (4 slots: ps .. ps3).  With
+   constrained transport (divb = ORC_DIVB_CT; uniform periodic meshes) each
+   block holds nw + ndim variables: the cell variables, then the face field
+   ws(ix, idim) of the reference (src/mod_physicaldata.t), stored at the
+   index of the cell whose upper idim face it sits on; the ghost fill and
+   every slot combination carry it like a cell variable. */
 int orc_set_w(orc_sim* s, int slot, const double* w);
 int orc_get_w(const orc_sim* s, int slot, double* w);
 

@@ -17,7 +17,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liborc.so")
 
 PHYS = {"hd": 1, "mhd": 2}
-DIVB = {"none": 0, "glm": 1, "linde": 4}
+DIVB = {"none": 0, "glm": 1, "linde": 4, "ct": 8}
+TYPE_CT = {"average": 1, "uct_contact": 2, "uct_hll": 3}
 
 
 class OrcParams(ctypes.Structure):
@@ -30,6 +31,7 @@ class OrcParams(ctypes.Structure):
         ("cada3_radius", ctypes.c_double), ("small_pressure", ctypes.c_double),
         ("small_density", ctypes.c_double), ("check_small_values", ctypes.c_int),
         ("nghost", ctypes.c_int), ("block_nx", ctypes.c_int * 3),
+        ("type_ct", ctypes.c_int),
     ]
 
 
@@ -117,6 +119,7 @@ def make_params(par, block_nx) -> OrcParams:
     p.nghost = par.nghost
     for d in range(3):
         p.block_nx[d] = block_nx[d] if d < par.ndim else 1
+    p.type_ct = TYPE_CT[par.type_ct] if par.physics == "mhd" and par.typedivbfix == "ct" else 0
     return p
 
 
@@ -149,7 +152,7 @@ class OracleFV:
             raise RuntimeError(f"orc_create failed: {rc}")
         self.h = h
         self.nw = par.nw
-        self.shape = (mesh.nblocks, self.nw) + mesh.nxg[::-1]
+        self.shape = (mesh.nblocks, par.nwt) + mesh.nxg[::-1]
 
     def __del__(self):
         if getattr(self, "h", None):

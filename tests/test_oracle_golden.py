@@ -50,8 +50,11 @@ class FieldLoopIntegrator(TimeIntegrator):
 
     def log_line(self):
         w = self.be.get_state(0)
-        ex = problems.field_loop_output_vars(self.mesh, w, self.state.global_time)
-        inner = self.mesh.interior(np.concatenate([w, ex], axis=1))
+        if self.par.stagger:
+            ex = problems.field_loop_output_vars_ct(self.mesh, w)
+        else:
+            ex = problems.field_loop_output_vars(self.mesh, w, self.state.global_time)
+        inner = self.mesh.interior(np.concatenate([w[:, :self.par.nw], ex], axis=1))
         m1 = inner.mean(axis=(0, 2, 3))
         m2 = (inner ** 2).mean(axis=(0, 2, 3))
         row = np.concatenate([[self.state.global_time], m1, m2])
@@ -65,6 +68,11 @@ def field_loop_setup(scheme: str = "3step_hll_cada"):
         kw = dict(typedivbfix="linde", flux_scheme="hll", typeboundspeed="cmaxmean")
     elif scheme == "3step_hlld_cada":  # the scheme file selects GLM cleaning
         kw = dict(typedivbfix="glm", flux_scheme="hlld", typeboundspeed="Einfeldt")
+    elif scheme.startswith("3step_hll_cada_ct") or scheme == "3step_hlld_cada_ct":
+        # constrained transport: type_ct 'uct_contact' unless the scheme names one
+        ct = {"3step_hll_cada_ct_average": "average", "3step_hll_cada_ct_hll": "uct_hll"}.get(scheme, "uct_contact")
+        kw = dict(typedivbfix="ct", type_ct=ct, flux_scheme="hlld" if "hlld" in scheme else "hll",
+                  typeboundspeed="cmaxmean" if "hlld" not in scheme else "Einfeldt")
     else:
         raise ValueError(scheme)
     par = Par(physics="mhd", ndim=2, ndir=3, time_stepper="threestep", limiter="cada3", courantpar=0.9,
@@ -76,7 +84,7 @@ def field_loop_setup(scheme: str = "3step_hll_cada"):
 def run_field_loop(backend_cls=OracleFV, scheme: str = "3step_hll_cada"):
     par, mesh = field_loop_setup(scheme)
     be = backend_cls(par, mesh)
-    w = problems.field_loop_2p5d(mesh)
+    w = problems.field_loop_ct(mesh) if par.stagger else problems.field_loop_2p5d(mesh)
     if par.nw == 9:  # psi of GLM starts at zero
         w = np.concatenate([w, np.zeros_like(w[:, :1])], axis=1)
     be.set_state(0, w)
@@ -97,6 +105,18 @@ def test_field_loop_3step_hlld_cada_glm():
     to 1e-9 or better except the field-error diagnostic (1e-6)."""
     log = run_field_loop(scheme="3step_hlld_cada")
     gold = read_log(os.path.join(GOLD, "fl_2d_3step_hlld_cada.log"))
+    assert log.shape == gold.shape
+    assert logs_match(log, gold)
+
+
+@pytest.mark.parametrize("scheme", ["3step_hll_cada_ct", "3step_hll_cada_ct_average", "3step_hll_cada_ct_hll",
+                                    "3step_hlld_cada_ct"])
+def test_field_loop_constrained_transport(scheme):
+    """Staggered face field with the uct_contact, average and uct_hll edge
+    fields (HLL) and uct_contact with HLLD: every column of the reference's
+    log to its 9 printed digits."""
+    log = run_field_loop(scheme=scheme)
+    gold = read_log(os.path.join(GOLD, f"fl_2d_{scheme}.log"))
     assert log.shape == gold.shape
     assert logs_match(log, gold)
 

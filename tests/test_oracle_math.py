@@ -644,7 +644,63 @@ def test_hlld_resolves_isolated_discontinuities():
     assert check_hlld_discontinuities() == ""
 
 
+# --------------------------------------------------------------------------
+# constrained transport (staggered face field)
+# --------------------------------------------------------------------------
+
+CT_SCHEMES = ("3step_hll_cada_ct", "3step_hll_cada_ct_average", "3step_hll_cada_ct_hll", "3step_hlld_cada_ct")
+
+
+def _ct_field_loop(scheme: str, lib_path=None, rows: int = 50):
+    from test_oracle_golden import FieldLoopIntegrator, field_loop_setup
+    par, mesh = field_loop_setup(scheme)
+    par.time_max = 0.01 * rows
+    be = OracleFV(par, mesh, lib_path=lib_path)
+    be.set_state(0, problems.field_loop_ct(mesh))
+    be.getbc(0)
+    st = FieldLoopIntegrator(par, mesh, be).run()
+    return par, mesh, be, np.array(st.log)
+
+
+def check_ct_golden(scheme: str, lib_path=None) -> str:
+    """The staggered field loop (tests/mhd/field_loop_2.5D with the scheme
+    file) reproduces the reference's regression log, all 50 rows."""
+    import os
+    from amrvac_amd.host import logs_match, read_log
+    _, _, _, log = _ct_field_loop(scheme, lib_path)
+    gold = read_log(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"fl_2d_{scheme}.log"))
+    if log.shape != gold.shape or not logs_match(log, gold):
+        return f"{scheme}: log differs from the reference's"
+    return ""
+
+
+def check_ct_divb(lib_path=None) -> str:
+    """Constrained transport keeps the face-field divergence at round-off
+    for every electric-field average (the defining property: the face
+    update is the circulation of one edge field, so the discrete
+    divergence of every cell telescopes to zero)."""
+    msgs = []
+    for scheme in CT_SCHEMES[:3]:
+        par, mesh, be, _ = _ct_field_loop(scheme, lib_path, rows=10)
+        w = be.get_state(0)
+        ex = problems.field_loop_output_vars_ct(mesh, w)
+        divb = np.abs(mesh.interior(ex)[:, 0]).max()
+        bscale = np.abs(mesh.interior(w)[:, 5:7]).max() / min(mesh.dx)
+        if not divb <= 1e-12 * bscale:
+            msgs.append(f"{scheme}: max|div B| {divb:.3e} vs |B|/dx {bscale:.3e}")
+    return "; ".join(msgs)
+
+
+def test_ct_keeps_div_b_zero():
+    assert check_ct_divb() == ""
+
+
 ALL_CHECKS = {
+    "ct_divb": check_ct_divb,
+    "ct_golden_contact": lambda p: check_ct_golden("3step_hll_cada_ct", p),
+    "ct_golden_average": lambda p: check_ct_golden("3step_hll_cada_ct_average", p),
+    "ct_golden_hll": lambda p: check_ct_golden("3step_hll_cada_ct_hll", p),
+    "ct_golden_hlld": lambda p: check_ct_golden("3step_hlld_cada_ct", p),
     "sod_hllc_vanleer": lambda p: check_sod("3step_hllc_vanleer", p),
     "sod_hll_cada3": lambda p: check_sod("3step_hll_cada3", p),
     "sod_tvdlf_minmod": lambda p: check_sod("2step_tvdlf_minmod", p),

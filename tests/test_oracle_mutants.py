@@ -19,6 +19,17 @@ ORACLE_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file
 # name: (text in amrvac_oracle.c, its replacement, occurrence to replace
 # (0-based), check that must fail)
 MUTANTS = {
+    # constrained transport
+    "ct_vbar_second_average": ("(cbmax * l2 + cbmin * r1) / (cbmax + cbmin);",
+                               "(cbmax * l2 + cbmin * wRp[i][s->mom[iE]]) / (cbmax + cbmin);", 0, "ct_golden_hll"),
+    "ct_average_factor": ("fE[idir * n + c] = quarter * (FM(c, c1, idim2)", "fE[idir * n + c] = half * (FM(c, c1, idim2)", 0,
+                          "ct_golden_average"),
+    "ct_contact_upwind": ("if (vn1[c] > 0.0) ELC = EL[c];", "if (vn1[c] > 0.0) ELC = EL[c1p];", 0, "ct_golden_contact"),
+    "ct_circulation_sign": ("if (l == 1) circ = circ + dE;", "if (l == 1) circ = circ - dE;", 0, "ct_divb"),
+    "ct_face_to_center": ("(ws[d * n + c] * sC + ws[d * n + SH(s, c, d, -1)] * sC)", "(ws[d * n + c] * sC + ws[d * n + c] * sC)",
+                          0, "ct_golden_contact"),
+    "ct_normal_field_of_prims": ("wLC[bi] = bn; wRC[bi] = bn; wLp[i][bi] = bn; wRp[i][bi] = bn;",
+                                 "wLC[bi] = bn; wRC[bi] = bn;", 0, "ct_golden_contact"),
     "hll_dissipation_sign": ("+ cmin * cmax * (wRC[iw] - wLC[iw])) /\n               (cmax - cmin);\n    }\n  }\n}",
                              "- cmin * cmax * (wRC[iw] - wLC[iw])) /\n               (cmax - cmin);\n    }\n  }\n}",
                              0, "sod_hll_cada3"),
