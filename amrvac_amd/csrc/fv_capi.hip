@@ -46,6 +46,12 @@ struct amrvac_fv_ctx {
   double dx[3] = {1, 1, 1};
   int bc[6] = {5, 5, 5, 5, 5, 5};
   double* slot[4] = {nullptr, nullptr, nullptr, nullptr};  // ps, ps1, ps2, ps3
+  // constrained transport: face field of each slot [nb][ndim][cells] (the
+  // upper faces of each cell), the per-stage face buffer, the new faces
+  int nws = 0;
+  double* wsl[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* d_ctf = nullptr;
+  double* d_wsn = nullptr;
   int* d_nbr = nullptr;
   int* d_crash = nullptr;
   double* d_red = nullptr;    // per-block reduction results
@@ -698,7 +704,8 @@ hipError_t dispatch_sweep(const amrvac_fv_ctx* c, const fvc::KernelKey& k, const
 template <template <class> class F, class... Args>
 hipError_t with_phys(const amrvac_fv_ctx* c, Args... args) {
   using namespace fv;
-  const int nd = c->cfg.ndim, ph = c->cfg.physics, nr = c->cfg.ndir, db = c->cfg.divb;
+  // constrained transport uses the ideal-MHD physics without a source term
+  const int nd = c->cfg.ndim, ph = c->cfg.physics, nr = c->cfg.ndir, db = c->cfg.divb == 8 ? 0 : c->cfg.divb;
   if (nd == 2 && ph == 1 && nr == 2) return F<Phys<2, 2, false, DIVB_NONE>>::run(args...);
   if (nd == 2 && ph == 2 && nr == 3 && db == DIVB_GLM) return F<Phys<2, 3, true, DIVB_GLM>>::run(args...);
   if (nd == 2 && ph == 2 && nr == 3 && db == DIVB_LINDE) return F<Phys<2, 3, true, DIVB_LINDE>>::run(args...);
@@ -733,8 +740,15 @@ int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
   if (c.physics != 1 && c.physics != 2) return fail(AMRVAC_FV_EINVAL, "physics must be 1 (hd) or 2 (mhd)");
   if (c.physics == 1 && c.ndir != c.ndim) return fail(AMRVAC_FV_EINVAL, "hd: ndir must equal ndim");
   if (c.physics == 2 && c.ndir != 3) return fail(AMRVAC_FV_EINVAL, "mhd: ndir must be 3");
-  if (c.physics == 2 && c.divb != 0 && c.divb != 1 && c.divb != 4)
-    return fail(AMRVAC_FV_EINVAL, "typedivbfix must be none (0), glm (1) or linde (4)");
+  if (c.physics == 2 && c.divb != 0 && c.divb != 1 && c.divb != 4 && c.divb != 8)
+    return fail(AMRVAC_FV_EINVAL, "typedivbfix must be none (0), glm (1), linde (4) or ct (8)");
+  if (c.physics == 2 && c.divb == 8) {
+    if (c.ndim != 2) return fail(AMRVAC_FV_EINVAL, "constrained transport is supported for ndim = 2");
+    if (c.type_ct < 1 || c.type_ct > 3) return fail(AMRVAC_FV_EINVAL, "type_ct must be 1, 2 or 3");
+    if (c.limiter == 16) return fail(AMRVAC_FV_EINVAL, "constrained transport takes a TVD limiter");
+    if (c.precision == AMRVAC_FV_PREC_MIXED)
+      return fail(AMRVAC_FV_EINVAL, "constrained transport runs in FP64 (precision 0 or 2)");
+  }
   if (c.physics == 2 && c.divb == 4 && c.ndim != 2)
     return fail(AMRVAC_FV_EINVAL, "Linde div B cleaning is supported for ndim = 2 only");
   if (c.block_nx[0] != 16 || c.block_nx[1] != 16)
@@ -751,6 +765,7 @@ int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
   amrvac_fv_ctx* x = new amrvac_fv_ctx();
   x->cfg = c;
   x->nw = c.physics == 1 ? c.ndir + 2 : 2 * c.ndir + 2 + (c.divb == 1 ? 1 : 0);
+  x->nws = (c.physics == 2 && c.divb == 8) ? c.ndim : 0;
   x->nnb = c.ndim == 2 ? 9 : 27;
   x->bnz = c.ndim == 3 ? c.block_nx[2] : 1;
   x->cells = (size_t)16 * 16 * x->bnz;
@@ -768,6 +783,11 @@ int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
 
 static void free_level(amrvac_fv_ctx* c) {
   for (int s = 0; s < 4; ++s) { if (c->slot[s]) (void)hipFree(c->slot[s]); c->slot[s] = nullptr; }
+  for (int s = 0; s < 4; ++s) { if (c->wsl[s]) (void)hipFree(c->wsl[s]); c->wsl[s] = nullptr; }
+  if (c->d_ctf) (void)hipFree(c->d_ctf);
+  if (c->d_wsn) (void)hipFree(c->d_wsn);
+  c->d_ctf = nullptr;
+  c->d_wsn = nullptr;
   if (c->d_nbr) (void)hipFree(c->d_nbr);
   if (c->d_crash) (void)hipFree(c->d_crash);
   if (c->d_red) (void)hipFree(c->d_red);
@@ -833,6 +853,9 @@ int amrvac_fv_set_mesh(amrvac_fv_ctx* c, int nblocks, const double* dx, const in
       return fail(AMRVAC_FV_EINVAL, "boundary type must be cont, symm, asymm or periodic");
   for (int d = 0; d < nd; ++d)
     if (!(dx[d] > 0.0)) return fail(AMRVAC_FV_EINVAL, "dx must be positive");
+  if (c->nws > 0)
+    for (int i = 0; i < nblocks * c->nnb; ++i)
+      if (neighbor[i] < 0) return fail(AMRVAC_FV_EINVAL, "constrained transport needs a periodic level");
   HIPCHK(hipStreamSynchronize(c->stream));
   free_level(c);
   return alloc_level(c, nblocks, nblocks, dx, neighbor, bc);
@@ -854,6 +877,22 @@ static int alloc_level(amrvac_fv_ctx* c, int nblocks, int capacity, const double
       return fail(AMRVAC_FV_ENOMEM, "state slot allocation failed");
     }
     HIPCHK(hipMemsetAsync(c->slot[s], 0, n * sizeof(double), c->stream));
+  }
+  if (c->nws > 0) {
+    const size_t nf = (size_t)capacity * c->nws * c->cells;
+    for (int s = 0; s < 4; ++s) {
+      if (hipMalloc(&c->wsl[s], nf * sizeof(double)) != hipSuccess) {
+        free_level(c);
+        return fail(AMRVAC_FV_ENOMEM, "face field allocation failed");
+      }
+      HIPCHK(hipMemsetAsync(c->wsl[s], 0, nf * sizeof(double), c->stream));
+    }
+    const size_t nfb = fv::ct_face_buffer_doubles(nblocks);
+    if (hipMalloc(&c->d_ctf, nfb * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->d_wsn, nf * sizeof(double)) != hipSuccess) {
+      free_level(c);
+      return fail(AMRVAC_FV_ENOMEM, "constrained-transport buffers allocation failed");
+    }
   }
   if (hipMalloc(&c->d_nbr, sizeof(int) * nblocks * c->nnb) != hipSuccess ||
       hipMalloc(&c->d_crash, sizeof(int)) != hipSuccess ||
@@ -900,12 +939,16 @@ int amrvac_fv_upload(amrvac_fv_ctx* c, int slot, const double* w) {
   if (!c || !w || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
   const int ng = c->cfg.nghost, nd = c->cfg.ndim;
-  const size_t n = (size_t)c->nb * c->nw * c->cells;
+  const int nwt = c->nw + c->nws;  // with constrained transport the face field follows the cell variables
+  const size_t n = (size_t)c->nb * nwt * c->cells;
   c->h_stage.resize(n);
   for (int b = 0; b < c->nb; ++b)
-    for (int iw = 0; iw < c->nw; ++iw) {
-      const double* src = w + ((size_t)b * c->nw + iw) * c->cells_g;
-      double* dst = c->h_stage.data() + ((size_t)b * c->nw + iw) * c->cells;
+    for (int iw = 0; iw < nwt; ++iw) {
+      const double* src = w + ((size_t)b * nwt + iw) * c->cells_g;
+      // cell variables first (all blocks), then the face components
+      double* dst = iw < c->nw ? c->h_stage.data() + ((size_t)b * c->nw + iw) * c->cells
+                               : c->h_stage.data() + (size_t)c->nb * c->nw * c->cells +
+                                     ((size_t)b * c->nws + (iw - c->nw)) * c->cells;
       for (int k = 0; k < c->bnz; ++k)
         for (int j = 0; j < 16; ++j) {
           const int kg = nd > 2 ? k + ng : 0;
@@ -913,7 +956,11 @@ int amrvac_fv_upload(amrvac_fv_ctx* c, int slot, const double* w) {
           std::memcpy(dst + ((size_t)k * 16 + j) * 16, row, 16 * sizeof(double));
         }
     }
-  HIPCHK(hipMemcpyAsync(c->slot[slot], c->h_stage.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const size_t nc = (size_t)c->nb * c->nw * c->cells;
+  HIPCHK(hipMemcpyAsync(c->slot[slot], c->h_stage.data(), nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (c->nws > 0)
+    HIPCHK(hipMemcpyAsync(c->wsl[slot], c->h_stage.data() + nc, (n - nc) * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return AMRVAC_FV_OK;
 }
@@ -1100,6 +1147,47 @@ static int stage_core(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
   return AMRVAC_FV_OK;
 }
 
+// One constrained-transport stage between slots (advect1 with stagger_grid,
+// uniform periodic 2-D level): face fluxes, edge fields, face and cell
+// updates, then the new face field into psb's slot.  ps's face field feeds
+// the HLLD star states; when psb is ps it holds the stage's initial
+// combination.
+static int ct_stage(amrvac_fv_ctx* c, int method, double dtfactor, double dt, int psa, int psb, int comb_slot,
+                    int mode, double comb_a, double comb_b, int* crash_out) {
+  int riem;
+  if (method_to_riem(method, riem) != 0) return fail(AMRVAC_FV_EINVAL, "unsupported flux scheme");
+  if (riem == fv::RIEM_HANCOCK) return fail(AMRVAC_FV_EINVAL, "constrained transport: no Hancock predictor");
+  fvc::SweepArgs a = base_args(c);
+  a.src = c->slot[psa];
+  a.dst = c->slot[psb];
+  a.base = comb_slot >= 0 ? c->slot[comb_slot] : nullptr;
+  a.comb_mode = mode;
+  a.comb_a = comb_a;
+  a.comb_b = comb_b;
+  const double qdt = dtfactor * dt;
+  a.qdt = qdt;
+  for (int d = 0; d < 3; ++d) a.dq[d] = -qdt / c->dx[d];
+  if (crash_out) HIPCHK(hipMemsetAsync(c->d_crash, 0, sizeof(int), c->stream));
+  const bool exact = c->cfg.precision == 2;
+  const double* wbase = comb_slot >= 0 ? c->wsl[comb_slot] : nullptr;
+  const int bx_mode = psb == 0 ? mode : -1;
+  hipError_t e = exact ? fvx::launch_ct_stage(riem, c->cfg.limiter, c->cfg.type_ct, a, c->wsl[psa], wbase,
+                                             c->wsl[psb], c->wsl[0], bx_mode, c->d_ctf, c->d_wsn, c->fp, c->stream)
+                       : fv::launch_ct_stage(riem, c->cfg.limiter, c->cfg.type_ct, a, c->wsl[psa], wbase,
+                                             c->wsl[psb], c->wsl[0], bx_mode, c->d_ctf, c->d_wsn, c->fp, c->stream);
+  if (e == hipErrorInvalidValue) return fail(AMRVAC_FV_EINVAL, "no constrained-transport kernel for this scheme");
+  if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("ct stage: ") + hipGetErrorString(e));
+  HIPCHK(hipMemcpyAsync(c->wsl[psb], c->d_wsn, (size_t)c->nb * c->nws * c->cells * sizeof(double),
+                        hipMemcpyDeviceToDevice, c->stream));
+  if (crash_out) {
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *crash_out = h;
+  }
+  return AMRVAC_FV_OK;
+}
+
 extern "C" {
 
 int amrvac_fv_download(amrvac_fv_ctx* c, int slot, double* w) {
@@ -1118,8 +1206,29 @@ int amrvac_fv_download(amrvac_fv_ctx* c, int slot, double* w) {
                        c->cfg.nghost, c->nxg[0], c->nxg[1], c->nxg[2], c->d_stage);
   HIPCHK(hipGetLastError());
   const size_t n = (size_t)c->nb * c->nw * c->cells_g;
-  HIPCHK(hipMemcpyAsync(w, c->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (c->nws == 0) {
+    HIPCHK(hipMemcpyAsync(w, c->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMRVAC_FV_OK;
+  }
+  // constrained transport: cell variables, then the face components of
+  // every block (ghost faces gathered from the neighbours like ghost cells)
+  const int nwt = c->nw + c->nws;
+  std::vector<double> hc(n), hf((size_t)c->nb * c->nws * c->cells_g);
+  HIPCHK(hipMemcpyAsync(hc.data(), c->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  a.src = c->wsl[slot];
+  hipLaunchKernelGGL((fv::fill_ghosted_kernel<2>), grid, dim3(fvc::NTHREADS), 0, c->stream, a, c->nws,
+                     c->cfg.nghost, c->nxg[0], c->nxg[1], c->nxg[2], c->d_stage);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(hf.data(), c->d_stage, hf.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < c->nb; ++b)
+    for (int iw = 0; iw < nwt; ++iw) {
+      const double* from = iw < c->nw ? hc.data() + ((size_t)b * c->nw + iw) * c->cells_g
+                                      : hf.data() + ((size_t)b * c->nws + (iw - c->nw)) * c->cells_g;
+      std::memcpy(w + ((size_t)b * nwt + iw) * c->cells_g, from, c->cells_g * sizeof(double));
+    }
   return AMRVAC_FV_OK;
 }
 
@@ -1143,6 +1252,7 @@ int amrvac_fv_advect1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
   if (comb_slot >= 0 && !check_slot(comb_slot)) return fail(AMRVAC_FV_EINVAL, "bad combination slot");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
   const int mode = comb_slot < 0 ? 0 : (comb_b == 0.0 ? (comb_slot == psa ? 3 : 1) : 2);
+  if (c->nws > 0) return ct_stage(c, method, dtfactor, dt, psa, psb, comb_slot, mode, comb_a, comb_b, crash_out);
   return stage_core(c, method, dtfactor, dt, cmax_global, c->slot[psa], c->slot[psb],
                     comb_slot >= 0 ? c->slot[comb_slot] : nullptr, mode, comb_a, comb_b, psb, crash_out);
 }
@@ -1152,6 +1262,7 @@ int amrvac_fv_stage(amrvac_fv_ctx* c, int method, double dtfactor, double dt, do
   if (!c || !u_in || !u_out) return fail(AMRVAC_FV_EINVAL, "bad argument");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
   if (c->amr) return fail(AMRVAC_FV_EINVAL, "state arrays are for a uniform level");
+  if (c->nws > 0) return fail(AMRVAC_FV_EINVAL, "constrained transport advances slots only");
   if (u_out == u_in) return fail(AMRVAC_FV_EINVAL, "u_out must not alias u_in");
   const int mode = base ? 2 : 3;
   return stage_core(c, method, dtfactor, dt, cmax_global, u_in, u_out, base, mode, comb_a, comb_b, -1, nullptr);
@@ -1188,6 +1299,13 @@ int amrvac_fv_combine(amrvac_fv_ctx* c, int dst_slot, int nops, const int* ops, 
   hipLaunchKernelGGL(fv::combine_kernel, dim3(grid), dim3(fvc::NTHREADS), 0, c->stream, c->slot[dst_slot],
                      c->slot[0], c->slot[1], c->slot[2], c->slot[3], n, p);
   HIPCHK(hipGetLastError());
+  if (c->nws > 0) {  // the face field takes the same combination (src/mod_advance.t:119-291)
+    const size_t nf = (size_t)c->nb * c->nws * c->cells;
+    const int gf = (int)std::min<size_t>((nf + fvc::NTHREADS - 1) / fvc::NTHREADS, 8192);
+    hipLaunchKernelGGL(fv::combine_kernel, dim3(gf), dim3(fvc::NTHREADS), 0, c->stream, c->wsl[dst_slot],
+                       c->wsl[0], c->wsl[1], c->wsl[2], c->wsl[3], nf, p);
+    HIPCHK(hipGetLastError());
+  }
   return AMRVAC_FV_OK;
 }
 
@@ -1197,6 +1315,9 @@ int amrvac_fv_copy(amrvac_fv_ctx* c, int dst, int src) {
   if (dst == src) return AMRVAC_FV_OK;
   HIPCHK(hipMemcpyAsync(c->slot[dst], c->slot[src], (size_t)c->nb * c->nw * c->cells * sizeof(double),
                         hipMemcpyDeviceToDevice, c->stream));
+  if (c->nws > 0)
+    HIPCHK(hipMemcpyAsync(c->wsl[dst], c->wsl[src], (size_t)c->nb * c->nws * c->cells * sizeof(double),
+                          hipMemcpyDeviceToDevice, c->stream));
   return AMRVAC_FV_OK;
 }
 
@@ -1243,6 +1364,7 @@ int amrvac_fv_set_mesh_amr(amrvac_fv_ctx* c, int nblocks, const double* dx1, con
   if (!c || !dx1 || !level || !ntype || !nbr || !nchild || !icpos || !xlohi || !bc)
     return fail(AMRVAC_FV_EINVAL, "null argument");
   if (nblocks < 1) return fail(AMRVAC_FV_EINVAL, "nblocks must be positive");
+  if (c->nws > 0) return fail(AMRVAC_FV_EINVAL, "constrained transport is provided on a uniform level");
   const int nd = c->cfg.ndim, nnb = c->nnb, nch = 1 << nd;
   for (int b = 0; b < nblocks; ++b) {
     if (level[b] < 1 || level[b] > fvc::MAXLEV) return fail(AMRVAC_FV_EINVAL, "level out of range (1..8)");
@@ -1448,6 +1570,7 @@ int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt,
                        double cb, double* lam_in) {
   if (!c || !u_in || !lam_out || !lam_in) return fail(AMRVAC_FV_EINVAL, "bad argument");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  if (c->nws > 0) return fail(AMRVAC_FV_EINVAL, "adjoint: no constrained transport");
   if (lam_in == lam_out || lam_in == u_in) return fail(AMRVAC_FV_EINVAL, "lam_in must not alias lam_out or u_in");
   if (c->cfg.physics == 2 && (c->cfg.divb != 0 || c->cfg.ndir != 3))
     return fail(AMRVAC_FV_EINVAL, "the MHD adjoint stage is provided for ndir 3 without divergence cleaning");

@@ -466,15 +466,19 @@ __device__ __forceinline__ void weno5(R wmm, R wm, R w0, R wp, R wpp, R wppp, R&
 // the single-star states w1 across the fast waves, the double-star states w2
 // across the Alfven waves, then the region of the fan at the face.  wLp/wRp
 // are the primitive states after modify_wLR, wLC/wRC the conserved ones.
-template <class P, class R, int ID>
+template <class P, class R, int ID, bool CT = false>
 __device__ __forceinline__ void hlld_flux(const R* wLC, const R* wRC, const R* wLp, const R* wRp, const R* fL,
-                                          const R* fR, R sR, R sL, R* fC) {
+                                          const R* fR, R sR, R sL, R* fC, R bx_ct = R(0)) {
   static_assert(P::MHD && P::NDIR == 3, "HLLD is provided for MHD with three vector components");
   constexpr int NW = P::NW, RHO = P::RHO, E = P::E, MG = P::MAG, MO = P::MOM;
   constexpr int I2 = (ID + 1) % 3, I3 = (ID + 2) % 3;
   constexpr int M1 = MO + ID, M2 = MO + I2, M3 = MO + I3;
   constexpr int B1 = MG + ID, B2 = MG + I2, B3 = MG + I3;
-  const R Bx = fdiv(sR * wRC[B1] - sL * wLC[B1], sR - sL);
+  // HLL estimate of the normal field (Li 2005 eq. 33); with constrained
+  // transport the face field (src/mod_finite_volume.t:536-537)
+  R Bx;
+  if constexpr (CT) Bx = bx_ct;
+  else Bx = fdiv(sR * wRC[B1] - sL * wLC[B1], sR - sL);
   const R ptR = wRp[E] + R(0.5) * (wRC[MG] * wRC[MG] + wRC[MG + 1] * wRC[MG + 1] + wRC[MG + 2] * wRC[MG + 2]);
   const R ptL = wLp[E] + R(0.5) * (wLC[MG] * wLC[MG] + wLC[MG + 1] * wLC[MG + 1] + wLC[MG + 2] * wLC[MG + 2]);
   const R vR = wRp[M1], vL = wLp[M1];
@@ -572,9 +576,18 @@ __device__ __forceinline__ void hlld_flux(const R* wLC, const R* wRC, const R* w
 // st[k][iw] are primitive values of cells i-NG+1+k (face between i and i+1,
 // NG = 2: st = w(i-1), w(i), w(i+1), w(i+2); NG = 3 adds w(i-2), w(i+3)).
 // ---------------------------------------------------------------------------
-template <class P, class R, int RIEM, int ID, int LIM, int NG>
+// Constrained transport: the face's normal field in (the stage input's face
+// field, and that of ps for the HLLD star states), the reconstructed
+// velocities and signal speeds out for the edge electric fields.
+struct CtIO {
+  double bn, bx;
+  double vL[3], vR[3];
+  double cmax, cmin;
+};
+
+template <class P, class R, int RIEM, int ID, int LIM, int NG, bool CT = false>
 __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& fp, R ch, R* fC,
-                                          int& crash) {
+                                          int& crash, CtIO* ctio = nullptr) {
   constexpr int NW = P::NW;
   constexpr int C0 = NG - 1;  // index of cell i in st
   R wLp[NW], wRp[NW];
@@ -628,12 +641,26 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
     wRC[E] = wRC[E] + R(0.5) * (wRC[B] * wRC[B]);
     wLC[E] = wLC[E] + R(0.5) * (wLC[B] * wLC[B]);
   }
+  if constexpr (CT) {
+    // mhd_modify_wLR with stagger_grid (src/mhd/mod_mhd_phys.t:6338-6345)
+    constexpr int B = P::MAG + ID;
+    wLC[B] = R(ctio->bn);
+    wRC[B] = wLC[B];
+    wLp[B] = wLC[B];
+    wRp[B] = wLC[B];
+#pragma unroll
+    for (int d = 0; d < P::NDIR; ++d) {
+      ctio->vL[d] = wLp[P::MOM + d];
+      ctio->vR[d] = wRp[P::MOM + d];
+    }
+  }
   R fL[NW], fR[NW];
   phys_flux<P, R, ID>(wLC, wLp, ch, fL);
   phys_flux<P, R, ID>(wRC, wRp, ch, fR);
   R cmax, cmin = R(0);
   if constexpr (RIEM == RIEM_TVDLF) {
     cbounds<P, R, ID, false>(wLC, wRC, wLp, wRp, fp, cmax, cmin, crash);
+    if constexpr (CT) { ctio->cmax = cmax; ctio->cmin = cmin; }
     const R fac = R(-0.5) * R(fp.tvdlfeps) * cmax;
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
@@ -642,11 +669,13 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
     }
   } else {
     cbounds<P, R, ID, true>(wLC, wRC, wLp, wRp, fp, cmax, cmin, crash);
+    if constexpr (CT) { ctio->cmax = cmax; ctio->cmin = cmin; }
     // flux_type: normal B gets Lax-Friedrichs dissipation only without GLM
     // (src/mhd/mod_mhd_phys.t:580-588)
     constexpr int BTV = (P::MHD && !P::GLM && P::NDIM > 1) ? (P::MAG + ID) : -1;
     if constexpr (RIEM == RIEM_HLLD) {
-      hlld_flux<P, R, ID>(wLC, wRC, wLp, wRp, fL, fR, cmax, cmin, fC);
+      if constexpr (CT) hlld_flux<P, R, ID, true>(wLC, wRC, wLp, wRp, fL, fR, cmax, cmin, fC, R(ctio->bx));
+      else hlld_flux<P, R, ID>(wLC, wRC, wLp, wRp, fL, fR, cmax, cmin, fC);
     } else if constexpr (RIEM == RIEM_HLL) {
       const R inv = frcp(cmax - cmin);
 #pragma unroll

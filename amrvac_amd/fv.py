@@ -13,7 +13,7 @@ import os
 
 import numpy as np
 
-from .host import BOUNDSPEED, DIVB, LIMITERS, Par
+from .host import BOUNDSPEED, DIVB, LIMITERS, TYPE_CT, Par
 from .mesh import UniformMesh
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -39,7 +39,7 @@ class FVConfig(ctypes.Structure):
         ("boundspeed", ctypes.c_int), ("tvdlfeps", ctypes.c_double), ("cada3_radius", ctypes.c_double),
         ("small_pressure", ctypes.c_double), ("small_density", ctypes.c_double),
         ("check_small_values", ctypes.c_int), ("nghost", ctypes.c_int),
-        ("block_nx", ctypes.c_int * 3), ("precision", ctypes.c_int),
+        ("block_nx", ctypes.c_int * 3), ("precision", ctypes.c_int), ("type_ct", ctypes.c_int),
     ]
 
 
@@ -125,6 +125,7 @@ def make_config(par: Par, mesh: UniformMesh, precision: str | None = None) -> FV
     for d in range(3):
         c.block_nx[d] = mesh.block_nx[d] if d < par.ndim else 1
     c.precision = PREC[precision or par.precision]
+    c.type_ct = TYPE_CT[par.type_ct] if par.stagger else 0
     return c
 
 
@@ -140,7 +141,7 @@ class DeviceFV:
         self.par = par
         self.mesh = mesh
         self.nw = par.nw
-        self.shape = (mesh.nblocks, self.nw) + mesh.nxg[::-1]
+        self.shape = (mesh.nblocks, par.nwt) + mesh.nxg[::-1]
         cfg = make_config(par, mesh, precision)
         h = ctypes.c_void_p()
         self._check(self.L.amrvac_fv_init(ctypes.byref(cfg), ctypes.byref(h)))

@@ -44,7 +44,7 @@ typedef struct amrvac_fv_config {
   int ndir;            /* vector components, ndim <= ndir <= 3 */
   int physics;         /* 1 = hd (src/hd/mod_hd_phys.t), 2 = mhd (src/mhd/mod_mhd_phys.t) */
   double gamma;        /* hd_gamma / mhd_gamma */
-  int divb;            /* typedivbfix: 0 none, 1 glm, 4 linde */
+  int divb;            /* typedivbfix: 0 none, 1 glm, 4 linde, 8 ct */
   double glm_alpha;    /* mhd_glm_alpha (0.5) */
   int glm_extended;    /* mhd_glm_extended (1) */
   double divbdiff;     /* Linde coefficient (0.8) */
@@ -57,6 +57,9 @@ typedef struct amrvac_fv_config {
   int nghost;          /* 2, or 3 for weno5 */
   int block_nx[3];     /* interior cells per block; x and y must be 16 */
   int precision;       /* AMRVAC_FV_PREC_* */
+  int type_ct;         /* with divb = 8 (constrained transport): 1 average,
+                          2 uct_contact, 3 uct_hll (type_ct,
+                          src/mhd/mod_mhd_phys.t:766-781) */
 } amrvac_fv_config;
 
 typedef struct amrvac_fv_ctx amrvac_fv_ctx;
