@@ -144,7 +144,12 @@ int amrvac_fv_set_fix_conserve(amrvac_fv_ctx* ctx, int on);
 /* Host <-> device copies of one slot in the reference layout (ghost cells
    included).  upload reads the interior; download fills the ghost cells as
    getbc would (same-level copies and physical boundaries,
-   src/mod_ghostcells_update.t:350-566) with a device kernel. */
+   src/mod_ghostcells_update.t:350-566) with a device kernel.
+   With constrained transport (divb = 8) each block holds nw + ndim
+   variables: the cell variables, then the face field ws(ix, idim) stored
+   at the index of the cell whose upper idim face it sits on; combine, copy
+   and advect1 carry the face field with the cells, the state-array entry
+   points (amrvac_fv_stage, amrvac_fv_slot_pointer) expose the cells only. */
 int amrvac_fv_upload(amrvac_fv_ctx* ctx, int slot, const double* w_host);
 int amrvac_fv_download(amrvac_fv_ctx* ctx, int slot, double* w_host);
 
