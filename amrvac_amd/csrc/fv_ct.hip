@@ -71,17 +71,21 @@ __device__ __forceinline__ void prim_at(const SweepArgs& a, int b, int x, int y,
   to_prim<P>(c, w, fp);
 }
 
+constexpr int CTS = BX + 4;  // staged cells -2..17 per direction
+
 template <class P, int RIEM, int LIM, int ID>
-__device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams& fp, int b, int i, int j,
-                            int& crash) {
+__device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams& fp, const double* lds, int b,
+                            int i, int j, int& crash) {
   using L = CtLayout<P>;
   constexpr int NW = P::NW;
   double st[4][NW];
-  // cells i-1 .. i+2 along ID through the upper ID face of cell (i, j)
+  // cells i-1 .. i+2 along ID through the upper ID face of cell (i, j),
+  // primitive, from the block's staged state
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int x = ID == 0 ? i - 1 + s : i, y = ID == 1 ? j - 1 + s : j;
-    prim_at<P>(a, b, x, y, fp, st[s]);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) st[s][iw] = lds[iw * CTS * CTS + (y + 2) * CTS + (x + 2)];
   }
   CtIO io;
   io.bn = ws_at(a, t.ws_src, b, ID, i, j);
@@ -105,17 +109,28 @@ __device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams&
   }
 }
 
-// grid: (block, direction); faces -1..15 along the direction, -tgc..15+tgc
-// across it
+// one workgroup per block: the primitive state of cells -2..17 (halo from
+// the neighbours) staged in LDS once, then both directions' faces: -1..15
+// along the direction, -tgc..15+tgc across it
 template <class P, int RIEM, int LIM>
 __global__ void __launch_bounds__(fvc::NTHREADS) ct_face_kernel(SweepArgs a, CtArgs t, FVParams fp) {
-  const int b = blockIdx.x >> 1, dim = blockIdx.x & 1;
-  const int nacross = BX + 2 * t.tgc, nalong = BX + 1;
+  __shared__ double lds[P::NW * CTS * CTS];
+  const int b = wg_block(a.per_xcd);
+  if (b >= a.nblocks) return;
+  for (int q = threadIdx.x; q < CTS * CTS; q += fvc::NTHREADS) {
+    double w[P::NW];
+    prim_at<P>(a, b, q % CTS - 2, q / CTS - 2, fp, w);
+#pragma unroll
+    for (int iw = 0; iw < P::NW; ++iw) lds[iw * CTS * CTS + q] = w[iw];
+  }
+  __syncthreads();
+  const int nacross = BX + 2 * t.tgc, nalong = BX + 1, nf = nacross * nalong;
   int crash = 0;
-  for (int q = threadIdx.x; q < nacross * nalong; q += fvc::NTHREADS) {
-    const int along = q % nalong - 1, across = q / nalong - t.tgc;
-    if (dim == 0) ct_one_face<P, RIEM, LIM, 0>(a, t, fp, b, along, across, crash);
-    else ct_one_face<P, RIEM, LIM, 1>(a, t, fp, b, across, along, crash);
+  for (int q = threadIdx.x; q < 2 * nf; q += fvc::NTHREADS) {
+    const int r = q < nf ? q : q - nf;
+    const int along = r % nalong - 1, across = r / nalong - t.tgc;
+    if (q < nf) ct_one_face<P, RIEM, LIM, 0>(a, t, fp, lds, b, along, across, crash);
+    else ct_one_face<P, RIEM, LIM, 1>(a, t, fp, lds, b, across, along, crash);
   }
   if (crash) *a.crash = 1;
 }
@@ -282,7 +297,7 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct_update_kernel(SweepArgs a, C
 
 template <class P, int RIEM, int LIM>
 hipError_t launch_ct_lim(const SweepArgs& a, const CtArgs& t, const FVParams& fp, hipStream_t s) {
-  hipLaunchKernelGGL((ct_face_kernel<P, RIEM, LIM>), dim3(2 * a.nblocks), dim3(fvc::NTHREADS), 0, s, a, t, fp);
+  hipLaunchKernelGGL((ct_face_kernel<P, RIEM, LIM>), dim3(8 * a.per_xcd), dim3(fvc::NTHREADS), 0, s, a, t, fp);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((ct_update_kernel<P, LIM>), dim3(8 * a.per_xcd), dim3(fvc::NTHREADS), 0, s, a, t, fp);

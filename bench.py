@@ -189,6 +189,41 @@ def amr_blast(precision: str, steps: int, noise: float = 0.01):
             "regrids": ti.regrids - regrids0}
 
 
+def ct_field_loop(steps: int, warmup: int = 2, n: int = 1024):
+    """Constrained transport: the 2.5-D advected field loop
+    (tests/mhd/field_loop_2.5D, staggered face field, type_ct uct_contact)
+    on n x n/2 cells in 16^2 blocks, threestep / HLL / cada3 / cmaxmean,
+    CFL 0.9, FP64; cell-updates per RK stage as for the main workload."""
+    from amrvac_amd import problems
+    from amrvac_amd.fv import DeviceFV
+    from amrvac_amd.host import Par, TimeIntegrator
+    from amrvac_amd.mesh import UniformMesh
+    out = {"workload": f"2.5-D MHD field loop with constrained transport, {n}x{n // 2} in 16^2 blocks, "
+                       "threestep/HLL/cada3/cmaxmean, FP64"}
+    for ct in ("uct_contact", "average", "uct_hll"):
+        par = Par(physics="mhd", ndim=2, ndir=3, gamma=5.0 / 3.0, typedivbfix="ct", type_ct=ct,
+                  time_stepper="threestep", flux_scheme="hll", limiter="cada3", typeboundspeed="cmaxmean",
+                  courantpar=0.9, precision="fp64")
+        mesh = UniformMesh(2, (n, n // 2), (16, 16), (-1.0, -0.5), (1.0, 0.5), ("periodic",) * 4)
+        be = DeviceFV(par, mesh)
+        be.set_state(0, problems.field_loop_ct(mesh))
+        ti = TimeIntegrator(par, mesh, be)
+        for _ in range(warmup):
+            ti.step()
+        be.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ti.step()
+        be.sync()
+        dt = time.perf_counter() - t0
+        cells = n * (n // 2)
+        print(f"[bench] ct {ct}: {steps} steps in {dt:.4f} s", file=sys.stderr, flush=True)
+        out[ct] = {"value": round(cells * 3 * steps / dt / 1e6, 2), "unit": "Mcell-updates/s",
+                   "ms_per_step": round(dt / steps * 1e3, 4), "steps": steps}
+        del be
+    return out
+
+
 def load_traffic(name: str):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -211,6 +246,7 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost legs")
     ap.add_argument("--no-amr", action="store_true", help="skip the AMR blast-wave leg")
     ap.add_argument("--amr-steps", type=int, default=200)
+    ap.add_argument("--no-ct", action="store_true", help="skip the constrained-transport leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy one state slot device-to-device (a plain copy of "
                          "known size, the reference point for HBM byte counters)")
@@ -286,6 +322,8 @@ def main() -> int:
                                            "Loehner regrid every step, twostep/TVDLF/minmod",
                                **{p: amr_blast(p, args.amr_steps) for p in ("fp64", "mixed")},
                                "fp64_noise_free": amr_blast("fp64", args.amr_steps, noise=0.0)}
+    if not args.no_ct:
+        result["ct_field_loop"] = ct_field_loop(20)
     if args.cpu_seconds > 0:
         try:
             result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
