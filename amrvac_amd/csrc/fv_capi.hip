@@ -746,8 +746,6 @@ int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
     if (c.ndim != 2) return fail(AMRVAC_FV_EINVAL, "constrained transport is supported for ndim = 2");
     if (c.type_ct < 1 || c.type_ct > 3) return fail(AMRVAC_FV_EINVAL, "type_ct must be 1, 2 or 3");
     if (c.limiter == 16) return fail(AMRVAC_FV_EINVAL, "constrained transport takes a TVD limiter");
-    if (c.precision == AMRVAC_FV_PREC_MIXED)
-      return fail(AMRVAC_FV_EINVAL, "constrained transport runs in FP64 (precision 0 or 2)");
   }
   if (c.physics == 2 && c.divb == 4 && c.ndim != 2)
     return fail(AMRVAC_FV_EINVAL, "Linde div B cleaning is supported for ndim = 2 only");
@@ -1171,9 +1169,10 @@ static int ct_stage(amrvac_fv_ctx* c, int method, double dtfactor, double dt, in
   const bool exact = c->cfg.precision == 2;
   const double* wbase = comb_slot >= 0 ? c->wsl[comb_slot] : nullptr;
   const int bx_mode = psb == 0 ? mode : -1;
-  hipError_t e = exact ? fvx::launch_ct_stage(riem, c->cfg.limiter, c->cfg.type_ct, a, c->wsl[psa], wbase,
+  const int mixed = c->cfg.precision == AMRVAC_FV_PREC_MIXED ? 1 : 0;
+  hipError_t e = exact ? fvx::launch_ct_stage(0, riem, c->cfg.limiter, c->cfg.type_ct, a, c->wsl[psa], wbase,
                                              c->wsl[psb], c->wsl[0], bx_mode, c->d_ctf, c->d_wsn, c->fp, c->stream)
-                       : fv::launch_ct_stage(riem, c->cfg.limiter, c->cfg.type_ct, a, c->wsl[psa], wbase,
+                       : fv::launch_ct_stage(mixed, riem, c->cfg.limiter, c->cfg.type_ct, a, c->wsl[psa], wbase,
                                              c->wsl[psb], c->wsl[0], bx_mode, c->d_ctf, c->d_wsn, c->fp, c->stream);
   if (e == hipErrorInvalidValue) return fail(AMRVAC_FV_EINVAL, "no constrained-transport kernel for this scheme");
   if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("ct stage: ") + hipGetErrorString(e));

@@ -73,12 +73,12 @@ __device__ __forceinline__ void prim_at(const SweepArgs& a, int b, int x, int y,
 
 constexpr int CTS = BX + 4;  // staged cells -2..17 per direction
 
-template <class P, int RIEM, int LIM, int ID>
-__device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams& fp, const double* lds, int b,
+template <class P, class R, int RIEM, int LIM, int ID>
+__device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams& fp, const R* lds, int b,
                             int i, int j, int& crash) {
   using L = CtLayout<P>;
   constexpr int NW = P::NW;
-  double st[4][NW];
+  R st[4][NW];
   // cells i-1 .. i+2 along ID through the upper ID face of cell (i, j),
   // primitive, from the block's staged state
 #pragma unroll
@@ -90,12 +90,12 @@ __device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams&
   CtIO io;
   io.bn = ws_at(a, t.ws_src, b, ID, i, j);
   io.bx = t.bx_mode < 0 ? ws_at(a, t.ws_block, b, ID, i, j) : ws_comb(a, t, t.bx_mode, b, ID, i, j);
-  double fC[NW];
-  face_flux<P, double, RIEM, ID, LIM, 2, true>(st, fp, 0.0, fC, crash, &io);
+  R fC[NW];
+  face_flux<P, R, RIEM, ID, LIM, 2, true>(st, fp, R(0), fC, crash, &io);
   double* f = t.faces + (size_t)b * L::per_block + (size_t)ID * L::per_dim + (size_t)(j + 2) * CTG + (i + 2);
   constexpr size_t K = (size_t)CTG * CTG;
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) f[iw * K] = fC[iw];
+  for (int iw = 0; iw < NW; ++iw) f[iw * K] = double(fC[iw]);
   if (t.ct == CT_CONTACT) f[L::VN * K] = 0.5 * (io.vL[ID] + io.vR[ID]);
   if (t.ct == CT_HLL) {
     const double cbmax = io.cmax > 0.0 ? io.cmax : 0.0;
@@ -110,18 +110,19 @@ __device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams&
 }
 
 // one workgroup per block: the primitive state of cells -2..17 (halo from
-// the neighbours) staged in LDS once, then both directions' faces: -1..15
-// along the direction, -tgc..15+tgc across it
-template <class P, int RIEM, int LIM>
+// the neighbours) staged in LDS once (in the face precision R: FP64, or FP32
+// for the mixed form), then both directions' faces: -1..15 along the
+// direction, -tgc..15+tgc across it
+template <class P, class R, int RIEM, int LIM>
 __global__ void __launch_bounds__(fvc::NTHREADS) ct_face_kernel(SweepArgs a, CtArgs t, FVParams fp) {
-  __shared__ double lds[P::NW * CTS * CTS];
+  __shared__ R lds[P::NW * CTS * CTS];
   const int b = wg_block(a.per_xcd);
   if (b >= a.nblocks) return;
   for (int q = threadIdx.x; q < CTS * CTS; q += fvc::NTHREADS) {
     double w[P::NW];
     prim_at<P>(a, b, q % CTS - 2, q / CTS - 2, fp, w);
 #pragma unroll
-    for (int iw = 0; iw < P::NW; ++iw) lds[iw * CTS * CTS + q] = w[iw];
+    for (int iw = 0; iw < P::NW; ++iw) lds[iw * CTS * CTS + q] = R(w[iw]);
   }
   __syncthreads();
   const int nacross = BX + 2 * t.tgc, nalong = BX + 1, nf = nacross * nalong;
@@ -129,8 +130,8 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct_face_kernel(SweepArgs a, CtA
   for (int q = threadIdx.x; q < 2 * nf; q += fvc::NTHREADS) {
     const int r = q < nf ? q : q - nf;
     const int along = r % nalong - 1, across = r / nalong - t.tgc;
-    if (q < nf) ct_one_face<P, RIEM, LIM, 0>(a, t, fp, lds, b, along, across, crash);
-    else ct_one_face<P, RIEM, LIM, 1>(a, t, fp, lds, b, across, along, crash);
+    if (q < nf) ct_one_face<P, R, RIEM, LIM, 0>(a, t, fp, lds, b, along, across, crash);
+    else ct_one_face<P, R, RIEM, LIM, 1>(a, t, fp, lds, b, across, along, crash);
   }
   if (crash) *a.crash = 1;
 }
@@ -296,8 +297,16 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct_update_kernel(SweepArgs a, C
 }
 
 template <class P, int RIEM, int LIM>
-hipError_t launch_ct_lim(const SweepArgs& a, const CtArgs& t, const FVParams& fp, hipStream_t s) {
-  hipLaunchKernelGGL((ct_face_kernel<P, RIEM, LIM>), dim3(8 * a.per_xcd), dim3(fvc::NTHREADS), 0, s, a, t, fp);
+hipError_t launch_ct_lim(int mixed, const SweepArgs& a, const CtArgs& t, const FVParams& fp, hipStream_t s) {
+  if (mixed) {
+    if constexpr (kExact) return hipErrorInvalidValue;  // the bit-reproducible build is FP64 only
+    else
+      hipLaunchKernelGGL((ct_face_kernel<P, float, RIEM, LIM>), dim3(8 * a.per_xcd), dim3(fvc::NTHREADS), 0, s, a,
+                         t, fp);
+  } else {
+    hipLaunchKernelGGL((ct_face_kernel<P, double, RIEM, LIM>), dim3(8 * a.per_xcd), dim3(fvc::NTHREADS), 0, s, a,
+                       t, fp);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((ct_update_kernel<P, LIM>), dim3(8 * a.per_xcd), dim3(fvc::NTHREADS), 0, s, a, t, fp);
@@ -305,19 +314,21 @@ hipError_t launch_ct_lim(const SweepArgs& a, const CtArgs& t, const FVParams& fp
 }
 
 template <class P, int RIEM>
-hipError_t launch_ct_riem(int lim, const SweepArgs& a, const CtArgs& t, const FVParams& fp, hipStream_t s) {
+hipError_t launch_ct_riem(int mixed, int lim, const SweepArgs& a, const CtArgs& t, const FVParams& fp,
+                          hipStream_t s) {
   switch (lim) {
-    case LIM_MINMOD: return launch_ct_lim<P, RIEM, LIM_MINMOD>(a, t, fp, s);
-    case LIM_VANLEER: return launch_ct_lim<P, RIEM, LIM_VANLEER>(a, t, fp, s);
-    case LIM_CADA3: return launch_ct_lim<P, RIEM, LIM_CADA3>(a, t, fp, s);
-    default: return launch_ct_lim<P, RIEM, 0>(a, t, fp, s);
+    case LIM_MINMOD: return launch_ct_lim<P, RIEM, LIM_MINMOD>(mixed, a, t, fp, s);
+    case LIM_VANLEER: return launch_ct_lim<P, RIEM, LIM_VANLEER>(mixed, a, t, fp, s);
+    case LIM_CADA3: return launch_ct_lim<P, RIEM, LIM_CADA3>(mixed, a, t, fp, s);
+    default: return launch_ct_lim<P, RIEM, 0>(mixed, a, t, fp, s);
   }
 }
 
 }  // namespace
 
-// one constrained-transport stage (2-D ideal MHD, ndir = 3, TVD limiters)
-hipError_t launch_ct_stage(int riem, int lim, int ct, const fvc::SweepArgs& a, const double* ws_src,
+// one constrained-transport stage (2-D ideal MHD, ndir = 3, TVD limiters;
+// mixed: FP32 reconstruction and Riemann solver, FP64 state and edge fields)
+hipError_t launch_ct_stage(int mixed, int riem, int lim, int ct, const fvc::SweepArgs& a, const double* ws_src,
                            const double* ws_base, const double* ws_dst, const double* ws_block, int bx_mode,
                            double* faces, double* ws_new, const fvc::FVParams& fp, hipStream_t s) {
   using P = Phys<2, 3, true, DIVB_NONE>;
@@ -333,10 +344,10 @@ hipError_t launch_ct_stage(int riem, int lim, int ct, const fvc::SweepArgs& a, c
   t.tgc = ct == CT_HLL ? 2 : 1;
   t.tvdlf = riem == RIEM_TVDLF;
   switch (riem) {
-    case RIEM_HLL: return launch_ct_riem<P, RIEM_HLL>(lim, a, t, fp, s);
-    case RIEM_TVDLF: return launch_ct_riem<P, RIEM_TVDLF>(lim, a, t, fp, s);
-    case RIEM_HLLC: return launch_ct_riem<P, RIEM_HLLC>(lim, a, t, fp, s);
-    case RIEM_HLLD: return launch_ct_riem<P, RIEM_HLLD>(lim, a, t, fp, s);
+    case RIEM_HLL: return launch_ct_riem<P, RIEM_HLL>(mixed, lim, a, t, fp, s);
+    case RIEM_TVDLF: return launch_ct_riem<P, RIEM_TVDLF>(mixed, lim, a, t, fp, s);
+    case RIEM_HLLC: return launch_ct_riem<P, RIEM_HLLC>(mixed, lim, a, t, fp, s);
+    case RIEM_HLLD: return launch_ct_riem<P, RIEM_HLLD>(mixed, lim, a, t, fp, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -49,7 +49,8 @@ using SweepLauncher = hipError_t (*)(const KernelKey&, const SweepArgs&, const F
                                 int, hipStream_t);                                                   \
   hipError_t launch_sweep_3d_mhd(const fvc::KernelKey&, const fvc::SweepArgs&, const fvc::FVParams&,\
                                  int, hipStream_t);                                                  \
-  hipError_t launch_ct_stage(int riem, int lim, int ct, const fvc::SweepArgs& a, const double* ws_src,  \
+  hipError_t launch_ct_stage(int mixed, int riem, int lim, int ct, const fvc::SweepArgs& a,             \
+                             const double* ws_src,                                                    \
                              const double* ws_base, const double* ws_dst, const double* ws_block,      \
                              int bx_mode, double* faces, double* ws_new, const fvc::FVParams& fp,      \
                              hipStream_t s);                                                         \

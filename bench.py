@@ -193,17 +193,18 @@ def ct_field_loop(steps: int, warmup: int = 2, n: int = 1024):
     """Constrained transport: the 2.5-D advected field loop
     (tests/mhd/field_loop_2.5D, staggered face field, type_ct uct_contact)
     on n x n/2 cells in 16^2 blocks, threestep / HLL / cada3 / cmaxmean,
-    CFL 0.9, FP64; cell-updates per RK stage as for the main workload."""
+    CFL 0.9, FP64 (and the mixed form for uct_contact); cell-updates per RK
+    stage as for the main workload."""
     from amrvac_amd import problems
     from amrvac_amd.fv import DeviceFV
     from amrvac_amd.host import Par, TimeIntegrator
     from amrvac_amd.mesh import UniformMesh
     out = {"workload": f"2.5-D MHD field loop with constrained transport, {n}x{n // 2} in 16^2 blocks, "
                        "threestep/HLL/cada3/cmaxmean, FP64"}
-    for ct in ("uct_contact", "average", "uct_hll"):
+    for ct, prec in (("uct_contact", "fp64"), ("average", "fp64"), ("uct_hll", "fp64"), ("uct_contact", "mixed")):
         par = Par(physics="mhd", ndim=2, ndir=3, gamma=5.0 / 3.0, typedivbfix="ct", type_ct=ct,
                   time_stepper="threestep", flux_scheme="hll", limiter="cada3", typeboundspeed="cmaxmean",
-                  courantpar=0.9, precision="fp64")
+                  courantpar=0.9, precision=prec)
         mesh = UniformMesh(2, (n, n // 2), (16, 16), (-1.0, -0.5), (1.0, 0.5), ("periodic",) * 4)
         be = DeviceFV(par, mesh)
         be.set_state(0, problems.field_loop_ct(mesh))
@@ -217,8 +218,8 @@ def ct_field_loop(steps: int, warmup: int = 2, n: int = 1024):
         be.sync()
         dt = time.perf_counter() - t0
         cells = n * (n // 2)
-        print(f"[bench] ct {ct}: {steps} steps in {dt:.4f} s", file=sys.stderr, flush=True)
-        out[ct] = {"value": round(cells * 3 * steps / dt / 1e6, 2), "unit": "Mcell-updates/s",
+        print(f"[bench] ct {ct} {prec}: {steps} steps in {dt:.4f} s", file=sys.stderr, flush=True)
+        out[ct if prec == "fp64" else f"{ct}_mixed"] = {"value": round(cells * 3 * steps / dt / 1e6, 2), "unit": "Mcell-updates/s",
                    "ms_per_step": round(dt / steps * 1e3, 4), "steps": steps}
         del be
     return out
