@@ -200,7 +200,7 @@ def ct_field_loop(steps: int, warmup: int = 2, n: int = 1024):
     from amrvac_amd.host import Par, TimeIntegrator
     from amrvac_amd.mesh import UniformMesh
     out = {"workload": f"2.5-D MHD field loop with constrained transport, {n}x{n // 2} in 16^2 blocks, "
-                       "threestep/HLL/cada3/cmaxmean, FP64"}
+                       "threestep/HLL/cada3/cmaxmean, FP64 (uct_contact_mixed: the mixed form)"}
     for ct, prec in (("uct_contact", "fp64"), ("average", "fp64"), ("uct_hll", "fp64"), ("uct_contact", "mixed")):
         par = Par(physics="mhd", ndim=2, ndir=3, gamma=5.0 / 3.0, typedivbfix="ct", type_ct=ct,
                   time_stepper="threestep", flux_scheme="hll", limiter="cada3", typeboundspeed="cmaxmean",
