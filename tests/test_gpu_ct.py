@@ -29,6 +29,8 @@ CASES = [
     ("3step_hll_cada_ct_hll", "tvdlf", "vanleer", None),
     ("3step_hll_cada_ct_average", "hllc", "koren", None),
     ("3step_hll_cada_ct_hll", "hlld", "woodward", "fourstep"),
+    ("3step_hll_cada_ct", "hll", "mcbeta", "fourstep:rk4"),
+    ("3step_hll_cada_ct_average", "tvdlf", "superbee", "fivestep"),
 ]
 
 
@@ -40,7 +42,9 @@ def _par(scheme, fs, lim, ts):
     if lim:
         kw["limiter"] = lim
     if ts:
-        kw["time_stepper"] = ts
+        kw["time_stepper"] = ts.split(":")[0]
+        if ":" in ts:
+            kw["time_integrator"] = ts.split(":")[1]
     return dataclasses.replace(par, **kw), mesh
 
 
