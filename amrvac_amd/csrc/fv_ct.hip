@@ -73,9 +73,20 @@ __device__ __forceinline__ void prim_at(const SweepArgs& a, int b, int x, int y,
 
 constexpr int CTS = BX + 4;  // staged cells -2..17 per direction
 
+// the stage input's face field of faces -2..17 of a block, staged in LDS
+__device__ __forceinline__ void stage_ws(const SweepArgs& a, const CtArgs& t, int b, double* wsl) {
+  for (int q = threadIdx.x; q < 2 * CTS * CTS; q += fvc::NTHREADS) {
+    const int d = q / (CTS * CTS), r = q % (CTS * CTS);
+    wsl[q] = ws_at(a, t.ws_src, b, d, r % CTS - 2, r / CTS - 2);
+  }
+}
+__device__ __forceinline__ double wsl_at(const double* wsl, int d, int x, int y) {
+  return wsl[d * CTS * CTS + (y + 2) * CTS + (x + 2)];
+}
+
 template <class P, class R, int RIEM, int LIM, int ID>
-__device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams& fp, const R* lds, int b,
-                            int i, int j, int& crash) {
+__device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams& fp, const R* lds,
+                            const double* wsl, int b, int i, int j, int& crash) {
   using L = CtLayout<P>;
   constexpr int NW = P::NW;
   R st[4][NW];
@@ -88,7 +99,7 @@ __device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams&
     for (int iw = 0; iw < NW; ++iw) st[s][iw] = lds[iw * CTS * CTS + (y + 2) * CTS + (x + 2)];
   }
   CtIO io;
-  io.bn = ws_at(a, t.ws_src, b, ID, i, j);
+  io.bn = wsl_at(wsl, ID, i, j);
   io.bx = t.bx_mode < 0 ? ws_at(a, t.ws_block, b, ID, i, j) : ws_comb(a, t, t.bx_mode, b, ID, i, j);
   R fC[NW];
   face_flux<P, R, RIEM, ID, LIM, 2, true>(st, fp, R(0), fC, crash, &io);
@@ -116,8 +127,10 @@ __device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams&
 template <class P, class R, int RIEM, int LIM>
 __global__ void __launch_bounds__(fvc::NTHREADS) ct_face_kernel(SweepArgs a, CtArgs t, FVParams fp) {
   __shared__ R lds[P::NW * CTS * CTS];
+  __shared__ double wsl[2 * CTS * CTS];
   const int b = wg_block(a.per_xcd);
   if (b >= a.nblocks) return;
+  stage_ws(a, t, b, wsl);
   for (int q = threadIdx.x; q < CTS * CTS; q += fvc::NTHREADS) {
     double w[P::NW];
     prim_at<P>(a, b, q % CTS - 2, q / CTS - 2, fp, w);
@@ -130,8 +143,8 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct_face_kernel(SweepArgs a, CtA
   for (int q = threadIdx.x; q < 2 * nf; q += fvc::NTHREADS) {
     const int r = q < nf ? q : q - nf;
     const int along = r % nalong - 1, across = r / nalong - t.tgc;
-    if (q < nf) ct_one_face<P, R, RIEM, LIM, 0>(a, t, fp, lds, b, along, across, crash);
-    else ct_one_face<P, R, RIEM, LIM, 1>(a, t, fp, lds, b, across, along, crash);
+    if (q < nf) ct_one_face<P, R, RIEM, LIM, 0>(a, t, fp, lds, wsl, b, along, across, crash);
+    else ct_one_face<P, R, RIEM, LIM, 1>(a, t, fp, lds, wsl, b, across, along, crash);
   }
   if (crash) *a.crash = 1;
 }
@@ -165,8 +178,13 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct_update_kernel(SweepArgs a, C
   __shared__ double wsx[NE * BX];  // x faces i = -1..15, rows 0..15
   __shared__ double wsy[BX * NE];  // y faces rows j = -1..15, columns 0..15
   __shared__ double ecc[(BX + 2) * (BX + 2)];  // cells -1..16
+  __shared__ double wsl[2 * CTS * CTS];         // stage input's faces -2..17 (uct_hll)
   const int b = wg_block(a.per_xcd);
   if (b >= a.nblocks) return;
+  if (t.ct == CT_HLL) {
+    stage_ws(a, t, b, wsl);
+    __syncthreads();
+  }
   if (t.ct == CT_CONTACT) {
     for (int q = threadIdx.x; q < (BX + 2) * (BX + 2); q += fvc::NTHREADS) {
       const int x = q % (BX + 2) - 1, y = q / (BX + 2) - 1;
@@ -228,10 +246,10 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct_update_kernel(SweepArgs a, C
       ct_recon<LIM>(fp, rx, fget<P>(t, b, 1, L::VB2, i - 1, j), fget<P>(t, b, 1, L::VB2, i, j),
                     fget<P>(t, b, 1, L::VB2, i + 1, j), fget<P>(t, b, 1, L::VB2, i + 2, j), vtL1, vtR1);
       // face field of the stage input: x faces along y, y faces along x
-      ct_recon<LIM>(fp, ry, ws_at(a, t.ws_src, b, 0, i, j - 1), ws_at(a, t.ws_src, b, 0, i, j),
-                    ws_at(a, t.ws_src, b, 0, i, j + 1), ws_at(a, t.ws_src, b, 0, i, j + 2), btLx, btRx);
-      ct_recon<LIM>(fp, rx, ws_at(a, t.ws_src, b, 1, i - 1, j), ws_at(a, t.ws_src, b, 1, i, j),
-                    ws_at(a, t.ws_src, b, 1, i + 1, j), ws_at(a, t.ws_src, b, 1, i + 2, j), btLy, btRy);
+      ct_recon<LIM>(fp, ry, wsl_at(wsl, 0, i, j - 1), wsl_at(wsl, 0, i, j), wsl_at(wsl, 0, i, j + 1),
+                    wsl_at(wsl, 0, i, j + 2), btLx, btRx);
+      ct_recon<LIM>(fp, rx, wsl_at(wsl, 1, i - 1, j), wsl_at(wsl, 1, i, j), wsl_at(wsl, 1, i + 1, j),
+                    wsl_at(wsl, 1, i + 2, j), btLy, btRy);
       auto mx = [](double u, double v) { return u > v ? u : v; };
       const double cm1 = mx(fget<P>(t, b, 0, L::CBMIN, i, j + 1), fget<P>(t, b, 0, L::CBMIN, i, j));
       const double cp1 = mx(fget<P>(t, b, 0, L::CBMAX, i, j + 1), fget<P>(t, b, 0, L::CBMAX, i, j));
