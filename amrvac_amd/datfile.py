@@ -13,7 +13,9 @@ The byte layout is the reference's (doc/fileformat.md):
   * per leaf in Morton order: the ghost-cell counts (0, as with the default
     save_physical_boundary = .false., block_shape_io
     src/io/mod_input_output_helper.t:54-84) and w(ixM^S, 1:nw) of the
-    conserved state, Fortran order.
+    conserved state, Fortran order; with a staggered face field
+    (constrained transport) followed by ws(ixM^L-1:ixM^L, 1:nws)
+    (src/io/mod_input_output.t:2100-2106), the header's staggered flag set.
 Integers are 4-byte, logicals 4-byte (0/1), offsets 8-byte, reals 8-byte,
 little-endian; names are blank-padded to 16 characters.
 
@@ -81,6 +83,7 @@ class Snapshot:
     level: np.ndarray = field(repr=False)       # [nleafs]
     ig: np.ndarray = field(repr=False)          # [nleafs, ndim], 1-based
     w: np.ndarray = field(repr=False)           # [nleafs, nw, (k,) j, i] interior cells
+    ws: np.ndarray = field(default=None, repr=False)  # [nleafs, nws, (k,) j, i] faces ixM-1..ixM (staggered)
     offset_tree: int = 0
     offset_block: int = 0
     block_offsets: np.ndarray = field(default=None, repr=False)
@@ -124,9 +127,18 @@ def write_snapshot(path: str, par, mesh, w: np.ndarray, it: int = 0, global_time
     leaf, level, ig = _tree(mesh)
     nleafs = int(level.shape[0])
     nparents = int(leaf.shape[0]) - nleafs
-    inner = np.ascontiguousarray(mesh.interior(np.asarray(w, dtype=np.float64)))
-    if inner.shape[0] != nleafs or inner.shape[1] != par.nw:
-        raise ValueError(f"state shape {inner.shape} does not match {nleafs} leaves x {par.nw} variables")
+    w = np.asarray(w, dtype=np.float64)
+    nws = getattr(par, "nws", 0)
+    if w.shape[0] != nleafs or w.shape[1] != par.nw + nws:
+        raise ValueError(f"state shape {w.shape} does not match {nleafs} leaves x {par.nw + nws} variables")
+    inner = np.ascontiguousarray(mesh.interior(w[:, :par.nw]))
+    faces = None
+    if nws:
+        # faces ixM-1..ixM in every dimension: the upper faces of the cells
+        # ng-1 .. ng+block_nx-1 (ghost faces as getbc left them)
+        ng = mesh.nghost
+        sl = tuple(slice(ng - 1, ng + mesh.block_nx[d]) for d in reversed(range(nd)))
+        faces = np.ascontiguousarray(w[(slice(None), slice(par.nw, par.nw + nws)) + sl])
     names = variable_names(par)
     periodic = tuple(mesh.bc[2 * d] == "periodic" for d in range(nd))
 
@@ -138,7 +150,7 @@ def write_snapshot(path: str, par, mesh, w: np.ndarray, it: int = 0, global_time
         h += struct.pack(f"<{nd}i", *mesh.domain_nx) + struct.pack(f"<{nd}i", *mesh.block_nx)
         h += struct.pack(f"<{nd}i", *[1 if p else 0 for p in periodic])
         h += _name(geometry_name(par))
-        h += struct.pack("<i", 0)  # staggered
+        h += struct.pack("<i", 1 if nws else 0)  # staggered
         h += b"".join(_name(n) for n in names)
         h += _name(par.physics)
         h += struct.pack("<i", 1) + struct.pack("<d", par.gamma) + _name("gamma")
@@ -148,7 +160,7 @@ def write_snapshot(path: str, par, mesh, w: np.ndarray, it: int = 0, global_time
     off_tree = len(header(0, 0))
     tree = leaf.astype("<i4").tobytes() + level.astype("<i4").tobytes() + ig.astype("<i4").tobytes()
     off_block = off_tree + len(tree) + 8 * nleafs
-    per_block = 2 * nd * 4 + inner[0].size * 8
+    per_block = 2 * nd * 4 + inner[0].size * 8 + (faces[0].size * 8 if nws else 0)
     offsets = off_block + per_block * np.arange(nleafs, dtype=np.int64)
     with open(path, "wb") as fh:
         fh.write(header(off_tree, off_block))
@@ -158,6 +170,8 @@ def write_snapshot(path: str, par, mesh, w: np.ndarray, it: int = 0, global_time
         for b in range(nleafs):
             fh.write(zeros)
             fh.write(inner[b].astype("<f8").tobytes())  # (iw, (k,) j, i) C order = Fortran w(i, j, (k,) iw)
+            if nws:
+                fh.write(faces[b].astype("<f8").tobytes())
     return read_snapshot(path)
 
 
@@ -207,7 +221,8 @@ def read_snapshot(path: str) -> Snapshot:
     p += 8 * nleafs
     if p != off_block:
         raise ValueError(f"block offset {off_block} does not follow the tree ({p})")
-    blocks = []
+    blocks, fblocks = [], []
+    nws = nd if stag else 0  # nws = ndim (src/mhd/mod_mhd_phys.t:558)
     for b in range(nleafs):
         q = int(offsets[b])
         ng = np.frombuffer(data, "<i4", 2 * nd, q)
@@ -215,11 +230,17 @@ def read_snapshot(path: str) -> Snapshot:
         n = nw * int(np.prod(shape))
         arr = np.frombuffer(data, "<f8", n, q + 8 * nd).reshape([nw] + shape[::-1])
         blocks.append(np.array(arr))
+        if nws:
+            fshape = [s_ + 1 for s_ in shape]
+            nf = nws * int(np.prod(fshape))
+            farr = np.frombuffer(data, "<f8", nf, q + 8 * nd + 8 * n).reshape([nws] + fshape[::-1])
+            fblocks.append(np.array(farr))
     return Snapshot(nw=nw, ndir=ndir, ndim=nd, levmax=levmax, it=it, global_time=global_time,
                     xprobmin=tuple(xmin), xprobmax=tuple(xmax), domain_nx=tuple(dnx), block_nx=tuple(bnx),
                     periodic=periodic, geometry=geom, staggered=bool(stag), w_names=w_names,
                     physics_type=phys, params=dict(zip(pnames, pvals)), snapshotnext=snext, slicenext=slnext,
                     collapsenext=cnext, leaf=leaf, level=level, ig=ig, w=np.stack(blocks),
+                    ws=np.stack(fblocks) if nws else None,
                     offset_tree=off_tree, offset_block=off_block, block_offsets=offsets)
 
 
@@ -247,8 +268,15 @@ def forest_from_snapshot(snap: Snapshot, bc, max_level: int, nghost: int = 2):
 
 def state_from_snapshot(snap: Snapshot, mesh) -> np.ndarray:
     """The snapshot's interior data in the backend's host layout with ghost
-    cells ([nb, nw, (k,) j, i]; ghost cells zero, to be filled by getbc)."""
-    shape = (snap.nleafs, snap.nw) + tuple(mesh.nxg[::-1])
+    cells ([nb, nw (+ nws), (k,) j, i]; ghost cells zero, to be filled by
+    getbc; a staggered snapshot's faces ixM-1..ixM after the cell
+    variables)."""
+    nws = 0 if snap.ws is None else snap.ws.shape[1]
+    shape = (snap.nleafs, snap.nw + nws) + tuple(mesh.nxg[::-1])
     w = np.zeros(shape, dtype=np.float64)
-    mesh.interior(w)[...] = snap.w
+    mesh.interior(w[:, :snap.nw])[...] = snap.w
+    if nws:
+        ng = mesh.nghost
+        sl = tuple(slice(ng - 1, ng + mesh.block_nx[d]) for d in reversed(range(snap.ndim)))
+        w[(slice(None), slice(snap.nw, snap.nw + nws)) + sl] = snap.ws
     return w

@@ -69,3 +69,33 @@ def test_device_amr_snapshot_restart(tmp_path):
     ti2.run(nsteps=4)
     assert [(n.level, n.ig) for n in f2.leaves] == [(n.level, n.ig) for n in f_ref.leaves]
     assert np.array_equal(f2.interior(ti2.be.get_state(0)), f_ref.interior(ref.be.get_state(0)))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_staggered_restart_continues_bitwise(tmp_path, precision):
+    """Constrained transport: snapshot of the device's cells and face field,
+    restart, and a bit-for-bit continuation."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_oracle_golden import field_loop_setup
+    from amrvac_amd.fv import DeviceFV
+    print(f"[datfile] 2.5-D CT field-loop restart, {precision}", flush=True)
+    par, mesh = field_loop_setup("3step_hll_cada_ct_hll")
+    w0 = problems.field_loop_ct(mesh)
+    ref = DeviceFV(par, mesh, precision=precision)
+    ref.set_state(0, w0)
+    TimeIntegrator(par, mesh, ref).run(nsteps=6)
+    be = DeviceFV(par, mesh, precision=precision)
+    be.set_state(0, w0)
+    ti = TimeIntegrator(par, mesh, be)
+    ti.run(nsteps=3)
+    path = os.path.join(tmp_path, "fl0000.dat")
+    snap = datfile.write_snapshot(path, par, mesh, be.get_state(0), it=ti.state.it,
+                                  global_time=ti.state.global_time)
+    assert snap.staggered
+    be2 = DeviceFV(par, mesh, precision=precision)
+    be2.set_state(0, datfile.state_from_snapshot(datfile.read_snapshot(path), mesh))
+    ti2 = TimeIntegrator(par, mesh, be2)
+    ti2.state.it, ti2.state.global_time = snap.it, snap.global_time
+    ti2.run(nsteps=3)
+    assert np.array_equal(mesh.interior(be2.get_state(0)), mesh.interior(ref.get_state(0)))
