@@ -58,7 +58,10 @@ def test_blast_wave_amr_golden_log(ndim, scheme):
 def test_blast_wave_3d_threestep_log_not_reproduced():
     """bw_3d_3step_hll_cada.log is NOT reproduced: mean(m^2) is 0.4% off
     from the first logged step while the 2-D log of the same scheme and the
-    3-D two-step log match to 1e-8.  Recorded as an open discrepancy."""
+    3-D two-step log match to 1e-8.  Recorded as an open discrepancy; the
+    directory (tests/hd/blast_wave_Cartesian_3D) is in none of the
+    reference's own regression lists, so its logs are not maintained against
+    the current code."""
     par, forest, ap = blast_setup(3, "3step_hll_cada", 0.003)
     ti = AMRIntegrator(par, forest, ap, lambda p, f: OracleFV(p, f, threads=4), blast_init)
     st = ti.run()
