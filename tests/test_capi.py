@@ -43,3 +43,38 @@ def test_init_rejects_bad_configs_without_touching_the_gpu():
     cfg.limiter = 16  # weno5 needs 3 ghost cells
     assert L.amrvac_fv_init(ctypes.byref(cfg), ctypes.byref(h)) == -1
     assert b"weno5" in L.amrvac_fv_last_error()
+
+
+def test_config_struct_layout_matches_header(tmp_path):
+    """The ctypes mirror of amrvac_fv_config has the header's size and field
+    offsets (a C program compiled against include/amrvac_fv.h prints them)."""
+    import subprocess
+    names = [f[0] for f in fv.FVConfig._fields_]
+    src = tmp_path / "layout.c"
+    body = "".join(f'  printf("{n} %zu\\n", offsetof(amrvac_fv_config, {n}));\n' for n in names)
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"amrvac_fv.h\"\nint main(void) {\n"
+                   '  printf("size %zu\\n", sizeof(amrvac_fv_config));\n' + body + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines())
+    assert int(out["size"]) == ctypes.sizeof(fv.FVConfig)
+    for n in names:
+        assert int(out[n]) == getattr(fv.FVConfig, n).offset, n
+
+
+def test_init_rejects_unsupported_constrained_transport():
+    if not os.path.exists(fv.LIB_PATH):
+        pytest.skip("library not built")
+    L = fv.load_library()
+    cfg = fv.FVConfig()
+    cfg.ndim, cfg.ndir, cfg.physics, cfg.gamma = 3, 3, 2, 5.0 / 3.0
+    cfg.limiter, cfg.boundspeed, cfg.nghost, cfg.precision = 9, 1, 2, 0
+    cfg.block_nx[0], cfg.block_nx[1], cfg.block_nx[2] = 16, 16, 16
+    cfg.divb, cfg.type_ct = 8, 2
+    h = ctypes.c_void_p()
+    assert L.amrvac_fv_init(ctypes.byref(cfg), ctypes.byref(h)) == -1
+    assert b"ndim = 2" in L.amrvac_fv_last_error()
+    cfg.ndim, cfg.block_nx[2], cfg.type_ct = 2, 1, 4
+    assert L.amrvac_fv_init(ctypes.byref(cfg), ctypes.byref(h)) == -1
+    assert b"type_ct" in L.amrvac_fv_last_error()
