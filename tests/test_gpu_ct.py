@@ -31,6 +31,8 @@ CASES = [
     ("3step_hll_cada_ct_hll", "hlld", "woodward", "fourstep"),
     ("3step_hll_cada_ct", "hll", "mcbeta", "fourstep:rk4"),
     ("3step_hll_cada_ct_average", "tvdlf", "superbee", "fivestep"),
+    ("3step_hll_cada_ct", "hll", "vanleer", "twostep"),          # predictor-corrector, HLL predictor
+    ("3step_hlld_cada_ct", "hlld", "minmod", "onestep"),
 ]
 
 
