@@ -676,7 +676,8 @@ fvc::KernelKey make_key(const amrvac_fv_ctx* c, int riem) {
   k.ndim = c->cfg.ndim;
   k.physics = c->cfg.physics;
   k.ndir = c->cfg.ndir;
-  k.divb = c->cfg.physics == 2 ? c->cfg.divb : 0;
+  // constrained transport: the cell kernels are those of ideal MHD
+  k.divb = c->cfg.physics == 2 ? (c->cfg.divb == 8 ? 0 : c->cfg.divb) : 0;
   k.mixed = c->cfg.precision == AMRVAC_FV_PREC_MIXED ? 1 : 0;
   k.riem = riem;
   k.weno = c->cfg.limiter == 16 ? 1 : 0;
@@ -1154,7 +1155,31 @@ static int ct_stage(amrvac_fv_ctx* c, int method, double dtfactor, double dt, in
                     int mode, double comb_a, double comb_b, int* crash_out) {
   int riem;
   if (method_to_riem(method, riem) != 0) return fail(AMRVAC_FV_EINVAL, "unsupported flux scheme");
-  if (riem == fv::RIEM_HANCOCK) return fail(AMRVAC_FV_EINVAL, "constrained transport: no Hancock predictor");
+  if (riem == fv::RIEM_HANCOCK) {
+    // hancock (src/mod_finite_volume.t:19-113) neither reads nor advances
+    // the face field: the cells take the ordinary predictor, the faces of
+    // psb the stage's initial combination
+    const int rc = stage_core(c, method, dtfactor, dt, 0.0, c->slot[psa], c->slot[psb],
+                              comb_slot >= 0 ? c->slot[comb_slot] : nullptr, mode, comb_a, comb_b, psb, crash_out);
+    if (rc != AMRVAC_FV_OK) return rc;
+    const size_t nf = (size_t)c->nb * c->nws * c->cells;
+    if (mode == 1 || mode == 3) {
+      const double* from = mode == 1 ? c->wsl[comb_slot] : c->wsl[psa];
+      HIPCHK(hipMemcpyAsync(c->wsl[psb], from, nf * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    } else if (mode == 2) {  // comb_a * ws[comb_slot] + comb_b * ws[psa]
+      fv::CombProg p{};
+      const int ops[7] = {fv::COP_CONST, comb_slot, fv::COP_MUL, fv::COP_CONST, psa, fv::COP_MUL, fv::COP_ADD};
+      p.nops = 7;
+      for (int q = 0; q < 7; ++q) p.ops[q] = ops[q];
+      p.k[0] = comb_a;
+      p.k[1] = comb_b;
+      const int gf = (int)std::min<size_t>((nf + fvc::NTHREADS - 1) / fvc::NTHREADS, 8192);
+      hipLaunchKernelGGL(fv::combine_kernel, dim3(gf), dim3(fvc::NTHREADS), 0, c->stream, c->wsl[psb],
+                         c->wsl[0], c->wsl[1], c->wsl[2], c->wsl[3], nf, p);
+      HIPCHK(hipGetLastError());
+    }
+    return AMRVAC_FV_OK;
+  }
   fvc::SweepArgs a = base_args(c);
   a.src = c->slot[psa];
   a.dst = c->slot[psb];

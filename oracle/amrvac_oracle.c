@@ -2072,7 +2072,8 @@ static void fix_conserve(orc_sim* s, int slot);
 int orc_advect1(orc_sim* s, int method, double dtfactor, double dt,
                 double cmax_global, int psa, int psb) {
   if (psa < 0 || psa > 3 || psb < 0 || psb > 3 || psa == psb) return 1;
-  if (s->ct && method == ORC_FS_HANCOCK) return 2; /* the predictor does not advance the face field */
+  /* with constrained transport hancock advances the cells only; the face
+     field of psb stays as the stage's initial combination left it */
   const double qdt = dtfactor * dt;
   int crash = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : crash) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())

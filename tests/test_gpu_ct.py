@@ -33,6 +33,7 @@ CASES = [
     ("3step_hll_cada_ct_average", "tvdlf", "superbee", "fivestep"),
     ("3step_hll_cada_ct", "hll", "vanleer", "twostep"),          # predictor-corrector, HLL predictor
     ("3step_hlld_cada_ct", "hlld", "minmod", "onestep"),
+    ("3step_hll_cada_ct_average", "tvdlf", "minmod", "twostep"),  # Hancock predictor
 ]
 
 
