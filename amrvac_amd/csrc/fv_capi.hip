@@ -1148,7 +1148,7 @@ static int stage_core(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
 
 // One constrained-transport stage between slots (advect1 with stagger_grid,
 // uniform periodic 2-D level): face fluxes, edge fields, face and cell
-// updates, then the new face field into psb's slot.  ps's face field feeds
+// updates; the new face field replaces psb's (an array swap).  ps's face field feeds
 // the HLLD star states; when psb is ps it holds the stage's initial
 // combination.
 static int ct_stage(amrvac_fv_ctx* c, int method, double dtfactor, double dt, int psa, int psb, int comb_slot,
@@ -1201,8 +1201,9 @@ static int ct_stage(amrvac_fv_ctx* c, int method, double dtfactor, double dt, in
                                              c->wsl[psb], c->wsl[0], bx_mode, c->d_ctf, c->d_wsn, c->fp, c->stream);
   if (e == hipErrorInvalidValue) return fail(AMRVAC_FV_EINVAL, "no constrained-transport kernel for this scheme");
   if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("ct stage: ") + hipGetErrorString(e));
-  HIPCHK(hipMemcpyAsync(c->wsl[psb], c->d_wsn, (size_t)c->nb * c->nws * c->cells * sizeof(double),
-                        hipMemcpyDeviceToDevice, c->stream));
+  // the update kernel wrote every upper face of every block: the buffer
+  // becomes psb's face field and psb's old array the next stage's buffer
+  std::swap(c->wsl[psb], c->d_wsn);
   if (crash_out) {
     int h = 0;
     HIPCHK(hipMemcpyAsync(&h, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));

@@ -1,7 +1,7 @@
 // Constrained transport on a uniform periodic 2-D level (ndir = 3, ideal MHD
 // with the staggered face field, typedivbfix = 'ct').
 //
-// One stage is three launches:
+// One stage is two launches:
 //   ct_face_kernel    every face flux of the block plus transverse_ghost_cells
 //                     rows (finite_volume src/mod_finite_volume.t:183-190 with
 //                     mhd_modify_wLR :6338-6345), with the face data the edge
@@ -11,8 +11,8 @@
 //                     _contact / _hll :6951-7479), the face-field update, the
 //                     flux-divergence update of the cells, and the cell field
 //                     from the faces (mhd_face_to_center :7586-7648);
-//   a copy of the new face field into its slot (the shared face of two
-//                     blocks is computed by both; each stores its upper faces).
+//   the new face field then replaces psb's (the shared face of two blocks
+//                     is computed by both; each stores its upper faces).
 //
 // Face field layout: ws[nb][2][16][16], component d of cell (i, j) is the
 // field on the upper d face of that cell; a face buffer holds per block and
