@@ -76,6 +76,7 @@ class Par:
     small_pressure: float = 0.0
     small_density: float = 0.0
     check_small_values: bool = True
+    fix_small_values: bool = False
     precision: str = "fp64"        # 'fp64' | 'mixed' (device backend only)
 
     @property

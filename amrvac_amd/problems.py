@@ -357,3 +357,75 @@ def blast_wave_blocks(forest, blocks, gamma: float = 5.0 / 3.0, rho0: float = 1.
     p = np.where(r2 < radius ** 2, p_in, p_out)
     comps = [rho] + [np.zeros_like(rho)] * nd + [p]
     return hd_to_conserved(np.stack(comps, axis=1), gamma, nd)
+
+
+def kelvin_helmholtz_2d_blocks(forest, blocks, gamma: float = 5.0 / 3.0, vextra: float = 0.0) -> np.ndarray:
+    """Kelvin-Helmholtz shear layers of tests/hd/Kelvin_Helmholtz_2D/mod_usr.t
+    (iprob=1): rho=10 for 0.25<=y<=0.75 else 1, p=2.5, vx=+-0.5 with linear
+    ramps of half-width 0.025 across y=0.25 and y=0.75, vy=0.01 sin(4 pi x)
+    times two Gaussians of variance 0.00125 centred on the layers."""
+    x = forest.cell_centers(blocks)
+    y = x[:, 1]
+    rhodens, rholight, kx, pint, dlin, sigma = 10.0, 1.0, 4.0 * DPI, 2.5, 0.025, 0.00125
+    rho = np.where((y < 0.25) | (y > 0.75), rholight, rhodens)
+    p = pint * np.ones_like(y)
+    vx = np.zeros_like(y)
+    vx = np.where((y <= 0.25 - dlin) | (y >= 0.75 + dlin), vextra - 0.5, vx)
+    vx = np.where((y >= 0.25 + dlin) & (y <= 0.75 - dlin), vextra + 0.5, vx)
+    ramp1 = (y - (0.25 - dlin)) * ((vextra + 0.5) - (vextra - 0.5)) / (2.0 * dlin) + (vextra - 0.5)
+    vx = np.where((y > 0.25 - dlin) & (y < 0.25 + dlin), ramp1, vx)
+    ramp2 = (y - (0.75 - dlin)) * ((vextra - 0.5) - (vextra + 0.5)) / (2.0 * dlin) + (vextra + 0.5)
+    vx = np.where((y > 0.75 - dlin) & (y < 0.75 + dlin), ramp2, vx)
+    vy = 0.01 * np.sin(kx * x[:, 0]) * (np.exp(-0.5 * (y - 0.25) ** 2 / sigma)
+                                        + np.exp(-0.5 * (y - 0.75) ** 2 / sigma))
+    return hd_to_conserved(np.stack([rho, vx, vy, p], axis=1), gamma, 2)
+
+
+def mhd_kelvin_helmholtz_3d_blocks(forest, blocks, gamma: float = 5.0 / 3.0, glm: bool = False) -> np.ndarray:
+    """MHD Kelvin-Helmholtz jet of tests/mhd/Kelvin_Helmholtz_3D/mod_usr.t
+    (Keppens & Toth): rho=1, p=1, B=(0.129, 0, 0), vx=0.645 tanh((r-0.5)/0.05)
+    with r the distance from the x axis, and a 0.01 perturbation of (vy, vz)
+    localised on the jet surface; nw=8, plus psi=0 with GLM cleaning."""
+    x = forest.cell_centers(blocks)
+    r = np.sqrt(x[:, 1] ** 2 + x[:, 2] ** 2)
+    phi = np.arctan2(x[:, 2], x[:, 1])
+    mpol, k1 = 1.0, 2.0 * DPI / (forest.xprobmax[0] - forest.xprobmin[0])
+    rjet, qv, b0, dv = 0.5, 0.645, 0.129, 0.01
+    width = 0.1 * rjet
+    one = np.ones_like(r)
+    vx = qv * np.tanh((r - rjet) / width)
+    env = dv * np.exp(-((r - rjet) / (4.0 * width)) ** 2) * np.cos(mpol * phi) * np.sin(k1 * x[:, 0])
+    vy = env * np.cos(phi)
+    vz = env * np.sin(phi)
+    comps = [one, vx, vy, vz, one, b0 * one, 0.0 * one, 0.0 * one] + ([0.0 * one] if glm else [])
+    return mhd_to_conserved(np.stack(comps, axis=1), gamma, 3)
+
+
+def mhd_blast_wave_3d_blocks(forest, blocks, gamma: float = 5.0 / 3.0, busr: float = 1.0,
+                             glm: bool = False) -> np.ndarray:
+    """MHD blast wave of tests/mhd/blast_wave_Cartesian_3D/mod_usr.t: rho=1,
+    v=0, p=100 inside r<0.2 around the domain centre and 1 outside, uniform
+    B=(Busr, Busr, Busr); nw=8, plus psi=0 with GLM cleaning."""
+    x = forest.cell_centers(blocks)
+    r2 = np.zeros_like(x[:, 0])
+    for d in range(3):
+        c = (forest.xprobmin[d] + forest.xprobmax[d]) * 0.5
+        r2 = r2 + (x[:, d] - c) ** 2
+    one = np.ones_like(r2)
+    p = np.where(r2 < 0.2 ** 2, 100.0, 1.0)
+    comps = [one, 0.0 * one, 0.0 * one, 0.0 * one, p, busr * one, busr * one, busr * one]
+    comps += [0.0 * one] if glm else []
+    return mhd_to_conserved(np.stack(comps, axis=1), gamma, 3)
+
+
+def orszag_tang_2d_blocks(forest, blocks, gamma: float = 5.0 / 3.0, glm: bool = False) -> np.ndarray:
+    """Orszag-Tang vortex of tests/mhd/Orszag_Tang_2D/mod_usr.t with two
+    vector components (ndir=2: nw=6, plus psi=0 with GLM cleaning)."""
+    x = forest.cell_centers(blocks)
+    rho0 = 25.0 / 36.0 / DPI
+    p0 = 5.0 / 12.0 / DPI
+    b0 = 1.0 / np.sqrt(4.0 * DPI)
+    one = np.ones_like(x[:, 0])
+    comps = [rho0 * one, -np.sin(2.0 * DPI * x[:, 1]), np.sin(2.0 * DPI * x[:, 0]), p0 * one,
+             -b0 * np.sin(2.0 * DPI * x[:, 1]), b0 * np.sin(4.0 * DPI * x[:, 0])] + ([0.0 * one] if glm else [])
+    return mhd_to_conserved(np.stack(comps, axis=1), gamma, 2)

@@ -72,6 +72,7 @@ struct orc_sim {
   double* bflux;   /* [nb][ndim][2][nw][nface] scaled boundary face fluxes */
   int bc[6];
   double* w[4];  /* ps, ps1, ps2, ps3 */
+  int cur_ib;      /* block of a per-block copy inside advect1 (-1 otherwise) */
   int crash;
   int ftype[3][MAXW];
   double inv_gamma_1, gamma_1;
@@ -84,6 +85,8 @@ int orc_nw(const orc_params* p) {
   if (p->phys == ORC_PHYS_HD) return p->ndir + 2;
   return 2 * p->ndir + 2 + (p->divb == ORC_DIVB_GLM ? 1 : 0);
 }
+
+static int dir_nidx(const orc_sim* s, const int* i);
 
 static inline long IX(const orc_sim* s, int i, int j, int k) {
   return (long)i + (long)s->nx[0] * ((long)j + (long)s->nx[1] * (long)k);
@@ -883,8 +886,9 @@ static void add_source_glm(const orc_sim* s, double qdt, const double* wCT,
       }
 }
 
-/* add_source_linde src/mhd/mod_mhd_phys.t:5599-5664 (single level: no level
-   jumps, so ixp = ixO) */
+/* add_source_linde src/mhd/mod_mhd_phys.t:5599-5664.  div B on ixO^LADD1;
+   the update range ixp is ixO, retreated by one cell layer from every face
+   whose neighbour is coarser or finer (the level-jump rule :5616-5641) */
 static void add_source_linde(const orc_sim* s, double qdt, const double* wCT,
                              double* w) {
   (void)qdt;
@@ -899,6 +903,17 @@ static void add_source_linde(const orc_sim* s, double qdt, const double* wCT,
   for (int k = (s->p.ndim > 2 ? lo[2] - 1 : 0); k <= (s->p.ndim > 2 ? hi[2] + 1 : 0); ++k)
     for (int j = (s->p.ndim > 1 ? lo[1] - 1 : 0); j <= (s->p.ndim > 1 ? hi[1] + 1 : 0); ++j)
       for (int i = lo[0] - 1; i <= hi[0] + 1; ++i) divb[IX(s, i, j, k)] = divb_c(s, wCT, i, j, k);
+  if (s->cur_ib >= 0)
+    for (int d = 0; d < s->p.ndim; ++d)
+      for (int side = 0; side < 2; ++side) {
+        int v[3] = {0, 0, 0};
+        v[d] = side == 0 ? -1 : 1;
+        const int t = s->ntype[(long)s->cur_ib * s->nnb + dir_nidx(s, v)];
+        if (t == 2 || t == 4) {
+          if (side == 0) lo[d] = lo[d] + 1;
+          else hi[d] = hi[d] - 1;
+        }
+      }
   double invds2 = 0.0;
   for (int d = 0; d < s->p.ndim; ++d) invds2 = invds2 + 1.0 / (s->dx[d] * s->dx[d]);
   for (int idim = 0; idim < s->p.ndim; ++idim) {
@@ -1888,6 +1903,7 @@ int orc_create(const orc_params* p, int nblocks, const double* dx,
                const int* neighbor, const double* xlohi, const int* bc,
                orc_sim** out) {
   orc_sim* s = (orc_sim*)calloc(1, sizeof(orc_sim));
+  s->cur_ib = -1;
   if (!s) return 1;
   s->p = *p;
   if (p->ndim < 1 || p->ndim > 3 || p->ndir < p->ndim || p->ndir > 3) { free(s); return 2; }
@@ -2084,6 +2100,7 @@ int orc_advect1(orc_sim* s, int method, double dtfactor, double dt,
     /* the block's own cell size (dxlevel of its level) */
     orc_sim sb = *s;
     for (int d = 0; d < 3; ++d) sb.dx[d] = s->bdx[ib * 3 + d];
+    sb.cur_ib = ib;
     double* bf = (s->amr && s->fix_conserve && method != ORC_FS_HANCOCK)
                      ? s->bflux + (long)ib * s->p.ndim * 2 * s->nw * s->nface
                      : NULL;  /* hancock stores no fluxes */

@@ -55,21 +55,23 @@ def test_blast_wave_amr_golden_log(ndim, scheme):
     assert logs_match(got, ref, rtol=1e-7, atol=1e-8)
 
 
-def test_blast_wave_3d_threestep_log_not_reproduced():
-    """bw_3d_3step_hll_cada.log is NOT reproduced: mean(m^2) is 0.4% off
-    from the first logged step while the 2-D log of the same scheme and the
-    3-D two-step log match to 1e-8.  Recorded as an open discrepancy; the
-    directory (tests/hd/blast_wave_Cartesian_3D) is in none of the
-    reference's own regression lists, so its logs are not maintained against
-    the current code."""
+HD_3D_STALE = ("tests/hd/blast_wave_Cartesian_3D is in none of the reference's regression lists (tests/Makefile "
+               "HD_DIRS); its multi-stage logs differ by 0.1-1% in the mean squares from the first row, while the "
+               "maintained 3-D logs of the same schemes (tests/mhd/blast_wave_Cartesian_3D, "
+               "tests/mhd/Kelvin_Helmholtz_3D, test_oracle_golden_amr.py) match to 1e-8")
+
+
+@pytest.mark.xfail(reason=HD_3D_STALE, strict=False)
+def test_blast_wave_3d_threestep_log():
+    """bw_3d_3step_hll_cada.log of tests/hd/blast_wave_Cartesian_3D, first
+    three rows; expected to fail (see HD_3D_STALE): mean rho and mean e
+    agree to 1e-7, mean m^2 is 0.4% off."""
     par, forest, ap = blast_setup(3, "3step_hll_cada", 0.003)
     ti = AMRIntegrator(par, forest, ap, lambda p, f: OracleFV(p, f, threads=4), blast_init)
     st = ti.run()
     ref = read_log(os.path.join(GOLDEN, "bw_3d_3step_hll_cada.log"))[:3]
     got = np.array(st.log)
-    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-8)
-    assert rel[:, [1, 5]].max() < 1e-7  # mean rho and mean e agree
-    assert rel[:, 7].max() > 1e-3       # mean m^2 does not
+    assert logs_match(got, ref)
 
 
 # ---------------------------------------------------------------------------

@@ -207,14 +207,16 @@ def test_fourstep_blast_2d_golden_log(scheme):
 
 
 @pytest.mark.parametrize("scheme", sorted(FOURSTEP))
-def test_fourstep_blast_3d_logs_not_reproduced(scheme):
-    """The 3-D logs of these schemes are not reproduced, like the 3-D
-    threestep log: mean rho and mean e agree to 1e-7 (conservation) while
-    the mean squares differ by 0.1-1% from the first logged step; the 2-D
-    logs of the same schemes and the 3-D two-step log match to 1e-8."""
+def test_fourstep_blast_3d_logs(scheme):
+    """The 3-D logs of tests/hd/blast_wave_Cartesian_3D for these schemes,
+    first three rows; expected to fail like the threestep one (see
+    test_amr.HD_3D_STALE): mean rho and mean e agree to 1e-7 while the mean
+    squares differ by 0.1-1%."""
     from oracle.oracle import OracleFV
+    from test_amr import HD_3D_STALE
     _, _, got = fourstep_blast(3, scheme, 0.003, lambda p, f: OracleFV(p, f, threads=4))
     ref = read_log(os.path.join(GOLDEN, f"bw_3d_{scheme}.log"))[:3]
     rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-8)
     assert rel[:, [1, 5]].max() < 1e-7
-    assert rel[:, 7:10].max() > 1e-3
+    if not logs_match(got, ref):
+        pytest.xfail(HD_3D_STALE)

@@ -361,6 +361,58 @@ def test_weno5_exact_for_quadratics():
     assert check_weno5_polynomials() == ""
 
 
+def check_fix_limiter(lib_path=None) -> str:
+    """fix_limiter (src/limiter/mod_weno.t:100-133): where the WENO5 left or
+    right face state fails the small-value check, both fall back to the
+    first-order states, the two cells next to the face.  With rho=1 and v=0
+    the TVDLF momentum flux is (p_L + p_R)/2 exactly, so the face states can
+    be read off the flux."""
+    msgs = []
+    gamma = 5.0 / 3.0
+
+    def mom_flux(p_line, small_pressure, v=0.0, rho_line=None):
+        par = Par(physics="hd", ndim=2, ndir=2, gamma=gamma, limiter="weno5", small_pressure=small_pressure)
+        sim = _sim(par, lib_path=lib_path)
+        n = len(p_line)
+        rho = np.ones(n) if rho_line is None else np.asarray(rho_line)
+        prims = np.stack([rho, v * np.ones(n), np.zeros(n), np.asarray(p_line)], axis=1)
+        f, _, _ = sim.face_flux(FS["tvdlf"], 0, prims)
+        return f
+    # a pressure minimum on the face: cell averages of p0 + (x - x_f)^2
+    # (h = 1), whose WENO5 face value is p0 = 0.1 exactly; every cell holds
+    # at least 0.1 + 1/4 + 1/12
+    xs = np.arange(6) - 2.5
+    p_avg = 0.1 + xs ** 2 + 1.0 / 12.0
+    f_weno = mom_flux(p_avg, 0.05)[1]
+    f_fixed = mom_flux(p_avg, 0.2)[1]
+    if abs(f_weno - 0.1) > 1e-12:
+        msgs.append(f"unflagged WENO5 face pressure {f_weno} != 0.1")
+    if abs(f_fixed - p_avg[2]) > 1e-12:
+        msgs.append(f"flagged face: momentum flux {f_fixed} != cell pressure {p_avg[2]}")
+    # the same for density: rho minimum on the face, v = 1 (mass flux rho_face)
+    g_weno = mom_flux(np.ones(6), 0.0, v=1.0, rho_line=p_avg)[0]
+    par = Par(physics="hd", ndim=2, ndir=2, gamma=gamma, limiter="weno5", small_density=0.2)
+    sim = _sim(par, lib_path=lib_path)
+    prims = np.stack([p_avg, np.ones(6), np.zeros(6), np.ones(6)], axis=1)
+    g_fixed = sim.face_flux(FS["tvdlf"], 0, prims)[0][0]
+    if abs(g_weno - 0.1) > 1e-12 or abs(g_fixed - p_avg[2]) > 1e-12:
+        msgs.append(f"density fallback: mass flux {g_weno} / {g_fixed}")
+    # only the right state fails (its WENO5 value is 0.297, the left one
+    # 1.777, every cell >= 0.6): both still fall back
+    p_line = [2.4, 0.75, 2.2, 0.6, 1.25, 1.35]
+    wl, wr = weno5_face(p_line, lib_path=lib_path)
+    f_one = mom_flux(p_line, 0.5)[1]
+    if not (wr < 0.5 < wl and min(p_line) > 0.5):
+        msgs.append(f"right-only case lost: {wl} {wr}")
+    if abs(f_one - 0.5 * (2.2 + 0.6)) > 1e-12:
+        msgs.append(f"right-only failure: momentum flux {f_one} != {0.5 * (2.2 + 0.6)}")
+    return "; ".join(msgs)
+
+
+def test_weno5_fix_limiter_falls_back_to_first_order():
+    assert check_fix_limiter() == ""
+
+
 # --------------------------------------------------------------------------
 # Whole-step invariants
 # --------------------------------------------------------------------------
@@ -695,7 +747,25 @@ def test_ct_keeps_div_b_zero():
     assert check_ct_divb() == ""
 
 
+def check_linde_level_jump(lib_path=None) -> str:
+    """Linde's source retreats one cell layer from faces with a coarser or
+    finer neighbour (src/mhd/mod_mhd_phys.t:5616-5641): the maintained 3-D
+    MHD blast log (levels 2 and 3 from the start) is reproduced to 1e-8
+    with the rule and leaves the compare_logs band without it."""
+    from amrvac_amd.amr import AMRIntegrator
+    from amrvac_amd.host import logs_match
+    from golden_setups import golden, setup
+    par, forest, ap, init = setup("bw3d", "3step_hll_cada")
+    ti = AMRIntegrator(par, forest, ap, lambda p, f: OracleFV(p, f, threads=8, lib_path=lib_path), init)
+    got = np.array(ti.run().log)
+    ref = golden("bw3d", "3step_hll_cada")
+    if got.shape != ref.shape or not logs_match(got, ref):
+        return "bw_3d_3step_hll_cada.log not reproduced"
+    return ""
+
+
 ALL_CHECKS = {
+    "linde_level_jump": check_linde_level_jump,
     "ct_divb": check_ct_divb,
     "ct_golden_contact": lambda p: check_ct_golden("3step_hll_cada_ct", p),
     "ct_golden_average": lambda p: check_ct_golden("3step_hll_cada_ct_average", p),
@@ -718,6 +788,7 @@ ALL_CHECKS = {
     "glm_interface": check_glm_interface_state,
     "limiters": check_limiters,
     "weno5_polynomials": check_weno5_polynomials,
+    "weno5_fix_limiter": check_fix_limiter,
     "glm_decay": check_glm_decay,
     "conservation": check_conservation_and_uniform_states,
 }

@@ -89,6 +89,14 @@ MUTANTS = {
                                "wnew[iw * n + c] + fC[i][iw] - fC[i][iw]", 0, "sod_hll_cada3"),
     "hancock_predictor_sign": ("dxinv * (fL[iw] - fR[iw])", "dxinv * (fR[iw] - fL[iw])", 0,
                                "smooth_twostep_vanleer"),
+    "weno5_fix_limiter_dropped": ("if (check_w_prim_fails(s, tl[i]) || check_w_prim_fails(s, tr[i])) continue;",
+                                  "if (check_w_prim_fails(s, tl[i]) || check_w_prim_fails(s, tr[i])) {}", 0,
+                                  "weno5_fix_limiter"),
+    "weno5_fix_limiter_left_only": ("if (check_w_prim_fails(s, tl[i]) || check_w_prim_fails(s, tr[i])) continue;",
+                                    "if (check_w_prim_fails(s, tl[i])) continue;", 0, "weno5_fix_limiter"),
+    "weno5_fix_limiter_pressure_unchecked": ("if (w[s->e_] < s->p.small_pressure) return 1;",
+                                             "if (w[s->e_] < -1.0e300) return 1;", 0, "weno5_fix_limiter"),
+    "linde_level_jump_rule_dropped": ("if (t == 2 || t == 4) {", "if (t == -7) {", 0, "linde_level_jump"),
     "tvdlf_no_dissipation": ("double fac = -0.5 * s->p.tvdlfeps * cmax;", "double fac = 0.0 * s->p.tvdlfeps * cmax;",
                              0, "sod_tvdlf_weno5"),
 }
