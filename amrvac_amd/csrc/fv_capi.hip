@@ -91,6 +91,8 @@ struct amrvac_fv_ctx {
   double* d_wts = nullptr;
   int nrt = 0, npt = 0, nft = 0;
   std::vector<int> h_level;
+  int crash_seen = 0;        // small-value flag copied with the last dt
+  bool crash_fresh = false;  // no stage launched since that copy
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
   size_t nev = 0;
@@ -101,7 +103,7 @@ namespace fv {
 template <class P>
 __global__ void __launch_bounds__(NTHREADS) getdt_kernel(const double* __restrict__ w, int nblocks, int bnz,
                                                          FVParams fp, const int* __restrict__ blevel,
-                                                         double* out) {
+                                                         double* out, int* crash) {
   const int b = blockIdx.x;
   if (b >= nblocks) return;
   if (blevel) {  // the block's level: 1/dx scales by 2^level exactly
@@ -111,10 +113,14 @@ __global__ void __launch_bounds__(NTHREADS) getdt_kernel(const double* __restric
   const size_t cells = (size_t)BX * BX * bnz;
   const double* wb = w + (size_t)b * P::NW * cells;
   double maxtot = -1.0e99, cmaxb = 0.0;
+  int small = 0;
   for (size_t c = threadIdx.x; c < cells; c += NTHREADS) {
     double cons[P::NW], pr[P::NW];
 #pragma unroll
     for (int iw = 0; iw < P::NW; ++iw) cons[iw] = wb[iw * cells + c];
+    // getdt_courant converts with phys_to_primitive (src/mod_dt.t:195),
+    // which checks the cell with fix_small_values
+    if (fp.fix_small && small_cons<P>(cons, fp)) small = 1;
     to_prim<P>(cons, pr, fp);
     // getdt_courant, maxsum, slab_uniform (src/mod_dt.t:203-221)
     double cm = cell_cmax<P, 0>(pr, fp);
@@ -132,6 +138,7 @@ __global__ void __launch_bounds__(NTHREADS) getdt_kernel(const double* __restric
     }
     maxtot = fmax(maxtot, tot);
   }
+  if (small) *crash = 1;
   // wave64 shuffle max, then across the 4 waves through LDS
   for (int off = 32; off > 0; off >>= 1) {
     maxtot = fmax(maxtot, __shfl_xor(maxtot, off, 64));
@@ -149,8 +156,10 @@ __global__ void __launch_bounds__(NTHREADS) getdt_kernel(const double* __restric
   }
 }
 
+// out[0] = max of the per-block maxtot, out[1] = max cmax, out[2] = the
+// context's small-value flag (it reaches the host with the dt copy)
 __global__ void __launch_bounds__(NTHREADS) reduce_max2_kernel(const double* __restrict__ in, int n,
-                                                               double* out) {
+                                                               double* out, const int* __restrict__ crash) {
   double m = -1.0e99, c = 0.0;
   for (int i = threadIdx.x; i < n; i += NTHREADS) {
     m = fmax(m, in[2 * i]);
@@ -168,6 +177,7 @@ __global__ void __launch_bounds__(NTHREADS) reduce_max2_kernel(const double* __r
     for (int i = 1; i < NTHREADS / 64; ++i) { s[0][0] = fmax(s[0][0], s[0][i]); s[1][0] = fmax(s[1][0], s[1][i]); }
     out[0] = s[0][0];
     out[1] = s[1][0];
+    out[2] = *crash ? 1.0 : 0.0;
   }
 }
 
@@ -517,8 +527,8 @@ __global__ void __launch_bounds__(NTHREADS) fixcons_kernel(double* w, int nw, in
 
 template <class P>
 hipError_t launch_getdt(const double* w, int nb, int bnz, const FVParams& fp, const int* blevel, double* out,
-                        hipStream_t s) {
-  hipLaunchKernelGGL((getdt_kernel<P>), dim3(nb), dim3(NTHREADS), 0, s, w, nb, bnz, fp, blevel, out);
+                        int* crash, hipStream_t s) {
+  hipLaunchKernelGGL((getdt_kernel<P>), dim3(nb), dim3(NTHREADS), 0, s, w, nb, bnz, fp, blevel, out, crash);
   return hipGetLastError();
 }
 
@@ -720,8 +730,8 @@ hipError_t with_phys(const amrvac_fv_ctx* c, Args... args) {
 template <class P>
 struct GetdtOp {
   static hipError_t run(const double* w, int nb, int bnz, const fvc::FVParams* fp, const int* blevel, double* out,
-                        hipStream_t s) {
-    return fv::launch_getdt<P>(w, nb, bnz, *fp, blevel, out, s);
+                        int* crash, hipStream_t s) {
+    return fv::launch_getdt<P>(w, nb, bnz, *fp, blevel, out, crash, s);
   }
 };
 
@@ -761,6 +771,10 @@ int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
   if (c.boundspeed < 1 || c.boundspeed > 3) return fail(AMRVAC_FV_EINVAL, "boundspeed must be 1, 2 or 3");
   if (c.precision < 0 || c.precision > 2) return fail(AMRVAC_FV_EINVAL, "precision must be 0, 1 or 2");
   if (c.gamma <= 0.0 || c.gamma == 1.0) return fail(AMRVAC_FV_EINVAL, "gamma must be > 0 and != 1");
+  if (c.fix_small_values != 0 && c.fix_small_values != 1)
+    return fail(AMRVAC_FV_EINVAL, "fix_small_values must be 0 or 1");
+  if (c.fix_small_values && c.physics == 2 && c.divb == 8)
+    return fail(AMRVAC_FV_EINVAL, "fix_small_values is not provided with constrained transport");
   amrvac_fv_ctx* x = new amrvac_fv_ctx();
   x->cfg = c;
   x->nw = c.physics == 1 ? c.ndir + 2 : 2 * c.ndir + 2 + (c.divb == 1 ? 1 : 0);
@@ -896,13 +910,15 @@ static int alloc_level(amrvac_fv_ctx* c, int nblocks, int capacity, const double
   if (hipMalloc(&c->d_nbr, sizeof(int) * nblocks * c->nnb) != hipSuccess ||
       hipMalloc(&c->d_crash, sizeof(int)) != hipSuccess ||
       hipMalloc(&c->d_red, sizeof(double) * (size_t)nblocks * 2 * (c->nw > 1 ? c->nw : 1)) != hipSuccess ||
-      hipMalloc(&c->d_red2, sizeof(double) * 2) != hipSuccess ||
+      hipMalloc(&c->d_red2, sizeof(double) * 3) != hipSuccess ||
       hipMalloc(&c->d_stage, sizeof(double) * (size_t)nblocks * c->nw * c->cells_g) != hipSuccess) {
     free_level(c);
     return fail(AMRVAC_FV_ENOMEM, "work buffer allocation failed");
   }
   HIPCHK(hipMemcpyAsync(c->d_nbr, neighbor, sizeof(int) * nblocks * c->nnb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->d_crash, 0, sizeof(int), c->stream));
+  c->crash_seen = 0;
+  c->crash_fresh = false;
   // scheme constants that depend on the mesh
   fvc::FVParams& fp = c->fp;
   const amrvac_fv_config& g = c->cfg;
@@ -918,6 +934,7 @@ static int alloc_level(amrvac_fv_ctx* c, int nblocks, int capacity, const double
   fp.small_pressure = g.small_pressure;
   fp.small_density = g.small_density;
   fp.check_small = g.check_small_values;
+  fp.fix_small = g.fix_small_values;
   fp.invds2 = 0.0;
   fp.mindx = c->dx[0];
   for (int d = 0; d < 3; ++d) {
@@ -1042,13 +1059,18 @@ static bool ensure_buf(T** buf, size_t* cap, size_t n) {
 
 static int getdt_core(amrvac_fv_ctx* c, const double* u, double courantpar, double* dtmin_out, double* cmax_out) {
   hipError_t e = with_phys<GetdtOp>(c, u, c->nb, c->bnz, (const fvc::FVParams*)&c->fp, (const int*)c->d_level,
-                                    c->d_red, c->stream);
+                                    c->d_red, c->d_crash, c->stream);
   if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("getdt launch: ") + hipGetErrorString(e));
-  hipLaunchKernelGGL(fv::reduce_max2_kernel, dim3(1), dim3(fvc::NTHREADS), 0, c->stream, c->d_red, c->nb, c->d_red2);
+  hipLaunchKernelGGL(fv::reduce_max2_kernel, dim3(1), dim3(fvc::NTHREADS), 0, c->stream, c->d_red, c->nb, c->d_red2,
+                     (const int*)c->d_crash);
   HIPCHK(hipGetLastError());
-  double r[2];
+  double r[3];
   HIPCHK(hipMemcpyAsync(r, c->d_red2, sizeof(r), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  // the small-value flag as of this reduction; valid until the next launch
+  // that may set it
+  c->crash_seen = r[2] != 0.0 ? 1 : 0;
+  c->crash_fresh = true;
   // min over blocks of courantpar/maxtot equals courantpar/max(maxtot): the
   // correctly rounded quotient is monotone in the divisor
   const double dt = r[0] > kSmall ? courantpar / r[0] : kBig;
@@ -1109,6 +1131,7 @@ static int stage_core(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
     }
   }
   const fvc::KernelKey k = make_key(c, riem);
+  c->crash_fresh = false;
   if (crash_out) HIPCHK(hipMemsetAsync(c->d_crash, 0, sizeof(int), c->stream));
   const int grid = 8 * a.per_xcd;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1190,6 +1213,7 @@ static int ct_stage(amrvac_fv_ctx* c, int method, double dtfactor, double dt, in
   const double qdt = dtfactor * dt;
   a.qdt = qdt;
   for (int d = 0; d < 3; ++d) a.dq[d] = -qdt / c->dx[d];
+  c->crash_fresh = false;
   if (crash_out) HIPCHK(hipMemsetAsync(c->d_crash, 0, sizeof(int), c->stream));
   const bool exact = c->cfg.precision == 2;
   const double* wbase = comb_slot >= 0 ? c->wsl[comb_slot] : nullptr;
@@ -1254,6 +1278,24 @@ int amrvac_fv_download(amrvac_fv_ctx* c, int slot, double* w) {
                                       : hf.data() + ((size_t)b * c->nws + (iw - c->nw)) * c->cells_g;
       std::memcpy(w + ((size_t)b * nwt + iw) * c->cells_g, from, c->cells_g * sizeof(double));
     }
+  return AMRVAC_FV_OK;
+}
+
+int amrvac_fv_crash(amrvac_fv_ctx* c, int clear, int* crash_out) {
+  if (!c || !crash_out) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->d_crash) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  if (!c->crash_fresh) {
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->crash_seen = h ? 1 : 0;
+    c->crash_fresh = true;
+  }
+  *crash_out = c->crash_seen;
+  if (clear) {
+    HIPCHK(hipMemsetAsync(c->d_crash, 0, sizeof(int), c->stream));
+    c->crash_seen = 0;
+  }
   return AMRVAC_FV_OK;
 }
 
@@ -1632,6 +1674,9 @@ int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt,
 int amrvac_fv_slot_copy(amrvac_fv_ctx* c, int slot, void* dptr, int to_slot) {
   if (!c || !check_slot(slot) || !dptr) return fail(AMRVAC_FV_EINVAL, "bad argument");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  // a slot of a constrained-transport context also holds its face field,
+  // which a cell-only copy would leave stale beside the restored cells
+  if (c->nws > 0) return fail(AMRVAC_FV_EINVAL, "slot copies move cell variables only: not with constrained transport");
   const size_t n = (size_t)c->nb * c->nw * c->cells * sizeof(double);
   if (to_slot) HIPCHK(hipMemcpyAsync(c->slot[slot], dptr, n, hipMemcpyDeviceToDevice, c->stream));
   else HIPCHK(hipMemcpyAsync(dptr, c->slot[slot], n, hipMemcpyDeviceToDevice, c->stream));

@@ -30,6 +30,7 @@ struct FVParams {
   double tvdlfeps;
   double small_pressure, small_density;
   int check_small;
+  int fix_small;       // fix_small_values (small_values_method 'error')
   double dx[3];
   double dxinv[3];     // 1/dx
   double rdelinv[3];   // 1/(cada3_radius*dx)^2, src/limiter/mod_limiter.t:254
@@ -164,6 +165,27 @@ __device__ __forceinline__ void to_prim(const double* c, double* w, const FVPara
   }
 }
 
+// phys_check_w on a conserved FP64 cell (hd_check_w src/hd/mod_hd_phys.t:
+// 648-674, mhd_check_w_origin src/mhd/mod_mhd_phys.t:1452-1473 with small_e
+// = small_pressure/(gamma-1)).  With fix_small_values and the default
+// small_values_method 'error' a flagged cell sets crash
+// (src/mod_small_values.t:25-87).
+template <class P>
+__device__ __forceinline__ bool small_cons(const double* c, const FVParams& fp) {
+  if (c[P::RHO] < fp.small_density) return true;
+  double msq = c[P::MOM] * c[P::MOM];
+#pragma unroll
+  for (int d = 1; d < P::NDIR; ++d) msq = msq + c[P::MOM + d] * c[P::MOM + d];
+  if constexpr (!P::MHD) {
+    return (fp.gamma - 1.0) * (c[P::E] - 0.5 * msq / c[P::RHO]) < fp.small_pressure;
+  } else {
+    double bsq = c[P::MAG] * c[P::MAG];
+#pragma unroll
+    for (int d = 1; d < P::NDIR; ++d) bsq = bsq + c[P::MAG + d] * c[P::MAG + d];
+    return c[P::E] - 0.5 * (msq / c[P::RHO] + bsq) < fp.small_pressure * fp.inv_gamma_1;
+  }
+}
+
 // hd_to_conserved src/hd/mod_hd_phys.t:677-700,
 // mhd_to_conserved_origin src/mhd/mod_mhd_phys.t:1562-1579
 template <class P, class R>
@@ -268,8 +290,12 @@ __device__ __forceinline__ void cbounds(const R* wLC, const R* wRC, const R* wLp
       }
       const R e = R(0.5) * (wLC[P::E] + wRC[P::E]);
       const R tmp1 = fdiv(mn, rho);
-      const R pth = (g - R(1)) * (e - fdiv(R(0.5) * msq, rho));
-      if (fp.check_small && pth < R(fp.small_pressure)) crash = 1;
+      R pth = (g - R(1)) * (e - fdiv(R(0.5) * msq, rho));
+      if (fp.fix_small) {
+        if (pth < R(fp.small_pressure)) pth = R(fp.small_pressure);
+      } else if (fp.check_small && pth < R(fp.small_pressure)) {
+        crash = 1;
+      }
       const R cs = fsqrt(fdiv(g * pth, rho));
       if constexpr (WANT_CMIN) {
         cmax = mx2(tmp1 + cs, R(0));

@@ -335,6 +335,10 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
       // advance the window and issue the lookahead plane k+NG
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) cons_k[iw] = Wc[0][iw];
+      // fix_small_values: phys_to_primitive checks every cell it converts
+      // (src/hd/mod_hd_phys.t:713-715, src/mhd/mod_mhd_phys.t:1750-1753);
+      // over the level, the blocks' own cells are every converted cell
+      if (fp.fix_small && small_cons<P>(cons_k, fp)) crash = 1;
 #pragma unroll
       for (int m = 0; m < NG - 1; ++m)
 #pragma unroll
@@ -349,6 +353,7 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
       for (int iw = 0; iw < NW; ++iw) sp[iw][ty + NG][tx + NG] = Wp[NG - 1][iw];
     } else {
       load_cons<P>(a.src, b, tx, ty, 0, 1, cons_k);
+      if (fp.fix_small && small_cons<P>(cons_k, fp)) crash = 1;
       R w[NW];
       prim_of<P, R>(cons_k, w, fp);
 #pragma unroll
@@ -511,6 +516,13 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
       if constexpr (NDIM == 3) hancock_dim(std::integral_constant<int, 2>(), zm, w0, zp);
     }
 
+    // ---- unphysical values after the flux update (finite_volume
+    //      :325-333, hancock :109-111: small_values_method 'error' sets
+    //      crash, after which the run stops).  The reconstructed face
+    //      states of the TVD limiters lie between their cells' values, so
+    //      the face checks of reconstruct_LR (:1213-1216) flag nothing more
+    if constexpr (!(HANCOCK && (P::GLM || P::LINDE)))  // hancock checks after its sources only
+      if (fp.fix_small && small_cons<P>(acc, fp)) crash = 1;
     // ---- sources: addsource2 -> mhd_add_source (src/mod_source.t:130-160,
     //      src/mhd/mod_mhd_phys.t:4565-4695) -----------------------------
     if constexpr (P::GLM) {
@@ -558,6 +570,9 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
       acc[P::E] = acc[P::E] + double(sp[P::MAG + 1][cy][cx]) * g;
     }
 
+    // the GLM / Linde sources check again (src/mhd/mod_mhd_phys.t:5529, 5662)
+    if constexpr (P::GLM || P::LINDE)
+      if (fp.fix_small && small_cons<P>(acc, fp)) crash = 1;
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) st_stream(acc[iw], a.dst + o + iw * cells);
     __syncthreads();  // LDS plane is rewritten next iteration

@@ -28,7 +28,7 @@ EXPORTS = ("amrvac_fv_init", "amrvac_fv_finalize", "amrvac_fv_set_stream", "amrv
            "amrvac_fv_sync", "amrvac_fv_sweep_timing", "amrvac_fv_last_error",
            "amrvac_fv_set_mesh_amr", "amrvac_fv_getbc", "amrvac_fv_set_fix_conserve", "amrvac_fv_adjoint1",
            "amrvac_fv_slot_copy", "amrvac_fv_stage", "amrvac_fv_getdt_state", "amrvac_fv_errest",
-           "amrvac_fv_regrid")
+           "amrvac_fv_regrid", "amrvac_fv_crash")
 
 
 class FVConfig(ctypes.Structure):
@@ -40,6 +40,7 @@ class FVConfig(ctypes.Structure):
         ("small_pressure", ctypes.c_double), ("small_density", ctypes.c_double),
         ("check_small_values", ctypes.c_int), ("nghost", ctypes.c_int),
         ("block_nx", ctypes.c_int * 3), ("precision", ctypes.c_int), ("type_ct", ctypes.c_int),
+        ("fix_small_values", ctypes.c_int),
     ]
 
 
@@ -92,6 +93,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                    ctypes.POINTER(ctypes.c_ubyte)]
     L.amrvac_fv_regrid.argtypes = [vp, ctypes.c_int, ip, dp, ip, ip, ip, ip, ip, dp, ip]
     L.amrvac_fv_slot_copy.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
+    L.amrvac_fv_crash.argtypes = [vp, ctypes.c_int, ip]
     _lib = L
     return L
 
@@ -126,6 +128,7 @@ def make_config(par: Par, mesh: UniformMesh, precision: str | None = None) -> FV
         c.block_nx[d] = mesh.block_nx[d] if d < par.ndim else 1
     c.precision = PREC[precision or par.precision]
     c.type_ct = TYPE_CT[par.type_ct] if par.stagger else 0
+    c.fix_small_values = 1 if par.fix_small_values else 0
     return c
 
 
@@ -162,7 +165,6 @@ class DeviceFV:
             dx = np.ascontiguousarray(mesh.dx, dtype=np.float64)
             nbr = np.ascontiguousarray(mesh.neighbor, dtype=np.int32)
             self._check(self.L.amrvac_fv_set_mesh(self.h, mesh.nblocks, _dp(dx), _ip(nbr), _dp(xl), _ip(bc)))
-        self._crash = False
 
     def _check(self, rc: int) -> None:
         if rc != 0:
@@ -300,26 +302,27 @@ class DeviceFV:
         k = np.ascontiguousarray(expr.consts + [0.0], dtype=np.float64)
         self._check(self.L.amrvac_fv_combine(self.h, dst, len(expr.ops), _ip(ops), len(expr.consts), _dp(k)))
 
-    def advect1(self, method, dtfactor, dt, cmax_global, psa, psb, comb, check_crash: bool = False) -> None:
+    def advect1(self, method, dtfactor, dt, cmax_global, psa, psb, comb) -> None:
         if comb is None:
             cs, ca, cb = -1, 0.0, 0.0
         elif comb[0] == "copy":
             cs, ca, cb = comb[1], 1.0, 0.0
         else:
             _, ca, cs, cb = comb
-        crash = ctypes.c_int(0)
         self._check(self.L.amrvac_fv_advect1(self.h, method, dtfactor, dt, cmax_global, psa, psb, cs, ca, cb,
-                                             ctypes.byref(crash) if check_crash else None))
-        if crash.value:
-            self._crash = True
+                                             None))
 
     def volume_average(self, slot: int) -> np.ndarray:
         out = np.zeros(2 * self.nw, dtype=np.float64)
         self._check(self.L.amrvac_fv_volume_average(self.h, slot, _dp(out)))
         return out
 
-    def crash(self) -> bool:
-        return self._crash
+    def crash(self, clear: bool = False) -> bool:
+        """The device's small-value flag (the reference's crash): set by any
+        stage or dt reduction since it was last cleared."""
+        flag = ctypes.c_int(0)
+        self._check(self.L.amrvac_fv_crash(self.h, 1 if clear else 0, ctypes.byref(flag)))
+        return bool(flag.value)
 
     # -- extras ---------------------------------------------------------------
     def sync(self) -> None:

@@ -60,6 +60,15 @@ typedef struct amrvac_fv_config {
   int type_ct;         /* with divb = 8 (constrained transport): 1 average,
                           2 uct_contact, 3 uct_hll (type_ct,
                           src/mhd/mod_mhd_phys.t:766-781) */
+  int fix_small_values; /* 0 or 1 (fix_small_values, methodlist) with the
+                          default small_values_method 'error': every cell a
+                          stage converts or updates is checked against
+                          small_density / small_pressure
+                          (src/mod_finite_volume.t:325-333) and a failure
+                          raises the crash flag; the HD cmaxmean bound
+                          speeds clip the pressure instead
+                          (src/hd/mod_hd_phys.t:1040-1045).  Not with
+                          constrained transport. */
 } amrvac_fv_config;
 
 typedef struct amrvac_fv_ctx amrvac_fv_ctx;
@@ -153,6 +162,17 @@ int amrvac_fv_set_fix_conserve(amrvac_fv_ctx* ctx, int on);
 int amrvac_fv_upload(amrvac_fv_ctx* ctx, int slot, const double* w_host);
 int amrvac_fv_download(amrvac_fv_ctx* ctx, int slot, double* w_host);
 
+/* Replaces: the crash flag of the reference (src/mod_global_parameters.t:
+   646) as the time loop reads it after advance (src/amrvac.t:311-318).
+   *crash_out = 1 when a small-value check of any stage, or of the dt
+   reduction, since the flag was last cleared would have set crash
+   (hd_get_pthermal with check_small_values, src/hd/mod_hd_phys.t:
+   1046-1062; the fix_small_values checks of the config).  The flag rides
+   the dt copy of amrvac_fv_getdt: read right after it, no synchronisation
+   is needed; after a stage the call waits for the stream.  clear != 0
+   resets it. */
+int amrvac_fv_crash(amrvac_fv_ctx* ctx, int clear, int* crash_out);
+
 /* Replaces: getdt_courant, maxsum (src/mod_dt.t:160-287) over every block,
    with the min over blocks (the MPI_ALLREDUCE of :81-86 on one rank).
    dtmin_out = min_blocks courantpar / max_cells sum_d cmax_d/dx_d;
@@ -234,7 +254,8 @@ int amrvac_fv_slot_pointer(amrvac_fv_ctx* ctx, int slot, void** dptr,
 /* Device-to-device copy between a slot (interior-only layout) and a device
    array of the same size: to_slot != 0 copies dptr into the slot, else the
    slot into dptr; ordered on the context's stream.  For checkpoints and
-   adjoint drivers that keep states in their own device memory. */
+   adjoint drivers that keep states in their own device memory.  Rejected
+   with constrained transport, whose slots also hold the face field. */
 int amrvac_fv_slot_copy(amrvac_fv_ctx* ctx, int slot, void* dptr, int to_slot);
 
 /* Wait for the context's stream. */

@@ -187,15 +187,59 @@ static double get_cmax(const orc_sim* s, const double* w, int idim) {
 }
 
 /* hd_get_pthermal src/hd/mod_hd_phys.t:1021-1064 with hd_kin_en :1396-1408
-   (conserved input; sets crash when below small_pressure) */
+   (conserved input): clipped to small_pressure with fix_small_values, else
+   sets crash when below small_pressure */
 static double hd_pthermal(const orc_sim* s, const double* w, int* crash) {
   const int nd = s->p.ndir;
   double msq = 0.0;
   for (int c = 0; c < nd; ++c) msq = msq + w[s->mom[c]] * w[s->mom[c]];
   double ke = 0.5 * msq / w[s->rho_];
   double pth = (s->p.gamma - 1.0) * (w[s->e_] - ke);
-  if (s->p.check_small_values && pth < s->p.small_pressure) *crash = 1;
+  if (s->p.fix_small_values) {
+    if (pth < s->p.small_pressure) pth = s->p.small_pressure;
+  } else if (s->p.check_small_values && pth < s->p.small_pressure) {
+    *crash = 1;
+  }
   return pth;
+}
+
+/* phys_check_w on a conserved cell: hd_check_w src/hd/mod_hd_phys.t:648-674,
+   mhd_check_w_origin src/mhd/mod_mhd_phys.t:1452-1473 (small_e =
+   small_pressure/(gamma-1), :1214).  With fix_small_values and the default
+   small_values_method 'error' a flagged cell sets crash
+   (small_values_error src/mod_small_values.t:25-87). */
+static int small_cons_fails(const orc_sim* s, const double* w) {
+  const int nd = s->p.ndir;
+  if (w[s->rho_] < s->p.small_density) return 1;
+  double msq = 0.0;
+  for (int c = 0; c < nd; ++c) msq = msq + w[s->mom[c]] * w[s->mom[c]];
+  if (s->p.phys == ORC_PHYS_HD) {
+    const double tmp = (s->p.gamma - 1.0) * (w[s->e_] - half * msq / w[s->rho_]);
+    return tmp < s->p.small_pressure;
+  }
+  double bsq = 0.0;
+  for (int c = 0; c < nd; ++c) bsq = bsq + w[s->mag[c]] * w[s->mag[c]];
+  const double small_e = s->p.small_pressure * s->inv_gamma_1;
+  return w[s->e_] - half * (msq / w[s->rho_] + bsq) < small_e;
+}
+
+/* phys_handle_small_values(.false., w, ixO) with small_values_method
+   'error' over the interior of a block (conserved variables) */
+static int small_values_interior(const orc_sim* s, const double* w) {
+  const long n = s->ncell;
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = d < s->p.ndim ? s->ng : 0;
+    hi[d] = d < s->p.ndim ? s->ng + s->bnx[d] - 1 : 0;
+  }
+  for (int k = lo[2]; k <= hi[2]; ++k)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int i = lo[0]; i <= hi[0]; ++i) {
+        double cell[MAXW];
+        for (int iw = 0; iw < s->nw; ++iw) cell[iw] = w[iw * n + IX(s, i, j, k)];
+        if (small_cons_fails(s, cell)) return 1;
+      }
+  return 0;
 }
 
 /* hd_get_cbounds src/hd/mod_hd_phys.t:891-1001;
@@ -749,11 +793,15 @@ static void riemann_hllc(const orc_sim* s, int idim, const double* wLC,
 /* per-block operators                                                 */
 /* ------------------------------------------------------------------ */
 
-static void block_to_primitive(const orc_sim* s, const double* wc, double* wp) {
+/* phys_to_primitive over the ghosted block (ixI); with fix_small_values
+   hd_to_primitive / mhd_to_primitive_origin first check every cell
+   (src/hd/mod_hd_phys.t:713-715, src/mhd/mod_mhd_phys.t:1750-1753) */
+static void block_to_primitive(const orc_sim* s, const double* wc, double* wp, int* crash) {
   const long n = s->ncell;
   double cell[MAXW];
   for (long c = 0; c < n; ++c) {
     for (int iw = 0; iw < s->nw; ++iw) cell[iw] = wc[iw * n + c];
+    if (s->p.fix_small_values && crash && small_cons_fails(s, cell)) *crash = 1;
     to_primitive_cell(s, cell);
     for (int iw = 0; iw < s->nw; ++iw) wp[iw * n + c] = cell[iw];
   }
@@ -1229,7 +1277,7 @@ static void finite_volume(const orc_sim* s, int method, double qdt,
   const int nw = s->nw;
   const long n = s->ncell;
   double* wprim = (double*)malloc(sizeof(double) * (size_t)(nw * n));
-  block_to_primitive(s, wCT, wprim);
+  block_to_primitive(s, wCT, wprim, crash);
   const int ct = s->ct;
   const int tgc = ct == ORC_CT_HLL ? 2 : ct ? 1 : 0;
   const double* wsCT = wCT + (long)nw * n;
@@ -1258,6 +1306,9 @@ static void finite_volume(const orc_sim* s, int method, double qdt,
         double fC[MAXLINE][MAXW];
         get_line(s, wprim, nw, idim, a, b, lw);
         reconstruct_line(s, idim, lw, ilo, ihi, wLp, wRp);
+        if (s->p.fix_small_values && s->p.limiter != ORC_LIM_WENO5)
+          for (int i = ilo; i <= ihi; ++i)  /* 'reconstruct left / right' :1213-1216 */
+            if (check_w_prim_fails(s, wLp[i]) || check_w_prim_fails(s, wRp[i])) *crash = 1;
         for (int i = ilo; i <= ihi; ++i) {
           double wLC[MAXW], wRC[MAXW], fL[MAXW], fR[MAXW];
           double cmax = 0.0, cmin = 0.0;
@@ -1329,17 +1380,24 @@ static void finite_volume(const orc_sim* s, int method, double qdt,
     ct_face_to_center(s, wnew);
     free(F.fm); free(F.vnorm); free(F.cbmin); free(F.cbmax); free(F.vbar);
   }
+  /* check unphysical values (:325-333; small_values_method 'error' only
+     sets crash, after which the run stops) */
+  if (s->p.fix_small_values && small_values_interior(s, wnew)) *crash = 1;
   add_sources(s, qdt, wCT, wnew, ch);
+  /* the Linde / GLM sources check again (src/mhd/mod_mhd_phys.t:5529, 5662) */
+  if (s->p.fix_small_values && s->p.phys == ORC_PHYS_MHD && s->p.ndim > 1 &&
+      (s->p.divb == ORC_DIVB_GLM || s->p.divb == ORC_DIVB_LINDE) && small_values_interior(s, wnew))
+    *crash = 1;
   free(wprim);
 }
 
 /* hancock src/mod_finite_volume.t:19-113 */
 static void hancock(const orc_sim* s, double qdt, const double* wCT,
-                    double* wnew, double ch) {
+                    double* wnew, double ch, int* crash) {
   const int nw = s->nw;
   const long n = s->ncell;
   double* wprim = (double*)malloc(sizeof(double) * (size_t)(nw * n));
-  block_to_primitive(s, wCT, wprim);
+  block_to_primitive(s, wCT, wprim, crash);
   for (int idim = 0; idim < s->p.ndim; ++idim) {
     int alo, ahi, blo, bhi;
     transverse_ranges(s, idim, &alo, &ahi, &blo, &bhi);
@@ -1360,6 +1418,7 @@ static void hancock(const orc_sim* s, double qdt, const double* wCT,
             wL[iw] = lw[iw][i] + half * ldw;
             wR[iw] = lw[iw][i] - half * rdw;
           }
+          if (s->p.fix_small_values && (check_w_prim_fails(s, wL) || check_w_prim_fails(s, wR))) *crash = 1;
           for (int iw = 0; iw < nw; ++iw) { wLC[iw] = wL[iw]; wRC[iw] = wR[iw]; }
           to_conserved_cell(s, wLC);
           to_conserved_cell(s, wRC);
@@ -1372,6 +1431,8 @@ static void hancock(const orc_sim* s, double qdt, const double* wCT,
       }
   }
   add_sources(s, qdt, wCT, wnew, ch);
+  /* :109-111 */
+  if (s->p.fix_small_values && small_values_interior(s, wnew)) *crash = 1;
   free(wprim);
 }
 
@@ -2046,7 +2107,8 @@ int orc_get_x(const orc_sim* s, double* x) {
 int orc_getdt_courant(orc_sim* s, int slot, double courantpar,
                       double* dtmin_out, double* cmax_out) {
   double dtmin = bigdouble, cmax_all = 0.0;
-#pragma omp parallel for schedule(static) reduction(min : dtmin) reduction(max : cmax_all) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
+  int crash_dt = 0;
+#pragma omp parallel for schedule(static) reduction(min : dtmin) reduction(max : cmax_all) reduction(| : crash_dt) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
   for (int ib = 0; ib < s->nb; ++ib) {
     const double* w = BLK(s, slot, ib);
     const long n = s->ncell;
@@ -2063,6 +2125,8 @@ int orc_getdt_courant(orc_sim* s, int slot, double courantpar,
           double cell[MAXW];
           long c = IX(s, i, j, k);
           for (int iw = 0; iw < s->nw; ++iw) cell[iw] = w[iw * n + c];
+          /* getdt_courant converts with phys_to_primitive (src/mod_dt.t:195) */
+          if (s->p.fix_small_values && small_cons_fails(s, cell)) crash_dt = 1;
           to_primitive_cell(s, cell);
           double tot = 0.0;
           for (int d = 0; d < s->p.ndim; ++d) {
@@ -2078,6 +2142,7 @@ int orc_getdt_courant(orc_sim* s, int slot, double courantpar,
     dtmin = MIN2(dtmin, dtnew);
     cmax_all = MAX2(cmax_all, cmaxb);
   }
+  if (crash_dt) s->crash = 1;
   *dtmin_out = dtmin;
   *cmax_out = cmax_all;
   return 0;
@@ -2105,7 +2170,7 @@ int orc_advect1(orc_sim* s, int method, double dtfactor, double dt,
                      ? s->bflux + (long)ib * s->p.ndim * 2 * s->nw * s->nface
                      : NULL;  /* hancock stores no fluxes */
     if (method == ORC_FS_HANCOCK)
-      hancock(&sb, qdt, wCT, wnew, cmax_global);
+      hancock(&sb, qdt, wCT, wnew, cmax_global, &c);
     else
       finite_volume(&sb, method, qdt, wCT, wnew, cmax_global, &c, bf,
                     s->ct ? BLK(s, 0, ib) + (long)s->nw * s->ncell : NULL);

@@ -73,6 +73,7 @@ typedef struct orc_params {
   int nghost;              /* 2 (3 for weno5) */
   int block_nx[3];         /* interior cells per block */
   int type_ct;             /* ORC_CT_* when divb = ORC_DIVB_CT */
+  int fix_small_values;    /* default 0; small_values_method 'error' */
 } orc_params;
 
 typedef struct orc_sim orc_sim;

@@ -31,7 +31,7 @@ class OrcParams(ctypes.Structure):
         ("cada3_radius", ctypes.c_double), ("small_pressure", ctypes.c_double),
         ("small_density", ctypes.c_double), ("check_small_values", ctypes.c_int),
         ("nghost", ctypes.c_int), ("block_nx", ctypes.c_int * 3),
-        ("type_ct", ctypes.c_int),
+        ("type_ct", ctypes.c_int), ("fix_small_values", ctypes.c_int),
     ]
 
 
@@ -120,6 +120,7 @@ def make_params(par, block_nx) -> OrcParams:
     for d in range(3):
         p.block_nx[d] = block_nx[d] if d < par.ndim else 1
     p.type_ct = TYPE_CT[par.type_ct] if par.physics == "mhd" and par.typedivbfix == "ct" else 0
+    p.fix_small_values = 1 if par.fix_small_values else 0
     return p
 
 

@@ -747,6 +747,83 @@ def test_ct_keeps_div_b_zero():
     assert check_ct_divb() == ""
 
 
+def check_small_values_crash(lib_path=None) -> str:
+    """fix_small_values with the default small_values_method 'error'
+    (src/mod_finite_volume.t:325-333, src/mod_small_values.t:25-87): a
+    run stops ("small value encountered", src/amrvac.t:311-318) right after
+    the first step whose updated state holds a cell below small_pressure.
+    Two smoothly separating streams open a rarefaction whose central
+    pressure falls; with onestep (the stage output is the step's state) and minmod
+    (face states bounded by their cells) the crash step is exactly the
+    first step whose minimum pressure is below the floor.  Without
+    fix_small_values the same run continues (MHD) or stops only where the
+    cmaxmean bound speed evaluates a small pressure (HD, hd_get_pthermal
+    src/hd/mod_hd_phys.t:1046-1062)."""
+    import dataclasses
+    msgs = []
+    gamma = 5.0 / 3.0
+    mesh = UniformMesh(2, (64, 16), (16, 16), (0.0, 0.0), (1.0, 0.25), ("cont", "cont", "periodic", "periodic"))
+    x = mesh.cell_centers()
+    vx = 0.6 * np.tanh((x[:, 0] - 0.5) / 0.05)
+    one = np.ones_like(vx)
+
+    def run(par, w0, nmax=60):
+        be = OracleFV(par, mesh, lib_path=lib_path)
+        be.set_state(0, w0)
+        be.getbc(0)
+        ti = TimeIntegrator(par, mesh, be)
+        pmin = []
+        crashed = False
+        try:
+            for _ in range(nmax):
+                ti.run(nsteps=1)
+                w = mesh.interior(be.get_state(0))
+                pmin.append(_pmin(w, par))
+                ti.state.final_dt_exit = False
+        except RuntimeError as e:
+            if "small value" not in str(e):
+                raise
+            crashed = True
+            pmin.append(_pmin(mesh.interior(be.get_state(0)), par))
+        return crashed, ti.state.it, pmin
+
+    def _pmin(w, par):
+        if par.physics == "hd":
+            return float(((gamma - 1) * (w[:, 3] - 0.5 * (w[:, 1] ** 2 + w[:, 2] ** 2) / w[:, 0])).min())
+        ke = 0.5 * (w[:, 1] ** 2 + w[:, 2] ** 2 + w[:, 3] ** 2) / w[:, 0]
+        return float(((gamma - 1) * (w[:, 4] - ke - 0.5 * (w[:, 5] ** 2 + w[:, 6] ** 2 + w[:, 7] ** 2))).min())
+    floor = 0.02
+    hd = Par(physics="hd", ndim=2, ndir=2, gamma=gamma, time_stepper="onestep", flux_scheme="tvdlf",
+             limiter="minmod", courantpar=0.4, small_pressure=floor, fix_small_values=True)
+    w_hd = problems.hd_to_conserved(np.stack([one, vx, 0 * one, 0.1 * one], axis=1), gamma, 2)
+    crashed, it, pmin = run(hd, w_hd)
+    first_low = next((k for k, p in enumerate(pmin) if p < floor), None)
+    if not crashed or first_low is None or it != first_low:
+        msgs.append(f"hd fix_small_values: crashed={crashed} at it={it}, first low step {first_low}")
+    # without fix_small_values the run goes on with Einfeldt bounds, and
+    # stops with cmaxmean bounds once a face mean falls below the floor
+    crashed0, _, _ = run(dataclasses.replace(hd, fix_small_values=False), w_hd)
+    crashed1, it1, _ = run(dataclasses.replace(hd, fix_small_values=False, typeboundspeed="cmaxmean"), w_hd)
+    if crashed0 or not crashed1 or first_low is None or not first_low - 3 <= it1 <= first_low + 3:
+        msgs.append(f"hd check_small_values only: Einfeldt crashed={crashed0}, cmaxmean crashed={crashed1} at {it1}")
+    mhd = Par(physics="mhd", ndim=2, ndir=3, gamma=gamma, typedivbfix="none", time_stepper="onestep",
+              flux_scheme="hll", limiter="minmod", courantpar=0.4, small_pressure=floor, fix_small_values=True)
+    w_mhd = problems.mhd_to_conserved(np.stack([one, vx, 0 * one, 0 * one, 0.1 * one, 0.1 * one, 0 * one,
+                                                0 * one], axis=1), gamma, 3)
+    crashed, it, pmin = run(mhd, w_mhd)
+    first_low = next((k for k, p in enumerate(pmin) if p < floor), None)
+    if not crashed or first_low is None or it != first_low:
+        msgs.append(f"mhd fix_small_values: crashed={crashed} at it={it}, first low step {first_low}")
+    crashed0, _, _ = run(dataclasses.replace(mhd, fix_small_values=False), w_mhd)
+    if crashed0:
+        msgs.append("mhd run with check_small_values only crashed")
+    return "; ".join(msgs)
+
+
+def test_small_values_stop_the_run():
+    assert check_small_values_crash() == ""
+
+
 def check_linde_level_jump(lib_path=None) -> str:
     """Linde's source retreats one cell layer from faces with a coarser or
     finer neighbour (src/mhd/mod_mhd_phys.t:5616-5641): the maintained 3-D
@@ -789,6 +866,7 @@ ALL_CHECKS = {
     "limiters": check_limiters,
     "weno5_polynomials": check_weno5_polynomials,
     "weno5_fix_limiter": check_fix_limiter,
+    "small_values_crash": check_small_values_crash,
     "glm_decay": check_glm_decay,
     "conservation": check_conservation_and_uniform_states,
 }

@@ -96,6 +96,11 @@ MUTANTS = {
                                     "if (check_w_prim_fails(s, tl[i])) continue;", 0, "weno5_fix_limiter"),
     "weno5_fix_limiter_pressure_unchecked": ("if (w[s->e_] < s->p.small_pressure) return 1;",
                                              "if (w[s->e_] < -1.0e300) return 1;", 0, "weno5_fix_limiter"),
+    "small_values_after_update_dropped": ("  if (s->p.fix_small_values && small_values_interior(s, wnew)) *crash = 1;\n  add_sources",
+                                          "  add_sources", 0, "small_values_crash"),
+    "small_values_mhd_magnetic_energy_dropped": ("return w[s->e_] - half * (msq / w[s->rho_] + bsq) < small_e;",
+                                                 "return w[s->e_] - half * (msq / w[s->rho_]) < small_e;", 0,
+                                                 "small_values_crash"),
     "linde_level_jump_rule_dropped": ("if (t == 2 || t == 4) {", "if (t == -7) {", 0, "linde_level_jump"),
     "tvdlf_no_dissipation": ("double fac = -0.5 * s->p.tvdlfeps * cmax;", "double fac = 0.0 * s->p.tvdlfeps * cmax;",
                              0, "sod_tvdlf_weno5"),
