@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,37 @@ int fail(int code, const std::string& msg) {
 constexpr double kBig = 1.0e99;     // src/mod_constants.t:11
 constexpr double kSmall = 1.0e-12;  // src/mod_constants.t:8
 
+// Scratch of the allocation-free dt reduction (amrvac_fv_getdt_state_noalloc):
+// per-block partial maxima, 2 doubles per block for at most kNoallocMaxBlocks
+// blocks, then the reduced triple.  Every GPU the library runs on holds its
+// own copy (512 KiB) until the process ends.  In host code the array's name
+// is not a device address: kernels get the address hipGetSymbolAddress
+// returned for the context's device (amrvac_fv_ctx::scratch).
+constexpr int kNoallocMaxBlocks = 32768;
+constexpr int kMaxDevices = 64;
+}  // namespace
+static __device__ __align__(256) double scratch[2 * kNoallocMaxBlocks + 4];
+namespace {
+double* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// the device address of `scratch` on the current device, looked up once
+int scratch_address(double** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(AMRVAC_FV_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(AMRVAC_FV_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return AMRVAC_FV_OK;
+}
+
 }  // namespace
 
 struct amrvac_fv_ctx {
@@ -39,6 +71,7 @@ struct amrvac_fv_ctx {
   int nw = 0;
   int nnb = 0;
   int nb = 0;
+  int bx = 16;       // block_nx(1) = block_nx(2)
   int bnz = 1;
   size_t cells = 0;  // interior cells per block
   size_t cells_g = 0;  // ghosted cells per block
@@ -55,6 +88,11 @@ struct amrvac_fv_ctx {
   int* d_nbr = nullptr;
   int* d_crash = nullptr;
   double* d_red = nullptr;    // per-block reduction results
+  // CFL partials of ps left by the last stage that wrote it (uniform level,
+  // production precision): getdt of ps then only reduces them
+  double* d_dtp = nullptr;
+  bool dt_valid = false;
+  bool slot0_exposed = false;  // ps handed out by amrvac_fv_slot_pointer
   double* d_red2 = nullptr;   // final reduction
   double* d_stage = nullptr;  // ghosted staging for download
   std::vector<double> h_stage;
@@ -91,6 +129,7 @@ struct amrvac_fv_ctx {
   double* d_wts = nullptr;
   int nrt = 0, npt = 0, nft = 0;
   std::vector<int> h_level;
+  double* scratch = nullptr; // this device's copy of `scratch` (no-allocation dt)
   int crash_seen = 0;        // small-value flag copied with the last dt
   bool crash_fresh = false;  // no stage launched since that copy
   bool timing = false;
@@ -101,42 +140,29 @@ struct amrvac_fv_ctx {
 namespace fv {
 
 template <class P>
-__global__ void __launch_bounds__(NTHREADS) getdt_kernel(const double* __restrict__ w, int nblocks, int bnz,
+__global__ void __launch_bounds__(NTHREADS) getdt_kernel(const double* __restrict__ w, int nblocks, int bx, int bnz,
                                                          FVParams fp, const int* __restrict__ blevel,
                                                          double* out, int* crash) {
+  const int BXr = bx;  // block_nx(1) = block_nx(2)
   const int b = blockIdx.x;
   if (b >= nblocks) return;
   if (blevel) {  // the block's level: 1/dx scales by 2^level exactly
     const double sc = double(1 << blevel[b]);
     for (int d = 0; d < 3; ++d) fp.dxinv[d] = fp.dxinv[d] * sc;
   }
-  const size_t cells = (size_t)BX * BX * bnz;
+  const size_t cells = (size_t)BXr * BXr * bnz;
   const double* wb = w + (size_t)b * P::NW * cells;
   double maxtot = -1.0e99, cmaxb = 0.0;
   int small = 0;
   for (size_t c = threadIdx.x; c < cells; c += NTHREADS) {
-    double cons[P::NW], pr[P::NW];
+    double cons[P::NW];
 #pragma unroll
     for (int iw = 0; iw < P::NW; ++iw) cons[iw] = wb[iw * cells + c];
     // getdt_courant converts with phys_to_primitive (src/mod_dt.t:195),
     // which checks the cell with fix_small_values
     if (fp.fix_small && small_cons<P>(cons, fp)) small = 1;
-    to_prim<P>(cons, pr, fp);
     // getdt_courant, maxsum, slab_uniform (src/mod_dt.t:203-221)
-    double cm = cell_cmax<P, 0>(pr, fp);
-    cmaxb = fmax(cmaxb, cm);
-    double tot = cm * fp.dxinv[0];
-    if constexpr (P::NDIM > 1) {
-      cm = cell_cmax<P, 1>(pr, fp);
-      cmaxb = fmax(cmaxb, cm);
-      tot = tot + cm * fp.dxinv[1];
-    }
-    if constexpr (P::NDIM > 2) {
-      cm = cell_cmax<P, 2>(pr, fp);
-      cmaxb = fmax(cmaxb, cm);
-      tot = tot + cm * fp.dxinv[2];
-    }
-    maxtot = fmax(maxtot, tot);
+    cfl_cell<P>(cons, fp, maxtot, cmaxb);
   }
   if (small) *crash = 1;
   // wave64 shuffle max, then across the 4 waves through LDS
@@ -253,9 +279,10 @@ __global__ void __launch_bounds__(NTHREADS) volsum_kernel(const double* __restri
 template <int NDIM>
 __global__ void __launch_bounds__(NTHREADS) fill_ghosted_kernel(SweepArgs a, int nw, int ng, int nxg0, int nxg1,
                                                                 int nxg2, double* out) {
+  const int BXr = a.bx;  // block_nx(1) = block_nx(2)
   const int b = blockIdx.y;
   const size_t cg = (size_t)nxg0 * nxg1 * nxg2;
-  const size_t cells = (size_t)BX * BX * a.bnz;
+  const size_t cells = (size_t)BXr * BXr * a.bnz;
   for (size_t c = blockIdx.x * (size_t)NTHREADS + threadIdx.x; c < cg; c += (size_t)gridDim.x * NTHREADS) {
     int i = int(c % nxg0);
     int j = int((c / nxg0) % nxg1);
@@ -263,7 +290,7 @@ __global__ void __launch_bounds__(NTHREADS) fill_ghosted_kernel(SweepArgs a, int
     int x = i - ng, y = NDIM > 1 ? j - ng : 0, z = NDIM > 2 ? k - ng : 0;
     double sign;
     const int sb = locate<NDIM>(a, b, x, y, z, sign);
-    const size_t so = (size_t)sb * nw * cells + ((size_t)z * BX + y) * BX + x;
+    const size_t so = (size_t)sb * nw * cells + ((size_t)z * BXr + y) * BXr + x;
     for (int iw = 0; iw < nw; ++iw) out[((size_t)b * nw + iw) * cg + c] = sign * a.src[so + iw * cells];
   }
 }
@@ -284,19 +311,20 @@ __device__ __forceinline__ void decode_dir(int k, int ndim, int* i) {
 // bc_fill_restrict src/mod_ghostcells_update.t:855-898).  Written into the
 // direction's virtual block in the neighbour's frame.
 template <int NDIM>
-__global__ void __launch_bounds__(NTHREADS) restrict_kernel(double* w, int nw, int bnz, int ng, int nnb,
+__global__ void __launch_bounds__(NTHREADS) restrict_kernel(double* w, int nw, int bx, int bnz, int ng, int nnb,
                                                             const int4* task, const int* nchild) {
+  const int BXr = bx;  // block_nx(1) = block_nx(2)
   const int4 t = task[blockIdx.x];
   const int C = t.x, k = t.y, V = t.z;
   int i[3];
   decode_dir(k, NDIM, i);
-  const int n[3] = {BX, BX, bnz};
+  const int n[3] = {BXr, BXr, bnz};
   int lo[3] = {0, 0, 0}, cnt[3] = {1, 1, 1};
   for (int d = 0; d < NDIM; ++d) {
     lo[d] = i[d] == -1 ? -ng : (i[d] == 0 ? 0 : n[d]);
     cnt[d] = i[d] == 0 ? n[d] : ng;
   }
-  const size_t cells = (size_t)BX * BX * bnz;
+  const size_t cells = (size_t)BXr * BXr * bnz;
   const double cofi = 1.0 / double(1 << NDIM);
   const int total = cnt[0] * cnt[1] * cnt[2];
   for (int c = threadIdx.x; c < total; c += NTHREADS) {
@@ -310,14 +338,14 @@ __global__ void __launch_bounds__(NTHREADS) restrict_kernel(double* w, int nw, i
     }
     const int F = nchild[((size_t)C * nnb + k) * (1 << NDIM) + cidx];
     const double* wf = w + (size_t)F * nw * cells;
-    double* wv = w + (size_t)V * nw * cells + ((size_t)gn[2] * BX + gn[1]) * BX + gn[0];
+    double* wv = w + (size_t)V * nw * cells + ((size_t)gn[2] * BXr + gn[1]) * BXr + gn[0];
     for (int iw = 0; iw < nw; ++iw) {
       double sum = 0.0;
       bool first = true;
       for (int dk = 0; dk <= (NDIM > 2 ? 1 : 0); ++dk)
         for (int dj = 0; dj <= 1; ++dj)
           for (int di = 0; di <= 1; ++di) {
-            const double v = wf[iw * cells + ((size_t)(f[2] + dk) * BX + (f[1] + dj)) * BX + f[0] + di];
+            const double v = wf[iw * cells + ((size_t)(f[2] + dk) * BXr + (f[1] + dj)) * BXr + f[0] + di];
             sum = first ? v : sum + v;
             first = false;
           }
@@ -335,7 +363,8 @@ __global__ void __launch_bounds__(NTHREADS) restrict_kernel(double* w, int nw, i
 template <int NDIM>
 __device__ double psc_value(const SweepArgs& a, int nw, int ng, int F, int C, const int* icF, const int* iC,
                             const bool (*bnd)[2], int p0, int p1, int p2, int iw) {
-  const int n[3] = {BX, BX, a.bnz};
+  const int BXr = a.bx;  // block_nx(1) = block_nx(2)
+  const int n[3] = {BXr, BXr, a.bnz};
   int q[3] = {p0, p1, p2};
   double sign = 1.0;
   bool inside = true;
@@ -352,7 +381,7 @@ __device__ double psc_value(const SweepArgs& a, int nw, int ng, int F, int C, co
     }
     if (q[d] < clo || q[d] > chi) inside = false;
   }
-  const size_t cells = (size_t)BX * BX * a.bnz;
+  const size_t cells = (size_t)BXr * BXr * a.bnz;
   if (inside) {
     int f[3] = {0, 0, 0};
     for (int d = 0; d < NDIM; ++d) f[d] = 2 * (q[d] - (ng + 1));
@@ -362,7 +391,7 @@ __device__ double psc_value(const SweepArgs& a, int nw, int ng, int F, int C, co
     for (int dk = 0; dk <= (NDIM > 2 ? 1 : 0); ++dk)
       for (int dj = 0; dj <= 1; ++dj)
         for (int di = 0; di <= 1; ++di) {
-          const double v = wf[((size_t)(f[2] + dk) * BX + (f[1] + dj)) * BX + f[0] + di];
+          const double v = wf[((size_t)(f[2] + dk) * BXr + (f[1] + dj)) * BXr + f[0] + di];
           sum = first ? v : sum + v;
           first = false;
         }
@@ -372,7 +401,7 @@ __device__ double psc_value(const SweepArgs& a, int nw, int ng, int F, int C, co
   for (int d = 0; d < NDIM; ++d) x[d] = q[d] + (icF[d] - 1) * (n[d] / 2) + n[d] * iC[d] - ng - 1;
   double s2;
   const int sb = locate<NDIM>(a, C, x[0], x[1], x[2], s2);
-  return sign * s2 * a.src[(size_t)sb * nw * cells + iw * cells + ((size_t)x[2] * BX + x[1]) * BX + x[0]];
+  return sign * s2 * a.src[(size_t)sb * nw * cells + iw * cells + ((size_t)x[2] * BXr + x[1]) * BXr + x[0]];
 }
 
 // Ghost cells of a fine block facing a coarser neighbour: interpolation_linear
@@ -383,6 +412,7 @@ template <int NDIM>
 __global__ void __launch_bounds__(NTHREADS) prolong_kernel(SweepArgs a, double* w, int nw, int ng, int nnb,
                                                            const int4* task, const int2* task2, const int* ntype,
                                                            const int* level, const double* xlo, FVParams fp) {
+  const int BXr = a.bx;  // block_nx(1) = block_nx(2)
   const int4 t = task[blockIdx.x];
   const int2 t2 = task2[blockIdx.x];
   const int F = t.x, k = t.y, V = t.z, C = t.w;
@@ -392,7 +422,7 @@ __global__ void __launch_bounds__(NTHREADS) prolong_kernel(SweepArgs a, double* 
     icF[d] = (t2.x >> (2 * d)) & 3;
     iC[d] = ((t2.y >> (2 * d)) & 3) - 1;
   }
-  const int n[3] = {BX, BX, a.bnz};
+  const int n[3] = {BXr, BXr, a.bnz};
   bool bnd[3][2] = {{false, false}, {false, false}, {false, false}};
   for (int d = 0; d < NDIM; ++d)
     for (int sd = 0; sd < 2; ++sd) {
@@ -416,7 +446,7 @@ __global__ void __launch_bounds__(NTHREADS) prolong_kernel(SweepArgs a, double* 
     xFimin[d] = rpxmin - double(ng) * dxFi[d];
     xComin[d] = rpxmin - double(ng) * dxCo[d];
   }
-  const size_t cells = (size_t)BX * BX * a.bnz;
+  const size_t cells = (size_t)BXr * BXr * a.bnz;
   const int total = cnt[0] * cnt[1] * cnt[2];
   for (int c = threadIdx.x; c < total; c += NTHREADS) {
     const int g[3] = {lo[0] + c % cnt[0], lo[1] + (c / cnt[0]) % cnt[1], lo[2] + c / (cnt[0] * cnt[1])};
@@ -430,7 +460,7 @@ __global__ void __launch_bounds__(NTHREADS) prolong_kernel(SweepArgs a, double* 
     }
     int gn[3] = {0, 0, 0};
     for (int d = 0; d < NDIM; ++d) gn[d] = g[d] - i[d] * n[d];
-    double* wv = w + (size_t)V * nw * cells + ((size_t)gn[2] * BX + gn[1]) * BX + gn[0];
+    double* wv = w + (size_t)V * nw * cells + ((size_t)gn[2] * BXr + gn[1]) * BXr + gn[0];
     for (int iw = 0; iw < nw; ++iw) {
       const double c0 = psc_value<NDIM>(a, nw, ng, F, C, icF, iC, bnd, co[0], co[1], co[2], iw);
       double val = c0;
@@ -456,12 +486,13 @@ __global__ void __launch_bounds__(NTHREADS) prolong_kernel(SweepArgs a, double* 
 // store_flux (:597-648): a coarse block's cells next to a finer neighbour
 // drop their own face flux and take the sum of the fine fluxes
 template <int NDIM>
-__global__ void __launch_bounds__(NTHREADS) fixcons_kernel(double* w, int nw, int bnz, int nnb, const int* ftask,
+__global__ void __launch_bounds__(NTHREADS) fixcons_kernel(double* w, int nw, int bx, int bnz, int nnb, const int* ftask,
                                                            const int* ntype, const int* nchild,
                                                            const int* fregoff, const double* freg, int nface) {
+  const int BXr = bx;  // block_nx(1) = block_nx(2)
   const int C = ftask[blockIdx.x];
-  const int n[3] = {BX, BX, bnz};
-  const size_t cells = (size_t)BX * BX * bnz;
+  const int n[3] = {BXr, BXr, bnz};
+  const size_t cells = (size_t)BXr * BXr * bnz;
   const double cofi = 1.0 / double(1 << NDIM);
   double* wc = w + (size_t)C * nw * cells;
   for (int idims = 0; idims < NDIM; ++idims) {
@@ -487,7 +518,7 @@ __global__ void __launch_bounds__(NTHREADS) fixcons_kernel(double* w, int nw, in
         if (tb >= 0) c[tb] = b2;
         const double f = own[(size_t)iw * nface + a2 + (size_t)na * b2];
         const double pflux = iside == 1 ? -f : f;
-        double& x = wc[iw * cells + ((size_t)c[2] * BX + c[1]) * BX + c[0]];
+        double& x = wc[iw * cells + ((size_t)c[2] * BXr + c[1]) * BXr + c[0]];
         x = x - pflux;
       }
       __syncthreads();
@@ -517,7 +548,7 @@ __global__ void __launch_bounds__(NTHREADS) fixcons_kernel(double* w, int nw, in
         c[idims] = ix;
         c[ta] = (ic[ta] - 1) * nca + a2;
         if (tb >= 0) c[tb] = (ic[tb] - 1) * ncb + b2;
-        double& x = wc[iw * cells + ((size_t)c[2] * BX + c[1]) * BX + c[0]];
+        double& x = wc[iw * cells + ((size_t)c[2] * BXr + c[1]) * BXr + c[0]];
         x = x + pf * cofi;
       }
       __syncthreads();
@@ -526,9 +557,9 @@ __global__ void __launch_bounds__(NTHREADS) fixcons_kernel(double* w, int nw, in
 }
 
 template <class P>
-hipError_t launch_getdt(const double* w, int nb, int bnz, const FVParams& fp, const int* blevel, double* out,
+hipError_t launch_getdt(const double* w, int nb, int bx, int bnz, const FVParams& fp, const int* blevel, double* out,
                         int* crash, hipStream_t s) {
-  hipLaunchKernelGGL((getdt_kernel<P>), dim3(nb), dim3(NTHREADS), 0, s, w, nb, bnz, fp, blevel, out, crash);
+  hipLaunchKernelGGL((getdt_kernel<P>), dim3(nb), dim3(NTHREADS), 0, s, w, nb, bx, bnz, fp, blevel, out, crash);
   return hipGetLastError();
 }
 
@@ -540,10 +571,105 @@ hipError_t launch_getdt(const double* w, int nb, int bnz, const FVParams& fp, co
 // table (the values download would write into the ghosted layout)
 template <int NDIM>
 __device__ __forceinline__ double cell_value(const SweepArgs& a, int nw, int b, int x, int y, int z, int iw) {
+  const int BXr = a.bx;  // block_nx(1) = block_nx(2)
   double sign;
   const int sb = locate<NDIM>(a, b, x, y, z, sign);
-  const size_t cells = (size_t)BX * BX * a.bnz;
-  return sign * a.src[(size_t)sb * nw * cells + iw * cells + ((size_t)z * BX + y) * BX + x];
+  const size_t cells = (size_t)BXr * BXr * a.bnz;
+  return sign * a.src[(size_t)sb * nw * cells + iw * cells + ((size_t)z * BXr + y) * BXr + x];
+}
+
+// add_source_linde (src/mhd/mod_mhd_phys.t:5599-5664) after the flux
+// update of a stage, for every block: Linde's div B diffusion of the stage
+// output dst from the stage input wCT = a.src (ghost cells through the
+// neighbour table, as the stage kernel reads them):
+//   g = divbdiff / (sum_d 1/dx_d^2) * (divB(x+e) - divB(x-e)) / (2 dx),
+//   B_idim += g,  e += B_idim(wCT) g,
+// dimension by dimension, with div B the central divvector of wCT
+// (get_divb src/physics/mod_functions_bfield.t:14-45, src/mod_geometry.t:
+// 582-655), and no update in the cell layer next to a face whose
+// neighbour is coarser or finer (the level-jump rule, :5616-5641).  One
+// workgroup per block; div B of the block and its one-cell rim in LDS.
+// Built without contraction, so every build rounds like the oracle.
+constexpr int kLindeLds = 7800;  // doubles: (bx+2)^2 (bnz+2) for bx <= 16, bnz <= 22
+template <class P>
+__global__ void __launch_bounds__(NTHREADS) linde_kernel(SweepArgs a, FVParams fp, const int* __restrict__ ntype,
+                                                         int nnb, double* __restrict__ dst) {
+  constexpr int NDIM = P::NDIM, NW = P::NW;
+  const int b = blockIdx.x;
+  if (b >= a.nblocks) return;
+  if (a.blevel && a.blevel[b]) {  // cell sizes of the block's level (exact powers of two)
+    const double sc = double(1 << a.blevel[b]), sc2 = sc * sc;
+    for (int d = 0; d < 3; ++d) {
+      fp.dx[d] = fp.dx[d] / sc;
+      fp.dxinv[d] = fp.dxinv[d] * sc;
+    }
+    fp.invds2 = fp.invds2 * sc2;
+  }
+  const int bx = a.bx, bz = NDIM == 3 ? a.bnz : 1;
+  const int ex = bx + 2, ez = NDIM == 3 ? bz + 2 : 1;
+  __shared__ double divb[kLindeLds];
+  for (int c = threadIdx.x; c < ex * ex * ez; c += NTHREADS) {
+    const int x = c % ex - 1, y = (c / ex) % ex - 1, z = NDIM == 3 ? c / (ex * ex) - 1 : 0;
+    double d = 0.0;
+#pragma unroll
+    for (int dd = 0; dd < NDIM; ++dd) {
+      const int hx = dd == 0, hy = dd == 1, hz = dd == 2;
+      const double bp = cell_value<NDIM>(a, NW, b, x + hx, y + hy, z + hz, P::MAG + dd);
+      const double bm = cell_value<NDIM>(a, NW, b, x - hx, y - hy, z - hz, P::MAG + dd);
+      d = d + 0.5 * (bp - bm) * fp.dxinv[dd];
+    }
+    divb[c] = d;
+  }
+  __syncthreads();
+  int lo[3] = {0, 0, 0}, hi[3] = {bx - 1, bx - 1, bz - 1};
+  if (ntype) {
+    for (int d = 0; d < NDIM; ++d)
+      for (int side = 0; side < 2; ++side) {
+        int k = 0, m = 1;
+        for (int e = 0; e < NDIM; ++e) { k += ((e == d ? (side ? 1 : -1) : 0) + 1) * m; m *= 3; }
+        const int t = ntype[(size_t)b * nnb + k];
+        if (t == 2 || t == 4) {
+          if (side == 0) lo[d] = lo[d] + 1;
+          else hi[d] = hi[d] - 1;
+        }
+      }
+  }
+  const size_t cells = (size_t)bx * bx * a.bnz;
+  double* wb = dst + (size_t)b * NW * cells;
+  const double* wct = a.src + (size_t)b * NW * cells;
+  int crash = 0;
+  for (int c = threadIdx.x; c < bx * bx * bz; c += NTHREADS) {
+    const int x = c % bx, y = (c / bx) % bx, z = NDIM == 3 ? c / (bx * bx) : 0;
+    if (x < lo[0] || x > hi[0] || y < lo[1] || y > hi[1] || z < lo[2] || z > hi[2]) continue;
+    const int l = (x + 1) + ex * ((y + 1) + ex * (z + 1));
+#pragma unroll
+    for (int idim = 0; idim < NDIM; ++idim) {
+      const int st = idim == 0 ? 1 : (idim == 1 ? ex : ex * ex);
+      double g = (divb[l + st] - divb[l - st]) * 0.5;  // gradient (src/mod_geometry.t:329-403)
+      g = g / fp.dx[idim];
+      g = g * fp.divbdiff / fp.invds2;
+      wb[(P::MAG + idim) * cells + c] = wb[(P::MAG + idim) * cells + c] + g;
+      wb[P::E * cells + c] = wb[P::E * cells + c] + wct[(P::MAG + idim) * cells + c] * g;
+    }
+    if (fp.fix_small) {  // add_source_linde checks the updated cells (:5662)
+      double w[NW];
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) w[iw] = wb[iw * cells + c];
+      if (small_cons<P>(w, fp)) crash = 1;
+    }
+  }
+  if (crash) *a.crash = 1;
+}
+
+template <class P>
+hipError_t launch_linde(const SweepArgs& a, const FVParams& fp, const int* ntype, int nnb, double* dst,
+                        hipStream_t s) {
+  if constexpr (!P::MHD || P::NDIM < 2) {
+    return hipErrorInvalidValue;
+  } else {
+    hipLaunchKernelGGL((linde_kernel<P>), dim3(a.nblocks), dim3(NTHREADS), 0, s, a, fp, ntype, nnb, dst);
+    return hipGetLastError();
+  }
 }
 
 // lohner_grid (src/amr/mod_errest.t:63-199, Cartesian, no B0, no log
@@ -559,11 +685,12 @@ template <int NDIM>
 __global__ void __launch_bounds__(NTHREADS) lohner_kernel(SweepArgs a, int nw, int nwt, const int* __restrict__ iws,
                                                           const double* __restrict__ wts, double wavefilter, double thr,
                                                           double cthr, unsigned char* __restrict__ flags) {
+  const int BXr = a.bx;  // block_nx(1) = block_nx(2)
   const int b = blockIdx.x;
-  const int cells = BX * BX * a.bnz;
+  const int cells = BXr * BXr * a.bnz;
   int any_r = 0, all_c = 1;
   for (int c = threadIdx.x; c < cells; c += NTHREADS) {
-    const int X[3] = {c % BX, (c / BX) % BX, c / (BX * BX)};
+    const int X[3] = {c % BXr, (c / BXr) % BXr, c / (BXr * BXr)};
     auto q = [&](int iw, int d1, int s1, int d2, int s2) {
       int p[3] = {X[0], X[1], X[2]};
       p[d1] += s1;
@@ -615,15 +742,16 @@ __global__ void __launch_bounds__(NTHREADS) lohner_kernel(SweepArgs a, int nw, i
 template <int NDIM>
 __global__ void __launch_bounds__(NTHREADS) remap_kernel(SweepArgs a, int nw, const int* __restrict__ plan,
                                                          double* __restrict__ out) {
+  const int BXr = a.bx;  // block_nx(1) = block_nx(2)
   constexpr int S = 2 + (1 << NDIM);
   const int b = blockIdx.x;
   const int* p = plan + (size_t)b * S;
   const int kind = p[0];
-  const int n[3] = {BX, BX, a.bnz};
-  const size_t cells = (size_t)BX * BX * a.bnz;
+  const int n[3] = {BXr, BXr, a.bnz};
+  const size_t cells = (size_t)BXr * BXr * a.bnz;
   double* o = out + (size_t)b * nw * cells;
   for (int c = threadIdx.x; c < (int)cells; c += NTHREADS) {
-    const int X[3] = {c % BX, (c / BX) % BX, c / (BX * BX)};
+    const int X[3] = {c % BXr, (c / BXr) % BXr, c / (BXr * BXr)};
     if (kind == 0) {
       const double* src = a.src + (size_t)p[1] * nw * cells;
       for (int iw = 0; iw < nw; ++iw) o[iw * cells + c] = src[iw * cells + c];
@@ -641,7 +769,7 @@ __global__ void __launch_bounds__(NTHREADS) remap_kernel(SweepArgs a, int nw, co
         for (int dk = 0; dk <= (NDIM > 2 ? 1 : 0); ++dk)
           for (int dj = 0; dj <= 1; ++dj)
             for (int di = 0; di <= 1; ++di) {
-              const double v = wf[iw * cells + ((size_t)(f[2] + dk) * BX + (f[1] + dj)) * BX + f[0] + di];
+              const double v = wf[iw * cells + ((size_t)(f[2] + dk) * BXr + (f[1] + dj)) * BXr + f[0] + di];
               sum = first ? v : sum + v;
               first = false;
             }
@@ -686,8 +814,9 @@ fvc::KernelKey make_key(const amrvac_fv_ctx* c, int riem) {
   k.ndim = c->cfg.ndim;
   k.physics = c->cfg.physics;
   k.ndir = c->cfg.ndir;
-  // constrained transport: the cell kernels are those of ideal MHD
-  k.divb = c->cfg.physics == 2 ? (c->cfg.divb == 8 ? 0 : c->cfg.divb) : 0;
+  // constrained transport and Linde's cleaning: the stage kernels are those
+  // of ideal MHD (Linde's source follows in linde_kernel)
+  k.divb = c->cfg.physics == 2 ? ((c->cfg.divb == 8 || c->cfg.divb == 4) ? 0 : c->cfg.divb) : 0;
   k.mixed = c->cfg.precision == AMRVAC_FV_PREC_MIXED ? 1 : 0;
   k.riem = riem;
   k.weno = c->cfg.limiter == 16 ? 1 : 0;
@@ -716,11 +845,14 @@ template <template <class> class F, class... Args>
 hipError_t with_phys(const amrvac_fv_ctx* c, Args... args) {
   using namespace fv;
   // constrained transport uses the ideal-MHD physics without a source term
-  const int nd = c->cfg.ndim, ph = c->cfg.physics, nr = c->cfg.ndir, db = c->cfg.divb == 8 ? 0 : c->cfg.divb;
+  // and so does Linde's cleaning, whose source is a kernel of its own
+  const int nd = c->cfg.ndim, ph = c->cfg.physics, nr = c->cfg.ndir;
+  const int db = (c->cfg.divb == 8 || c->cfg.divb == 4) ? 0 : c->cfg.divb;
   if (nd == 2 && ph == 1 && nr == 2) return F<Phys<2, 2, false, DIVB_NONE>>::run(args...);
   if (nd == 2 && ph == 2 && nr == 3 && db == DIVB_GLM) return F<Phys<2, 3, true, DIVB_GLM>>::run(args...);
-  if (nd == 2 && ph == 2 && nr == 3 && db == DIVB_LINDE) return F<Phys<2, 3, true, DIVB_LINDE>>::run(args...);
   if (nd == 2 && ph == 2 && nr == 3 && db == DIVB_NONE) return F<Phys<2, 3, true, DIVB_NONE>>::run(args...);
+  if (nd == 2 && ph == 2 && nr == 2 && db == DIVB_GLM) return F<Phys<2, 2, true, DIVB_GLM>>::run(args...);
+  if (nd == 2 && ph == 2 && nr == 2 && db == DIVB_NONE) return F<Phys<2, 2, true, DIVB_NONE>>::run(args...);
   if (nd == 3 && ph == 1 && nr == 3) return F<Phys<3, 3, false, DIVB_NONE>>::run(args...);
   if (nd == 3 && ph == 2 && nr == 3 && db == DIVB_GLM) return F<Phys<3, 3, true, DIVB_GLM>>::run(args...);
   if (nd == 3 && ph == 2 && nr == 3 && db == DIVB_NONE) return F<Phys<3, 3, true, DIVB_NONE>>::run(args...);
@@ -728,10 +860,18 @@ hipError_t with_phys(const amrvac_fv_ctx* c, Args... args) {
 }
 
 template <class P>
+struct LindeOp {
+  static hipError_t run(const fvc::SweepArgs* a, const fvc::FVParams* fp, const int* ntype, int nnb, double* dst,
+                        hipStream_t s) {
+    return fv::launch_linde<P>(*a, *fp, ntype, nnb, dst, s);
+  }
+};
+
+template <class P>
 struct GetdtOp {
-  static hipError_t run(const double* w, int nb, int bnz, const fvc::FVParams* fp, const int* blevel, double* out,
-                        int* crash, hipStream_t s) {
-    return fv::launch_getdt<P>(w, nb, bnz, *fp, blevel, out, crash, s);
+  static hipError_t run(const double* w, int nb, int bx, int bnz, const fvc::FVParams* fp, const int* blevel,
+                        double* out, int* crash, hipStream_t s) {
+    return fv::launch_getdt<P>(w, nb, bx, bnz, *fp, blevel, out, crash, s);
   }
 };
 
@@ -750,19 +890,31 @@ int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
   if (c.ndir < c.ndim || c.ndir > 3) return fail(AMRVAC_FV_EINVAL, "need ndim <= ndir <= 3");
   if (c.physics != 1 && c.physics != 2) return fail(AMRVAC_FV_EINVAL, "physics must be 1 (hd) or 2 (mhd)");
   if (c.physics == 1 && c.ndir != c.ndim) return fail(AMRVAC_FV_EINVAL, "hd: ndir must equal ndim");
-  if (c.physics == 2 && c.ndir != 3) return fail(AMRVAC_FV_EINVAL, "mhd: ndir must be 3");
+  if (c.physics == 2 && !(c.ndir == 3 || (c.ndir == 2 && c.ndim == 2)))
+    return fail(AMRVAC_FV_EINVAL, "mhd: ndir must be 3 (or 2 in 2-D)");
   if (c.physics == 2 && c.divb != 0 && c.divb != 1 && c.divb != 4 && c.divb != 8)
     return fail(AMRVAC_FV_EINVAL, "typedivbfix must be none (0), glm (1), linde (4) or ct (8)");
   if (c.physics == 2 && c.divb == 8) {
     if (c.ndim != 2) return fail(AMRVAC_FV_EINVAL, "constrained transport is supported for ndim = 2");
+    if (c.ndir != 3) return fail(AMRVAC_FV_EINVAL, "constrained transport needs ndir = 3");
+    if (c.block_nx[0] != 16) return fail(AMRVAC_FV_EINVAL, "constrained transport needs block_nx = 16");
     if (c.type_ct < 1 || c.type_ct > 3) return fail(AMRVAC_FV_EINVAL, "type_ct must be 1, 2 or 3");
     if (c.limiter == 16) return fail(AMRVAC_FV_EINVAL, "constrained transport takes a TVD limiter");
   }
-  if (c.physics == 2 && c.divb == 4 && c.ndim != 2)
-    return fail(AMRVAC_FV_EINVAL, "Linde div B cleaning is supported for ndim = 2 only");
-  if (c.block_nx[0] != 16 || c.block_nx[1] != 16)
-    return fail(AMRVAC_FV_EINVAL, "block_nx(1) and block_nx(2) must be 16");
+  if (c.block_nx[0] != c.block_nx[1])
+    return fail(AMRVAC_FV_EINVAL, "block_nx(1) and block_nx(2) must be equal");
+  if (c.block_nx[0] != 16) {
+    if (c.block_nx[0] != 8 && c.block_nx[0] != 10)
+      return fail(AMRVAC_FV_EINVAL, "block_nx(1) = block_nx(2) must be 16, or 8 or 10 in 3-D");
+    if (c.ndim != 3) return fail(AMRVAC_FV_EINVAL, "block_nx(1) = block_nx(2) must be 16 in 2-D");
+    if (c.precision == AMRVAC_FV_PREC_MIXED)
+      return fail(AMRVAC_FV_EINVAL, "the mixed form needs block_nx(1) = block_nx(2) = 16");
+  }
   if (c.ndim == 3 && c.block_nx[2] < 4) return fail(AMRVAC_FV_EINVAL, "block_nx(3) must be at least 4");
+  if (c.physics == 2 && c.divb == 4 &&
+      (c.block_nx[0] + 2) * (c.block_nx[0] + 2) * (c.ndim == 3 ? c.block_nx[2] + 2 : 1) > fv::kLindeLds)
+    return fail(AMRVAC_FV_EINVAL, "Linde cleaning: block_nx(3) too large for the div B tile (at most 22 with "
+                                  "16^2 cross-sections)");
   const bool weno = c.limiter == 16;
   if (!weno && !(c.limiter >= 1 && c.limiter <= 9))
     return fail(AMRVAC_FV_EINVAL, "unsupported limiter code");
@@ -781,7 +933,8 @@ int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
   x->nws = (c.physics == 2 && c.divb == 8) ? c.ndim : 0;
   x->nnb = c.ndim == 2 ? 9 : 27;
   x->bnz = c.ndim == 3 ? c.block_nx[2] : 1;
-  x->cells = (size_t)16 * 16 * x->bnz;
+  x->bx = c.block_nx[0];
+  x->cells = (size_t)x->bx * x->bx * x->bnz;
   for (int d = 0; d < 3; ++d) x->nxg[d] = d < c.ndim ? c.block_nx[d] + 2 * c.nghost : 1;
   x->cells_g = (size_t)x->nxg[0] * x->nxg[1] * x->nxg[2];
   hipError_t e = hipStreamCreate(&x->stream);
@@ -790,6 +943,12 @@ int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
     return fail(AMRVAC_FV_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
   x->own_stream = true;
+  const int rc = scratch_address(&x->scratch);
+  if (rc != AMRVAC_FV_OK) {
+    (void)hipStreamDestroy(x->stream);
+    delete x;
+    return rc;
+  }
   *out = x;
   return AMRVAC_FV_OK;
 }
@@ -805,6 +964,9 @@ static void free_level(amrvac_fv_ctx* c) {
   if (c->d_crash) (void)hipFree(c->d_crash);
   if (c->d_red) (void)hipFree(c->d_red);
   if (c->d_red2) (void)hipFree(c->d_red2);
+  if (c->d_dtp) (void)hipFree(c->d_dtp);
+  c->d_dtp = nullptr;
+  c->dt_valid = false;
   if (c->d_stage) (void)hipFree(c->d_stage);
   c->d_nbr = nullptr; c->d_crash = nullptr; c->d_red = nullptr; c->d_red2 = nullptr; c->d_stage = nullptr;
   void* amr_bufs[] = {c->d_level, c->d_ntype, c->d_nchild, c->d_rtask, c->d_ptask, c->d_ptask2,
@@ -911,6 +1073,7 @@ static int alloc_level(amrvac_fv_ctx* c, int nblocks, int capacity, const double
       hipMalloc(&c->d_crash, sizeof(int)) != hipSuccess ||
       hipMalloc(&c->d_red, sizeof(double) * (size_t)nblocks * 2 * (c->nw > 1 ? c->nw : 1)) != hipSuccess ||
       hipMalloc(&c->d_red2, sizeof(double) * 3) != hipSuccess ||
+      hipMalloc(&c->d_dtp, sizeof(double) * (size_t)nblocks * 2) != hipSuccess ||
       hipMalloc(&c->d_stage, sizeof(double) * (size_t)nblocks * c->nw * c->cells_g) != hipSuccess) {
     free_level(c);
     return fail(AMRVAC_FV_ENOMEM, "work buffer allocation failed");
@@ -954,6 +1117,7 @@ static int alloc_level(amrvac_fv_ctx* c, int nblocks, int capacity, const double
 int amrvac_fv_upload(amrvac_fv_ctx* c, int slot, const double* w) {
   if (!c || !w || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  if (slot == 0) c->dt_valid = false;
   const int ng = c->cfg.nghost, nd = c->cfg.ndim;
   const int nwt = c->nw + c->nws;  // with constrained transport the face field follows the cell variables
   const size_t n = (size_t)c->nb * nwt * c->cells;
@@ -966,10 +1130,10 @@ int amrvac_fv_upload(amrvac_fv_ctx* c, int slot, const double* w) {
                                : c->h_stage.data() + (size_t)c->nb * c->nw * c->cells +
                                      ((size_t)b * c->nws + (iw - c->nw)) * c->cells;
       for (int k = 0; k < c->bnz; ++k)
-        for (int j = 0; j < 16; ++j) {
+        for (int j = 0; j < c->bx; ++j) {
           const int kg = nd > 2 ? k + ng : 0;
           const double* row = src + ((size_t)kg * c->nxg[1] + (j + ng)) * c->nxg[0] + ng;
-          std::memcpy(dst + ((size_t)k * 16 + j) * 16, row, 16 * sizeof(double));
+          std::memcpy(dst + ((size_t)k * c->bx + j) * c->bx, row, c->bx * sizeof(double));
         }
     }
   const size_t nc = (size_t)c->nb * c->nw * c->cells;
@@ -985,6 +1149,7 @@ static fvc::SweepArgs base_args(const amrvac_fv_ctx* c) {
   fvc::SweepArgs a{};
   a.nbr = c->d_nbr;
   a.nblocks = c->nb;
+  a.bx = c->bx;
   a.bnz = c->bnz;
   a.per_xcd = (c->nb + 7) / 8;
   for (int s = 0; s < 6; ++s) a.bc[s] = c->bc[s];
@@ -1004,10 +1169,10 @@ static int getbc_amr(amrvac_fv_ctx* c, int slot) {
   if (c->nrt) {
     if (nd == 2)
       hipLaunchKernelGGL((fv::restrict_kernel<2>), dim3(c->nrt), dim3(fvc::NTHREADS), 0, c->stream, c->slot[slot],
-                         c->nw, c->bnz, ng, c->nnb, c->d_rtask, c->d_nchild);
+                         c->nw, c->bx, c->bnz, ng, c->nnb, c->d_rtask, c->d_nchild);
     else
       hipLaunchKernelGGL((fv::restrict_kernel<3>), dim3(c->nrt), dim3(fvc::NTHREADS), 0, c->stream, c->slot[slot],
-                         c->nw, c->bnz, ng, c->nnb, c->d_rtask, c->d_nchild);
+                         c->nw, c->bx, c->bnz, ng, c->nnb, c->d_rtask, c->d_nchild);
     HIPCHK(hipGetLastError());
   }
   if (c->npt) {
@@ -1028,11 +1193,11 @@ static int fix_conserve_amr(amrvac_fv_ctx* c, int slot) {
   if (!c->amr || !c->fix_conserve || !c->nft) return AMRVAC_FV_OK;
   if (c->cfg.ndim == 2)
     hipLaunchKernelGGL((fv::fixcons_kernel<2>), dim3(c->nft), dim3(fvc::NTHREADS), 0, c->stream, c->slot[slot],
-                       c->nw, c->bnz, c->nnb, c->d_ftask, c->d_ntype, c->d_nchild, c->d_fregoff, c->d_freg,
+                       c->nw, c->bx, c->bnz, c->nnb, c->d_ftask, c->d_ntype, c->d_nchild, c->d_fregoff, c->d_freg,
                        c->nface);
   else
     hipLaunchKernelGGL((fv::fixcons_kernel<3>), dim3(c->nft), dim3(fvc::NTHREADS), 0, c->stream, c->slot[slot],
-                       c->nw, c->bnz, c->nnb, c->d_ftask, c->d_ntype, c->d_nchild, c->d_fregoff, c->d_freg,
+                       c->nw, c->bx, c->bnz, c->nnb, c->d_ftask, c->d_ntype, c->d_nchild, c->d_fregoff, c->d_freg,
                        c->nface);
   HIPCHK(hipGetLastError());
   return AMRVAC_FV_OK;
@@ -1057,15 +1222,27 @@ static bool ensure_buf(T** buf, size_t* cap, size_t n) {
   return true;
 }
 
-static int getdt_core(amrvac_fv_ctx* c, const double* u, double courantpar, double* dtmin_out, double* cmax_out) {
-  hipError_t e = with_phys<GetdtOp>(c, u, c->nb, c->bnz, (const fvc::FVParams*)&c->fp, (const int*)c->d_level,
-                                    c->d_red, c->d_crash, c->stream);
+// reduction of the per-block CFL maxima part[2*nb]: their max and the
+// small-value flag into fin[3], copied to the host; dt and cmax from them
+static int getdt_reduce(amrvac_fv_ctx* c, double courantpar, double* dtmin_out, double* cmax_out, double* part,
+                        double* fin);
+
+// dt of state u: per-block maxima into part[2*nb], then getdt_reduce
+static int getdt_core(amrvac_fv_ctx* c, const double* u, double courantpar, double* dtmin_out, double* cmax_out,
+                      double* part, double* fin) {
+  hipError_t e = with_phys<GetdtOp>(c, u, c->nb, c->bx, c->bnz, (const fvc::FVParams*)&c->fp, (const int*)c->d_level,
+                                    part, c->d_crash, c->stream);
   if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("getdt launch: ") + hipGetErrorString(e));
-  hipLaunchKernelGGL(fv::reduce_max2_kernel, dim3(1), dim3(fvc::NTHREADS), 0, c->stream, c->d_red, c->nb, c->d_red2,
+  return getdt_reduce(c, courantpar, dtmin_out, cmax_out, part, fin);
+}
+
+static int getdt_reduce(amrvac_fv_ctx* c, double courantpar, double* dtmin_out, double* cmax_out, double* part,
+                        double* fin) {
+  hipLaunchKernelGGL(fv::reduce_max2_kernel, dim3(1), dim3(fvc::NTHREADS), 0, c->stream, part, c->nb, fin,
                      (const int*)c->d_crash);
   HIPCHK(hipGetLastError());
   double r[3];
-  HIPCHK(hipMemcpyAsync(r, c->d_red2, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(r, fin, sizeof(r), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   // the small-value flag as of this reduction; valid until the next launch
   // that may set it
@@ -1132,6 +1309,15 @@ static int stage_core(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
   }
   const fvc::KernelKey k = make_key(c, riem);
   c->crash_fresh = false;
+  // ps written on a uniform level by the production kernels: the stage also
+  // leaves the CFL partials of its output for the next getdt (the dt
+  // reduction of setdt, src/mod_dt.t:160-287, without a pass over ps)
+  const bool linde = c->cfg.physics == 2 && c->cfg.divb == 4;
+  a.linde = linde ? 1 : 0;
+  const bool fuse_dt =
+      dst_slot == 0 && !c->amr && c->nws == 0 && !linde && c->cfg.precision != 2 && !c->slot0_exposed;
+  if (fuse_dt) a.dt_part = c->d_dtp;
+  if (dst_slot == 0 || dst_slot < 0) c->dt_valid = false;
   if (crash_out) HIPCHK(hipMemsetAsync(c->d_crash, 0, sizeof(int), c->stream));
   const int grid = 8 * a.per_xcd;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1151,6 +1337,12 @@ static int stage_core(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
   if (e == hipErrorInvalidValue) return fail(AMRVAC_FV_EINVAL, "no stage kernel for this configuration");
   if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("stage kernel: ") + hipGetErrorString(e));
   if (c->timing) HIPCHK(hipEventRecord(e1, c->stream));
+  if (linde) {  // addsource2 -> add_source_linde, before refluxing
+    e = with_phys<LindeOp>(c, (const fvc::SweepArgs*)&a, (const fvc::FVParams*)&c->fp,
+                           (const int*)(c->amr ? c->d_ntype : nullptr), c->nnb, dst, c->stream);
+    if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("Linde source kernel: ") + hipGetErrorString(e));
+  }
+  if (fuse_dt) c->dt_valid = true;
   if (c->amr && dst_slot >= 0) {
     // refluxing, then the ghost cells of dst (advect1: fix_conserve, getbc)
     if (riem != fv::RIEM_HANCOCK) {
@@ -1214,6 +1406,7 @@ static int ct_stage(amrvac_fv_ctx* c, int method, double dtfactor, double dt, in
   a.qdt = qdt;
   for (int d = 0; d < 3; ++d) a.dq[d] = -qdt / c->dx[d];
   c->crash_fresh = false;
+  if (psb == 0) c->dt_valid = false;
   if (crash_out) HIPCHK(hipMemsetAsync(c->d_crash, 0, sizeof(int), c->stream));
   const bool exact = c->cfg.precision == 2;
   const double* wbase = comb_slot >= 0 ? c->wsl[comb_slot] : nullptr;
@@ -1302,7 +1495,8 @@ int amrvac_fv_crash(amrvac_fv_ctx* c, int clear, int* crash_out) {
 int amrvac_fv_getdt(amrvac_fv_ctx* c, int slot, double courantpar, double* dtmin_out, double* cmax_out) {
   if (!c || !check_slot(slot)) return fail(AMRVAC_FV_EINVAL, "bad argument");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
-  return getdt_core(c, c->slot[slot], courantpar, dtmin_out, cmax_out);
+  if (slot == 0 && c->dt_valid) return getdt_reduce(c, courantpar, dtmin_out, cmax_out, c->d_dtp, c->d_red2);
+  return getdt_core(c, c->slot[slot], courantpar, dtmin_out, cmax_out, c->d_red, c->d_red2);
 }
 
 int amrvac_fv_getdt_state(amrvac_fv_ctx* c, const double* u, double courantpar, double* dtmin_out,
@@ -1310,7 +1504,21 @@ int amrvac_fv_getdt_state(amrvac_fv_ctx* c, const double* u, double courantpar, 
   if (!c || !u) return fail(AMRVAC_FV_EINVAL, "bad argument");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
   if (c->amr) return fail(AMRVAC_FV_EINVAL, "state arrays are for a uniform level");
-  return getdt_core(c, u, courantpar, dtmin_out, cmax_out);
+  return getdt_core(c, u, courantpar, dtmin_out, cmax_out, c->d_red, c->d_red2);
+}
+
+int amrvac_fv_getdt_state_noalloc(amrvac_fv_ctx* c, const double* u, double courantpar, double* dtmin_out,
+                                  double* cmax_out) {
+  if (!c || !u) return fail(AMRVAC_FV_EINVAL, "bad argument");
+  if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
+  if (c->amr) return fail(AMRVAC_FV_EINVAL, "state arrays are for a uniform level");
+  if (c->nb > kNoallocMaxBlocks)
+    return fail(AMRVAC_FV_EINVAL, "amrvac_fv_getdt_state_noalloc takes at most 32768 blocks "
+                                  "(its scratch array holds two partial maxima per block)");
+  if (!c->scratch) return fail(AMRVAC_FV_ESTATE, "no scratch address for this device");
+  // every per-block entry and the reduced triple are overwritten: nothing
+  // in the scratch needs clearing between calls
+  return getdt_core(c, u, courantpar, dtmin_out, cmax_out, c->scratch, c->scratch + 2 * kNoallocMaxBlocks);
 }
 
 int amrvac_fv_advect1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, double cmax_global, int psa,
@@ -1360,6 +1568,7 @@ int amrvac_fv_combine(amrvac_fv_ctx* c, int dst_slot, int nops, const int* ops, 
     p.ops[q] = op;
   }
   if (depth != 1 || nk != nconst) return fail(AMRVAC_FV_EINVAL, "combination: malformed program");
+  if (dst_slot == 0) c->dt_valid = false;
   for (int i = 0; i < nconst; ++i) p.k[i] = consts[i];
   const size_t n = (size_t)c->nb * c->nw * c->cells;
   const int grid = (int)std::min<size_t>((n + fvc::NTHREADS - 1) / fvc::NTHREADS, 8192);
@@ -1380,6 +1589,7 @@ int amrvac_fv_copy(amrvac_fv_ctx* c, int dst, int src) {
   if (!c || !check_slot(dst) || !check_slot(src)) return fail(AMRVAC_FV_EINVAL, "bad slots");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
   if (dst == src) return AMRVAC_FV_OK;
+  if (dst == 0) c->dt_valid = false;
   HIPCHK(hipMemcpyAsync(c->slot[dst], c->slot[src], (size_t)c->nb * c->nw * c->cells * sizeof(double),
                         hipMemcpyDeviceToDevice, c->stream));
   if (c->nws > 0)
@@ -1422,6 +1632,12 @@ int amrvac_fv_slot_pointer(amrvac_fv_ctx* c, int slot, void** dptr, long long* n
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
   *dptr = c->slot[slot];
   if (nbytes) *nbytes = (long long)((size_t)c->nb * c->nw * c->cells * sizeof(double));
+  // writes through the pointer bypass the context: ps's CFL partials are
+  // no longer kept by the stages of this context
+  if (slot == 0) {
+    c->slot0_exposed = true;
+    c->dt_valid = false;
+  }
   return AMRVAC_FV_OK;
 }
 
@@ -1432,6 +1648,8 @@ int amrvac_fv_set_mesh_amr(amrvac_fv_ctx* c, int nblocks, const double* dx1, con
     return fail(AMRVAC_FV_EINVAL, "null argument");
   if (nblocks < 1) return fail(AMRVAC_FV_EINVAL, "nblocks must be positive");
   if (c->nws > 0) return fail(AMRVAC_FV_EINVAL, "constrained transport is provided on a uniform level");
+  if (c->bx % 2 != 0 || (c->cfg.ndim == 3 && c->bnz % 2 != 0))
+    return fail(AMRVAC_FV_EINVAL, "an AMR mesh needs even block_nx");
   const int nd = c->cfg.ndim, nnb = c->nnb, nch = 1 << nd;
   for (int b = 0; b < nblocks; ++b) {
     if (level[b] < 1 || level[b] > fvc::MAXLEV) return fail(AMRVAC_FV_EINVAL, "level out of range (1..8)");
@@ -1509,7 +1727,7 @@ int amrvac_fv_set_mesh_amr(amrvac_fv_ctx* c, int nblocks, const double* dx1, con
   c->nrt = (int)rt.size();
   c->npt = (int)pt.size();
   // refluxing registers on faces with a level jump, coarse blocks to fix
-  c->nface = 16 * (c->bnz > 16 ? c->bnz : 16);
+  c->nface = c->bx * (c->bnz > c->bx ? c->bnz : c->bx);
   std::vector<int> fregoff((size_t)nblocks * 2 * nd, -1), ftask;
   int nreg = 0;
   for (int b = 0; b < nblocks; ++b) {
@@ -1638,6 +1856,7 @@ int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt,
   if (!c || !u_in || !lam_out || !lam_in) return fail(AMRVAC_FV_EINVAL, "bad argument");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
   if (c->nws > 0) return fail(AMRVAC_FV_EINVAL, "adjoint: no constrained transport");
+  if (c->bx != 16) return fail(AMRVAC_FV_EINVAL, "the adjoint stage needs block_nx = 16");
   if (lam_in == lam_out || lam_in == u_in) return fail(AMRVAC_FV_EINVAL, "lam_in must not alias lam_out or u_in");
   if (c->cfg.physics == 2 && (c->cfg.divb != 0 || c->cfg.ndir != 3))
     return fail(AMRVAC_FV_EINVAL, "the MHD adjoint stage is provided for ndir 3 without divergence cleaning");
@@ -1677,6 +1896,7 @@ int amrvac_fv_slot_copy(amrvac_fv_ctx* c, int slot, void* dptr, int to_slot) {
   // a slot of a constrained-transport context also holds its face field,
   // which a cell-only copy would leave stale beside the restored cells
   if (c->nws > 0) return fail(AMRVAC_FV_EINVAL, "slot copies move cell variables only: not with constrained transport");
+  if (to_slot && slot == 0) c->dt_valid = false;
   const size_t n = (size_t)c->nb * c->nw * c->cells * sizeof(double);
   if (to_slot) HIPCHK(hipMemcpyAsync(c->slot[slot], dptr, n, hipMemcpyDeviceToDevice, c->stream));
   else HIPCHK(hipMemcpyAsync(dptr, c->slot[slot], n, hipMemcpyDeviceToDevice, c->stream));

@@ -829,14 +829,68 @@ __device__ __forceinline__ void face_flux(const R (*st)[P::NW], const FVParams& 
 }
 
 // cell-centred cmax for dt: hd_get_cmax src/hd/mod_hd_phys.t:787-811,
-// mhd_get_cmax_origin src/mhd/mod_mhd_phys.t:2482-2550 (FP64, primitive)
+// mhd_get_cmax_origin src/mhd/mod_mhd_phys.t:2482-2550 (FP64, primitive
+// w), written out without contraction (see cfl_cell)
 template <class P, int ID>
-__device__ __forceinline__ double cell_cmax(const double* w, const FVParams& fp) {
+__device__ __forceinline__ double cfl_cmax(const double* w, const FVParams& fp) {
+#pragma clang fp contract(off)
   if constexpr (!P::MHD) {
     return fabs(w[P::MOM + ID]) + sqrt(fp.gamma * w[P::E] / w[P::RHO]);
   } else {
-    return fabs(w[P::MOM + ID]) + fast_speed<P, double, ID>(w, fp.gamma);
+    // mhd_get_csound_prim, as fast_speed
+    const double inv_rho = frcp(w[P::RHO]);
+    const double cs = fp.gamma * w[P::E] * inv_rho;
+    double b2 = w[P::MAG] * w[P::MAG];
+#pragma unroll
+    for (int d = 1; d < P::NDIR; ++d) b2 = b2 + w[P::MAG + d] * w[P::MAG + d];
+    const double cfast2 = b2 * inv_rho + cs;
+    const double bn = w[P::MAG + ID];
+    double av = cfast2 * cfast2 - 4.0 * cs * (bn * bn) * inv_rho;
+    if (av < 0.0) av = 0.0;
+    return fabs(w[P::MOM + ID]) + fsqrt(0.5 * (cfast2 + fsqrt(av)));
   }
+}
+
+// getdt_courant (maxsum, slab_uniform; src/mod_dt.t:160-287) of one
+// conserved cell folded into the running maxima maxtot (of sum_d
+// cmax_d/dx_d) and cmaxb (of cmax).  Every operation is written here
+// without contraction whatever the translation unit's setting, so that the
+// dt kernel and the stage kernel's fused CFL partials round alike.
+template <class P>
+__device__ __forceinline__ void cfl_cell(const double* cons, const FVParams& fp, double& maxtot, double& cmaxb) {
+#pragma clang fp contract(off)
+  double w[P::NW];
+#pragma unroll
+  for (int iw = 0; iw < P::NW; ++iw) w[iw] = cons[iw];
+  // to_prim (hd_to_primitive / mhd_to_primitive_origin)
+  const double inv_rho = 1.0 / w[P::RHO];
+#pragma unroll
+  for (int d = 0; d < P::NDIR; ++d) w[P::MOM + d] = w[P::MOM + d] * inv_rho;
+  double vsq = w[P::MOM] * w[P::MOM];
+#pragma unroll
+  for (int d = 1; d < P::NDIR; ++d) vsq = vsq + w[P::MOM + d] * w[P::MOM + d];
+  if constexpr (!P::MHD) {
+    w[P::E] = (fp.gamma - 1.0) * (w[P::E] - 0.5 * w[P::RHO] * vsq);
+  } else {
+    double bsq = w[P::MAG] * w[P::MAG];
+#pragma unroll
+    for (int d = 1; d < P::NDIR; ++d) bsq = bsq + w[P::MAG + d] * w[P::MAG + d];
+    w[P::E] = fp.gamma_1 * (w[P::E] - 0.5 * (w[P::RHO] * vsq + bsq));
+  }
+  double cm = cfl_cmax<P, 0>(w, fp);
+  cmaxb = fmax(cmaxb, cm);
+  double tot = cm * fp.dxinv[0];
+  if constexpr (P::NDIM > 1) {
+    cm = cfl_cmax<P, 1>(w, fp);
+    cmaxb = fmax(cmaxb, cm);
+    tot = tot + cm * fp.dxinv[1];
+  }
+  if constexpr (P::NDIM > 2) {
+    cm = cfl_cmax<P, 2>(w, fp);
+    cmaxb = fmax(cmaxb, cm);
+    tot = tot + cm * fp.dxinv[2];
+  }
+  maxtot = fmax(maxtot, tot);
 }
 
 }  // namespace FV_NS

@@ -1,11 +1,13 @@
-// 2-D ideal MHD stage kernels (ndir = 3: 2.5-D), GLM or Linde div B control.
+// 2-D ideal MHD stage kernels with three vector components (2.5-D), GLM or
+// no div B control in the stage (Linde's source is a kernel of its own);
+// two components go to fv_k2_mhd_ndir2.hip.
 #include "fv_launch.h"
 namespace FV_NS {
 hipError_t launch_sweep_2d_mhd(const KernelKey& k, const SweepArgs& a, const FVParams& fp,
                                         int grid, hipStream_t s) {
+  if (k.ndir == 2) return launch_sweep_2d_mhd_ndir2(k, a, fp, grid, s);
   if (k.ndir != 3) return hipErrorInvalidValue;
   if (k.divb == DIVB_GLM) return launch_prec<Phys<2, 3, true, DIVB_GLM>>(k, a, fp, grid, s);
-  if (k.divb == DIVB_LINDE) return launch_prec<Phys<2, 3, true, DIVB_LINDE>>(k, a, fp, grid, s);
   if (k.divb == DIVB_NONE) return launch_prec<Phys<2, 3, true, DIVB_NONE>>(k, a, fp, grid, s);
   return hipErrorInvalidValue;
 }

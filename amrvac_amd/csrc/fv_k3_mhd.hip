@@ -1,4 +1,5 @@
-// 3-D ideal MHD stage kernels (ndir = 3), GLM div B control or none.
+// 3-D ideal MHD stage kernels (ndir = 3), GLM div B control or none in the
+// stage (Linde's source is a kernel of its own).
 #include "fv_launch.h"
 namespace FV_NS {
 hipError_t launch_sweep_3d_mhd(const KernelKey& k, const SweepArgs& a, const FVParams& fp,

@@ -45,6 +45,8 @@ using SweepLauncher = hipError_t (*)(const KernelKey&, const SweepArgs&, const F
                                 int, hipStream_t);                                                   \
   hipError_t launch_sweep_2d_mhd(const fvc::KernelKey&, const fvc::SweepArgs&, const fvc::FVParams&,\
                                  int, hipStream_t);                                                  \
+  hipError_t launch_sweep_2d_mhd_ndir2(const fvc::KernelKey&, const fvc::SweepArgs&,                  \
+                                       const fvc::FVParams&, int, hipStream_t);                      \
   hipError_t launch_sweep_3d_hd(const fvc::KernelKey&, const fvc::SweepArgs&, const fvc::FVParams&, \
                                 int, hipStream_t);                                                   \
   hipError_t launch_sweep_3d_mhd(const fvc::KernelKey&, const fvc::SweepArgs&, const fvc::FVParams&,\
@@ -73,15 +75,42 @@ using fvc::KernelKey;
 template <class P, class R, int RIEM, int LIM, int NG>
 hipError_t launch_one(const SweepArgs& a, const FVParams& fp, int grid, hipStream_t s) {
   if (a.blevel)
-    hipLaunchKernelGGL((sweep_kernel<P, R, RIEM, LIM, NG, true>), dim3(grid), dim3(NTHREADS), 0, s, a, fp);
+    hipLaunchKernelGGL((sweep_kernel<P, R, RIEM, LIM, NG, true, 16>), dim3(grid), dim3(kLanes<16>), 0, s, a, fp);
   else
-    hipLaunchKernelGGL((sweep_kernel<P, R, RIEM, LIM, NG, false>), dim3(grid), dim3(NTHREADS), 0, s, a, fp);
+    hipLaunchKernelGGL((sweep_kernel<P, R, RIEM, LIM, NG, false, 16>), dim3(grid), dim3(kLanes<16>), 0, s, a,
+                       fp);
+  return hipGetLastError();
+}
+
+// 8^3 and 10^3 blocks (the block sizes of the reference's 3-D regression
+// setups): FP64, the runtime-switched TVD limiters or WENO5, one kernel for
+// uniform and AMR levels
+template <class P, class R, int RIEM, int NG, int BXc>
+hipError_t launch_small(const SweepArgs& a, const FVParams& fp, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((sweep_kernel<P, R, RIEM, NG == 3 ? LIM_WENO5 : 0, NG, true, BXc>), dim3(grid),
+                     dim3(kLanes<BXc>), 0, s, a, fp);
   return hipGetLastError();
 }
 
 // TVD (runtime limiter) or WENO5 for one physics / precision / solver
 template <class P, class R, int RIEM>
 hipError_t launch_lim(const KernelKey& k, const SweepArgs& a, const FVParams& fp, int grid, hipStream_t s) {
+  if (a.bx != 16) {
+    if constexpr (!std::is_same<R, double>::value || P::NDIM != 3) {
+      return hipErrorInvalidValue;
+    } else if constexpr (RIEM == RIEM_HANCOCK) {
+      if (k.weno) return hipErrorInvalidValue;
+      if (a.bx == 8) return launch_small<P, R, RIEM, 2, 8>(a, fp, grid, s);
+      if (a.bx == 10) return launch_small<P, R, RIEM, 2, 10>(a, fp, grid, s);
+      return hipErrorInvalidValue;
+    } else {
+      if (a.bx == 8) return k.weno ? launch_small<P, R, RIEM, 3, 8>(a, fp, grid, s)
+                                   : launch_small<P, R, RIEM, 2, 8>(a, fp, grid, s);
+      if (a.bx == 10) return k.weno ? launch_small<P, R, RIEM, 3, 10>(a, fp, grid, s)
+                                    : launch_small<P, R, RIEM, 2, 10>(a, fp, grid, s);
+      return hipErrorInvalidValue;
+    }
+  }
   if (k.weno) {
     if constexpr (RIEM == RIEM_HANCOCK) return hipErrorInvalidValue;
     else return launch_one<P, R, RIEM, LIM_WENO5, 3>(a, fp, grid, s);

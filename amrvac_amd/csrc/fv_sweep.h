@@ -1,7 +1,8 @@
 // fv_sweep.h -- the fused finite-volume stage kernel.
 //
-// One workgroup (256 lanes = 16x16) updates one 16x16(x16) block of the
-// level.  The block's stencil data never round-trips through HBM ghost
+// One workgroup updates one block of the level; a lane owns one (x, y)
+// column of the block (BX x BX lanes: 256 for 16^2 blocks, rounded up to
+// whole waves for 8^2 and 10^2).  The block's stencil data never round-trips through HBM ghost
 // cells: halo cells are read straight from the face neighbours' interiors
 // through the level's neighbour table (or from a physical-boundary rule),
 // converted to primitive variables once and staged in LDS.  In 3-D the
@@ -23,7 +24,7 @@
 #include "fv_device.h"
 
 namespace fvc {
-constexpr int BX = 16;  // block_nx(1) = block_nx(2) = 16
+constexpr int BX = 16;  // block_nx(1) = block_nx(2) of the fastest kernels
 constexpr int NTHREADS = 256;
 
 struct SweepArgs {
@@ -32,6 +33,7 @@ struct SweepArgs {
   double* dst;         // psb
   const int* nbr;      // [nb][3^ndim]
   int nblocks;
+  int bx;              // block_nx(1) = block_nx(2)
   int bnz;             // block_nx(3) (1 in 2-D)
   int per_xcd;         // ceil(nblocks/8)
   int comb_mode;       // 0: init = dst, 1: init = base, 2: init = a*base + b*src,
@@ -49,6 +51,11 @@ struct SweepArgs {
   double* freg;             // scaled block-boundary face fluxes (refluxing)
   const int* fregoff;       // [nb][2*ndim] register offset of a face, -1: none
   int nface;                // transverse cells of a face (register stride)
+  int linde;                // Linde's source follows the stage (linde_kernel)
+  // CFL partials of the output (uniform level): per block the max over
+  // cells of sum_d cmax_d/dx_d and of cmax, as getdt_courant needs them;
+  // nullptr: not wanted
+  double* dt_part;
 };
 
 constexpr int MAXLEV = 8;
@@ -59,6 +66,10 @@ namespace FV_NS {
 using fvc::BX;
 using fvc::NTHREADS;
 using fvc::SweepArgs;
+
+// lanes of the stage kernel for BX x BX blocks (whole waves)
+template <int BXc>
+constexpr int kLanes = (BXc * BXc + 63) / 64 * 64;
 
 // Block id of a workgroup: consecutive blocks of the Morton-ordered level go
 // to the same XCD (workgroups are dealt round-robin over the 8 XCDs), so that
@@ -71,10 +82,12 @@ __device__ __forceinline__ int wg_block(int per_xcd) {
 // Resolve the source of cell (x,y,z) (block-local, may lie outside the
 // block by up to NG) for block b: returns the source block and rewrites the
 // coordinates into it; *sign = -1 for an antisymmetric boundary.
-template <int NDIM>
+// (BXc = 0: the block size a.bx of the arguments; else the compile-time one)
+template <int NDIM, int BXc = 0>
 __device__ __forceinline__ int locate(const SweepArgs& a, int b, int& x, int& y, int& z, double& sign) {
   int c[3] = {x, y, z};
-  const int n[3] = {BX, BX, a.bnz};
+  const int bx = BXc ? BXc : a.bx;
+  const int n[3] = {bx, bx, a.bnz};
   int dir[3] = {0, 0, 0};
   sign = 1.0;
 #pragma unroll
@@ -123,21 +136,21 @@ __device__ __forceinline__ int locate(const SweepArgs& a, int b, int& x, int& y,
 }
 
 template <class P>
-__device__ __forceinline__ void load_cons(const double* __restrict__ w, int b, int x, int y, int z,
+__device__ __forceinline__ void load_cons(const double* __restrict__ w, int b, int x, int y, int z, int bx,
                                           int bnz, double* c) {
-  const size_t cells = (size_t)BX * BX * bnz;
-  const size_t off = (size_t)b * P::NW * cells + ((size_t)z * BX + y) * BX + x;
+  const size_t cells = (size_t)bx * bx * bnz;
+  const size_t off = (size_t)b * P::NW * cells + ((size_t)z * bx + y) * bx + x;
 #pragma unroll
   for (int iw = 0; iw < P::NW; ++iw) c[iw] = w[off + iw * cells];
 }
 
 // conserved state of an arbitrary (possibly halo) cell, with the sign of an
 // antisymmetric boundary applied
-template <class P, int NDIM>
+template <class P, int NDIM, int BXc = 0>
 __device__ __forceinline__ void load_cons_at(const SweepArgs& a, int b, int x, int y, int z, double* c) {
   double sign;
-  const int sb = locate<NDIM>(a, b, x, y, z, sign);
-  load_cons<P>(a.src, sb, x, y, z, a.bnz, c);
+  const int sb = locate<NDIM, BXc>(a, b, x, y, z, sign);
+  load_cons<P>(a.src, sb, x, y, z, BXc ? BXc : a.bx, a.bnz, c);
   if (sign < 0.0) {
 #pragma unroll
     for (int iw = 0; iw < P::NW; ++iw) c[iw] = -c[iw];
@@ -154,11 +167,11 @@ __device__ __forceinline__ void prim_of(const double* c, R* out, const FVParams&
 }
 
 // primitive state of an arbitrary (possibly halo) cell
-template <class P, class R, int NDIM>
+template <class P, class R, int NDIM, int BXc = 0>
 __device__ __forceinline__ void load_prim(const SweepArgs& a, const FVParams& fp, int b, int x, int y,
                                           int z, R* w) {
   double c[P::NW];
-  load_cons_at<P, NDIM>(a, b, x, y, z, c);
+  load_cons_at<P, NDIM, BXc>(a, b, x, y, z, c);
   prim_of<P, R>(c, w, fp);
 }
 
@@ -170,7 +183,7 @@ __device__ __forceinline__ void st_stream(double v, double* p) { __builtin_nonte
 
 // Cell t of the four NG-deep halo strips of a plane (x and y block-local,
 // outside the block on one side), t < 4*NG*BX; then the 4 NG x NG corners.
-template <int NG>
+template <int NG, int BX>
 __device__ __forceinline__ void strip_cell(int t, int& x, int& y) {
   constexpr int NSTRIP = 4 * NG * BX;
   if (t < NSTRIP) {
@@ -220,17 +233,18 @@ __device__ __forceinline__ void level_params(const SweepArgs& a, int b, FVParams
 // AMR: level-dependent scheme constants and the refluxing registers; the
 // uniform-level kernels are compiled without them (their registers would
 // spill in the 3-D MHD kernels).
-template <class P, class R, int RIEM, int LIM, int NG, bool AMR>
-__global__ void __launch_bounds__(NTHREADS, kMinWaves<R>)
+template <class P, class R, int RIEM, int LIM, int NG, bool AMR, int BXc>
+__global__ void __launch_bounds__(kLanes<BXc>, kMinWaves<R>)
 sweep_kernel(SweepArgs a, FVParams fp0) {
+  constexpr int BX = BXc;                 // block_nx(1) = block_nx(2)
+  constexpr int NTHREADS = kLanes<BXc>;   // lanes of the workgroup
+  constexpr bool FULL = BX * BX == NTHREADS;
   constexpr int NDIM = P::NDIM;
   constexpr int NW = P::NW;
   constexpr int PX = BX + 2 * NG;
-  constexpr bool CORNERS = P::LINDE;  // Linde needs div B one cell out
   constexpr bool HANCOCK = RIEM == RIEM_HANCOCK;
-  static_assert(!(P::LINDE && NDIM == 3), "Linde cleaning is 2-D only on the device path");
   constexpr int NSTRIP = 4 * NG * BX;
-  constexpr int NHALO = NSTRIP + (CORNERS ? 4 * NG * NG : 0);
+  constexpr int NHALO = NSTRIP;
   // 3-D, FP32 reconstruction: the halo strips of the NG planes ahead are
   // loaded together with the window's lookahead plane and kept in an LDS ring
   // (the face neighbours read the same planes at the same time, so the loads
@@ -258,10 +272,15 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
     glm_decay = a.glm_decay;
   }
   const int tid = threadIdx.x;
-  const int tx = tid & (BX - 1);
-  const int ty = tid >> 4;
+  // lane -> column; the lanes past BX*BX (8^2 and 10^2 blocks) repeat
+  // column 0's arithmetic and store nothing of it
+  const bool own = FULL || tid < BX * BX;
+  const int cid = own ? tid : 0;
+  const int tx = cid % BX;
+  const int ty = cid / BX;
   const R ch = R(a.ch);
   int crash = 0;
+  double dt_maxtot = -1.0e99, dt_cmax = 0.0;  // CFL partials of this lane's column
   const size_t cells = (size_t)BX * BX * a.bnz;
   const size_t boff = (size_t)b * NW * cells;
 
@@ -286,16 +305,16 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
     // prologue: planes -NG .. NG-1, halo ring of planes 0 .. NG-1, carried
     // face -1/2
 #pragma unroll
-    for (int m = 0; m < NG; ++m) load_prim<P, R, NDIM>(a, fp, b, tx, ty, m - NG, Wp[m]);
+    for (int m = 0; m < NG; ++m) load_prim<P, R, NDIM, BX>(a, fp, b, tx, ty, m - NG, Wp[m]);
 #pragma unroll
-    for (int m = 0; m < NG; ++m) load_cons_at<P, NDIM>(a, b, tx, ty, m, Wc[m]);
+    for (int m = 0; m < NG; ++m) load_cons_at<P, NDIM, BX>(a, b, tx, ty, m, Wc[m]);
     if constexpr (HRING) {
       for (int p = 0; p < NG && p < a.bnz; ++p)
         for (int t = tid; t < NSTRIP; t += NTHREADS) {
           int x, y;
-          strip_cell<NG>(t, x, y);
+          strip_cell<NG, BX>(t, x, y);
           R w[NW];
-          load_prim<P, R, NDIM>(a, fp, b, x, y, p, w);
+          load_prim<P, R, NDIM, BX>(a, fp, b, x, y, p, w);
 #pragma unroll
           for (int iw = 0; iw < NW; ++iw) hring[p][iw][t] = w[iw];
         }
@@ -348,32 +367,34 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
       for (int m = 0; m < NG - 1; ++m)
 #pragma unroll
         for (int iw = 0; iw < NW; ++iw) Wc[m][iw] = Wc[m + 1][iw];
-      load_cons_at<P, NDIM>(a, b, tx, ty, k + NG, Wc[NG - 1]);
+      load_cons_at<P, NDIM, BX>(a, b, tx, ty, k + NG, Wc[NG - 1]);
+      if (own)
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) sp[iw][ty + NG][tx + NG] = Wp[NG - 1][iw];
+        for (int iw = 0; iw < NW; ++iw) sp[iw][ty + NG][tx + NG] = Wp[NG - 1][iw];
     } else {
-      load_cons<P>(a.src, b, tx, ty, 0, 1, cons_k);
+      load_cons<P>(a.src, b, tx, ty, 0, BX, 1, cons_k);
       if (fp.fix_small && small_cons<P>(cons_k, fp)) crash = 1;
       R w[NW];
       prim_of<P, R>(cons_k, w, fp);
+      if (own)
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) sp[iw][ty + NG][tx + NG] = w[iw];
+        for (int iw = 0; iw < NW; ++iw) sp[iw][ty + NG][tx + NG] = w[iw];
     }
     if constexpr (HRING) {
       const int slot = k % NG;
       for (int t = tid; t < NSTRIP; t += NTHREADS) {
         int x, y;
-        strip_cell<NG>(t, x, y);
+        strip_cell<NG, BX>(t, x, y);
 #pragma unroll
         for (int iw = 0; iw < NW; ++iw) sp[iw][y + NG][x + NG] = hring[slot][iw][t];
       }
     } else {
-      // halo strips (+ corners for Linde) of plane k
+      // halo strips of plane k
       for (int t = tid; t < NHALO; t += NTHREADS) {
         int x, y;
-        strip_cell<NG>(t, x, y);
+        strip_cell<NG, BX>(t, x, y);
         R w[NW];
-        load_prim<P, R, NDIM>(a, fp, b, x, y, k, w);
+        load_prim<P, R, NDIM, BX>(a, fp, b, x, y, k, w);
 #pragma unroll
         for (int iw = 0; iw < NW; ++iw) sp[iw][y + NG][x + NG] = w[iw];
       }
@@ -385,9 +406,9 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
         const int slot = k % NG;
         for (int t = tid; t < NSTRIP; t += NTHREADS) {
           int x, y;
-          strip_cell<NG>(t, x, y);
+          strip_cell<NG, BX>(t, x, y);
           R w[NW];
-          load_prim<P, R, NDIM>(a, fp, b, x, y, k + NG, w);
+          load_prim<P, R, NDIM, BX>(a, fp, b, x, y, k + NG, w);
 #pragma unroll
           for (int iw = 0; iw < NW; ++iw) hring[slot][iw][t] = w[iw];
         }
@@ -404,9 +425,9 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
 
     if constexpr (!HANCOCK) {
       // ---- face fluxes of this plane --------------------------------------
-      constexpr int NXF = BX * (BX + 1);      // 272 x faces
-      constexpr int XSLOTS = 320;             // padded to a wave multiple
-      constexpr int NSLOT = XSLOTS + NXF;     // + 272 y faces
+      constexpr int NXF = BX * (BX + 1);             // x faces (272 for 16^2)
+      constexpr int XSLOTS = (NXF + 63) / 64 * 64;   // padded to a wave multiple
+      constexpr int NSLOT = XSLOTS + NXF;            // + the y faces
       for (int t = tid; t < NSLOT; t += NTHREADS) {
         if (t < NXF) {
           const int y = t / (BX + 1), f = t % (BX + 1);  // face between f-1 and f
@@ -446,7 +467,7 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
       __syncthreads();
       // ---- store_flux (src/mod_fix_conserve.t:597-648): the scaled fluxes
       //      of the block-boundary faces, for refluxing --------------------
-      if (AMR && a.freg) {
+      if (AMR && a.freg && own) {
         const size_t nf = (size_t)a.nface;
 #pragma unroll
         for (int iw = 0; iw < NW; ++iw) {
@@ -521,8 +542,9 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
     //      crash, after which the run stops).  The reconstructed face
     //      states of the TVD limiters lie between their cells' values, so
     //      the face checks of reconstruct_LR (:1213-1216) flag nothing more
-    if constexpr (!(HANCOCK && (P::GLM || P::LINDE)))  // hancock checks after its sources only
-      if (fp.fix_small && small_cons<P>(acc, fp)) crash = 1;
+    // (hancock checks only after its sources, GLM here or Linde's kernel)
+    if constexpr (!(HANCOCK && P::GLM))
+      if (!(HANCOCK && a.linde) && fp.fix_small && small_cons<P>(acc, fp)) crash = 1;
     // ---- sources: addsource2 -> mhd_add_source (src/mod_source.t:130-160,
     //      src/mhd/mod_mhd_phys.t:4565-4695) -----------------------------
     if constexpr (P::GLM) {
@@ -549,35 +571,37 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
         for (int d = 0; d < P::NDIR; ++d) acc[P::MOM + d] = acc[P::MOM + d] - a.qdt * bn[d] * divb;
       }
     }
-    if constexpr (P::LINDE) {
-      // add_source_linde src/mhd/mod_mhd_phys.t:5599-5664 (2-D, one level)
-      const int cx = tx + NG, cy = ty + NG;
-      auto divb_at = [&](int x, int y) {
-        double d = 0.0;
-        d = d + 0.5 * (double(sp[P::MAG][y][x + 1]) - double(sp[P::MAG][y][x - 1])) * fp.dxinv[0];
-        d = d + 0.5 * (double(sp[P::MAG + 1][y + 1][x]) - double(sp[P::MAG + 1][y - 1][x])) * fp.dxinv[1];
-        return d;
-      };
-      const double dxp = divb_at(cx + 1, cy), dxm = divb_at(cx - 1, cy);
-      const double dyp = divb_at(cx, cy + 1), dym = divb_at(cx, cy - 1);
-      double g = ((dxp - dxm) * 0.5) / fp.dx[0];
-      g = g * fp.divbdiff / fp.invds2;
-      acc[P::MAG] = acc[P::MAG] + g;
-      acc[P::E] = acc[P::E] + double(sp[P::MAG][cy][cx]) * g;
-      g = ((dyp - dym) * 0.5) / fp.dx[1];
-      g = g * fp.divbdiff / fp.invds2;
-      acc[P::MAG + 1] = acc[P::MAG + 1] + g;
-      acc[P::E] = acc[P::E] + double(sp[P::MAG + 1][cy][cx]) * g;
-    }
-
-    // the GLM / Linde sources check again (src/mhd/mod_mhd_phys.t:5529, 5662)
-    if constexpr (P::GLM || P::LINDE)
+    // Linde's source (add_source_linde) follows in linde_kernel: div B at
+    // the neighbours needs the field beyond this kernel's planes and rim
+    // the GLM source checks again (src/mhd/mod_mhd_phys.t:5529)
+    if constexpr (P::GLM)
       if (fp.fix_small && small_cons<P>(acc, fp)) crash = 1;
+    if (own)
 #pragma unroll
-    for (int iw = 0; iw < NW; ++iw) st_stream(acc[iw], a.dst + o + iw * cells);
+      for (int iw = 0; iw < NW; ++iw) st_stream(acc[iw], a.dst + o + iw * cells);
+    if constexpr (!AMR)
+      if (a.dt_part) cfl_cell<P>(acc, fp, dt_maxtot, dt_cmax);
     __syncthreads();  // LDS plane is rewritten next iteration
   }
   if (crash) *a.crash = 1;
+  if constexpr (!AMR) {
+    if (a.dt_part) {
+      // block max: wave64 shuffles, then the 4 waves through LDS
+      for (int off = 32; off > 0; off >>= 1) {
+        dt_maxtot = fmax(dt_maxtot, __shfl_xor(dt_maxtot, off, 64));
+        dt_cmax = fmax(dt_cmax, __shfl_xor(dt_cmax, off, 64));
+      }
+      __shared__ double dtr[2][NTHREADS / 64];
+      if ((tid & 63) == 0) { dtr[0][tid >> 6] = dt_maxtot; dtr[1][tid >> 6] = dt_cmax; }
+      __syncthreads();
+      if (tid == 0) {
+        double m = dtr[0][0], c = dtr[1][0];
+        for (int i = 1; i < NTHREADS / 64; ++i) { m = fmax(m, dtr[0][i]); c = fmax(c, dtr[1][i]); }
+        a.dt_part[2 * b] = m;
+        a.dt_part[2 * b + 1] = c;
+      }
+    }
+  }
 }
 
 }  // namespace FV_NS

@@ -28,7 +28,7 @@ EXPORTS = ("amrvac_fv_init", "amrvac_fv_finalize", "amrvac_fv_set_stream", "amrv
            "amrvac_fv_sync", "amrvac_fv_sweep_timing", "amrvac_fv_last_error",
            "amrvac_fv_set_mesh_amr", "amrvac_fv_getbc", "amrvac_fv_set_fix_conserve", "amrvac_fv_adjoint1",
            "amrvac_fv_slot_copy", "amrvac_fv_stage", "amrvac_fv_getdt_state", "amrvac_fv_errest",
-           "amrvac_fv_regrid", "amrvac_fv_crash")
+           "amrvac_fv_regrid", "amrvac_fv_crash", "amrvac_fv_getdt_state_noalloc")
 
 
 class FVConfig(ctypes.Structure):
@@ -94,6 +94,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.amrvac_fv_regrid.argtypes = [vp, ctypes.c_int, ip, dp, ip, ip, ip, ip, ip, dp, ip]
     L.amrvac_fv_slot_copy.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
     L.amrvac_fv_crash.argtypes = [vp, ctypes.c_int, ip]
+    L.amrvac_fv_getdt_state_noalloc.argtypes = [vp, vp, dbl, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
     _lib = L
     return L
 

@@ -211,6 +211,17 @@ int amrvac_fv_stage(amrvac_fv_ctx* ctx, int method, double dtfactor, double dt,
 int amrvac_fv_getdt_state(amrvac_fv_ctx* ctx, const double* u, double courantpar,
                           double* dtmin_out, double* cmax_out);
 
+/* amrvac_fv_getdt_state without any allocation (no counterpart in the
+   reference): for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Its per-block partial maxima live in a file-scope device array of the
+   library (512 KiB per GPU, held until the process ends) instead of the
+   context's buffers; at most 32768 blocks (a larger level is rejected).
+   All calls on one GPU share that array: calls that overlap on one GPU
+   race. */
+int amrvac_fv_getdt_state_noalloc(amrvac_fv_ctx* ctx, const double* u, double courantpar,
+                                  double* dtmin_out, double* cmax_out);
+
 /* Adjoint of one stage (no counterpart in the reference, which is not
    differentiable; the reverse of finite_volume's flux divergence,
    src/mod_finite_volume.t:116-339, on a uniform periodic level with

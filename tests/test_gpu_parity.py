@@ -202,3 +202,28 @@ def test_download_ghosts_match_oracle_getbc():
     d = device_backend(par, mesh, "fp64")
     d.set_state(0, w0)
     assert np.array_equal(d.get_state(0), o.get_state(0))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_cfl_partials_leave_the_dt_sequence_unchanged(precision):
+    """On a uniform level the stage that writes ps also leaves the per-block
+    CFL maxima of its output, and the next getdt only reduces them.  The dt
+    sequence and the state must equal those of the separate dt pass bit
+    for bit (a context whose ps was handed out by amrvac_fv_slot_pointer
+    keeps to the separate pass)."""
+    from amrvac_amd.fv import DeviceFV
+    print(f"[parity] fused CFL partials, 3-D MHD-GLM {precision}", flush=True)
+    par = Par(physics="mhd", ndim=3, ndir=3, gamma=5.0 / 3.0, typedivbfix="glm", time_stepper="threestep",
+              flux_scheme="hll", limiter="cada3", courantpar=0.6)
+    mesh = UniformMesh(3, (64, 64, 48), (16, 16, 16), (0.0,) * 3, (1.0, 1.0, 0.75), ("periodic",) * 6)
+    w0 = problems.orszag_tang_3d(mesh)
+    out = {}
+    for fused in (True, False):
+        be = DeviceFV(par, mesh, precision=precision)
+        if not fused:
+            be.slot_pointer(0)
+        w, st = run(par, mesh, w0, be, 6)
+        out[fused] = (w, st.dts, st.cmax_global)
+    assert out[True][1] == out[False][1]
+    assert out[True][2] == out[False][2]
+    assert np.array_equal(out[True][0], out[False][0])
