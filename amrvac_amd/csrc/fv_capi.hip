@@ -641,7 +641,8 @@ __global__ void __launch_bounds__(NTHREADS) linde_kernel(SweepArgs a, FVParams f
   for (int c = threadIdx.x; c < bx * bx * bz; c += NTHREADS) {
     const int x = c % bx, y = (c / bx) % bx, z = NDIM == 3 ? c / (bx * bx) : 0;
     if (x < lo[0] || x > hi[0] || y < lo[1] || y > hi[1] || z < lo[2] || z > hi[2]) continue;
-    const int l = (x + 1) + ex * ((y + 1) + ex * (z + 1));
+    // the tile's layout as filled above: no z rim in 2-D
+    const int l = (x + 1) + ex * ((y + 1) + ex * (NDIM == 3 ? z + 1 : 0));
 #pragma unroll
     for (int idim = 0; idim < NDIM; ++idim) {
       const int st = idim == 0 ? 1 : (idim == 1 ? ex : ex * ex);
@@ -1309,13 +1310,16 @@ static int stage_core(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
   }
   const fvc::KernelKey k = make_key(c, riem);
   c->crash_fresh = false;
-  // ps written on a uniform level by the production kernels: the stage also
+  // ps written on a uniform level by the mixed-form kernels: the stage also
   // leaves the CFL partials of its output for the next getdt (the dt
-  // reduction of setdt, src/mod_dt.t:160-287, without a pass over ps)
+  // reduction of setdt, src/mod_dt.t:160-287, without a pass over ps).
+  // The FP64 kernels do not compute them (the extra work in their z loop
+  // costs more than the separate pass), nor do the AMR-capable kernels of
+  // 8^2 / 10^2 cross-sections.
   const bool linde = c->cfg.physics == 2 && c->cfg.divb == 4;
   a.linde = linde ? 1 : 0;
-  const bool fuse_dt =
-      dst_slot == 0 && !c->amr && c->nws == 0 && !linde && c->cfg.precision != 2 && !c->slot0_exposed;
+  const bool fuse_dt = dst_slot == 0 && !c->amr && c->bx == 16 && c->nws == 0 && !linde &&
+                       c->cfg.precision == AMRVAC_FV_PREC_MIXED && !c->slot0_exposed;
   if (fuse_dt) a.dt_part = c->d_dtp;
   if (dst_slot == 0 || dst_slot < 0) c->dt_valid = false;
   if (crash_out) HIPCHK(hipMemsetAsync(c->d_crash, 0, sizeof(int), c->stream));

@@ -116,7 +116,9 @@ hipError_t launch_lim(const KernelKey& k, const SweepArgs& a, const FVParams& fp
     else return launch_one<P, R, RIEM, LIM_WENO5, 3>(a, fp, grid, s);
   }
   // compile-time limiter for the schemes of the benchmark configurations,
-  // a runtime switch for the others
+  // a runtime switch for the others and for runs with fix_small_values
+  // (the compile-time-limiter kernels carry no small-value checks)
+  if (fp.fix_small) return launch_one<P, R, RIEM, 0, 2>(a, fp, grid, s);
   switch (k.limiter) {
     case LIM_MINMOD: return launch_one<P, R, RIEM, LIM_MINMOD, 2>(a, fp, grid, s);
     case LIM_VANLEER: return launch_one<P, R, RIEM, LIM_VANLEER, 2>(a, fp, grid, s);

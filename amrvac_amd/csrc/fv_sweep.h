@@ -243,6 +243,15 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
   constexpr int NW = P::NW;
   constexpr int PX = BX + 2 * NG;
   constexpr bool HANCOCK = RIEM == RIEM_HANCOCK;
+  // Any extra runtime branch in the z loop of the specialised kernels (a
+  // compile-time TVD limiter on a uniform level of 16^2 cross-sections)
+  // costs the FP64 form about 12% (measured): they carry no small-value
+  // checks (runs with fix_small_values take the runtime-limiter kernel),
+  // and only the FP32-reconstruction form, where the fused CFL partials cost
+  // 2% and save the separate pass over ps, computes them.
+  constexpr bool SPECIAL = LIM != 0 && LIM != LIM_WENO5 && !AMR && BXc == 16;
+  constexpr bool SMALLCHK = !SPECIAL;
+  constexpr bool CFL = !AMR && BXc == 16 && std::is_same<R, float>::value;
   constexpr int NSTRIP = 4 * NG * BX;
   constexpr int NHALO = NSTRIP;
   // 3-D, FP32 reconstruction: the halo strips of the NG planes ahead are
@@ -357,7 +366,7 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
       // fix_small_values: phys_to_primitive checks every cell it converts
       // (src/hd/mod_hd_phys.t:713-715, src/mhd/mod_mhd_phys.t:1750-1753);
       // over the level, the blocks' own cells are every converted cell
-      if (fp.fix_small && small_cons<P>(cons_k, fp)) crash = 1;
+      if (SMALLCHK && fp.fix_small && small_cons<P>(cons_k, fp)) crash = 1;
 #pragma unroll
       for (int m = 0; m < NG - 1; ++m)
 #pragma unroll
@@ -373,7 +382,7 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
         for (int iw = 0; iw < NW; ++iw) sp[iw][ty + NG][tx + NG] = Wp[NG - 1][iw];
     } else {
       load_cons<P>(a.src, b, tx, ty, 0, BX, 1, cons_k);
-      if (fp.fix_small && small_cons<P>(cons_k, fp)) crash = 1;
+      if (SMALLCHK && fp.fix_small && small_cons<P>(cons_k, fp)) crash = 1;
       R w[NW];
       prim_of<P, R>(cons_k, w, fp);
       if (own)
@@ -544,7 +553,7 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
     //      the face checks of reconstruct_LR (:1213-1216) flag nothing more
     // (hancock checks only after its sources, GLM here or Linde's kernel)
     if constexpr (!(HANCOCK && P::GLM))
-      if (!(HANCOCK && a.linde) && fp.fix_small && small_cons<P>(acc, fp)) crash = 1;
+      if (!(HANCOCK && a.linde) && SMALLCHK && fp.fix_small && small_cons<P>(acc, fp)) crash = 1;
     // ---- sources: addsource2 -> mhd_add_source (src/mod_source.t:130-160,
     //      src/mhd/mod_mhd_phys.t:4565-4695) -----------------------------
     if constexpr (P::GLM) {
@@ -575,16 +584,16 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
     // the neighbours needs the field beyond this kernel's planes and rim
     // the GLM source checks again (src/mhd/mod_mhd_phys.t:5529)
     if constexpr (P::GLM)
-      if (fp.fix_small && small_cons<P>(acc, fp)) crash = 1;
+      if (SMALLCHK && fp.fix_small && small_cons<P>(acc, fp)) crash = 1;
     if (own)
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) st_stream(acc[iw], a.dst + o + iw * cells);
-    if constexpr (!AMR)
+    if constexpr (CFL)
       if (a.dt_part) cfl_cell<P>(acc, fp, dt_maxtot, dt_cmax);
     __syncthreads();  // LDS plane is rewritten next iteration
   }
   if (crash) *a.crash = 1;
-  if constexpr (!AMR) {
+  if constexpr (CFL) {
     if (a.dt_part) {
       // block max: wave64 shuffles, then the 4 waves through LDS
       for (int off = 32; off > 0; off >>= 1) {

@@ -206,11 +206,12 @@ def test_download_ghosts_match_oracle_getbc():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_fused_cfl_partials_leave_the_dt_sequence_unchanged(precision):
-    """On a uniform level the stage that writes ps also leaves the per-block
-    CFL maxima of its output, and the next getdt only reduces them.  The dt
-    sequence and the state must equal those of the separate dt pass bit
-    for bit (a context whose ps was handed out by amrvac_fv_slot_pointer
-    keeps to the separate pass)."""
+    """On a uniform level the mixed-form stage that writes ps also leaves the
+    per-block CFL maxima of its output, and the next getdt only reduces
+    them (the FP64 form keeps the separate pass).  The dt sequence and the
+    state must equal those of the separate dt pass bit for bit (a context
+    whose ps was handed out by amrvac_fv_slot_pointer keeps to the separate
+    pass)."""
     from amrvac_amd.fv import DeviceFV
     print(f"[parity] fused CFL partials, 3-D MHD-GLM {precision}", flush=True)
     par = Par(physics="mhd", ndim=3, ndir=3, gamma=5.0 / 3.0, typedivbfix="glm", time_stepper="threestep",
