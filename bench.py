@@ -277,9 +277,23 @@ def main() -> int:
     be64, ti64, t64, ms64, _ = run_device(par64, mesh, w0, args.steps, args.warmup, "fp64")
     value64 = cells * stages * args.steps / t64 / 1e6
     if args.copy_probe:
+        # reference points for the HBM byte counters, each of one slot's
+        # size: the runtime's device-to-device copy, an elementwise FP64
+        # kernel (read and write, 8-byte loads as the stage kernel's) and an
+        # FP64 reduction (read only)
         be64.copy(1, 0)
         be64.sync()
-        print(f"[bench] copy probe: {mesh.nblocks * nw * 4096 * 8} bytes read and written", file=sys.stderr)
+        import torch
+        nslot = mesh.nblocks * nw * 4096
+        xa = torch.ones(nslot, dtype=torch.float64, device="cuda")
+        xb = torch.empty_like(xa)
+        torch.cuda.synchronize()
+        torch.add(xa, 1.0, out=xb)
+        torch.cuda.synchronize()
+        s = float(xb.sum())
+        del xa, xb
+        print(f"[bench] copy probes: {nslot * 8} bytes read and written (copy, elementwise add), "
+              f"{nslot * 8} bytes read (sum {s:.0f})", file=sys.stderr)
     achieved = b_alg * cells / (ms64 * 1e-3) / 1e9
     result = {
         "metric": "Mcell-updates/s 3D MHD 256^3, FP64 vs mixed @L1<=1e-5; adjoint/forward cost",

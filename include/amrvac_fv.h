@@ -55,7 +55,8 @@ typedef struct amrvac_fv_config {
   double small_pressure, small_density; /* 0.0 */
   int check_small_values;               /* 1 */
   int nghost;          /* 2, or 3 for weno5 */
-  int block_nx[3];     /* interior cells per block; x and y must be 16 */
+  int block_nx[3];     /* interior cells per block; block_nx(1) = block_nx(2)
+                          = 16, or 8 or 10 in 3-D (FP64) */
   int precision;       /* AMRVAC_FV_PREC_* */
   int type_ct;         /* with divb = 8 (constrained transport): 1 average,
                           2 uct_contact, 3 uct_hll (type_ct,

@@ -150,12 +150,18 @@ def test_adjoint_mixed_precision_close_to_fp64():
     g64 = p64.grad()[1]
     p32 = _Problem(128, 200, 20, precision="mixed")
     g32 = p32.grad()[1]
+    # The mixed adjoint is the exact reverse of the FP32-face map, not of
+    # the FP64 one: the two gradients differ by 3.7e-4 to 1.1e-3 in L2
+    # (rho, vx, vy), and that difference, not the adjoint, bounds the
+    # agreement with FP64 differences (measured 1.2e-3 normwise).  The FP64
+    # adjoint meets the 1e-4 target against the same differences
+    # (test_adjoint_finite_differences_kh_128).
     for a, b in zip(g32, g64):
         rel = ((a - b).norm() / b.norm()).item()
         print(f"[adjoint] mixed vs fp64 rel L2 {rel:.3e}", flush=True)
-        assert rel < 1e-2
+        assert rel < 3e-3
     p64.dts = p32.dts  # FP64 differences along the mixed run's dt sequence
     fd, an = _fd_directions(p64, g32, ndir=8)
     rel = np.linalg.norm(fd - an) / np.linalg.norm(fd)
     print(f"[adjoint] mixed adjoint vs FP64 differences, normwise {rel:.3e}", flush=True)
-    assert rel < 1e-2
+    assert rel < 3e-3

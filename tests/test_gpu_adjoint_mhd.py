@@ -144,8 +144,14 @@ def test_mhd_adjoint_fp64_finite_differences_32():
     pb = _MHDProblem(32, 10, "fp64", every=3)
     _, gb, gr = pb.grad()
     r = _fd(pb, gb, gr, eps_random=1.0e-6)
-    # random directions: limiter kinks of the noisy 32^3 field are crossed
-    # within the step (about 3e-3 normwise); the gradient direction is not
+    # b0 and the gradient direction meet the 1e-4 target (measured 4e-9 and
+    # 5e-6).  A random unit direction cannot: it is nearly orthogonal to the
+    # gradient, so its directional derivative is about |grad|/sqrt(N) (5e-8
+    # against |grad| 4e-5 here) and the absolute error of the difference
+    # quotient, from the limiter kinks of the noisy 32^3 field crossed within
+    # +-eps, is magnified by about sqrt(N) relative to it (measured 3.0e-3).
+    # The adjoint itself equals the autograd twin of the same map to 1e-13
+    # at this size (test_adjoint_3d_matches_torch_autograd).
     assert r["b0"] < 1.0e-6 and r["grad_dir"] < 1.0e-4 and r["random"] < 1.0e-2
 
 
@@ -161,11 +167,19 @@ def test_mhd_adjoint_config_128():
     torch.cuda.synchronize()
     print(f"[adjoint-mhd] fp64 forward+backward {time.perf_counter() - t0:.3f} s", flush=True)
     r = _fd(pb, gb, gr)
-    assert r["b0"] < 1.0e-6 and r["grad_dir"] < 1.0e-4 and r["random"] < 1.0e-3
+    # the 1e-4 target in every check (measured: b0 1.5e-10, gradient
+    # direction 4.7e-8, random directions 7.6e-5 normwise; the runs are
+    # deterministic)
+    assert r["b0"] < 1.0e-6 and r["grad_dir"] < 1.0e-4 and r["random"] < 1.0e-4
     pm = _MHDProblem(128, 100, "mixed")
     _, gbm, grm = pm.grad()
     pb.dts = pm.dts
     rm = _fd(pb, gbm, grm)
     print(f"[adjoint-mhd] mixed vs fp64 density gradient rel L2 {((grm - gr).norm() / gr.norm()).item():.3e}",
           flush=True)
-    assert rm["b0"] < 1.0e-6 and rm["grad_dir"] < 1.0e-2 and rm["random"] < 1.0e-2
+    # The mixed adjoint is the exact reverse of the FP32-face map, and the
+    # differences are of the FP64 map: the two maps' gradients differ by
+    # 1.5e-4 in L2 (printed above), which bounds how well the random
+    # directions can agree (measured 3.8e-4); b0 and the gradient direction
+    # still meet 1e-4 (4.8e-9, 7.2e-7).
+    assert rm["b0"] < 1.0e-6 and rm["grad_dir"] < 1.0e-4 and rm["random"] < 1.0e-3
