@@ -79,9 +79,9 @@ class SSPRK3Run:
 
     def forward(self, u0: torch.Tensor, dts: list[float] | None = None) -> torch.Tensor:
         """Final state; keeps checkpoints for backward.  With ``dts`` the
-        given time steps are replayed instead of the CFL ones.  The last
-        segment runs on the stage tape, so the backward pass starts
-        without recomputing it."""
+        given time steps are replayed instead of the CFL ones.  The last two
+        segments run on the two stage tapes the backward pass double-buffers,
+        so it starts without recomputing them."""
         if u0.shape != self.shape or u0.dtype != torch.float64 or u0.is_cuda != self.device_backend:
             raise ValueError(f"initial state must be a float64 {'CUDA ' if self.device_backend else ''}tensor "
                              f"of shape {self.shape}")
@@ -93,29 +93,43 @@ class SSPRK3Run:
         u2 = torch.empty_like(u)
         self.checkpoints = {0: u.clone()}
         n_last = ((self.nsteps - 1) // self.every) * self.every
-        tape = self._tape(u)
+        # first step on a tape: the segment before the last one (second
+        # tape), if there is one
+        n_tape = n_last - self.every if n_last >= self.every else n_last
+        tapes = {n_last: self._tape(u)}
+        if n_tape < n_last:
+            tapes[n_tape] = self._tape2(u)
         out = torch.empty_like(u)
         used = []
         for n in range(self.nsteps):
-            if n < n_last:
+            if n < n_tape:
                 cur = u.data_ptr()
             else:
-                if n == n_last:
+                s0 = n_last if n >= n_last else n_tape
+                tape, j = tapes[s0], n - s0
+                if n == n_tape:
                     tape[0, 0].copy_(u)
-                cur = tape[n - n_last, 0].data_ptr()
+                cur = tape[j, 0].data_ptr()
             dt = dts[n] if dts is not None else self.be.getdt_state(cur, self.par.courantpar)[0]
             used.append(dt)
-            if n < n_last:
+            if n < n_tape:
                 # the third stage overwrites u in place (its RK base, read per cell)
                 self._step(cur, u1.data_ptr(), u2.data_ptr(), cur, dt)
                 if (n + 1) % self.every == 0:
                     self.checkpoints[n + 1] = u.clone()
             else:
-                j = n - n_last
-                nxt = out.data_ptr() if n + 1 == self.nsteps else tape[j + 1, 0].data_ptr()
+                if n + 1 == self.nsteps:
+                    nxt = out.data_ptr()
+                elif n + 1 == n_last:  # into the last segment's tape
+                    nxt = tapes[n_last][0, 0].data_ptr()
+                else:
+                    nxt = tape[j + 1, 0].data_ptr()
                 self._step(cur, tape[j, 1].data_ptr(), tape[j, 2].data_ptr(), nxt, dt)
+                if n + 1 == n_last:
+                    self.checkpoints[n_last] = tapes[n_last][0, 0].clone()
         self.dts = used
         self._tape_segment = n_last
+        self._tape2_segment = n_tape if n_tape < n_last else -1
         return out
 
     def _recompute(self, be, tape: torch.Tensor, n0: int, n1: int) -> None:
@@ -158,7 +172,8 @@ class SSPRK3Run:
         starts = sorted(self.checkpoints)
         segs = [(n0, starts[i + 1] if i + 1 < len(starts) else self.nsteps) for i, n0 in enumerate(starts)][::-1]
         have_last = segs[0][0] == self._tape_segment
-        self._tape_segment = -1  # the tapes are reused from here on
+        have_prev = len(segs) > 1 and segs[1][0] == getattr(self, "_tape2_segment", -1)
+        self._tape_segment = self._tape2_segment = -1  # the tapes are reused from here on
         if not self.device_backend:
             tape = self._tape(lam)
             for i, (n0, n1) in enumerate(segs):
@@ -186,7 +201,7 @@ class SSPRK3Run:
         if not have_last:
             launch_recompute(0)
         for i, (n0, n1) in enumerate(segs):
-            if i + 1 < len(segs):
+            if i + 1 < len(segs) and not (i + 1 == 1 and have_prev):
                 launch_recompute(i + 1)
             if ready[i] is not None:
                 main.wait_event(ready[i])

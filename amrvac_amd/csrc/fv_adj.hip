@@ -170,65 +170,83 @@ __global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R>)) adj_plane_kernel(
   }
   __syncthreads();
 
-  // Face j of a round goes to lane (j - j0 + 64 r) mod 256: the last,
-  // partial round of each dimension (NFACE - NTHREADS faces) falls to wave
-  // r, with r different for x and y and varying over the planes, so that no
-  // wave reverses the partial rounds of both dimensions.  Which lane reverses
-  // a face does not change the order of the per-cell sums.
-  const int rx = k & 3, ry = (k + 2) & 3;
-  // ---- x faces: face j = (f, row), f = j % NF - 1 ------------------------
-  for (int j0 = 0; j0 < NFACE; j0 += NTHREADS) {
-    const int j = j0 + ((t + (NTHREADS - 64 * rx)) & (NTHREADS - 1));
-    const bool on = j < NFACE;
-    const int f = j % NF - 1, row = j / NF;
-    double wb[4][NW];
-    if (on) {
-      R st[4][NW];
+  // Faces of dimension D: x face j = (f, row) with f = j % NF - 1, row =
+  // j / NF; y face j = (col, f) with col = j % BX, f = j / BX - 1.  A face's
+  // primitive adjoints go to the cells at stencil positions s = 0..3.
+  auto reverse_face = [&](auto Dc, int j, double (*wb)[NW]) {
+    constexpr int D = decltype(Dc)::value;
+    R st[4][NW];
+    R mu[NW];
+    if constexpr (D == 0) {
+      const int f = j % NF - 1, row = j / NF;
 #pragma unroll
       for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int iw = 0; iw < NW; ++iw) st[s][iw] = Wl[iw][(row + AH) * AW + f - 2 + s + AH];
-      R mu[NW];
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw)
         mu[iw] = R(a.dq[0] * (double(Ll[iw][(row + LH) * LW + f - 1 + LH]) - double(Ll[iw][(row + LH) * LW + f + LH])));
-      face_adj_prim<P, R, RIEM, 0, LIM>(st, fp, mu, wb);
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      __syncthreads();
-      const int x = f - 2 + s;
-      if (on && x >= 0 && x < BX)
-#pragma unroll
-        for (int iw = 0; iw < NW; ++iw) Acc[iw][row * BX + x] += wb[s][iw];
-    }
-  }
-  // ---- y faces: face j = (col, f), col = j % BX, f = j / BX - 1 ----------
-  for (int j0 = 0; j0 < NFACE; j0 += NTHREADS) {
-    const int j = j0 + ((t + (NTHREADS - 64 * ry)) & (NTHREADS - 1));
-    const bool on = j < NFACE;
-    const int col = j % BX, f = j / BX - 1;
-    double wb[4][NW];
-    if (on) {
-      R st[4][NW];
+    } else {
+      const int col = j % BX, f = j / BX - 1;
 #pragma unroll
       for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int iw = 0; iw < NW; ++iw) st[s][iw] = Wl[iw][(f - 2 + s + AH) * AW + col + AH];
-      R mu[NW];
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw)
         mu[iw] = R(a.dq[1] * (double(Ll[iw][(f - 1 + LH) * LW + col + LH]) - double(Ll[iw][(f + LH) * LW + col + LH])));
-      face_adj_prim<P, R, RIEM, 1, LIM>(st, fp, mu, wb);
     }
+    face_adj_prim<P, R, RIEM, D, LIM>(st, fp, mu, wb);
+  };
+  // the cell of stencil position s of face j (-1: outside the tile)
+  auto cell_of = [&](auto Dc, int j, int s) {
+    constexpr int D = decltype(Dc)::value;
+    if constexpr (D == 0) {
+      const int x = j % NF - 3 + s, row = j / NF;
+      return (x >= 0 && x < BX) ? row * BX + x : -1;
+    } else {
+      const int y = j / BX - 3 + s, col = j % BX;
+      return (y >= 0 && y < BX) ? y * BX + col : -1;
+    }
+  };
+  // four passes over the stencil positions: within a pass the faces of one
+  // dimension and round write distinct cells, so the order of every cell's
+  // sum is fixed
+  auto scatter = [&](auto Dc, int j, bool on, double (*wb)[NW]) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       __syncthreads();
-      const int y = f - 2 + s;
-      if (on && y >= 0 && y < BX)
+      const int c = on ? cell_of(Dc, j, s) : -1;
+      if (c >= 0)
 #pragma unroll
-        for (int iw = 0; iw < NW; ++iw) Acc[iw][y * BX + col] += wb[s][iw];
+        for (int iw = 0; iw < NW; ++iw) Acc[iw][c] += wb[s][iw];
     }
+  };
+  constexpr auto DX = std::integral_constant<int, 0>();
+  constexpr auto DY = std::integral_constant<int, 1>();
+  static_assert(NFACE > NTHREADS && NFACE - NTHREADS <= 64, "one full round and a partial wave per dimension");
+  // Rounds: the first NTHREADS x faces, the first NTHREADS y faces, then
+  // the remaining NFACE - NTHREADS faces of both dimensions together, the
+  // x ones in wave rx and the y ones in wave ry (a different wave, varying
+  // over the planes), so that no workgroup spends a round on one partial
+  // wave per dimension and no wave takes both code paths.
+  {
+    double wb[4][NW];
+    reverse_face(DX, t, wb);
+    scatter(DX, t, true, wb);
+    reverse_face(DY, t, wb);
+    scatter(DY, t, true, wb);
+  }
+  {
+    const int rx = k & 3, ry = (k + 2) & 3;
+    const int wave = t >> 6, lane = t & 63;
+    const int j = NTHREADS + lane;
+    const bool onx = wave == rx && j < NFACE, ony = wave == ry && j < NFACE;
+    double wb[4][NW];
+    if (onx) reverse_face(DX, j, wb);
+    if (ony) reverse_face(DY, j, wb);
+    scatter(DX, j, onx, wb);
+    scatter(DY, j, ony, wb);
   }
   __syncthreads();
   double wsum[NW];
