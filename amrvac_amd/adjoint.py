@@ -6,7 +6,8 @@ checkpoint-and-recompute in time.
 The time-step sequence is taken from the forward pass (CFL condition of
 src/mod_dt.t:160-287) and held fixed in the backward pass, like the limiter
 and Riemann branches: the gradient is that of the scheme with the forward
-run's dt sequence.  Checkpoints of the state every ``every`` steps are kept
+run's dt sequence (and, with GLM, its sequence of cleaning speeds c_h =
+cmax_global, src/mod_dt.t:128-131, which set the psi decay of each stage).  Checkpoints of the state every ``every`` steps are kept
 in device memory; in the backward pass each segment is recomputed once,
 keeping its stage states on a tape, and then reversed stage by stage (three
 adjoint launches per step, the RK combination fused into the last).
@@ -54,6 +55,13 @@ class SSPRK3Run:
         self.method = FS[par.flux_scheme]
         self.c = ssprk3_coefficients()
         self.dts: list[float] | None = None
+        self.chs: list[float] | None = None
+        # GLM: the psi decay of a stage, exp(-qdt c_h alpha / min dx)
+        # (add_source_glm, src/mhd/mod_mhd_phys.t:5491-5495), is a diagonal
+        # factor on the psi row of the adjoint
+        self.glm = par.physics == "mhd" and par.typedivbfix == "glm"
+        self.psi = 2 * par.ndir + 2
+        self.mindx = min(mesh.dx[:mesh.ndim])
         self.checkpoints: dict[int, torch.Tensor] = {}
         self._tape_buf: torch.Tensor | None = None
         self._tape_segment = -1
@@ -62,12 +70,17 @@ class SSPRK3Run:
         if self.device_backend:
             self.be.set_stream(torch.cuda.current_stream().cuda_stream)
 
-    def _step(self, u: int, u1: int, u2: int, out: int, dt: float) -> None:
+    def _step(self, u: int, u1: int, u2: int, out: int, dt: float, ch: float) -> None:
         """out = ssprk3 step of u (pointers); u1, u2 receive the stage states."""
         c, m, be = self.c, self.method, self.be
-        be.stage(m, c["beta11"], dt, 0.0, u, u1)
-        be.stage(m, c["beta22"], dt, 0.0, u1, u2, base=u, comb_a=c["alfa21"], comb_b=c["alfa22"])
-        be.stage(m, c["beta33"], dt, 0.0, u2, out, base=u, comb_a=c["alfa31"], comb_b=c["alfa33"])
+        be.stage(m, c["beta11"], dt, ch, u, u1)
+        be.stage(m, c["beta22"], dt, ch, u1, u2, base=u, comb_a=c["alfa21"], comb_b=c["alfa22"])
+        be.stage(m, c["beta33"], dt, ch, u2, out, base=u, comb_a=c["alfa31"], comb_b=c["alfa33"])
+
+    def _psi_decay(self, lam: torch.Tensor, beta: float, dt: float, ch: float) -> None:
+        """With GLM: lam's psi row times the stage's psi decay, in place."""
+        if self.glm:
+            lam[:, self.psi].mul_(math.exp(-beta * dt * ch * self.par.mhd_glm_alpha / self.mindx))
 
     def _tape(self, like: torch.Tensor) -> torch.Tensor:
         """Stage states of one segment: tape[j] = (u0, u1, u2) of its step j
@@ -77,9 +90,12 @@ class SSPRK3Run:
                                          device=like.device)
         return self._tape_buf
 
-    def forward(self, u0: torch.Tensor, dts: list[float] | None = None) -> torch.Tensor:
+    def forward(self, u0: torch.Tensor, dts: list[float] | None = None,
+                chs: list[float] | None = None) -> torch.Tensor:
         """Final state; keeps checkpoints for backward.  With ``dts`` the
-        given time steps are replayed instead of the CFL ones.  The last two
+        given time steps are replayed instead of the CFL ones, and with
+        them the GLM speeds ``chs`` (by default those of the previous
+        forward pass).  The last two
         segments run on the two stage tapes the backward pass double-buffers,
         so it starts without recomputing them."""
         if u0.shape != self.shape or u0.dtype != torch.float64 or u0.is_cuda != self.device_backend:
@@ -87,6 +103,10 @@ class SSPRK3Run:
                              f"of shape {self.shape}")
         if dts is not None and len(dts) != self.nsteps:
             raise ValueError("dts must hold one time step per step")
+        if dts is not None and chs is None:
+            chs = self.chs if self.chs is not None else [0.0] * self.nsteps
+        if chs is not None and len(chs) != self.nsteps:
+            raise ValueError("chs must hold one GLM speed per step")
         self._bind_stream()
         u = u0.detach().contiguous().clone()
         u1 = torch.empty_like(u)
@@ -100,7 +120,7 @@ class SSPRK3Run:
         if n_tape < n_last:
             tapes[n_tape] = self._tape2(u)
         out = torch.empty_like(u)
-        used = []
+        used, used_ch = [], []
         for n in range(self.nsteps):
             if n < n_tape:
                 cur = u.data_ptr()
@@ -110,11 +130,15 @@ class SSPRK3Run:
                 if n == n_tape:
                     tape[0, 0].copy_(u)
                 cur = tape[j, 0].data_ptr()
-            dt = dts[n] if dts is not None else self.be.getdt_state(cur, self.par.courantpar)[0]
+            if dts is not None:
+                dt, ch = dts[n], chs[n]
+            else:
+                dt, ch = self.be.getdt_state(cur, self.par.courantpar)
             used.append(dt)
+            used_ch.append(ch)
             if n < n_tape:
                 # the third stage overwrites u in place (its RK base, read per cell)
-                self._step(cur, u1.data_ptr(), u2.data_ptr(), cur, dt)
+                self._step(cur, u1.data_ptr(), u2.data_ptr(), cur, dt, ch)
                 if (n + 1) % self.every == 0:
                     self.checkpoints[n + 1] = u.clone()
             else:
@@ -124,10 +148,11 @@ class SSPRK3Run:
                     nxt = tapes[n_last][0, 0].data_ptr()
                 else:
                     nxt = tape[j + 1, 0].data_ptr()
-                self._step(cur, tape[j, 1].data_ptr(), tape[j, 2].data_ptr(), nxt, dt)
+                self._step(cur, tape[j, 1].data_ptr(), tape[j, 2].data_ptr(), nxt, dt, ch)
                 if n + 1 == n_last:
                     self.checkpoints[n_last] = tapes[n_last][0, 0].clone()
         self.dts = used
+        self.chs = used_ch
         self._tape_segment = n_last
         self._tape2_segment = n_tape if n_tape < n_last else -1
         return out
@@ -137,27 +162,31 @@ class SSPRK3Run:
         c, m = self.c, self.method
         tape[0, 0].copy_(self.checkpoints[n0])
         for j in range(n1 - n0):
-            dt = self.dts[n0 + j]
+            dt, ch = self.dts[n0 + j], self.chs[n0 + j]
             u0, u1, u2 = (tape[j, q].data_ptr() for q in range(3))
-            be.stage(m, c["beta11"], dt, 0.0, u0, u1)
-            be.stage(m, c["beta22"], dt, 0.0, u1, u2, base=u0, comb_a=c["alfa21"], comb_b=c["alfa22"])
+            be.stage(m, c["beta11"], dt, ch, u0, u1)
+            be.stage(m, c["beta22"], dt, ch, u1, u2, base=u0, comb_a=c["alfa21"], comb_b=c["alfa22"])
             if j + 1 < n1 - n0:
-                be.stage(m, c["beta33"], dt, 0.0, u2, tape[j + 1, 0].data_ptr(), base=u0,
+                be.stage(m, c["beta33"], dt, ch, u2, tape[j + 1, 0].data_ptr(), base=u0,
                          comb_a=c["alfa31"], comb_b=c["alfa33"])
 
     def _reverse(self, tape: torch.Tensor, n0: int, n1: int, lam, l1, l2) -> None:
         """The adjoint stages of steps n1-1..n0 from the stage tape."""
         c, m, be = self.c, self.method, self.be
         for j in range(n1 - n0 - 1, -1, -1):
-            dt = self.dts[n0 + j]
+            dt, ch = self.dts[n0 + j], self.chs[n0 + j]
             u0, u1, u2 = (tape[j, q].data_ptr() for q in range(3))
-            # u3 = a31 u0 + a33 u2 + L3(u2)
-            be.adjoint1(m, c["beta33"], dt, u2, lam.data_ptr(), c["alfa33"], l2.data_ptr())
+            # u3 = a31 u0 + a33 u2 + L3(u2) (psi row: times the decay D3)
+            self._psi_decay(lam, c["beta33"], dt, ch)
+            be.adjoint1(m, c["beta33"], dt, u2, lam.data_ptr(), c["alfa33"], l2.data_ptr(), ch=ch)
             # u2 = a21 u0 + a22 u1 + L2(u1)
-            be.adjoint1(m, c["beta22"], dt, u1, l2.data_ptr(), c["alfa22"], l1.data_ptr())
-            # u1 = u0 + L1(u0); lam_u0 = l1 + L1^T l1 + a31 lam + a21 l2 (in place)
+            self._psi_decay(l2, c["beta22"], dt, ch)
+            be.adjoint1(m, c["beta22"], dt, u1, l2.data_ptr(), c["alfa22"], l1.data_ptr(), ch=ch)
+            # u1 = u0 + L1(u0); lam_u0 = l1 + L1^T l1 + a31 lam + a21 l2 (in
+            # place; lam and l2 already carry their stages' decay)
+            self._psi_decay(l1, c["beta11"], dt, ch)
             be.adjoint1(m, c["beta11"], dt, u0, l1.data_ptr(), 1.0, lam.data_ptr(),
-                        add_a=lam.data_ptr(), ca=c["alfa31"], add_b=l2.data_ptr(), cb=c["alfa21"])
+                        add_a=lam.data_ptr(), ca=c["alfa31"], add_b=l2.data_ptr(), cb=c["alfa21"], ch=ch)
 
     def backward(self, lam: torch.Tensor) -> torch.Tensor:
         """Adjoint of the whole run: d(final)/d(initial)^T lam.
@@ -229,19 +258,21 @@ class SSPRK3Run:
 
 class _RunFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u0, run: SSPRK3Run, dts):
+    def forward(ctx, u0, run: SSPRK3Run, dts, chs):
         ctx.run = run
-        return run.forward(u0, dts)
+        return run.forward(u0, dts, chs)
 
     @staticmethod
     def backward(ctx, grad):
-        return ctx.run.backward(grad), None, None
+        return ctx.run.backward(grad), None, None, None
 
 
-def run_differentiable(u0: torch.Tensor, run: SSPRK3Run, dts: list[float] | None = None) -> torch.Tensor:
+def run_differentiable(u0: torch.Tensor, run: SSPRK3Run, dts: list[float] | None = None,
+                       chs: list[float] | None = None) -> torch.Tensor:
     """Final state of ``run`` from ``u0`` (block layout [nb, nw, (z,) y, x]),
-    differentiable w.r.t. u0 through the adjoint kernels."""
-    return _RunFunction.apply(u0, run, dts)
+    differentiable w.r.t. u0 through the adjoint kernels; ``dts`` (and the
+    GLM speeds ``chs``) replay a time-step sequence."""
+    return _RunFunction.apply(u0, run, dts, chs)
 
 
 # ---------------------------------------------------------------------------
@@ -320,8 +351,9 @@ def kh_fields(mesh: UniformMesh, seed: int, amp: float = 1.0e-3, device=None, rh
 
 
 def mhd_conserved(rho: torch.Tensor, v: list[torch.Tensor], p: torch.Tensor, b: list[torch.Tensor],
-                  gamma: float) -> torch.Tensor:
-    """mhd_to_conserved on fields (no B0): [rho, m_1..3, e, b_1..3]."""
+                  gamma: float, psi: torch.Tensor | None = None) -> torch.Tensor:
+    """mhd_to_conserved on fields (no B0): [rho, m_1..3, e, b_1..3] (and
+    psi with GLM)."""
     vsq = v[0] * v[0]
     for c in v[1:]:
         vsq = vsq + c * c
@@ -329,7 +361,7 @@ def mhd_conserved(rho: torch.Tensor, v: list[torch.Tensor], p: torch.Tensor, b: 
     for c in b[1:]:
         bsq = bsq + c * c
     e = p * (1.0 / (gamma - 1.0)) + 0.5 * (vsq * rho + bsq)
-    return torch.stack([rho] + [rho * c for c in v] + [e] + list(b))
+    return torch.stack([rho] + [rho * c for c in v] + [e] + list(b) + ([psi] if psi is not None else []))
 
 
 def orszag_tang_fields(mesh: UniformMesh, seed: int, rho_amp: float = 1.0e-2, vz_amp: float = 1.0e-3, device=None,

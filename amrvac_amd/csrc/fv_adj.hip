@@ -55,8 +55,11 @@ __device__ __forceinline__ void load_lam(const SweepArgs& a, const double* __res
 // are added to a window of four running per-cell sums.  A cell's sum is
 // complete after its fourth face (faces in increasing z, so the order is
 // fixed) and is written once: gz[iw * ncells + cell], in the face precision.
+// (the FP32-face kernels are held to 2 waves per SIMD, the launch's one
+// round; the FP64 ones may take the whole register file)
 template <class P, class R, int RIEM, int LIM>
-__global__ void __launch_bounds__(NTHREADS) adj_zface_kernel(SweepArgs a, FVParams fp, AdjArgs g) {
+__global__ void __launch_bounds__(NTHREADS, (std::is_same<R, float>::value ? 2 : 1))
+    adj_zface_kernel(SweepArgs a, FVParams fp, AdjArgs g) {
   constexpr int NW = P::NW, NDIM = P::NDIM;
   static_assert(NDIM == 3, "z faces exist in 3-D");
   const int nch = a.bnz / ZC;
@@ -66,17 +69,25 @@ __global__ void __launch_bounds__(NTHREADS) adj_zface_kernel(SweepArgs a, FVPara
   const size_t cells = (size_t)BX * BX * a.bnz;
   const size_t ntot = (size_t)a.nblocks * cells;
   R* gz = static_cast<R*>(g.gz);
-  // st[s] = cell k-2+s of the face k; ll/lr = lam_out(k-1), (k);
-  // sum[m] = running sum of cell k-2+m
+  // st[s] = cell k-2+s of the face k; ll/lr = lam_out(k-1), (k), in the
+  // face precision as the plane kernel's tile holds them; sum[m] = running
+  // sum of cell k-2+m, in the face precision it is stored in
   R st[4][NW];
-  double ll[NW], lr[NW], sum[4][NW];
+  R ll[NW], lr[NW];
+  R sum[4][NW];
 #pragma unroll
   for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int iw = 0; iw < NW; ++iw) sum[m][iw] = 0.0;
+    for (int iw = 0; iw < NW; ++iw) sum[m][iw] = R(0);
 #pragma unroll
   for (int s = 0; s < 3; ++s) load_prim<P, R, NDIM>(a, fp, b, tx, ty, k0 - 3 + s, st[s + 1]);
-  load_lam<NW, NDIM>(a, g.lam_out, b, tx, ty, k0 - 2, lr);
+  auto load_lam_r = [&](int z, R* l) {
+    double t[NW];
+    load_lam<NW, NDIM>(a, g.lam_out, b, tx, ty, z, t);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) l[iw] = R(t[iw]);
+  };
+  load_lam_r(k0 - 2, lr);
   for (int k = k0 - 1; k <= k0 + ZC + 1; ++k) {
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
@@ -86,33 +97,114 @@ __global__ void __launch_bounds__(NTHREADS) adj_zface_kernel(SweepArgs a, FVPara
       ll[iw] = lr[iw];
     }
     load_prim<P, R, NDIM>(a, fp, b, tx, ty, k + 1, st[3]);
-    load_lam<NW, NDIM>(a, g.lam_out, b, tx, ty, k, lr);
+    load_lam_r(k, lr);
     R mu[NW];
 #pragma unroll
-    for (int iw = 0; iw < NW; ++iw) mu[iw] = R(a.dq[2] * (ll[iw] - lr[iw]));
-    face_adj_prim<P, R, RIEM, 2, LIM, true>(st, fp, mu, sum);
+    for (int iw = 0; iw < NW; ++iw) mu[iw] = R(a.dq[2] * (double(ll[iw]) - double(lr[iw])));
+    face_adj_prim<P, R, RIEM, 2, LIM, true, R>(st, fp, R(a.ch), mu, sum);
     // cell k-2 has all four of its faces
     if (k - 2 >= k0) {
       const size_t o = (size_t)b * cells + ((size_t)(k - 2) * BX + ty) * BX + tx;
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) gz[(size_t)iw * ntot + o] = R(sum[0][iw]);
+      for (int iw = 0; iw < NW; ++iw) gz[(size_t)iw * ntot + o] = sum[0][iw];
     }
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
       sum[0][iw] = sum[1][iw];
       sum[1][iw] = sum[2][iw];
       sum[2][iw] = sum[3][iw];
-      sum[3][iw] = 0.0;
+      sum[3][iw] = R(0);
     }
   }
 }
 
-// occupancy: the FP32-face kernels keep to 3 workgroups per CU
+// Reverse of the extended GLM sources of a stage (add_source_glm with
+// mhd_glm_extended, src/mhd/mod_mhd_phys.t:5472-5531, as the stage kernel
+// adds them to the updated cell from the stage input's primitive state):
+//   e   -= qdt sum_d B_d (psi(+e_d) - psi(-e_d)) / (2 dx_d),
+//   m_k -= qdt B_k divB,  divB = sum_d (B_d(+e_d) - B_d(-e_d)) / (2 dx_d).
+// They are linear in B and psi (conserved = primitive), so the own cell's
+// B and the neighbours' B and psi collect, for cell c and its neighbours
+// p = c + e_d, m = c - e_d:
+//   lam[B_d](c)  += -qdt g_d(c) lam_e(c) - qdt divB(c) lam_m_d(c)
+//                   + qdt/(2 dx_d) sum_k (B_k lam_m_k)(p) - (B_k lam_m_k)(m),
+//   lam[psi](c)  += qdt/(2 dx_d) sum_d (B_d lam_e)(p) - (B_d lam_e)(m).
+// x / y neighbours come from the workgroup's tiles, z neighbours from the
+// planes above and below.  The psi decay of the stage is a diagonal factor
+// on the psi row that the caller applies to lam_out beforehand.
 template <class P, class R>
-constexpr int kAdjWaves = std::is_same<R, float>::value ? 3 : (P::NW > 5 ? 1 : 2);
+__device__ __forceinline__ void glm_source_rev(const SweepArgs& a, const FVParams& fp, const AdjArgs& g,
+                                               const R (*Wl)[AW * AW], const R (*Ll)[LW * LW], const double* lo,
+                                               int b, int k, int tx, int ty, double* acc) {
+  constexpr int NDIM = P::NDIM, ND = P::NDIR, E = P::E, MAG = P::MAG, MOM = P::MOM, S = P::PSI;
+  const double qdt = a.qdt;
+  // B_0..ND-1, psi and lam_e, lam_m_0..ND-1 at the own cell's neighbours
+  // n = 0: -e_d, 1: +e_d per dimension d
+  double Bn[3][2][3], psin[3][2], le[3][2], lm[3][2][3];
+#pragma unroll
+  for (int d = 0; d < 2; ++d)
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+      const int sx = d == 0 ? 2 * n - 1 : 0, sy = d == 1 ? 2 * n - 1 : 0;
+      const int wi = (ty + AH + sy) * AW + tx + AH + sx, li = (ty + LH + sy) * LW + tx + LH + sx;
+#pragma unroll
+      for (int q = 0; q < ND; ++q) {
+        Bn[d][n][q] = double(Wl[MAG + q][wi]);
+        lm[d][n][q] = double(Ll[MOM + q][li]);
+      }
+      psin[d][n] = double(Wl[S][wi]);
+      le[d][n] = double(Ll[E][li]);
+    }
+  if constexpr (NDIM == 3) {
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+      R w[P::NW];
+      load_prim<P, R, NDIM>(a, fp, b, tx, ty, k + 2 * n - 1, w);
+      double l[P::NW];
+      load_lam<P::NW, NDIM>(a, g.lam_out, b, tx, ty, k + 2 * n - 1, l);
+#pragma unroll
+      for (int q = 0; q < ND; ++q) {
+        Bn[2][n][q] = double(w[MAG + q]);
+        lm[2][n][q] = l[MOM + q];
+      }
+      psin[2][n] = double(w[S]);
+      le[2][n] = l[E];
+    }
+  }
+  double gp[3], divb = 0.0;
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    gp[d] = ((psin[d][1] - psin[d][0]) * 0.5) / fp.dx[d];
+    divb = divb + 0.5 * (Bn[d][1][d] - Bn[d][0][d]) * fp.dxinv[d];
+  }
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) acc[MAG + d] = acc[MAG + d] - qdt * gp[d] * lo[E];
+#pragma unroll
+  for (int q = 0; q < ND; ++q) acc[MAG + q] = acc[MAG + q] - qdt * divb * lo[MOM + q];
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    const double h = 0.5 * qdt / fp.dx[d];
+    acc[S] = acc[S] + h * (Bn[d][1][d] * le[d][1] - Bn[d][0][d] * le[d][0]);
+    double sp = 0.0, sm = 0.0;
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+      sp = sp + Bn[d][1][q] * lm[d][1][q];
+      sm = sm + Bn[d][0][q] * lm[d][0][q];
+    }
+    acc[MAG + d] = acc[MAG + d] + 0.5 * qdt * fp.dxinv[d] * (sp - sm);
+  }
+}
+
+// occupancy: the FP32-face kernels keep to 3 workgroups per CU, except
+// those of GLM or cada3 (their reverse needs more registers; held to 3 they
+// spill)
+template <class P, class R, int LIM>
+constexpr int kAdjWaves = std::is_same<R, float>::value ? ((P::GLM || LIM == LIM_CADA3) ? 2 : 3)
+                                                        : (P::NW > 5 ? 1 : 2);
 
 template <class P, class R, int RIEM, int LIM>
-__global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R>)) adj_plane_kernel(SweepArgs a, FVParams fp, AdjArgs g) {
+__global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R, LIM>))
+    adj_plane_kernel(SweepArgs a, FVParams fp, AdjArgs g) {
   constexpr int NW = P::NW, NDIM = P::NDIM;
   // lam_out is kept in the face precision: mu is rounded to R anyway
   __shared__ R Wl[NW][AW * AW];
@@ -196,7 +288,7 @@ __global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R>)) adj_plane_kernel(
       for (int iw = 0; iw < NW; ++iw)
         mu[iw] = R(a.dq[1] * (double(Ll[iw][(f - 1 + LH) * LW + col + LH]) - double(Ll[iw][(f + LH) * LW + col + LH])));
     }
-    face_adj_prim<P, R, RIEM, D, LIM>(st, fp, mu, wb);
+    face_adj_prim<P, R, RIEM, D, LIM>(st, fp, R(a.ch), mu, wb);
   };
   // the cell of stencil position s of face j (-1: outside the tile)
   auto cell_of = [&](auto Dc, int j, int s) {
@@ -258,6 +350,9 @@ __global__ void __launch_bounds__(NTHREADS, (kAdjWaves<P, R>)) adj_plane_kernel(
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) cons[iw] = a.src[o + iw * cells];
   prim_rev<P>(cons, wsum, fp, acc);
+  if constexpr (P::GLM) {
+    if (fp.glm_extended) glm_source_rev<P, R>(a, fp, g, Wl, Ll, lo, b, k, tx, ty, acc);
+  }
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) g.lam_in[o + iw * cells] = acc[iw];
 }
@@ -274,23 +369,25 @@ hipError_t launch_adj(const SweepArgs& a, const FVParams& fp, const AdjArgs& g, 
   return hipGetLastError();
 }
 
+template <class P, class R, int RIEM>
+hipError_t adj_lim(int lim, const SweepArgs& a, const FVParams& fp, const AdjArgs& g, hipStream_t s) {
+  switch (lim) {
+    case LIM_VANLEER: return launch_adj<P, R, RIEM, LIM_VANLEER>(a, fp, g, s);
+    case LIM_MINMOD: return launch_adj<P, R, RIEM, LIM_MINMOD>(a, fp, g, s);
+    case LIM_CADA3: return launch_adj<P, R, RIEM, LIM_CADA3>(a, fp, g, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <class P, class R>
 hipError_t adj_riem(int riem, int lim, const SweepArgs& a, const FVParams& fp, const AdjArgs& g, hipStream_t s) {
-  if (lim != LIM_VANLEER && lim != LIM_MINMOD) return hipErrorInvalidValue;
-  const bool vl = lim == LIM_VANLEER;
   switch (riem) {
-    case RIEM_HLL:
-      return vl ? launch_adj<P, R, RIEM_HLL, LIM_VANLEER>(a, fp, g, s) : launch_adj<P, R, RIEM_HLL, LIM_MINMOD>(a, fp, g, s);
+    case RIEM_HLL: return adj_lim<P, R, RIEM_HLL>(lim, a, fp, g, s);
     case RIEM_HLLC:
       if constexpr (P::MHD) return hipErrorInvalidValue;
-      else
-        return vl ? launch_adj<P, R, RIEM_HLLC, LIM_VANLEER>(a, fp, g, s)
-                  : launch_adj<P, R, RIEM_HLLC, LIM_MINMOD>(a, fp, g, s);
-    case RIEM_TVDLF:
-      return vl ? launch_adj<P, R, RIEM_TVDLF, LIM_VANLEER>(a, fp, g, s)
-                : launch_adj<P, R, RIEM_TVDLF, LIM_MINMOD>(a, fp, g, s);
-    default:
-      return hipErrorInvalidValue;
+      else return adj_lim<P, R, RIEM_HLLC>(lim, a, fp, g, s);
+    case RIEM_TVDLF: return adj_lim<P, R, RIEM_TVDLF>(lim, a, fp, g, s);
+    default: return hipErrorInvalidValue;
   }
 }
 
@@ -301,14 +398,17 @@ hipError_t adj_prec(int mixed, int riem, int lim, const SweepArgs& a, const FVPa
   return mixed ? adj_riem<P, float>(riem, lim, a, fp, g, s) : adj_riem<P, double>(riem, lim, a, fp, g, s);
 }
 
-// hydrodynamics (physics 1) and ideal MHD without divergence cleaning
-// (physics 2, ndir 3), 2-D / 3-D
-hipError_t launch_adjoint_stage(int ndim, int physics, int mixed, int riem, int lim, const SweepArgs& a,
+// hydrodynamics (physics 1) and ideal MHD (physics 2, ndir 3) without
+// divergence control or with GLM (divb 1), 2-D / 3-D
+hipError_t launch_adjoint_stage(int ndim, int physics, int divb, int mixed, int riem, int lim, const SweepArgs& a,
                                 const FVParams& fp, const AdjArgs& g, hipStream_t s) {
   if (physics == 1) {
     if (ndim == 2) return adj_prec<Phys<2, 2, false, DIVB_NONE>>(mixed, riem, lim, a, fp, g, s);
     if (ndim == 3) return adj_prec<Phys<3, 3, false, DIVB_NONE>>(mixed, riem, lim, a, fp, g, s);
-  } else if (physics == 2) {
+  } else if (physics == 2 && divb == DIVB_GLM) {
+    if (ndim == 2) return adj_prec<Phys<2, 3, true, DIVB_GLM>>(mixed, riem, lim, a, fp, g, s);
+    if (ndim == 3) return adj_prec<Phys<3, 3, true, DIVB_GLM>>(mixed, riem, lim, a, fp, g, s);
+  } else if (physics == 2 && divb == DIVB_NONE) {
     if (ndim == 2) return adj_prec<Phys<2, 3, true, DIVB_NONE>>(mixed, riem, lim, a, fp, g, s);
     if (ndim == 3) return adj_prec<Phys<3, 3, true, DIVB_NONE>>(mixed, riem, lim, a, fp, g, s);
   }

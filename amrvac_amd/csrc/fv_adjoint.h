@@ -38,21 +38,84 @@ __device__ __forceinline__ R lim_grad(R a, R b, R& da, R& db) {
   }
 }
 
+// cada_limited (the limiter_cada branch of dwlimiter2, src/limiter/
+// mod_limiter.t:233-251) of theta and its derivative, along the branch the
+// forward min / max take
+template <class R>
+__device__ __forceinline__ R cada_limited_grad(R th, R& dth) {
+  const R cadalfa = R(0.5), cadbeta = R(2), cadgamma = R(1.6);
+  const R A = (R(2) + th) * R(1.0 / 3.0);
+  R v = cadbeta * th, dv = cadbeta;                  // min(beta theta, A)
+  if (A < v) { v = A; dv = R(1.0 / 3.0); }
+  if (cadgamma < v) { v = cadgamma; dv = R(0); }     // min(., gamma)
+  if (-cadalfa * th > v) { v = -cadalfa * th; dv = -cadalfa; }  // max(-alfa theta, .)
+  if (A < v) { v = A; dv = R(1.0 / 3.0); }           // min(A, .)
+  if (R(0) > v) { v = R(0); dv = R(0); }             // max(0, .)
+  dth = dv;
+  return v;
+}
+
+// One face side of cada3 (limiter_cada3, src/limiter/mod_limiter.t:253-290,
+// as face_limit evaluates it): F(x, d0) = phi(x / (d0 + sign(eps, d0)),
+// (d0^2 + x^2) rdelinv) * d0, x the outer difference (dwC(i-1) for the
+// left state, dwC(i+1) for the right one) and d0 = dwC(i); phi = cada3_side.
+// Returns F and its partials in x and d0.
+template <class R>
+__device__ __forceinline__ R cada3_grad(R x, R d0, R rdel, R& dx, R& dd0) {
+  const R eps = R(1.4901161193847656e-08);
+  const R q = frcp(d0 + copysign(eps, d0));
+  const R th = x * q;
+  const R eta = (d0 * d0 + x * x) * rdel;
+  const R A = (R(2) + th) * R(1.0 / 3.0);
+  R lth;
+  const R lim = cada_limited_grad(th, lth);
+  R phi, pth, peta;
+  if (eta <= R(1) - R(1.0e-14)) {
+    phi = A; pth = R(1.0 / 3.0); peta = R(0);
+  } else if (eta >= R(1) + R(1.0e-14)) {
+    phi = lim; pth = lth; peta = R(0);
+  } else {  // the narrow blend of the two, linear in eta
+    const R t2 = (eta - R(1)) * R(1.0e14);
+    phi = R(0.5) * ((R(1) - t2) * A + (R(1) + t2) * lim);
+    pth = R(0.5) * ((R(1) - t2) * R(1.0 / 3.0) + (R(1) + t2) * lth);
+    peta = R(0.5e14) * (lim - A);
+  }
+  dx = d0 * (pth * q + peta * R(2) * x * rdel);
+  dd0 = phi + d0 * (-pth * x * (q * q) + peta * R(2) * d0 * rdel);
+  return phi * d0;
+}
+
+// Limited slopes of the two states of a face and their partials: t1 = ldw
+// of the left cell from (dm, d0), t2 = rdw of the right cell from (d0, dp);
+// d1a, d1b = dt1/d(dm, d0); d2a, d2b = dt2/d(d0, dp)
+template <class R, int LIM>
+__device__ __forceinline__ void face_slopes(R dm, R d0, R dp, R rdel, R& t1, R& t2, R& d1a, R& d1b, R& d2a,
+                                            R& d2b) {
+  if constexpr (LIM == LIM_CADA3) {
+    t1 = cada3_grad(dm, d0, rdel, d1a, d1b);
+    t2 = cada3_grad(dp, d0, rdel, d2b, d2a);
+  } else {
+    t1 = lim_grad<R, LIM>(dm, d0, d1a, d1b);
+    t2 = lim_grad<R, LIM>(d0, dp, d2a, d2b);
+  }
+}
+
 // Reverse of the face flux in primitive form: from the primitive states w of
 // the face's four cells (as the sweep rounds them to R) and the flux adjoint
 // fbar, writes the primitive adjoints wb[c] = (d fC / d w_c)^T fbar (or adds
 // them to wb[c] when ACC).
-template <class P, class R, int RIEM, int ID, int LIM, bool ACC = false>
-__device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const R* fbar, double (*wb)[P::NW]) {
+template <class P, class R, int RIEM, int ID, int LIM, bool ACC = false, class W = double>
+__device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, R ch, const R* fbar, W (*wb)[P::NW]) {
   static_assert(!P::MHD, "hydrodynamics");
   constexpr int NW = P::NW, ND = P::NDIR, RHO = P::RHO, E = P::E, MN = P::MOM + ID;
   const R gam = R(fp.gamma), inv_g1 = R(fp.inv_gamma_1);
   // ---------------- forward ----------------
+  const R rdel = R(fp.rdelinv[ID]);
   R wL[NW], wR[NW], d1a[NW], d1b[NW], d2a[NW], d2b[NW];
   for (int iw = 0; iw < NW; ++iw) {
     const R dm = w[1][iw] - w[0][iw], d0 = w[2][iw] - w[1][iw], dp = w[3][iw] - w[2][iw];
-    const R t1 = lim_grad<R, LIM>(dm, d0, d1a[iw], d1b[iw]);
-    const R t2 = lim_grad<R, LIM>(d0, dp, d2a[iw], d2b[iw]);
+    R t1, t2;
+    face_slopes<R, LIM>(dm, d0, dp, rdel, t1, t2, d1a[iw], d1b[iw], d2a[iw], d2b[iw]);
     wL[iw] = w[1][iw] + R(0.5) * t1;
     wR[iw] = w[2][iw] - R(0.5) * t2;
   }
@@ -352,15 +415,15 @@ __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const 
     const R d0b = t1b * d1b[iw] + t2b * d2a[iw];
     const R dpb = t2b * d2b[iw];
     if constexpr (ACC) {
-      wb[0][iw] += double(-dmb);
-      wb[1][iw] += double(wLb[iw] + dmb - d0b);
-      wb[2][iw] += double(wRb[iw] + d0b - dpb);
-      wb[3][iw] += double(dpb);
+      wb[0][iw] += W(-dmb);
+      wb[1][iw] += W(wLb[iw] + dmb - d0b);
+      wb[2][iw] += W(wRb[iw] + d0b - dpb);
+      wb[3][iw] += W(dpb);
     } else {
-      wb[0][iw] = double(-dmb);
-      wb[1][iw] = double(wLb[iw] + dmb - d0b);
-      wb[2][iw] = double(wRb[iw] + d0b - dpb);
-      wb[3][iw] = double(dpb);
+      wb[0][iw] = W(-dmb);
+      wb[1][iw] = W(wLb[iw] + dmb - d0b);
+      wb[2][iw] = W(wRb[iw] + d0b - dpb);
+      wb[3][iw] = W(dpb);
     }
   }
 }
@@ -371,22 +434,38 @@ __device__ void face_adj_prim_hd(const R (*w)[P::NW], const FVParams& fp, const 
 // the fast speed (mhd_get_cbounds :3037-3127, mhd_get_csound_prim
 // :3329-3379) -> HLL or TVDLF; the normal field takes Lax-Friedrichs
 // dissipation under HLL (flux_type, :580-588).
-template <class P, class R, int RIEM, int ID, int LIM, bool ACC = false>
-__device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const R* fbar, double (*wb)[P::NW]) {
-  static_assert(P::MHD && !P::GLM, "ideal MHD without GLM");
+template <class P, class R, int RIEM, int ID, int LIM, bool ACC = false, class W = double>
+__device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, R ch, const R* fbar,
+                                  W (*wb)[P::NW]) {
+  static_assert(P::MHD && !P::LINDE, "ideal MHD, without divergence control in the face or with GLM");
   static_assert(RIEM == RIEM_HLL || RIEM == RIEM_TVDLF, "MHD faces take hll or tvdlf");
   constexpr int NW = P::NW, ND = P::NDIR, RHO = P::RHO, E = P::E, MN = P::MOM + ID, BN = P::MAG + ID;
-  constexpr int BTV = (RIEM == RIEM_HLL && P::NDIM > 1) ? BN : -1;
+  // flux_type: the normal field takes Lax-Friedrichs dissipation under HLL
+  // only without GLM (src/mhd/mod_mhd_phys.t:580-588)
+  constexpr int BTV = (RIEM == RIEM_HLL && P::NDIM > 1 && !P::GLM) ? BN : -1;
   const R gam = R(fp.gamma), inv_g1 = R(fp.inv_gamma_1), eps = R(fp.tvdlfeps);
   // ---------------- forward ----------------
+  const R rdel = R(fp.rdelinv[ID]);
   R wL[NW], wR[NW], d1a[NW], d1b[NW], d2a[NW], d2b[NW];
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
     const R dm = w[1][iw] - w[0][iw], d0 = w[2][iw] - w[1][iw], dp = w[3][iw] - w[2][iw];
-    const R t1 = lim_grad<R, LIM>(dm, d0, d1a[iw], d1b[iw]);
-    const R t2 = lim_grad<R, LIM>(d0, dp, d2a[iw], d2b[iw]);
+    R t1, t2;
+    face_slopes<R, LIM>(dm, d0, dp, rdel, t1, t2, d1a[iw], d1b[iw], d2a[iw], d2b[iw]);
     wL[iw] = w[1][iw] + R(0.5) * t1;
     wR[iw] = w[2][iw] - R(0.5) * t2;
+  }
+  // GLM (mhd_modify_wLR, src/mhd/mod_mhd_phys.t:6326-6377): both states
+  // take the same normal field and psi; the conserved states below are
+  // those of the modified primitive ones, as the forward's energy
+  // correction makes them
+  if constexpr (P::GLM) {
+    constexpr int S = P::PSI;
+    const R dB = wR[BN] - wL[BN], dPsi = wR[S] - wL[S];
+    const R bs = R(0.5) * (wR[BN] + wL[BN] - fdiv(dPsi, ch));
+    const R ps = R(0.5) * (wR[S] + wL[S] - dB * ch);
+    wL[BN] = bs; wR[BN] = bs;
+    wL[S] = ps; wR[S] = ps;
   }
   R wLC[NW], wRC[NW];
 #pragma unroll
@@ -394,8 +473,8 @@ __device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const
   to_cons<P, R>(wLC, inv_g1);
   to_cons<P, R>(wRC, inv_g1);
   R fL[NW], fR[NW];
-  phys_flux<P, R, ID>(wLC, wL, R(0), fL);
-  phys_flux<P, R, ID>(wRC, wR, R(0), fR);
+  phys_flux<P, R, ID>(wLC, wL, ch, fL);
+  phys_flux<P, R, ID>(wRC, wR, ch, fR);
   // fast speeds, kept with their intermediates for the reverse pass
   struct Fast { R ir, cs, b2, cf2, av, sq, c; };
   auto fast = [&](const R* x) {
@@ -559,18 +638,24 @@ __device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       const int m = P::MOM + d, b = P::MAG + d;
+      // the induction row of the normal field is GLM's psi flux instead
+      const R fbb = (P::GLM && d == ID) ? R(0) : fb[b];
       // f[m] = xC[MN] v_d - bn B_d
       cmnb = cmnb + fb[m] * x[m];
-      xb[m] = xb[m] + fb[m] * xC[MN] + bvb * x[b] - fb[b] * bn;
-      bnb = bnb - fb[m] * x[b] - fb[b] * x[m];
+      xb[m] = xb[m] + fb[m] * xC[MN] + bvb * x[b] - fbb * bn;
+      bnb = bnb - fb[m] * x[b] - fbb * x[m];
       // f[b] = vn B_d - bn v_d; bv; ptot
-      vnb = vnb + fb[b] * x[b];
-      xb[b] = xb[b] - fb[m] * bn + bvb * x[m] + fb[b] * vn + ptotb * x[b];
+      vnb = vnb + fbb * x[b];
+      xb[b] = xb[b] - fb[m] * bn + bvb * x[m] + fbb * vn + ptotb * x[b];
     }
     xCb[MN] = xCb[MN] + cmnb;
     xb[E] = xb[E] + ptotb;
     xb[MN] = xb[MN] + vnb;
     xb[BN] = xb[BN] + bnb;
+    if constexpr (P::GLM) {  // f[BN] = psi, f[psi] = ch^2 B_n
+      xb[P::PSI] = xb[P::PSI] + fb[BN];
+      xb[BN] = xb[BN] + ch * ch * fb[P::PSI];
+    }
   };
   flux_rev(wL, wLC, fLb, wLb, wLCb);
   flux_rev(wR, wRC, fRb, wRb, wRCb);
@@ -589,9 +674,20 @@ __device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const
       xb[m] = xb[m] + R(2) * x[m] * vsqb + x[RHO] * xCb[m];
       xb[b] = xb[b] + xCb[b] + x[b] * xCb[E];
     }
+    if constexpr (P::GLM) xb[P::PSI] = xb[P::PSI] + xCb[P::PSI];
   };
   cons_rev(wL, wLCb, wLb);
   cons_rev(wR, wRCb, wRb);
+  if constexpr (P::GLM) {
+    // reverse of the shared interface state (B_n*, psi*) of both sides
+    constexpr int S = P::PSI;
+    const R bsb = wLb[BN] + wRb[BN], psb = wLb[S] + wRb[S];
+    const R bq = fdiv(R(0.5) * bsb, ch), pq = R(0.5) * ch * psb;
+    wLb[BN] = R(0.5) * bsb + pq;
+    wRb[BN] = R(0.5) * bsb - pq;
+    wLb[S] = bq + R(0.5) * psb;
+    wRb[S] = -bq + R(0.5) * psb;
+  }
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
     const R t1b = R(0.5) * wLb[iw], t2b = R(-0.5) * wRb[iw];
@@ -599,24 +695,24 @@ __device__ void face_adj_prim_mhd(const R (*w)[P::NW], const FVParams& fp, const
     const R d0b = t1b * d1b[iw] + t2b * d2a[iw];
     const R dpb = t2b * d2b[iw];
     if constexpr (ACC) {
-      wb[0][iw] += double(-dmb);
-      wb[1][iw] += double(wLb[iw] + dmb - d0b);
-      wb[2][iw] += double(wRb[iw] + d0b - dpb);
-      wb[3][iw] += double(dpb);
+      wb[0][iw] += W(-dmb);
+      wb[1][iw] += W(wLb[iw] + dmb - d0b);
+      wb[2][iw] += W(wRb[iw] + d0b - dpb);
+      wb[3][iw] += W(dpb);
     } else {
-      wb[0][iw] = double(-dmb);
-      wb[1][iw] = double(wLb[iw] + dmb - d0b);
-      wb[2][iw] = double(wRb[iw] + d0b - dpb);
-      wb[3][iw] = double(dpb);
+      wb[0][iw] = W(-dmb);
+      wb[1][iw] = W(wLb[iw] + dmb - d0b);
+      wb[2][iw] = W(wRb[iw] + d0b - dpb);
+      wb[3][iw] = W(dpb);
     }
   }
 }
 
-template <class P, class R, int RIEM, int ID, int LIM, bool ACC = false>
-__device__ __forceinline__ void face_adj_prim(const R (*w)[P::NW], const FVParams& fp, const R* fbar,
-                                              double (*wb)[P::NW]) {
-  if constexpr (P::MHD) face_adj_prim_mhd<P, R, RIEM, ID, LIM, ACC>(w, fp, fbar, wb);
-  else face_adj_prim_hd<P, R, RIEM, ID, LIM, ACC>(w, fp, fbar, wb);
+template <class P, class R, int RIEM, int ID, int LIM, bool ACC = false, class W = double>
+__device__ __forceinline__ void face_adj_prim(const R (*w)[P::NW], const FVParams& fp, R ch, const R* fbar,
+                                              W (*wb)[P::NW]) {
+  if constexpr (P::MHD) face_adj_prim_mhd<P, R, RIEM, ID, LIM, ACC, W>(w, fp, ch, fbar, wb);
+  else face_adj_prim_hd<P, R, RIEM, ID, LIM, ACC, W>(w, fp, ch, fbar, wb);
 }
 
 // Reverse of hd_to_primitive (FP64) for one cell: cbar += (d w / d cons)^T wb
@@ -639,6 +735,7 @@ __device__ __forceinline__ void prim_rev(const double* cons, const double* wb, c
   if constexpr (P::MHD) {  // p = (gamma-1)(e - (rho vsq + bsq)/2)
 #pragma unroll
     for (int k = 0; k < ND; ++k) cbar[P::MAG + k] += wb[P::MAG + k] + 2.0 * tt * cons[P::MAG + k];
+    if constexpr (P::GLM) cbar[P::PSI] += wb[P::PSI];
   }
   double rhob = wb[RHO] + tt * vsq;
   const double vsqb = tt * rho;
@@ -656,7 +753,7 @@ __device__ __forceinline__ void prim_rev(const double* cons, const double* wb, c
 // Conserved form: recomputes the forward face from the conserved values of
 // its four cells and accumulates (d fC / d cons)^T fbar into cbar.
 template <class P, class R, int RIEM, int ID, int LIM>
-__device__ void face_flux_adj(const double (*cons)[P::NW], const FVParams& fp, const R* fbar,
+__device__ void face_flux_adj(const double (*cons)[P::NW], const FVParams& fp, R ch, const R* fbar,
                               double (*cbar)[P::NW]) {
   constexpr int NW = P::NW;
   R w[4][NW];
@@ -666,7 +763,7 @@ __device__ void face_flux_adj(const double (*cons)[P::NW], const FVParams& fp, c
     for (int iw = 0; iw < NW; ++iw) w[c][iw] = R(t[iw]);
   }
   double wb[4][NW];
-  face_adj_prim<P, R, RIEM, ID, LIM>(w, fp, fbar, wb);
+  face_adj_prim<P, R, RIEM, ID, LIM>(w, fp, ch, fbar, wb);
   for (int c = 0; c < 4; ++c) prim_rev<P>(cons[c], wb[c], fp, cbar[c]);
 }
 

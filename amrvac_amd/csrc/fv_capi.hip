@@ -1854,22 +1854,26 @@ int amrvac_fv_set_fix_conserve(amrvac_fv_ctx* c, int on) {
 }
 
 
-int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, const double* u_in,
-                       const double* lam_out, double coef, const double* add_a, double ca, const double* add_b,
-                       double cb, double* lam_in) {
+int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt, double cmax_global,
+                       const double* u_in, const double* lam_out, double coef, const double* add_a, double ca,
+                       const double* add_b, double cb, double* lam_in) {
   if (!c || !u_in || !lam_out || !lam_in) return fail(AMRVAC_FV_EINVAL, "bad argument");
   if (!c->slot[0]) return fail(AMRVAC_FV_ESTATE, "no mesh");
   if (c->nws > 0) return fail(AMRVAC_FV_EINVAL, "adjoint: no constrained transport");
   if (c->bx != 16) return fail(AMRVAC_FV_EINVAL, "the adjoint stage needs block_nx = 16");
   if (lam_in == lam_out || lam_in == u_in) return fail(AMRVAC_FV_EINVAL, "lam_in must not alias lam_out or u_in");
-  if (c->cfg.physics == 2 && (c->cfg.divb != 0 || c->cfg.ndir != 3))
-    return fail(AMRVAC_FV_EINVAL, "the MHD adjoint stage is provided for ndir 3 without divergence cleaning");
+  if (c->cfg.physics == 2 && ((c->cfg.divb != 0 && c->cfg.divb != 1) || c->cfg.ndir != 3))
+    return fail(AMRVAC_FV_EINVAL, "the MHD adjoint stage is provided for ndir 3, without divergence control "
+                                  "or with GLM");
+  const bool glm = c->cfg.physics == 2 && c->cfg.divb == 1;
+  if (glm && !(cmax_global > 0.0)) return fail(AMRVAC_FV_EINVAL, "GLM needs cmax_global > 0");
   if (c->cfg.physics != 1 && c->cfg.physics != 2) return fail(AMRVAC_FV_EINVAL, "unknown physics");
   if (c->amr) return fail(AMRVAC_FV_EINVAL, "the adjoint stage needs a uniform level");
   for (int sd = 0; sd < 2 * c->cfg.ndim; ++sd)
     if (c->bc[sd] != 5) return fail(AMRVAC_FV_EINVAL, "the adjoint stage needs periodic boundaries");
   if (c->cfg.boundspeed != 1) return fail(AMRVAC_FV_EINVAL, "the adjoint stage needs Einfeldt bounds");
-  if (c->cfg.limiter != 1 && c->cfg.limiter != 5) return fail(AMRVAC_FV_EINVAL, "the adjoint needs minmod or vanleer");
+  if (c->cfg.limiter != 1 && c->cfg.limiter != 5 && c->cfg.limiter != 9)
+    return fail(AMRVAC_FV_EINVAL, "the adjoint needs minmod, vanleer or cada3");
   int riem;
   if (method_to_riem(method, riem) != 0 || riem == fv::RIEM_HANCOCK || riem == fv::RIEM_HLLD)
     return fail(AMRVAC_FV_EINVAL, "adjoint: flux scheme must be hll, hllc or tvdlf");
@@ -1884,10 +1888,11 @@ int amrvac_fv_adjoint1(amrvac_fv_ctx* c, int method, double dtfactor, double dt,
   const double qdt = dtfactor * dt;
   a.qdt = qdt;
   for (int d = 0; d < 3; ++d) a.dq[d] = -qdt / c->dx[d];
+  a.ch = cmax_global;
   fvc::AdjArgs g{lam_out, coef, add_a, ca, add_b, cb, lam_in, c->d_adjG};
   const int mixed = c->cfg.precision == AMRVAC_FV_PREC_MIXED ? 1 : 0;
-  hipError_t e =
-      fv::launch_adjoint_stage(c->cfg.ndim, c->cfg.physics, mixed, riem, c->cfg.limiter, a, c->fp, g, c->stream);
+  hipError_t e = fv::launch_adjoint_stage(c->cfg.ndim, c->cfg.physics, glm ? 1 : 0, mixed, riem, c->cfg.limiter, a,
+                                          c->fp, g, c->stream);
   if (e == hipErrorInvalidValue) return fail(AMRVAC_FV_EINVAL, "no adjoint kernel for this configuration");
   if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("adjoint kernels: ") + hipGetErrorString(e));
   return AMRVAC_FV_OK;

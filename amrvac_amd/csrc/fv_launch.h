@@ -62,10 +62,11 @@ FV_DECLARE_LAUNCHERS(fv)
 FV_DECLARE_LAUNCHERS(fvx)
 
 namespace fv {
-// adjoint stage (hydrodynamics or ideal MHD without divergence cleaning,
-// 2-D / 3-D, FP64 or FP32 faces)
-hipError_t launch_adjoint_stage(int ndim, int physics, int mixed, int riem, int lim, const fvc::SweepArgs& a,
-                                const fvc::FVParams& fp, const fvc::AdjArgs& g, hipStream_t s);
+// adjoint stage (hydrodynamics, or ideal MHD without divergence control or
+// with GLM, 2-D / 3-D, FP64 or FP32 faces)
+hipError_t launch_adjoint_stage(int ndim, int physics, int divb, int mixed, int riem, int lim,
+                                const fvc::SweepArgs& a, const fvc::FVParams& fp, const fvc::AdjArgs& g,
+                                hipStream_t s);
 }
 
 namespace FV_NS {

@@ -86,7 +86,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.amrvac_fv_getbc.argtypes = [vp, ctypes.c_int]
     L.amrvac_fv_set_fix_conserve.argtypes = [vp, ctypes.c_int]
     dbl = ctypes.c_double
-    L.amrvac_fv_adjoint1.argtypes = [vp, ctypes.c_int, dbl, dbl, vp, vp, dbl, vp, dbl, vp, dbl, vp]
+    L.amrvac_fv_adjoint1.argtypes = [vp, ctypes.c_int, dbl, dbl, dbl, vp, vp, dbl, vp, dbl, vp, dbl, vp]
     L.amrvac_fv_stage.argtypes = [vp, ctypes.c_int, dbl, dbl, dbl, vp, vp, dbl, dbl, vp]
     L.amrvac_fv_getdt_state.argtypes = [vp, vp, dbl, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
     L.amrvac_fv_errest.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, dp, dbl, dbl, dbl,
@@ -211,11 +211,13 @@ class DeviceFV:
         self._check(self.L.amrvac_fv_slot_copy(self.h, slot, ctypes.c_void_p(dptr), 1))
 
     def adjoint1(self, method: int, dtfactor: float, dt: float, u_in: int, lam_out: int, coef: float,
-                 lam_in: int, add_a: int = 0, ca: float = 0.0, add_b: int = 0, cb: float = 0.0) -> None:
+                 lam_in: int, add_a: int = 0, ca: float = 0.0, add_b: int = 0, cb: float = 0.0,
+                 ch: float = 0.0) -> None:
         """lam_in = coef*lam_out + ca*add_a + cb*add_b + (d stage/d u_in)^T lam_out
-        (device pointers; 0 for an absent add term)."""
+        (device pointers; 0 for an absent add term); ch: the GLM speed of
+        the step."""
         v = ctypes.c_void_p
-        self._check(self.L.amrvac_fv_adjoint1(self.h, method, dtfactor, dt, v(u_in), v(lam_out), coef,
+        self._check(self.L.amrvac_fv_adjoint1(self.h, method, dtfactor, dt, ch, v(u_in), v(lam_out), coef,
                                               v(add_a) if add_a else None, ca, v(add_b) if add_b else None, cb,
                                               v(lam_in)))
 

@@ -316,6 +316,7 @@ class RunState:
     tsavelast_log: float = 0.0
     log: list = field(default_factory=list)
     dts: list = field(default_factory=list)
+    chs: list = field(default_factory=list)  # cmax_global of each step (GLM c_h)
 
 
 class TimeIntegrator:
@@ -458,6 +459,7 @@ class TimeIntegrator:
                 break
             self.advect()
             st.dts.append(st.dt)
+            st.chs.append(st.cmax_global)
             if st.crash or self.be.crash():
                 st.crash = True
                 raise RuntimeError("Error: small value encountered, run crash.")
@@ -471,6 +473,7 @@ class TimeIntegrator:
         self.setdt()
         self.advect()
         self.state.dts.append(self.state.dt)
+        self.state.chs.append(self.state.cmax_global)
         self.state.it += 1
         self.state.global_time = self.state.global_time + self.state.dt
 

@@ -115,21 +115,28 @@ def adjoint_cost(kind: str, reps: int = 2):
             return ((adjoint.run_differentiable(u0, run)[:, 0] - target) ** 2).mean()
         workload = "2-D HD Kelvin-Helmholtz 128^2 (16^2 blocks), threestep/HLLC/vanleer, 200 steps, checkpoints every 20"
     else:
+        # ot3d: the differentiable 3-D workload (vanleer, no divergence
+        # control); ot3d_glm_cada3: the same with the benchmark's scheme
+        # (GLM, cada3)
+        glm = kind == "ot3d_glm_cada3"
         n, nsteps = 128, 100
         mesh = UniformMesh(3, (n, n, n), (16, 16, 16), (0.0,) * 3, (1.0,) * 3, ("periodic",) * 6)
-        par = Par(physics="mhd", ndim=3, ndir=3, gamma=5.0 / 3.0, typedivbfix="none", time_stepper="threestep",
-                  flux_scheme="hll", limiter="vanleer", courantpar=0.6)
+        par = Par(physics="mhd", ndim=3, ndir=3, gamma=5.0 / 3.0, typedivbfix="glm" if glm else "none",
+                  time_stepper="threestep", flux_scheme="hll", limiter="cada3" if glm else "vanleer",
+                  courantpar=0.6)
         idx = torch.as_tensor(adjoint.block_index(mesh), device=dev)
         run = adjoint.SSPRK3Run(par, mesh, nsteps, precision="mixed")
         rho, v, p, bs = adjoint.orszag_tang_fields(mesh, 20261202, device=dev)
+        psi = torch.zeros_like(rho) if glm else None
         xs = (torch.tensor(1.0, dtype=torch.float64, device=dev), rho)
 
         def loss(xs):
-            u0 = adjoint.to_blocks(adjoint.mhd_conserved(xs[1], v, p, [xs[0] * c for c in bs], 5.0 / 3.0), idx)
+            u0 = adjoint.to_blocks(adjoint.mhd_conserved(xs[1], v, p, [xs[0] * c for c in bs], 5.0 / 3.0, psi=psi),
+                                   idx)
             bf = adjoint.run_differentiable(u0, run)[:, 5:8]
             return 0.5 * (bf * bf).sum() / n ** 3
-        workload = (f"3-D ideal MHD Orszag-Tang 128^3 (16^3 blocks), threestep/HLL/vanleer, 100 steps, "
-                    f"checkpoints every {run.every}")
+        workload = (f"3-D ideal MHD Orszag-Tang 128^3 (16^3 blocks), threestep/HLL/"
+                    f"{'cada3 + GLM' if glm else 'vanleer'}, 100 steps, checkpoints every {run.every}")
     tf = tg = float("inf")
     for _ in range(reps + 1):  # the first pass warms up
         torch.cuda.synchronize()
@@ -341,7 +348,7 @@ def main() -> int:
                            "traffic": load_traffic("mixed")}
         del w64, wmx
     if not args.no_adjoint:
-        result["adjoint_forward_cost"] = {k: adjoint_cost(k) for k in ("kh2d", "ot3d")}
+        result["adjoint_forward_cost"] = {k: adjoint_cost(k) for k in ("kh2d", "ot3d", "ot3d_glm_cada3")}
     if not args.no_amr:
         result["amr_blast"] = {"workload": "2-D HD blast wave, 64^2 base in 16^2 blocks, 4 levels (512^2 effective), "
                                            "Loehner regrid every step, twostep/TVDLF/minmod",

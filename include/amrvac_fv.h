@@ -224,19 +224,25 @@ int amrvac_fv_getdt_state_noalloc(amrvac_fv_ctx* ctx, const double* u, double co
                                   double* dtmin_out, double* cmax_out);
 
 /* Adjoint of one stage (no counterpart in the reference, which is not
-   differentiable; the reverse of finite_volume's flux divergence,
-   src/mod_finite_volume.t:116-339, on a uniform periodic level with
-   Einfeldt bounds and minmod / vanleer: hd with hll / hllc / tvdlf, or
-   ideal mhd (ndir 3, typedivbfix 'none') with hll / tvdlf):
+   differentiable; the reverse of finite_volume's flux divergence and
+   sources, src/mod_finite_volume.t:116-339, on a uniform periodic level with
+   Einfeldt bounds and minmod / vanleer / cada3: hd with hll / hllc / tvdlf,
+   or ideal mhd (ndir 3) with hll / tvdlf, without divergence control or
+   with GLM (typedivbfix 'glm': mhd_modify_wLR, the psi / B_n fluxes and the
+   extended sources, src/mhd/mod_mhd_phys.t:6326-6377, 5472-5531)):
      lam_in = coef*lam_out + ca*add_a + cb*add_b
               + (d (qdt L(u)) / d u)^T lam_out  at u = u_in,
-   qdt = dtfactor*dt; add_a / add_b may be NULL.  All arrays are device
-   arrays in the slot layout; lam_in must not alias lam_out or u_in (it may
-   alias add_a or add_b).  Limiter and Riemann branches are differentiated
-   along the branch the forward state takes; the result is deterministic. */
+   qdt = dtfactor*dt; cmax_global is the GLM speed c_h of the step (held
+   fixed, as dt is).  With GLM the stage's psi decay exp(-qdt c_h alpha /
+   min dx) is a diagonal factor on the psi row that the caller applies to
+   lam_out (and to the add terms of other stages) beforehand.  add_a /
+   add_b may be NULL.  All arrays are device arrays in the slot layout;
+   lam_in must not alias lam_out or u_in (it may alias add_a or add_b).
+   Limiter and Riemann branches are differentiated along the branch the
+   forward state takes; the result is deterministic. */
 int amrvac_fv_adjoint1(amrvac_fv_ctx* ctx, int method, double dtfactor, double dt,
-                       const double* u_in, const double* lam_out, double coef,
-                       const double* add_a, double ca, const double* add_b,
+                       double cmax_global, const double* u_in, const double* lam_out,
+                       double coef, const double* add_a, double ca, const double* add_b,
                        double cb, double* lam_in);
 
 /* ps[dst] = ps[src] (interior cells) */

@@ -57,7 +57,8 @@ class FakeBackend:
         init = comb_a * _view(base, self.shape) + comb_b * x if base else x
         _view(u_out, self.shape)[...] = init + qdt * (_S(x) + C * x * x)
 
-    def adjoint1(self, method, dtfactor, dt, u_in, lam_out, coef, lam_in, add_a=0, ca=0.0, add_b=0, cb=0.0):
+    def adjoint1(self, method, dtfactor, dt, u_in, lam_out, coef, lam_in, add_a=0, ca=0.0, add_b=0, cb=0.0,
+                 ch=0.0):
         self.calls["adjoint1"] += 1
         qdt = dtfactor * dt
         x = _view(u_in, self.shape)

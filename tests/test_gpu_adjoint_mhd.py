@@ -28,10 +28,10 @@ FIELD_AMP = 1.0e-3
 class _MHDProblem:
     """loss(b0, rho0) = sum over cells of |B_final|^2 / 2 * dV"""
 
-    def __init__(self, n, nsteps, precision="fp64", every=None, field_amp=FIELD_AMP):
+    def __init__(self, n, nsteps, precision="fp64", every=None, field_amp=FIELD_AMP, divb="none", lim="vanleer"):
         self.mesh = UniformMesh(3, (n, n, n), (16, 16, 16), (0.0,) * 3, (1.0,) * 3, ("periodic",) * 6)
-        self.par = Par(physics="mhd", ndim=3, ndir=3, gamma=GAMMA, typedivbfix="none", time_stepper="threestep",
-                       flux_scheme="hll", limiter="vanleer", courantpar=0.6)
+        self.par = Par(physics="mhd", ndim=3, ndir=3, gamma=GAMMA, typedivbfix=divb, time_stepper="threestep",
+                       flux_scheme="hll", limiter=lim, courantpar=0.6)
         self.dev = torch.device("cuda")
         self.idx = torch.as_tensor(adjoint.block_index(self.mesh), device=self.dev)
         self.run = adjoint.SSPRK3Run(self.par, self.mesh, nsteps, every=every, precision=precision)
@@ -42,8 +42,9 @@ class _MHDProblem:
         self.dts = None
 
     def loss(self, b0, rho, dts=None):
-        u0 = adjoint.to_blocks(adjoint.mhd_conserved(rho, self.v, self.p, [b0 * c for c in self.bshape], GAMMA),
-                               self.idx)
+        psi = torch.zeros_like(rho) if self.par.typedivbfix == "glm" else None
+        u0 = adjoint.to_blocks(adjoint.mhd_conserved(rho, self.v, self.p, [b0 * c for c in self.bshape], GAMMA,
+                                                     psi=psi), self.idx)
         uf = adjoint.run_differentiable(u0, self.run, dts)
         bf = uf[:, 5:8]
         return 0.5 * (bf * bf).sum() * self.dV
@@ -58,19 +59,23 @@ class _MHDProblem:
 
 
 @pytest.mark.parametrize("physics,riemann,lim", [("mhd", "hll", "vanleer"), ("mhd", "tvdlf", "minmod"),
-                                                 ("hd", "hllc", "vanleer")])
+                                                 ("hd", "hllc", "vanleer"), ("mhd-glm", "hll", "cada3"),
+                                                 ("mhd-glm", "tvdlf", "vanleer"), ("mhd-glm", "hll", "vanleer"),
+                                                 ("mhd", "hll", "cada3")])
 def test_adjoint_3d_matches_torch_autograd(physics, riemann, lim):
     """3-D gradients of sum(u_final * r) (r seeded) w.r.t. the initial
     conserved state against torch autograd through the FP64 PyTorch
-    restatement, on the CPU, for the dt sequence of the device run."""
+    restatement, on the CPU, for the dt (and GLM c_h) sequence of the
+    device run.  mhd-glm with hll / cada3 is the benchmark's scheme."""
     print(f"[adjoint-3d] {physics} {riemann}/{lim} 32^3 vs torch autograd", flush=True)
     mesh = UniformMesh(3, (32, 32, 32), (16, 16, 16), (0.0,) * 3, (1.0,) * 3, ("periodic",) * 6)
+    glm = physics == "mhd-glm"
     if physics == "hd":
         par = Par(physics="hd", ndim=3, ndir=3, gamma=1.4, time_stepper="threestep", flux_scheme=riemann,
                   limiter=lim, courantpar=0.8)
     else:
-        par = Par(physics="mhd", ndim=3, ndir=3, gamma=GAMMA, typedivbfix="none", time_stepper="threestep",
-                  flux_scheme=riemann, limiter=lim, courantpar=0.6)
+        par = Par(physics="mhd", ndim=3, ndir=3, gamma=GAMMA, typedivbfix="glm" if glm else "none",
+                  time_stepper="threestep", flux_scheme=riemann, limiter=lim, courantpar=0.6)
     dev = torch.device("cuda")
     idx = torch.as_tensor(adjoint.block_index(mesh), device=dev)
     rho, v, p, b = adjoint.orszag_tang_fields(mesh, SEED, device=dev)
@@ -79,22 +84,29 @@ def test_adjoint_3d_matches_torch_autograd(physics, riemann, lim):
     # (every slope there sits on a limiter kink), so all fields get noise
     noise = lambda q: q + 1.0e-3 * torch.randn(q.shape, generator=gen, dtype=torch.float64).to(dev)
     v, b, p = [noise(c) for c in v], [noise(c) for c in b], noise(p)
+    psi = 1.0e-3 * torch.randn(rho.shape, generator=gen, dtype=torch.float64).to(dev) if glm else None
     u0 = (adjoint.hd_conserved(rho, v, p, par.gamma) if physics == "hd"
-          else adjoint.mhd_conserved(rho, v, p, b, par.gamma))
+          else adjoint.mhd_conserved(rho, v, p, b, par.gamma, psi=psi))
     r = torch.randn(u0.shape, generator=gen, dtype=torch.float64)
     run = adjoint.SSPRK3Run(par, mesh, 4, every=2)
     x = u0.clone().requires_grad_(True)
     uf = adjoint.from_blocks(adjoint.run_differentiable(adjoint.to_blocks(x, idx), run), idx, (32, 32, 32))
     g_dev, = torch.autograd.grad((uf * r.to(dev)).sum(), x)
-    sch = torch_fv_ref.Scheme(3, physics, par.gamma, lim, riemann)
+    sch = torch_fv_ref.Scheme(3, "hd" if physics == "hd" else "mhd", par.gamma, lim, riemann, glm=glm)
     xc = u0.detach().cpu().clone().requires_grad_(True)
-    ufc = sch.run(xc, run.dts, mesh.dx)
+    ufc = sch.run(xc, run.dts, mesh.dx, run.chs)
     assert torch.allclose(ufc.detach(), uf.detach().cpu(), rtol=1e-10, atol=1e-12)
     g_ref, = torch.autograd.grad((ufc * r).sum(), xc)
     rel = ((g_dev.cpu() - g_ref).norm() / g_ref.norm()).item()
     worst = ((g_dev.cpu() - g_ref).abs().max() / g_ref.abs().max()).item()
     print(f"[adjoint-3d] rel L2 {rel:.3e}, max {worst:.3e}", flush=True)
-    assert rel < 1e-9
+    # cada3's gradient is ill-conditioned (theta = x / (d0 + sign(eps, d0))
+    # with eps = 1.5e-8): a 1e-14 relative change of the state moves the
+    # stage gradient by 3e-7 (tests/test_torch_ref.py), and the device and
+    # torch forward runs differ at rounding level, so over four steps the
+    # two gradients agree to 4e-8 (GLM) / 6e-9 (none); at one stage, at the
+    # same state, to 2e-10 (test_adjoint_stage_matches_torch_autograd)
+    assert rel < (1e-7 if lim == "cada3" else 1e-9)
 
 
 def _fd(pb: _MHDProblem, gb: float, gr: torch.Tensor, ndir: int = 3, eps_random: float = 1.0e-5):
@@ -183,3 +195,70 @@ def test_mhd_adjoint_config_128():
     # directions can agree (measured 3.8e-4); b0 and the gradient direction
     # still meet 1e-4 (4.8e-9, 7.2e-7).
     assert rm["b0"] < 1.0e-6 and rm["grad_dir"] < 1.0e-4 and rm["random"] < 1.0e-3
+
+
+@pytest.mark.parametrize("divb,riemann,lim", [("glm", "hll", "cada3"), ("none", "hll", "cada3"),
+                                              ("glm", "hll", "vanleer")])
+def test_adjoint_stage_matches_torch_autograd(divb, riemann, lim):
+    """One stage at one point: the device adjoint (J^T lam at u_in, with the
+    psi row of lam scaled by the stage's decay as the driver does) against
+    torch autograd of the FP64 PyTorch stage at the same u_in, 32^3.  No
+    trajectory lies between the two, so limiter and Riemann branches are
+    those of the same state."""
+    print(f"[adjoint-stage] {divb} {riemann}/{lim} 32^3", flush=True)
+    mesh = UniformMesh(3, (32, 32, 32), (16, 16, 16), (0.0,) * 3, (1.0,) * 3, ("periodic",) * 6)
+    par = Par(physics="mhd", ndim=3, ndir=3, gamma=GAMMA, typedivbfix=divb, time_stepper="threestep",
+              flux_scheme=riemann, limiter=lim, courantpar=0.6)
+    glm = divb == "glm"
+    dev = torch.device("cuda")
+    idx = torch.as_tensor(adjoint.block_index(mesh), device=dev)
+    rho, v, p, b = adjoint.orszag_tang_fields(mesh, SEED, device=dev)
+    gen = torch.Generator(device="cpu").manual_seed(SEED + 5)
+    noise = lambda q: q + 1.0e-3 * torch.randn(q.shape, generator=gen, dtype=torch.float64).to(dev)
+    v, b, p = [noise(c) for c in v], [noise(c) for c in b], noise(p)
+    psi = 1.0e-3 * torch.randn(rho.shape, generator=gen, dtype=torch.float64).to(dev) if glm else None
+    u = adjoint.mhd_conserved(rho, v, p, b, par.gamma, psi=psi)
+    lam = torch.randn(u.shape, generator=gen, dtype=torch.float64).to(dev)
+    run = adjoint.SSPRK3Run(par, mesh, 1)
+    be = run.be
+    ub = adjoint.to_blocks(u, idx).contiguous()
+    dt, ch = be.getdt_state(ub.data_ptr(), par.courantpar)
+    beta = 2.0 / 3.0
+    lb = adjoint.to_blocks(lam, idx).contiguous()
+    lam_scaled = lb.clone()
+    run._psi_decay(lam_scaled, beta, dt, ch)
+    out = torch.empty_like(lb)
+    be.adjoint1(run.method, beta, dt, ub.data_ptr(), lam_scaled.data_ptr(), 0.0, out.data_ptr(), ch=ch)
+    torch.cuda.synchronize()
+    g_dev = adjoint.from_blocks(out, idx, (32, 32, 32)).cpu()
+    sch = torch_fv_ref.Scheme(3, "mhd", par.gamma, lim, riemann, glm=glm)
+    xc = u.detach().cpu().clone().requires_grad_(True)
+    oc = sch.stage(xc, torch.zeros_like(xc), beta * dt, mesh.dx, ch)
+    g_ref, = torch.autograd.grad((oc * lam.cpu()).sum(), xc)
+    rel = ((g_dev - g_ref).norm() / g_ref.norm()).item()
+    worst = ((g_dev - g_ref).abs().max() / g_ref.abs().max()).item()
+    print(f"[adjoint-stage] rel L2 {rel:.3e}, max {worst:.3e}", flush=True)
+    # the device's primitive states round differently from torch's (refined
+    # hardware reciprocal, FMA), which the ill-conditioned cada3 gradient
+    # turns into 1e-10 (measured 1.0e-10 GLM, 1.9e-10 none); vanleer 1.7e-14
+    assert rel < (1e-9 if lim == "cada3" else 1e-12)
+
+
+
+def test_glm_adjoint_finite_differences():
+    """3-D MHD + GLM (interface states, psi / B_n fluxes, psi decay and the
+    extended sources) with HLL, Einfeldt and vanleer at 64^3 for 10 steps,
+    FP64: the adjoint against FP64 central differences with the dt and c_h
+    sequences of the unperturbed run held fixed (measured: b0 5e-12, the
+    gradient direction 1.0e-7, random directions 4e-4 normwise; the random
+    directions as at 32^3 above).  The benchmark's cada3 limiter is not
+    used here: limiter_cada3 switches between its third-order and limited
+    forms at eta = 1 within a band of 2e-14, a jump that differences of a
+    run across 1e8 limiter evaluations cross; its reverse is checked face by
+    face against differences (tests/test_face_adjoint_cpu.py, jump points
+    skipped) and stage by stage against torch autograd."""
+    print("[adjoint-glm] 64^3, 10 steps, GLM / HLL / vanleer, FP64", flush=True)
+    pb = _MHDProblem(64, 10, "fp64", every=3, divb="glm", lim="vanleer")
+    _, gb, gr = pb.grad()
+    r = _fd(pb, gb, gr, eps_random=1.0e-6)
+    assert r["b0"] < 1.0e-4 and r["grad_dir"] < 1.0e-4 and r["random"] < 1.0e-2

@@ -1,12 +1,13 @@
 """FP64 PyTorch restatement of the finite-volume stage on a periodic uniform
 grid (test infrastructure), for 2-D / 3-D hydrodynamics and ideal MHD
-without divergence cleaning: threestep (SSPRK3), TVD reconstruction of the
-primitive variables with vanleer or minmod, Einfeldt bounds, and the TVDLF,
-HLL or (hydrodynamics) HLLC Riemann flux, as in src/mod_finite_volume.t
-(:116-339, :348-504), src/limiter/mod_limiter.t, src/hd/mod_hd_phys.t,
-src/hd/mod_hd_hllc.t and src/mhd/mod_mhd_phys.t (mhd_get_flux :3810-3889,
-mhd_get_cbounds :3037-3127, mhd_get_csound_prim :3329-3379, flux_type
-:580-588).  Written with torch tensor operations on global fields so that
+without divergence cleaning or with GLM: threestep (SSPRK3), TVD
+reconstruction of the primitive variables with vanleer, minmod or cada3,
+Einfeldt bounds, and the TVDLF, HLL or (hydrodynamics) HLLC Riemann flux, as
+in src/mod_finite_volume.t (:116-339, :348-504), src/limiter/mod_limiter.t
+(limiter_cada3 :253-290), src/hd/mod_hd_phys.t, src/hd/mod_hd_hllc.t and
+src/mhd/mod_mhd_phys.t (mhd_get_flux :3810-3889, mhd_get_cbounds
+:3037-3127, mhd_get_csound_prim :3329-3379, flux_type :580-588; GLM:
+mhd_modify_wLR :6326-6377, add_source_glm :5472-5531).  Written with torch tensor operations on global fields so that
 torch.autograd differentiates it: the independent reverse-mode reference
 for the hand-written adjoint kernels.
 
@@ -19,16 +20,22 @@ import torch
 
 
 class Scheme:
-    def __init__(self, ndim, physics="hd", gamma=1.4, lim="vanleer", riemann="hllc", ndir=None):
+    def __init__(self, ndim, physics="hd", gamma=1.4, lim="vanleer", riemann="hllc", ndir=None, glm=False,
+                 glm_alpha=0.5, glm_extended=True, cada3_radius=0.1):
         self.ndim = ndim
         self.mhd = physics == "mhd"
         self.ndir = ndir or (3 if self.mhd else ndim)
         self.gamma = gamma
         self.lim = lim
         self.riemann = riemann
+        self.glm = self.mhd and glm
+        self.glm_alpha = glm_alpha
+        self.glm_extended = glm_extended
+        self.cada3_radius = cada3_radius
         self.E = self.ndir + 1
         self.MAG = self.ndir + 2
-        self.nw = 2 * self.ndir + 2 if self.mhd else self.ndir + 2
+        self.PSI = 2 * self.ndir + 2
+        self.nw = (2 * self.ndir + 2 + (1 if self.glm else 0)) if self.mhd else self.ndir + 2
 
     # -- conversions -----------------------------------------------------------
     def to_prim(self, u):
@@ -45,7 +52,7 @@ class Scheme:
             for k in range(1, nd):
                 bsq = bsq + b[k] * b[k]
             p = (self.gamma - 1.0) * (u[E] - 0.5 * (rho * vsq + bsq))
-            return torch.stack([rho] + v + [p] + b)
+            return torch.stack([rho] + v + [p] + b + ([u[self.PSI]] if self.glm else []))
         p = (self.gamma - 1.0) * (u[E] - 0.5 * rho * vsq)
         return torch.stack([rho] + v + [p])
 
@@ -60,12 +67,13 @@ class Scheme:
             for k in range(1, nd):
                 bsq = bsq + w[self.MAG + k] * w[self.MAG + k]
             e = w[E] * (1.0 / (self.gamma - 1.0)) + 0.5 * (vsq * w[0] + bsq)
-            return torch.stack([w[0]] + m + [e] + [w[self.MAG + k] for k in range(nd)])
+            return torch.stack([w[0]] + m + [e] + [w[self.MAG + k] for k in range(nd)]
+                               + ([w[self.PSI]] if self.glm else []))
         e = w[E] * (1.0 / (self.gamma - 1.0)) + 0.5 * vsq * w[0]
         return torch.stack([w[0]] + m + [e])
 
     # -- fluxes and speeds -----------------------------------------------------
-    def flux(self, wc, w, idim):
+    def flux(self, wc, w, idim, ch=0.0):
         nd, E, mn = self.ndir, self.E, 1 + idim
         vn = w[mn]
         if not self.mhd:
@@ -84,6 +92,9 @@ class Scheme:
         f[mn] = f[mn] + ptot
         f.append(vn * (wc[E] + ptot) - bn * bv)
         f += [vn * w[M + k] - bn * w[1 + k] for k in range(nd)]
+        if self.glm:  # the normal field carries psi, psi carries ch^2 B_n
+            f[M + idim] = w[self.PSI]
+            f.append(ch * ch * bn)
         return torch.stack(f)
 
     def csq(self, w, idim):
@@ -110,15 +121,45 @@ class Scheme:
         s = torch.where(s == 0, torch.ones_like(s), s)
         return s * torch.clamp(torch.minimum(torch.abs(b), s * a), min=0.0)
 
-    def face_flux(self, w, idim):
+    @staticmethod
+    def cada3_side(x, d0, rdel):
+        """One face side of limiter_cada3 as face_limit forms it:
+        phi(x / (d0 + sign(eps, d0)), (d0^2 + x^2) rdel) * d0."""
+        eps = 1.4901161193847656e-08
+        th = x / (d0 + torch.copysign(torch.full_like(d0, eps), d0))
+        eta = (d0 * d0 + x * x) * rdel
+        A = (2.0 + th) * (1.0 / 3.0)
+        lim = torch.clamp(torch.minimum(A, torch.maximum(-0.5 * th, torch.clamp(torch.minimum(2.0 * th, A),
+                                                                                max=1.6))), min=0.0)
+        t2 = (eta - 1.0) * 1.0e14
+        blend = 0.5 * ((1.0 - t2) * A + (1.0 + t2) * lim)
+        phi = torch.where(eta <= 1.0 - 1.0e-14, A, torch.where(eta >= 1.0 + 1.0e-14, lim, blend))
+        return phi * d0
+
+    def face_flux(self, w, idim, dx=None, ch=0.0):
         """Flux at the face between cell i and i+1 along idim, indexed by i."""
         ax = self.ndim - idim
         sh = lambda k: torch.roll(w, shifts=-k, dims=ax)  # value at i+k
         wm, w0, wp, wpp = sh(-1), w, sh(1), sh(2)
-        wL = w0 + 0.5 * self.limiter(w0 - wm, wp - w0)
-        wR = wp - 0.5 * self.limiter(wp - w0, wpp - wp)
+        if self.lim == "cada3":
+            rdel = 1.0 / (self.cada3_radius * dx[idim]) ** 2
+            wL = w0 + 0.5 * self.cada3_side(w0 - wm, wp - w0, rdel)
+            wR = wp - 0.5 * self.cada3_side(wpp - wp, wp - w0, rdel)
+        else:
+            wL = w0 + 0.5 * self.limiter(w0 - wm, wp - w0)
+            wR = wp - 0.5 * self.limiter(wp - w0, wpp - wp)
+        if self.glm:  # shared interface state of the normal field and psi
+            bn, S = self.MAG + idim, self.PSI
+            bs = 0.5 * (wR[bn] + wL[bn] - (wR[S] - wL[S]) / ch)
+            ps = 0.5 * (wR[S] + wL[S] - (wR[bn] - wL[bn]) * ch)
+            rows = [wL[i] for i in range(self.nw)]
+            rows[bn], rows[S] = bs, ps
+            wL = torch.stack(rows)
+            rows = [wR[i] for i in range(self.nw)]
+            rows[bn], rows[S] = bs, ps
+            wR = torch.stack(rows)
         cL, cR = self.to_cons(wL), self.to_cons(wR)
-        fL, fR = self.flux(cL, wL, idim), self.flux(cR, wR, idim)
+        fL, fR = self.flux(cL, wL, idim, ch), self.flux(cR, wR, idim, ch)
         mn = 1 + idim
         t1, t2 = torch.sqrt(wL[0]), torch.sqrt(wR[0])
         t3 = 1.0 / (t1 + t2)
@@ -135,7 +176,7 @@ class Scheme:
         fhll = (cmax * fL - cmin * fR + cmin * cmax * (cR - cL)) / D
         hll = torch.where(cmin >= 0, fL, torch.where(cmax <= 0, fR, fhll))
         if self.riemann == "hll":
-            if self.mhd and self.ndim > 1:
+            if self.mhd and self.ndim > 1 and not self.glm:
                 # normal field: Lax-Friedrichs dissipation (flux_type)
                 bnv = self.MAG + idim
                 M = torch.maximum(cmax, torch.abs(cmin))
@@ -178,27 +219,44 @@ class Scheme:
         return torch.where(cmin >= 0, fL, torch.where(cmax <= 0, fR, inner))
 
     # -- time stepping -----------------------------------------------------------
-    def stage(self, u_in, init, qdt, dx):
-        """init + sum_d (-qdt/dx_d)(fC(i) - fC(i-1)), dimension by dimension."""
+    def stage(self, u_in, init, qdt, dx, ch=0.0):
+        """init + sum_d (-qdt/dx_d)(fC(i) - fC(i-1)), dimension by dimension;
+        with GLM the psi decay and the extended sources follow."""
         w = self.to_prim(u_in)
         acc = init
         for idim in range(self.ndim):
             ax = self.ndim - idim
-            f = (-qdt / dx[idim]) * self.face_flux(w, idim)
+            f = (-qdt / dx[idim]) * self.face_flux(w, idim, dx, ch)
             acc = acc + f - torch.roll(f, shifts=1, dims=ax)
-        return acc
+        if not self.glm:
+            return acc
+        import math
+        rows = [acc[i] for i in range(self.nw)]
+        rows[self.PSI] = math.exp(-qdt * ch * self.glm_alpha / min(dx[:self.ndim])) * rows[self.PSI]
+        if self.glm_extended:
+            M, S = self.MAG, self.PSI
+            nb = lambda q, d, k: torch.roll(q, shifts=-k, dims=self.ndim - 1 - d)  # field q at i + k e_d
+            gp = [((nb(w[S], d, 1) - nb(w[S], d, -1)) * 0.5) / dx[d] for d in range(self.ndim)]
+            divb = 0.5 * (nb(w[M], 0, 1) - nb(w[M], 0, -1)) * (1.0 / dx[0])
+            for d in range(1, self.ndim):
+                divb = divb + 0.5 * (nb(w[M + d], d, 1) - nb(w[M + d], d, -1)) * (1.0 / dx[d])
+            for d in range(self.ndim):
+                rows[self.E] = rows[self.E] - qdt * w[M + d] * gp[d]
+            for k in range(self.ndir):
+                rows[1 + k] = rows[1 + k] - qdt * w[M + k] * divb
+        return torch.stack(rows)
 
-    def ssprk3_step(self, u, dt, dx):
+    def ssprk3_step(self, u, dt, dx, ch=0.0):
         a21, a31 = 3.0 / 4.0, 1.0 / 3.0
         a22, a33 = 1.0 - a21, 1.0 - a31
-        u1 = self.stage(u, u, 1.0 * dt, dx)
-        u2 = self.stage(u1, a21 * u + a22 * u1, 0.25 * dt, dx)
-        return self.stage(u2, a31 * u + a33 * u2, (2.0 / 3.0) * dt, dx)
+        u1 = self.stage(u, u, 1.0 * dt, dx, ch)
+        u2 = self.stage(u1, a21 * u + a22 * u1, 0.25 * dt, dx, ch)
+        return self.stage(u2, a31 * u + a33 * u2, (2.0 / 3.0) * dt, dx, ch)
 
-    def run(self, u0, dts, dx):
+    def run(self, u0, dts, dx, chs=None):
         u = u0
-        for dt in dts:
-            u = self.ssprk3_step(u, dt, dx)
+        for n, dt in enumerate(dts):
+            u = self.ssprk3_step(u, dt, dx, chs[n] if chs is not None else 0.0)
         return u
 
 
