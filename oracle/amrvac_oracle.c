@@ -1973,10 +1973,10 @@ int orc_create(const orc_params* p, int nblocks, const double* dx,
   s->nws = 0;
   s->ct = 0;
   if (p->phys == ORC_PHYS_MHD && p->divb == ORC_DIVB_CT) {
-    /* constrained transport: uniform periodic 2-D meshes (the edge loops
-       are written for three edge directions, but only the 2-D form is
-       checked against reference logs) */
-    if (p->ndim != 2 || p->type_ct < ORC_CT_AVERAGE || p->type_ct > ORC_CT_HLL) { free(s); return 6; }
+    /* constrained transport: uniform periodic 2-D and 3-D meshes (the 2-D
+       form is checked against the reference's logs, the 3-D one against
+       the 2-D one on z-invariant states and by axis permutation) */
+    if (p->ndim < 2 || p->type_ct < ORC_CT_AVERAGE || p->type_ct > ORC_CT_HLL) { free(s); return 6; }
     for (int i = 0; i < 2 * p->ndim; ++i)
       if (bc[i] != ORC_BC_PERIODIC) { free(s); return 6; }
     s->nws = p->ndim;

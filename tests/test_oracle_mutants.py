@@ -12,7 +12,10 @@ import subprocess
 
 import pytest
 
+from test_oracle_ct3d import CT3D_CHECKS
 from test_oracle_math import ALL_CHECKS
+
+CHECKS = {**ALL_CHECKS, **CT3D_CHECKS}
 
 ORACLE_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
 
@@ -30,6 +33,16 @@ MUTANTS = {
                           0, "ct_golden_contact"),
     "ct_normal_field_of_prims": ("wLC[bi] = bn; wRC[bi] = bn; wLp[i][bi] = bn; wRp[i][bi] = bn;",
                                  "wLC[bi] = bn; wRC[bi] = bn;", 0, "ct_golden_contact"),
+    # an edge field taken one cell off along its own edge (3-D only); the
+    # extrusions see it only where the shift reaches the block edge, the
+    # mirror check everywhere
+    "ct3d_edge_offset_along_edge": (
+        "const double dE = fE[idir * n + c] - fE[idir * n + SH(s, c, idim2, -1)];",
+        "const long ce = nd == 3 ? SH(s, c, idir, 1) : c;\n"
+        "              const double dE = fE[idir * n + ce] - fE[idir * n + SH(s, ce, idim2, -1)];", 0, "ct3d_mirror"),
+    "ct3d_contact_cell_efield_sign": (
+        "ECC[0 * n + c] = wp[b2 * n + c] * wp[m3 * n + c] - wp[b3 * n + c] * wp[m2 * n + c];",
+        "ECC[0 * n + c] = wp[b3 * n + c] * wp[m2 * n + c] - wp[b2 * n + c] * wp[m3 * n + c];", 0, "ct3d_extruded"),
     "hll_dissipation_sign": ("+ cmin * cmax * (wRC[iw] - wLC[iw])) /\n               (cmax - cmin);\n    }\n  }\n}",
                              "- cmin * cmax * (wRC[iw] - wLC[iw])) /\n               (cmax - cmin);\n    }\n  }\n}",
                              0, "sod_hll_cada3"),
@@ -135,7 +148,7 @@ def test_planted_mistake_is_caught(name, oracle_source, tmp_path):
     import numpy as np
     with np.errstate(all="ignore"):
         try:
-            msg = ALL_CHECKS[check](str(so_path))
+            msg = CHECKS[check](str(so_path))
         except RuntimeError as e:  # the run crashed: caught as well
             msg = str(e)
     assert msg != "", f"check {check!r} passes with the planted mistake {name!r}"
