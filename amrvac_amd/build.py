@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libamrvac_fv.so")
 
 ARCH = os.environ.get("AMRVAC_FV_ARCH", "gfx950")
-KERNEL_UNITS = ["fv_k2_hd", "fv_k2_mhd", "fv_k2_mhd_ndir2", "fv_k3_hd", "fv_k3_mhd", "fv_ct"]
+KERNEL_UNITS = ["fv_k2_hd", "fv_k2_mhd", "fv_k2_mhd_ndir2", "fv_k3_hd", "fv_k3_mhd", "fv_ct", "fv_ct3"]
 
 
 def hipcc() -> str:

@@ -896,9 +896,9 @@ int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
   if (c.physics == 2 && c.divb != 0 && c.divb != 1 && c.divb != 4 && c.divb != 8)
     return fail(AMRVAC_FV_EINVAL, "typedivbfix must be none (0), glm (1), linde (4) or ct (8)");
   if (c.physics == 2 && c.divb == 8) {
-    if (c.ndim != 2) return fail(AMRVAC_FV_EINVAL, "constrained transport is supported for ndim = 2");
     if (c.ndir != 3) return fail(AMRVAC_FV_EINVAL, "constrained transport needs ndir = 3");
-    if (c.block_nx[0] != 16) return fail(AMRVAC_FV_EINVAL, "constrained transport needs block_nx = 16");
+    if (c.block_nx[0] != 16 || (c.ndim == 3 && c.block_nx[2] != 16))
+      return fail(AMRVAC_FV_EINVAL, "constrained transport needs block_nx = 16");
     if (c.type_ct < 1 || c.type_ct > 3) return fail(AMRVAC_FV_EINVAL, "type_ct must be 1, 2 or 3");
     if (c.limiter == 16) return fail(AMRVAC_FV_EINVAL, "constrained transport takes a TVD limiter");
   }
@@ -1063,7 +1063,7 @@ static int alloc_level(amrvac_fv_ctx* c, int nblocks, int capacity, const double
       }
       HIPCHK(hipMemsetAsync(c->wsl[s], 0, nf * sizeof(double), c->stream));
     }
-    const size_t nfb = fv::ct_face_buffer_doubles(nblocks);
+    const size_t nfb = nd == 3 ? fv::ct3_scratch_doubles(nblocks) : fv::ct_face_buffer_doubles(nblocks);
     if (hipMalloc(&c->d_ctf, nfb * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->d_wsn, nf * sizeof(double)) != hipSuccess) {
       free_level(c);
@@ -1366,7 +1366,7 @@ static int stage_core(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
 }
 
 // One constrained-transport stage between slots (advect1 with stagger_grid,
-// uniform periodic 2-D level): face fluxes, edge fields, face and cell
+// uniform periodic 2-D or 3-D level): face fluxes, edge fields, face and cell
 // updates; the new face field replaces psb's (an array swap).  ps's face field feeds
 // the HLLD star states; when psb is ps it holds the stage's initial
 // combination.
@@ -1416,10 +1416,10 @@ static int ct_stage(amrvac_fv_ctx* c, int method, double dtfactor, double dt, in
   const double* wbase = comb_slot >= 0 ? c->wsl[comb_slot] : nullptr;
   const int bx_mode = psb == 0 ? mode : -1;
   const int mixed = c->cfg.precision == AMRVAC_FV_PREC_MIXED ? 1 : 0;
-  hipError_t e = exact ? fvx::launch_ct_stage(0, riem, c->cfg.limiter, c->cfg.type_ct, a, c->wsl[psa], wbase,
-                                             c->wsl[psb], c->wsl[0], bx_mode, c->d_ctf, c->d_wsn, c->fp, c->stream)
-                       : fv::launch_ct_stage(mixed, riem, c->cfg.limiter, c->cfg.type_ct, a, c->wsl[psa], wbase,
-                                             c->wsl[psb], c->wsl[0], bx_mode, c->d_ctf, c->d_wsn, c->fp, c->stream);
+  const auto launch = c->cfg.ndim == 3 ? (exact ? fvx::launch_ct3_stage : fv::launch_ct3_stage)
+                                       : (exact ? fvx::launch_ct_stage : fv::launch_ct_stage);
+  hipError_t e = launch(exact ? 0 : mixed, riem, c->cfg.limiter, c->cfg.type_ct, a, c->wsl[psa], wbase, c->wsl[psb],
+                        c->wsl[0], bx_mode, c->d_ctf, c->d_wsn, c->fp, c->stream);
   if (e == hipErrorInvalidValue) return fail(AMRVAC_FV_EINVAL, "no constrained-transport kernel for this scheme");
   if (e != hipSuccess) return fail(AMRVAC_FV_EHIP, std::string("ct stage: ") + hipGetErrorString(e));
   // the update kernel wrote every upper face of every block: the buffer
@@ -1464,8 +1464,12 @@ int amrvac_fv_download(amrvac_fv_ctx* c, int slot, double* w) {
   HIPCHK(hipMemcpyAsync(hc.data(), c->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   a.src = c->wsl[slot];
-  hipLaunchKernelGGL((fv::fill_ghosted_kernel<2>), grid, dim3(fvc::NTHREADS), 0, c->stream, a, c->nws,
-                     c->cfg.nghost, c->nxg[0], c->nxg[1], c->nxg[2], c->d_stage);
+  if (c->cfg.ndim == 2)
+    hipLaunchKernelGGL((fv::fill_ghosted_kernel<2>), grid, dim3(fvc::NTHREADS), 0, c->stream, a, c->nws,
+                       c->cfg.nghost, c->nxg[0], c->nxg[1], c->nxg[2], c->d_stage);
+  else
+    hipLaunchKernelGGL((fv::fill_ghosted_kernel<3>), grid, dim3(fvc::NTHREADS), 0, c->stream, a, c->nws,
+                       c->cfg.nghost, c->nxg[0], c->nxg[1], c->nxg[2], c->d_stage);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(hf.data(), c->d_stage, hf.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

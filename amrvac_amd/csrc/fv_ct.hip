@@ -18,34 +18,19 @@
 // field on the upper d face of that cell; a face buffer holds per block and
 // direction a 20 x 20 grid (faces -2..17 in both indices) of CT_FK values:
 // the NW unscaled fluxes, then vnorm, cbarmin, cbarmax, vbarC(1), vbarC(2).
-#include "fv_launch.h"
+#include "fv_ct_common.h"
 
 namespace FV_NS {
 namespace {
 
 using fvc::BX;
 constexpr int CTG = 20;            // face grid edge
-constexpr int CT_AVERAGE = 1, CT_CONTACT = 2, CT_HLL = 3;
-
 template <class P>
 struct CtLayout {
   static constexpr int FK = P::NW + 5;
   static constexpr int VN = P::NW, CBMIN = P::NW + 1, CBMAX = P::NW + 2, VB1 = P::NW + 3, VB2 = P::NW + 4;
   static constexpr size_t per_dim = (size_t)FK * CTG * CTG;
   static constexpr size_t per_block = 2 * per_dim;
-};
-
-struct CtArgs {
-  const double* ws_src;    // face field of the stage input
-  const double* ws_base;   // of the RK base slot (comb_mode 1, 2)
-  const double* ws_dst;    // of psb before the stage (comb_mode 0)
-  const double* ws_block;  // of ps (HLLD normal field) when bx_mode < 0
-  int bx_mode;             // < 0: ws_block as is; else the comb_mode whose result ps holds
-  double* faces;           // [nb][2][FK][20][20]
-  double* ws_new;          // [nb][2][16][16] new upper faces
-  int ct;                  // CT_*
-  int tgc;                 // transverse_ghost_cells
-  int tvdlf;               // 1: no cmin (cbarmin = cbarmax)
 };
 
 __device__ __forceinline__ double ws_at(const SweepArgs& a, const double* ws, int b, int d, int x, int y) {
@@ -154,19 +139,6 @@ __device__ __forceinline__ double fget(const CtArgs& t, int b, int dim, int k, i
   using L = CtLayout<P>;
   return t.faces[(size_t)b * L::per_block + (size_t)dim * L::per_dim + (size_t)k * CTG * CTG + (size_t)(j + 2) * CTG +
                  (i + 2)];
-}
-
-// reconstruct src/mod_constrained_transport.t:177-237 (TVD limiters) of four
-// values q(-1..2) around the edge between q(0) and q(1)
-template <int LIM>
-__device__ __forceinline__ void ct_recon(const FVParams& fp, double rdel, double qm, double q0, double q1, double q2,
-                                         double& qL, double& qR) {
-  const double dm = q0 - qm, d0 = q1 - q0, dp = q2 - q1;
-  double ldq, rdq, dummy;
-  dwlimiter<double, LIM>(fp.limiter, dm, d0, rdel, ldq, dummy);
-  dwlimiter<double, LIM>(fp.limiter, d0, dp, rdel, dummy, rdq);
-  qL = q0 + 0.5 * ldq;
-  qR = q1 - 0.5 * rdq;
 }
 
 template <class P, int LIM>

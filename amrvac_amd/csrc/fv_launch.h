@@ -57,6 +57,11 @@ using SweepLauncher = hipError_t (*)(const KernelKey&, const SweepArgs&, const F
                              int bx_mode, double* faces, double* ws_new, const fvc::FVParams& fp,      \
                              hipStream_t s);                                                         \
   size_t ct_face_buffer_doubles(int nblocks);                                                        \
+  hipError_t launch_ct3_stage(int mixed, int riem, int lim, int ct, const fvc::SweepArgs& a,            \
+                              const double* ws_src, const double* ws_base, const double* ws_dst,       \
+                              const double* ws_block, int bx_mode, double* scratch, double* ws_new,    \
+                              const fvc::FVParams& fp, hipStream_t s);                                 \
+  size_t ct3_scratch_doubles(int nblocks);                                                           \
   }
 FV_DECLARE_LAUNCHERS(fv)
 FV_DECLARE_LAUNCHERS(fvx)

@@ -58,8 +58,10 @@ typedef struct amrvac_fv_config {
   int block_nx[3];     /* interior cells per block; block_nx(1) = block_nx(2)
                           = 16, or 8 or 10 in 3-D (FP64) */
   int precision;       /* AMRVAC_FV_PREC_* */
-  int type_ct;         /* with divb = 8 (constrained transport): 1 average,
-                          2 uct_contact, 3 uct_hll (type_ct,
+  int type_ct;         /* with divb = 8 (constrained transport on a
+                          uniform periodic 2-D or 3-D level, ndir = 3,
+                          16-cell blocks): 1 average, 2 uct_contact,
+                          3 uct_hll (type_ct,
                           src/mhd/mod_mhd_phys.t:766-781) */
   int fix_small_values; /* 0 or 1 (fix_small_values, methodlist) with the
                           default small_values_method 'error': every cell a

@@ -146,39 +146,43 @@ def check_ct3d_extruded(lib_path=None, cases=EXTRUSIONS) -> str:
 
 # -- a fully 3-D state ------------------------------------------------------
 
-def vector_potential_faces(nx: int, amp: float):
-    """Face field ws[3, nx, nx, nx] (upper faces of each cell) and the cell
-    field by face averaging, on the unit cube with nx^3 cells, from a
-    periodic vector potential with one component per axis.  As in
-    b_from_vector_potentialA, each component is sampled at the centre of its
-    edges (the cell centre shifted by half a cell in the two other
-    directions) times the edge length, and a face's field is the
-    circulation around it over its area, with the edge orientation of
-    lvc."""
-    h = 1.0 / nx
-    Z, Y, X = np.meshgrid(*[(np.arange(nx) + 0.5) * h] * 3, indexing="ij")
-    ax = amp * np.sin(TP * (Y + 0.5 * h)) * np.cos(TP * (Z + 0.5 * h)) * h
-    ay = amp * np.sin(TP * (Z + 0.5 * h)) * np.cos(2 * TP * (X + 0.5 * h)) * h
-    az = amp * np.cos(TP * (X + 0.5 * h)) * np.sin(TP * (Y + 0.5 * h) + 0.3) * h
+def vector_potential_faces(n, lengths, amp: float):
+    """Face field ws[3, nz, ny, nx] (upper faces of each cell) and the cell
+    field by face averaging, on the periodic box [0, L_d] with n = (nx, ny,
+    nz) cells, from a periodic vector potential with one component per axis.
+    As in b_from_vector_potentialA, each component is sampled at the centre
+    of its edges (the cell centre shifted by half a cell in the two other
+    directions) times the edge length, and a face's field is the circulation
+    around it over its area, with the edge orientation of lvc."""
+    h = [lengths[d] / n[d] for d in range(3)]
+    Z, Y, X = np.meshgrid(*[(np.arange(n[d]) + 0.5) / n[d] for d in (2, 1, 0)], indexing="ij")
+    hx, hy, hz = (h[d] / lengths[d] for d in range(3))  # half-cell shifts in units of the box
+    ax = amp * np.sin(TP * (Y + 0.5 * hy)) * np.cos(TP * (Z + 0.5 * hz)) * h[0]
+    ay = amp * np.sin(TP * (Z + 0.5 * hz)) * np.cos(2 * TP * (X + 0.5 * hx)) * h[1]
+    az = amp * np.cos(TP * (X + 0.5 * hx)) * np.sin(TP * (Y + 0.5 * hy) + 0.3) * h[2]
     m = _roll_lower
-    area = h * h
-    bx = ((az - m(az, 1)) - (ay - m(ay, 2))) / area
-    by = ((ax - m(ax, 2)) - (az - m(az, 0))) / area
-    bz = ((ay - m(ay, 0)) - (ax - m(ax, 1))) / area
+    area = (h[1] * h[2], h[0] * h[2], h[0] * h[1])
+    bx = ((az - m(az, 1)) - (ay - m(ay, 2))) / area[0]
+    by = ((ax - m(ax, 2)) - (az - m(az, 0))) / area[1]
+    bz = ((ay - m(ay, 0)) - (ax - m(ax, 1))) / area[2]
     ws = np.stack([bx, by, bz])
-    cells = [0.5 / area * (ws[d] * area + m(ws[d], d) * area) for d in range(3)]
+    cells = [0.5 / area[d] * (ws[d] * area[d] + m(ws[d], d) * area[d]) for d in range(3)]
     return ws, cells
 
 
 def div_faces(ws, h):
-    return sum((ws[d] - _roll_lower(ws[d], d)) / h for d in range(3))
+    """Face-field divergence; h: the cell size, or one per direction."""
+    h = (h,) * 3 if np.isscalar(h) else h
+    return sum((ws[d] - _roll_lower(ws[d], d)) / h[d] for d in range(3))
 
 
-def sheared_state(nx: int, amp: float):
-    """Global state [11, nx, nx, nx] on the unit cube: a smooth, sheared,
-    compressible flow with no symmetry, and the vector-potential field."""
-    ws, (b1, b2, b3) = vector_potential_faces(nx, amp)
-    Z, Y, X = np.meshgrid(*[(np.arange(nx) + 0.5) / nx] * 3, indexing="ij")
+def sheared_state(n, amp: float, lengths=(1.0, 1.0, 1.0)):
+    """Global state [11, nz, ny, nx] on the periodic box [0, L_d]: a smooth,
+    sheared, compressible flow with no symmetry, and the vector-potential
+    field; n: cells per direction (one int for a cube)."""
+    n = (n,) * 3 if np.isscalar(n) else tuple(n)
+    ws, (b1, b2, b3) = vector_potential_faces(n, lengths, amp)
+    Z, Y, X = np.meshgrid(*[(np.arange(n[d]) + 0.5) / n[d] for d in (2, 1, 0)], indexing="ij")
     rho = 1.0 + 0.2 * np.sin(TP * X) * np.cos(TP * Z) + 0.1 * np.cos(TP * Y)
     v1 = 0.3 * np.sin(TP * Y) * np.cos(TP * (X + 0.1))
     v2 = 0.3 * np.sin(TP * (Z + 0.2)) * np.cos(TP * X)
@@ -276,3 +280,18 @@ def test_uct_hll_is_not_mirror_symmetric():
 def test_ct3d_keeps_div_b_at_round_off(ct):
     msg = check_ct3d_divb(cts=(ct,))
     assert msg == "", msg
+
+
+def test_ct3d_div_b_with_unequal_cell_sizes():
+    """Edge lengths and face areas differ per direction (dx, dy, dz =
+    1/32, 0.75/16, 2/48): div B stays at round-off."""
+    n, lengths = (32, 16, 48), (1.0, 0.75, 2.0)
+    mesh = UniformMesh(3, n, (16, 16, 16), (0.0,) * 3, lengths, ("periodic",) * 6)
+    g = sheared_state(n, 1e-2, lengths)
+    h = mesh.dx
+    assert np.abs(div_faces(g[8:], h)).max() < 1e-12
+    for ct in CT_TYPES:
+        gf, _ = run(ct_par(3, ct), mesh, g, nsteps=4)
+        bmax = np.abs(gf[8:]).max()
+        assert np.abs(div_faces(gf[8:], h)).max() * min(h) / bmax < 1e-13, ct
+        assert np.abs(gf[8:] - g[8:]).max() > 1e-3 * bmax, ct
