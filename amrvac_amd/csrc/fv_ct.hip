@@ -85,7 +85,8 @@ __device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams&
   }
   CtIO io;
   io.bn = wsl_at(wsl, ID, i, j);
-  io.bx = t.bx_mode < 0 ? ws_at(a, t.ws_block, b, ID, i, j) : ws_comb(a, t, t.bx_mode, b, ID, i, j);
+  if constexpr (RIEM == RIEM_HLLD)
+    io.bx = t.bx_mode < 0 ? ws_at(a, t.ws_block, b, ID, i, j) : ws_comb(a, t, t.bx_mode, b, ID, i, j);
   R fC[NW];
   face_flux<P, R, RIEM, ID, LIM, 2, true>(st, fp, R(0), fC, crash, &io);
   double* f = t.faces + (size_t)b * L::per_block + (size_t)ID * L::per_dim + (size_t)(j + 2) * CTG + (i + 2);

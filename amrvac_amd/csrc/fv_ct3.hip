@@ -114,7 +114,8 @@ __device__ void ct3_one_face(const SweepArgs& a, const CtArgs& t, const FVParams
   }
   CtIO io;
   io.bn = ws_at3(a, t.ws_src, b, ID, i, j, k);
-  io.bx = t.bx_mode < 0 ? ws_at3(a, t.ws_block, b, ID, i, j, k) : ws_comb3(a, t, t.bx_mode, b, ID, i, j, k);
+  if constexpr (RIEM == RIEM_HLLD)
+    io.bx = t.bx_mode < 0 ? ws_at3(a, t.ws_block, b, ID, i, j, k) : ws_comb3(a, t, t.bx_mode, b, ID, i, j, k);
   R fC[NW];
   face_flux<P, R, RIEM, ID, LIM, 2, true>(st, fp, R(0), fC, crash, &io);
   const size_t c = gidx(i, j, k);
@@ -134,19 +135,34 @@ __device__ void ct3_one_face(const SweepArgs& a, const CtArgs& t, const FVParams
   }
 }
 
+// thread q of a block's faces (or edges) -> cell (x, y, z), x fastest so
+// that neighbouring lanes touch neighbouring addresses for every direction;
+// lo / n: first index and count per direction
+__device__ __forceinline__ bool box_cell(int q, const int* lo, const int* n, int& x, int& y, int& z) {
+  if (q >= n[0] * n[1] * n[2]) return false;
+  x = q % n[0] + lo[0];
+  y = q / n[0] % n[1] + lo[1];
+  z = q / (n[0] * n[1]) + lo[2];
+  return true;
+}
+
 // blockIdx.y: direction, blockIdx.z: block; faces -1..15 along the
-// direction (fastest), -tgc..15+tgc across it
+// direction, -tgc..15+tgc across it
 template <class P, class R, int RIEM, int LIM>
 __global__ void __launch_bounds__(fvc::NTHREADS) ct3_face_kernel(SweepArgs a, CtArgs t, FVParams fp) {
   const int d = blockIdx.y, b = blockIdx.z;
-  const int nac = BX + 2 * t.tgc, nal = BX + 1;
-  const int q = blockIdx.x * fvc::NTHREADS + threadIdx.x;
-  if (q >= nal * nac * nac) return;
-  const int along = q % nal - 1, u = q / nal % nac - t.tgc, v = q / (nal * nac) - t.tgc;
+  int lo[3], n[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    lo[e] = e == d ? -1 : -t.tgc;
+    n[e] = e == d ? BX + 1 : BX + 2 * t.tgc;
+  }
+  int x, y, z;
+  if (!box_cell(blockIdx.x * fvc::NTHREADS + threadIdx.x, lo, n, x, y, z)) return;
   int crash = 0;
-  if (d == 0) ct3_one_face<P, R, RIEM, LIM, 0>(a, t, fp, b, along, u, v, crash);
-  else if (d == 1) ct3_one_face<P, R, RIEM, LIM, 1>(a, t, fp, b, u, along, v, crash);
-  else ct3_one_face<P, R, RIEM, LIM, 2>(a, t, fp, b, u, v, along, crash);
+  if (d == 0) ct3_one_face<P, R, RIEM, LIM, 0>(a, t, fp, b, x, y, z, crash);
+  else if (d == 1) ct3_one_face<P, R, RIEM, LIM, 1>(a, t, fp, b, x, y, z, crash);
+  else ct3_one_face<P, R, RIEM, LIM, 2>(a, t, fp, b, x, y, z, crash);
   if (crash) *a.crash = 1;
 }
 
@@ -157,14 +173,15 @@ template <class P, int LIM>
 __global__ void __launch_bounds__(fvc::NTHREADS) ct3_edge_kernel(SweepArgs a, CtArgs t, FVParams fp) {
   using L = Ct3Layout<P>;
   const int idir = blockIdx.y, b = blockIdx.z;
-  const int q = blockIdx.x * fvc::NTHREADS + threadIdx.x;
-  constexpr int NA = BX + 1;
-  if (q >= BX * NA * NA) return;
-  const int idim1 = (idir + 1) % 3, idim2 = (idir + 2) % 3;
+  int lo[3], n[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    lo[e] = e == idir ? 0 : -1;
+    n[e] = e == idir ? BX : BX + 1;
+  }
   int c[3];
-  c[idir] = q % BX;
-  c[idim1] = q / BX % NA - 1;
-  c[idim2] = q / (BX * NA) - 1;
+  if (!box_cell(blockIdx.x * fvc::NTHREADS + threadIdx.x, lo, n, c[0], c[1], c[2])) return;
+  const int idim1 = (idir + 1) % 3, idim2 = (idir + 2) % 3;
   int e1[3] = {0, 0, 0}, e2[3] = {0, 0, 0};
   e1[idim1] = 1;
   e2[idim2] = 1;

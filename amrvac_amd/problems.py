@@ -270,6 +270,76 @@ def field_loop_ct(mesh: UniformMesh, gamma: float = 5.0 / 3.0) -> np.ndarray:
     return np.concatenate([mhd_to_conserved(prim, gamma, 3), ws], axis=1)
 
 
+def field_loop_3d_ct(mesh: UniformMesh, gamma: float = 5.0 / 3.0, a0: float = 1.0e-3,
+                     r0: float = 0.3) -> np.ndarray:
+    """The 3-D advected field loop of Gardiner & Stone (2008, J. Comput.
+    Phys. 227, 4123, sec. 5.4) on a staggered mesh over the periodic box
+    [-0.5, 0.5] x [-0.5, 0.5] x [-1, 1]: a cylinder of field lines along the
+    axis e3 = (-1, 0, 2)/sqrt(5), vector potential A = A0 (R0 - r) e3 inside
+    r <= R0, where r is the distance from that axis (periodic images by
+    wrapping x1 = (2x + z)/sqrt(5) and y), advected with v = (1, 1, 2) at
+    rho = p = 1.  The face field is built as b_from_vector_potentialA does
+    (src/mod_constrained_transport.t:103-173): each potential component at
+    the centre of its edges times the edge length, then per face the
+    circulation in lvc order over the face area; the cell field is the face
+    average (mhd_face_to_center src/mhd/mod_mhd_phys.t:7586-7648).  Returns
+    the backend state [nb, 8 + 3, nz, ny, nx] (ghost cells and ghost faces
+    zero, for getbc)."""
+    assert mesh.ndim == 3
+    x = mesh.cell_centers()
+    dx = mesh.dx
+    s5 = np.sqrt(5.0)
+    p1 = 2.0 / s5  # period of x1 over the box
+
+    def a3(xe, ye, ze):
+        x1 = (2.0 * xe + ze) / s5
+        x1 = x1 - p1 * np.floor(x1 / p1 + 0.5)
+        x2 = ye - np.floor(ye + 0.5)
+        r = np.sqrt(x1 * x1 + x2 * x2)
+        return np.where(r <= r0, a0 * (r0 - r), 0.0)
+
+    xc, yc, zc = x[:, 0], x[:, 1], x[:, 2]
+    h = 0.5 * np.asarray(dx)
+    # edge potentials times edge length: A_x at (x, y + dy/2, z + dz/2),
+    # A_y = 0, A_z at (x + dx/2, y + dy/2, z)
+    edge = [-a3(xc, yc + h[1], zc + h[2]) / s5 * dx[0],
+            np.zeros_like(xc),
+            2.0 * a3(xc + h[0], yc + h[1], zc) / s5 * dx[2]]
+    lvc = {(0, 1, 2): 1, (1, 2, 0): 1, (2, 0, 1): 1, (0, 2, 1): -1, (2, 1, 0): -1, (1, 0, 2): -1}
+    area = (dx[1] * dx[2], dx[0] * dx[2], dx[0] * dx[1])
+    ng = mesh.nghost
+    nz, ny, nx = x.shape[-3:]
+    # faces of the interior cells and their lower neighbours
+    nb = x.shape[0]
+    ws = np.zeros((nb, 3, nz, ny, nx))
+    for d in range(3):
+        lo = [ng, ng, ng]
+        lo[d] = ng - 1
+        sl = tuple(slice(lo[e], (nx, ny, nz)[e] - ng) for e in range(3))[::-1]
+        circ = np.zeros_like(xc[(slice(None),) + sl])
+        for idim2 in range(3):
+            for idir in range(3):
+                l = lvc.get((d, idim2, idir), 0)
+                if l == 0:
+                    continue
+                lo2 = list(lo)
+                lo2[idim2] -= 1
+                sl2 = tuple(slice(lo2[e], lo2[e] + (sl[2 - e].stop - sl[2 - e].start)) for e in range(3))[::-1]
+                da = edge[idir][(slice(None),) + sl] - edge[idir][(slice(None),) + sl2]
+                circ = circ + da if l == 1 else circ - da
+        ws[(slice(None), d) + sl] = circ / area[d]
+    inner = tuple(slice(ng, n - ng) for n in (nz, ny, nx))
+    b = np.zeros((nb, 3, nz, ny, nx))
+    for d in range(3):
+        lower = list(inner)
+        lower[2 - d] = slice(ng - 1, (nz, ny, nx)[2 - d] - ng - 1)
+        b[(slice(None), d) + inner] = 0.5 / area[d] * (ws[(slice(None), d) + inner] * area[d] +
+                                                       ws[(slice(None), d) + tuple(lower)] * area[d])
+    one = np.ones_like(xc)
+    prim = np.stack([one, 1.0 * one, 1.0 * one, 2.0 * one, 1.0 * one, b[:, 0], b[:, 1], b[:, 2]], axis=1)
+    return np.concatenate([mhd_to_conserved(prim, gamma, 3), ws], axis=1)
+
+
 def field_loop_output_vars_ct(mesh: UniformMesh, w: np.ndarray) -> np.ndarray:
     """set_output_vars of the field-loop module on a staggered mesh: div B
     of the face field (get_divb src/physics/mod_functions_bfield.t:30-37,

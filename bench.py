@@ -232,6 +232,44 @@ def ct_field_loop(steps: int, warmup: int = 2, n: int = 1024):
     return out
 
 
+def ct3d_field_loop(steps: int, warmup: int = 2, n: int = 128):
+    """Constrained transport in 3-D: the Gardiner & Stone tilted field loop
+    on n x n x 2n cells in 16^3 blocks, threestep / HLL / cada3 / cmaxmean,
+    CFL 0.4, FP64 (and the mixed form for uct_contact); cell-updates per RK
+    stage as for the main workload."""
+    from amrvac_amd import problems
+    from amrvac_amd.fv import DeviceFV
+    from amrvac_amd.host import Par, TimeIntegrator
+    from amrvac_amd.mesh import UniformMesh
+    out = {"workload": f"3-D MHD field loop (Gardiner & Stone 2008) with constrained transport, "
+                       f"{n}x{n}x{2 * n} in 16^3 blocks, threestep/HLL/cada3/cmaxmean, FP64 "
+                       "(uct_contact_mixed: the mixed form)"}
+    mesh = UniformMesh(3, (n, n, 2 * n), (16, 16, 16), (-0.5, -0.5, -1.0), (0.5, 0.5, 1.0), ("periodic",) * 6)
+    w0 = problems.field_loop_3d_ct(mesh)
+    for ct, prec in (("uct_contact", "fp64"), ("average", "fp64"), ("uct_hll", "fp64"), ("uct_contact", "mixed")):
+        par = Par(physics="mhd", ndim=3, ndir=3, gamma=5.0 / 3.0, typedivbfix="ct", type_ct=ct,
+                  time_stepper="threestep", flux_scheme="hll", limiter="cada3", typeboundspeed="cmaxmean",
+                  courantpar=0.4, precision=prec)
+        be = DeviceFV(par, mesh)
+        be.set_state(0, w0)
+        ti = TimeIntegrator(par, mesh, be)
+        for _ in range(warmup):
+            ti.step()
+        be.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ti.step()
+        be.sync()
+        dt = time.perf_counter() - t0
+        cells = n * n * 2 * n
+        print(f"[bench] ct3d {ct} {prec}: {steps} steps in {dt:.4f} s", file=sys.stderr, flush=True)
+        out[ct if prec == "fp64" else f"{ct}_mixed"] = {"value": round(cells * 3 * steps / dt / 1e6, 2),
+                                                        "unit": "Mcell-updates/s",
+                                                        "ms_per_step": round(dt / steps * 1e3, 4), "steps": steps}
+        del be
+    return out
+
+
 def load_traffic(name: str):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -356,6 +394,7 @@ def main() -> int:
                                "fp64_noise_free": amr_blast("fp64", args.amr_steps, noise=0.0)}
     if not args.no_ct:
         result["ct_field_loop"] = ct_field_loop(20)
+        result["ct3d_field_loop"] = ct3d_field_loop(10)
     if args.cpu_seconds > 0:
         try:
             result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

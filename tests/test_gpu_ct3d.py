@@ -114,3 +114,18 @@ def test_ct3d_fma_and_mixed_forms(ct):
             assert l1 <= bound, (prec, iw, l1)
         bmax = np.abs(g[8:]).max()
         assert np.abs(div_faces(g[8:], h)).max() * min(h) / bmax < 1e-13, prec
+
+
+def test_ct3d_field_loop_exact_kernels_match_oracle_bitwise():
+    """The Gardiner & Stone 3-D field loop (the bench workload, 32 x 32 x 64)."""
+    from amrvac_amd import problems
+    from amrvac_amd.fv import DeviceFV
+    print("[ct3d] field loop", flush=True)
+    par = ct_par(3, "uct_contact")
+    mesh = UniformMesh(3, (32, 32, 64), (16, 16, 16), (-0.5, -0.5, -1.0), (0.5, 0.5, 1.0), ("periodic",) * 6)
+    w0 = problems.field_loop_3d_ct(mesh)
+    runs = _runs(par, mesh, w0, (("oracle", lambda: OracleFV(par, mesh, threads=8)),
+                                 ("device", lambda: DeviceFV(par, mesh, precision="fp64_exact"))), 5)
+    (wo, dto), (wd, dtd) = runs["oracle"], runs["device"]
+    assert dto == dtd
+    assert np.array_equal(mesh.interior(wd), mesh.interior(wo))

@@ -295,3 +295,26 @@ def test_ct3d_div_b_with_unequal_cell_sizes():
         bmax = np.abs(gf[8:]).max()
         assert np.abs(div_faces(gf[8:], h)).max() * min(h) / bmax < 1e-13, ct
         assert np.abs(gf[8:] - g[8:]).max() > 1e-3 * bmax, ct
+
+
+def test_field_loop_3d_setup():
+    """The Gardiner & Stone 3-D field loop (problems.field_loop_3d_ct):
+    divergence-free faces; |B| = A0 inside the loop; B perpendicular to the
+    loop's axis (-1, 0, 2)/sqrt(5) (to the discretisation error of the
+    face average); no field outside; and div B stays at round-off as the
+    oracle advects it."""
+    mesh = UniformMesh(3, (32, 32, 64), (16, 16, 16), (-0.5, -0.5, -1.0), (0.5, 0.5, 1.0), ("periodic",) * 6)
+    w = problems.field_loop_3d_ct(mesh)
+    g = mesh.to_global(w)
+    h = mesh.dx
+    assert np.abs(div_faces(g[8:], h)).max() < 1e-15
+    b = g[5:8]
+    bmag = np.sqrt((b * b).sum(axis=0))
+    assert abs(bmag.max() - 1e-3) < 1e-6
+    axis = np.array([-1.0, 0.0, 2.0]) / np.sqrt(5.0)
+    along = np.abs(np.tensordot(axis, b, axes=1))
+    inside = bmag > 0.99e-3
+    assert inside.mean() > 0.2 and along[inside].max() < 0.01e-3
+    assert (bmag == 0.0).mean() > 0.6  # the loop's cylinder fills 28% of the box
+    gf, _ = run(ct_par(3, "uct_contact"), mesh, w, nsteps=4)
+    assert np.abs(div_faces(gf[8:], h)).max() * min(h) / np.abs(gf[8:]).max() < 1e-13
