@@ -99,3 +99,34 @@ def test_device_staggered_restart_continues_bitwise(tmp_path, precision):
     ti2.state.it, ti2.state.global_time = snap.it, snap.global_time
     ti2.run(nsteps=3)
     assert np.array_equal(mesh.interior(be2.get_state(0)), mesh.interior(ref.get_state(0)))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_staggered_restart_continues_bitwise_in_3d(tmp_path, precision):
+    """Constrained transport in 3-D (the Gardiner & Stone field loop):
+    snapshot of the device's cells and three face components, restart, and
+    a bit-for-bit continuation."""
+    from amrvac_amd.fv import DeviceFV
+    print(f"[datfile] 3-D CT field-loop restart, {precision}", flush=True)
+    par = Par(physics="mhd", ndim=3, ndir=3, gamma=5.0 / 3.0, typedivbfix="ct", type_ct="uct_contact",
+              time_stepper="threestep", flux_scheme="hll", limiter="cada3", typeboundspeed="cmaxmean",
+              courantpar=0.4)
+    mesh = UniformMesh(3, (32, 32, 64), (16, 16, 16), (-0.5, -0.5, -1.0), (0.5, 0.5, 1.0), ("periodic",) * 6)
+    w0 = problems.field_loop_3d_ct(mesh)
+    ref = DeviceFV(par, mesh, precision=precision)
+    ref.set_state(0, w0)
+    TimeIntegrator(par, mesh, ref).run(nsteps=4)
+    be = DeviceFV(par, mesh, precision=precision)
+    be.set_state(0, w0)
+    ti = TimeIntegrator(par, mesh, be)
+    ti.run(nsteps=2)
+    path = os.path.join(tmp_path, "fl3d0000.dat")
+    snap = datfile.write_snapshot(path, par, mesh, be.get_state(0), it=ti.state.it,
+                                  global_time=ti.state.global_time)
+    assert snap.staggered and snap.ws.shape[1:] == (3, 17, 17, 17)
+    be2 = DeviceFV(par, mesh, precision=precision)
+    be2.set_state(0, datfile.state_from_snapshot(datfile.read_snapshot(path), mesh))
+    ti2 = TimeIntegrator(par, mesh, be2)
+    ti2.state.it, ti2.state.global_time = snap.it, snap.global_time
+    ti2.run(nsteps=2)
+    assert np.array_equal(mesh.interior(be2.get_state(0)), mesh.interior(ref.get_state(0)))
