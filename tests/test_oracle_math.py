@@ -747,6 +747,57 @@ def test_ct_keeps_div_b_zero():
     assert check_ct_divb() == ""
 
 
+def check_hancock_ct(lib_path=None) -> str:
+    """The Hancock predictor with constrained transport (twostep with
+    TVDLF, typepred1 = 'hancock', src/io/mod_input_output.t:788-810).
+    hancock (src/mod_finite_volume.t:19-113) neither reads nor updates the
+    face field, and advect copies ps into ps1 before the predictor
+    (src/mod_advance.t:69-74).  So after the predictor ps1's faces equal
+    ps's bit for bit, and ps1's cells equal those of the same predictor run
+    without a face field (typedivbfix 'none') on the same cells, bit for
+    bit.  The corrector then keeps div B at round-off."""
+    import dataclasses
+    from amrvac_amd.host import FS, TimeIntegrator
+    from test_oracle_golden import field_loop_setup
+    par, mesh = field_loop_setup("3step_hll_cada_ct")
+    par = dataclasses.replace(par, time_stepper="twostep", flux_scheme="tvdlf", limiter="minmod")
+    if par.typepred1 != "hancock":
+        return "twostep with tvdlf does not select the Hancock predictor"
+    w0 = problems.field_loop_ct(mesh)
+    be = OracleFV(par, mesh, lib_path=lib_path)
+    be.set_state(0, w0)
+    be.getbc(0)
+    ti = TimeIntegrator(par, mesh, be)
+    ti.setdt()
+    dt = ti.state.dt
+    be.advect1(FS["hancock"], 0.5, dt, 0.0, 0, 1, ("copy", 0))
+    w1 = be.get_state(1)
+    plain = dataclasses.replace(par, typedivbfix="none")
+    bp = OracleFV(plain, mesh, lib_path=lib_path)
+    bp.set_state(0, np.ascontiguousarray(w0[:, :plain.nw]))
+    bp.getbc(0)
+    bp.advect1(FS["hancock"], 0.5, dt, 0.0, 0, 1, ("copy", 0))
+    wp = bp.get_state(1)
+    msgs = []
+    if not np.array_equal(mesh.interior(w1)[:, par.nw:], mesh.interior(be.get_state(0))[:, par.nw:]):
+        msgs.append("the predictor changed the face field")
+    if not np.array_equal(mesh.interior(w1)[:, :par.nw], mesh.interior(wp)):
+        msgs.append("the predictor's cells differ from the run without a face field")
+    be.advect1(FS["tvdlf"], 1.0, dt, 0.0, 1, 0, None)
+    w = be.get_state(0)
+    ex = problems.field_loop_output_vars_ct(mesh, w)
+    divb = np.abs(mesh.interior(ex)[:, 0]).max()
+    if not divb <= 1e-12 * np.abs(mesh.interior(w)[:, 5:7]).max() / min(mesh.dx):
+        msgs.append(f"div B {divb:.3e} after the corrector")
+    if np.array_equal(mesh.interior(w)[:, par.nw:], mesh.interior(w0)[:, par.nw:]):
+        msgs.append("the corrector left the face field unchanged")
+    return "; ".join(msgs)
+
+
+def test_hancock_predictor_with_constrained_transport():
+    assert check_hancock_ct() == ""
+
+
 def check_small_values_crash(lib_path=None) -> str:
     """fix_small_values with the default small_values_method 'error'
     (src/mod_finite_volume.t:325-333, src/mod_small_values.t:25-87): a
@@ -844,6 +895,7 @@ def check_linde_level_jump(lib_path=None) -> str:
 ALL_CHECKS = {
     "linde_level_jump": check_linde_level_jump,
     "ct_divb": check_ct_divb,
+    "hancock_ct": check_hancock_ct,
     "ct_golden_contact": lambda p: check_ct_golden("3step_hll_cada_ct", p),
     "ct_golden_average": lambda p: check_ct_golden("3step_hll_cada_ct_average", p),
     "ct_golden_hll": lambda p: check_ct_golden("3step_hll_cada_ct_hll", p),

@@ -43,6 +43,10 @@ MUTANTS = {
     "ct3d_contact_cell_efield_sign": (
         "ECC[0 * n + c] = wp[b2 * n + c] * wp[m3 * n + c] - wp[b3 * n + c] * wp[m2 * n + c];",
         "ECC[0 * n + c] = wp[b3 * n + c] * wp[m2 * n + c] - wp[b2 * n + c] * wp[m3 * n + c];", 0, "ct3d_extruded"),
+    # the cell field re-derived from the faces after the Hancock predictor
+    "hancock_ct_face_to_center": ("      hancock(&sb, qdt, wCT, wnew, cmax_global, &c);",
+                                  "      { hancock(&sb, qdt, wCT, wnew, cmax_global, &c);\n"
+                                  "        if (s->ct) ct_face_to_center(&sb, wnew); }", 0, "hancock_ct"),
     "hll_dissipation_sign": ("+ cmin * cmax * (wRC[iw] - wLC[iw])) /\n               (cmax - cmin);\n    }\n  }\n}",
                              "- cmin * cmax * (wRC[iw] - wLC[iw])) /\n               (cmax - cmin);\n    }\n  }\n}",
                              0, "sod_hll_cada3"),
