@@ -19,6 +19,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import amr_native
+
 NB_BOUNDARY, NB_COARSE, NB_SIBLING, NB_FINE = 1, 2, 3, 4
 
 
@@ -137,6 +139,10 @@ class AMRForest:
         for b, n in enumerate(leaves):
             n.block = b
         self.leaves = leaves
+        # the leaves in the order the forest created them (the level lists
+        # proper_nesting walks)
+        self.leaf_order = np.fromiter((n.block for n in self.nodes.values() if n.leaf), dtype=np.int32,
+                                      count=len(leaves))
         self._connectivity()
 
     @property
@@ -145,72 +151,23 @@ class AMRForest:
 
     @property
     def levmin(self) -> int:
-        return min(n.level for n in self.leaves)
+        return int(self.level.min())
 
     @property
     def levmax(self) -> int:
-        return max(n.level for n in self.leaves)
-
-    def node_at(self, level: int, ig: tuple[int, ...]) -> "Node | None | str":
-        """The node at (level, ig) with periodic wrap; 'out' outside the domain."""
-        ng = self.ng(level)
-        g = list(ig)
-        for d in range(self.ndim):
-            if g[d] < 1 or g[d] > ng[d]:
-                if self.periodic[d]:
-                    g[d] = 1 + (g[d] - 1) % ng[d]
-                else:
-                    return "out"
-        return self.nodes.get((level, tuple(g)))
-
-    def find_neighbor(self, node: Node, i: tuple[int, ...]) -> tuple[int, "Node | None"]:
-        """(type, node): the sibling / coarse neighbour leaf, or for a fine
-        neighbour the same-level node whose children are leaves or deeper."""
-        tgt = tuple(node.ig[d] + i[d] for d in range(self.ndim))
-        n = self.node_at(node.level, tgt)
-        if n == "out":
-            return NB_BOUNDARY, None
-        if n is None:
-            # no node at this level: the parent's neighbour is a leaf
-            ptgt = tuple((tgt[d] - 1) // 2 + 1 for d in range(self.ndim))
-            p = self.node_at(node.level - 1, ptgt)
-            if p == "out" or p is None:
-                return NB_BOUNDARY, None
-            return NB_COARSE, p
-        return (NB_SIBLING if n.leaf else NB_FINE), n
+        return int(self.level.max())
 
     def _connectivity(self) -> None:
+        """Neighbour tables of the leaves (build_connectivity,
+        src/mod_functions_connectivity.t:55-431) from the native tree
+        routines: per leaf and direction the type, the same-level or coarser
+        leaf, and the children of a finer neighbour that touch the leaf."""
         nd = self.ndim
         nb = self.nblocks
-        nn = 3 ** nd
-        nc = 2 ** nd
-        self.ntype = np.zeros((nb, nn), dtype=np.int32)
-        self.nbr = np.full((nb, nn), -1, dtype=np.int32)            # sibling / coarse leaf
-        self.nchild = np.full((nb, nn, nc), -1, dtype=np.int32)     # fine: children of the node
         self.level = np.array([n.level for n in self.leaves], dtype=np.int32)
         self.ig = np.array([n.ig for n in self.leaves], dtype=np.int32).reshape(nb, nd)
         self.icpos = np.array([n.ic() for n in self.leaves], dtype=np.int32).reshape(nb, nd)
-        for n in self.leaves:
-            for i in itertools.product(*[(-1, 0, 1)] * nd):
-                i = i[::-1]
-                if all(v == 0 for v in i):
-                    self.ntype[n.block, dir_index(i)] = 0
-                    self.nbr[n.block, dir_index(i)] = n.block
-                    continue
-                t, m = self.find_neighbor(n, i)
-                k = dir_index(i)
-                self.ntype[n.block, k] = t
-                if t in (NB_SIBLING, NB_COARSE):
-                    self.nbr[n.block, k] = m.block
-                elif t == NB_FINE:
-                    # the children of the neighbour node that touch this block
-                    for j, c in enumerate(m.children):
-                        icj = tuple(1 + ((j >> d) & 1) for d in range(nd))
-                        if any((i[d] == -1 and icj[d] != 2) or (i[d] == 1 and icj[d] != 1) for d in range(nd)):
-                            continue
-                        if not c.leaf:
-                            raise RuntimeError("level jump larger than one between neighbours")
-                        self.nchild[n.block, k, j] = c.block
+        self.ntype, self.nbr, self.nchild = amr_native.connectivity(nd, self.ng1, self.periodic, self.level, self.ig)
         # block extents (alloc_node, src/amr/mod_amr_solution_node.t:150-160)
         self.xlohi = np.zeros((nb, nd, 2), dtype=np.float64)
         self.dxb = np.zeros((nb, nd), dtype=np.float64)
@@ -380,63 +337,14 @@ def errest_flags(forest: AMRForest, w: np.ndarray, ap: AMRPar) -> tuple[np.ndarr
     return refine, coarsen
 
 
-def _outward_dirs(ic: tuple[int, ...]):
-    for i in itertools.product(*[range(c - 2, c) for c in ic[::-1]]):
-        i = i[::-1]
-        if any(v != 0 for v in i):
-            yield i
-
-
 def proper_nesting(forest: AMRForest, ap: AMRPar, refine: np.ndarray, coarsen: np.ndarray) -> None:
-    """2:1 balance: refining a block refines its parent's outward leaf
-    neighbours; a coarsening needs all siblings flagged and no neighbour
-    refining or staying finer (in place)."""
-    nodes_by_level: dict[int, list[Node]] = {}
-    for n in forest.nodes.values():
-        nodes_by_level.setdefault(n.level, []).append(n)
-    levmin, levmax = forest.levmin, forest.levmax
-    for level in range(min(levmax, ap.refine_max_level - 1), levmin, -1):
-        for n in nodes_by_level.get(level, []):
-            if not n.leaf or not refine[n.block]:
-                continue
-            p = n.parent
-            for inp in _outward_dirs(n.ic()):
-                m = forest.node_at(level - 1, tuple(p.ig[d] + inp[d] for d in range(forest.ndim)))
-                if m is None or m == "out":
-                    continue
-                if m.leaf:
-                    refine[m.block] = True
-    coarsen &= ~refine
-    for level in range(levmax, max(2, levmin) - 1, -1):
-        for n in nodes_by_level.get(level, []):
-            if not n.leaf or not coarsen[n.block]:
-                continue
-            parent = n.parent
-            sibs = parent.children
-            if not all(c.leaf and coarsen[c.block] for c in sibs):
-                for c in sibs:
-                    if c.leaf:
-                        coarsen[c.block] = False
-                continue
-            ok = True
-            for c in sibs:
-                for i in _outward_dirs(c.ic()):
-                    t, m = forest.find_neighbor(c, i)
-                    if t == NB_SIBLING:
-                        if refine[m.block]:
-                            ok = False
-                    elif t == NB_FINE:
-                        c1 = m.children[0]
-                        if not (c1.leaf and coarsen[c1.block]):
-                            ok = False
-                    if not ok:
-                        break
-                if not ok:
-                    break
-            if not ok:
-                for c in sibs:
-                    if c.leaf:
-                        coarsen[c.block] = False
+    """2:1 balance (proper_nesting, src/amr/mod_coarsen_refine.t:186-319,
+    native): refining a block refines its parent's outward leaf neighbours;
+    a coarsening needs all siblings flagged and no neighbour refining or
+    staying finer (flags updated in place).  The coarsening checks visit the
+    leaves of a level in the order the forest created them."""
+    amr_native.proper_nesting(forest.ndim, forest.ng1, forest.periodic, forest.level, forest.ig, forest.leaf_order,
+                              ap.refine_max_level, refine, coarsen)
 
 
 def apply_regrid(forest: AMRForest, refine: np.ndarray, coarsen: np.ndarray) -> list[tuple]:
@@ -560,14 +468,17 @@ class AMRIntegrator(TimeIntegrator):
     def mesh(self, value):
         pass
 
-    def regrid(self, time_advance: bool = True) -> None:
+    def regrid(self, time_advance: bool = True, flags=None) -> None:
         """errest + amr_coarsen_refine (+ getbc) on ps.  A backend with a
         device regrid (amr_device_regrid) estimates and moves the data
         without a round trip through the host; the new leaves of the
-        initial tree (time_advance False) take the initial condition."""
+        initial tree (time_advance False) take the initial condition.
+        flags: the (refine, coarsen) estimate of ps if already made."""
         forest, par = self.forest, self.par
         device = getattr(self.be, "amr_device_regrid", False)
-        if device:
+        if flags is not None:
+            refine, coarsen = flags
+        elif device:
             refine, coarsen = self.be.errest_flags(forest, self.ap)
         else:
             w = self.be.get_state(0)
@@ -635,11 +546,16 @@ class AMRIntegrator(TimeIntegrator):
                 break
             self.advect()
             st.dts.append(st.dt)
+            # the device estimate's flag copy also brings the small-value
+            # flag, so the crash check after it waits on nothing more
+            regrid_due = self.ap.refine_max_level > 1 and (st.it + 1) % self.ap.ditregrid == 0
+            flags = self.be.errest_flags(self.forest, self.ap) \
+                if regrid_due and getattr(self.be, "amr_device_regrid", False) else None
             if st.crash or self.be.crash():
                 st.crash = True
                 raise RuntimeError("Error: small value encountered, run crash.")
-            if self.ap.refine_max_level > 1 and (st.it + 1) % self.ap.ditregrid == 0:
-                self.regrid()
+            if regrid_due:
+                self.regrid(flags=flags)
             st.it += 1
             st.global_time = st.global_time + st.dt
             steps += 1

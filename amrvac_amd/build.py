@@ -1,4 +1,5 @@
-"""Build the HIP library libamrvac_fv.so in-tree for gfx950.
+"""Build the HIP library libamrvac_fv.so in-tree for gfx950, and the host
+library libamrvac_amr.so (the native AMR tree routines).
 
 Every stage-kernel translation unit is compiled twice: once with the default
 FMA contraction (the production FP64 and mixed kernels) and once with
@@ -17,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libamrvac_fv.so")
+AMR_LIB = os.path.join(HERE, "libamrvac_amr.so")
 
 ARCH = os.environ.get("AMRVAC_FV_ARCH", "gfx950")
 KERNEL_UNITS = ["fv_k2_hd", "fv_k2_mhd", "fv_k2_mhd_ndir2", "fv_k3_hd", "fv_k3_mhd", "fv_ct", "fv_ct3"]
@@ -78,5 +80,20 @@ def build(verbose: bool = False, jobs: int | None = None, csrc: str = CSRC, buil
     return lib
 
 
+def build_host(lib: str = AMR_LIB, cxx: str | None = None) -> str:
+    """Compile the native AMR tree routines (csrc/amr_tree.cpp, host code
+    only) into ``lib`` with the host C++ compiler."""
+    cxx = cxx or os.environ.get("CXX") or shutil.which("g++") or "g++"
+    tmp = lib + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I",
+           os.path.abspath(os.path.join(HERE, "..", "include")), os.path.join(CSRC, "amr_tree.cpp"), "-o", tmp]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("AMR tree library build failed:\n" + p.stdout + p.stderr)
+    os.replace(tmp, lib)
+    return lib
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_host())

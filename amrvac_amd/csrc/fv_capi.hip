@@ -1798,7 +1798,13 @@ int amrvac_fv_errest(amrvac_fv_ctx* c, int slot, int nwt, const int* iws, const 
                        c->d_iws, c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(flags, c->d_flags, (size_t)c->nb, hipMemcpyDeviceToHost, c->stream));
+  // the small-value flag rides the same synchronisation (amrvac_fv_crash
+  // then reads it without another one)
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  c->crash_seen = h ? 1 : 0;
+  c->crash_fresh = true;
   return AMRVAC_FV_OK;
 }
 

@@ -129,7 +129,9 @@ int amrvac_fv_getbc(amrvac_fv_ctx* ctx, int slot);
    flags[nb]: bit 0 = some cell has error >= threshold (refine candidate),
    bit 1 = every cell has error < threshold and <= coarsen_threshold
    (derefine_ratio*threshold; coarsen candidate).  The level limits and
-   proper nesting stay with the host. */
+   proper nesting stay with the host.  The copy of the flags also fetches
+   the small-value flag, so amrvac_fv_crash right after it does not wait on
+   the device again. */
 int amrvac_fv_errest(amrvac_fv_ctx* ctx, int slot, int nwt, const int* iws,
                      const double* weights, double wavefilter, double threshold,
                      double coarsen_threshold, unsigned char* flags);
