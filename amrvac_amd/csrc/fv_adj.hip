@@ -195,11 +195,11 @@ __device__ __forceinline__ void glm_source_rev(const SweepArgs& a, const FVParam
   }
 }
 
-// occupancy: the FP32-face kernels keep to 3 workgroups per CU, except
-// those of GLM or cada3 (their reverse needs more registers; held to 3 they
-// spill)
+// occupancy: the FP32-face kernels keep to 3 workgroups per CU (GLM's
+// reverse then spills 48 B per lane, still 1.1x faster than at 2), except
+// those of cada3 (held to 3, its reverse with GLM spills 312 B per lane)
 template <class P, class R, int LIM>
-constexpr int kAdjWaves = std::is_same<R, float>::value ? ((P::GLM || LIM == LIM_CADA3) ? 2 : 3)
+constexpr int kAdjWaves = std::is_same<R, float>::value ? (LIM == LIM_CADA3 ? 2 : 3)
                                                         : (P::NW > 5 ? 1 : 2);
 
 template <class P, class R, int RIEM, int LIM>
