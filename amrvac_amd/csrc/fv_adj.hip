@@ -156,19 +156,23 @@ __device__ __forceinline__ void glm_source_rev(const SweepArgs& a, const FVParam
       le[d][n] = double(Ll[E][li]);
     }
   if constexpr (NDIM == 3) {
+    // the planes above and below: only the field and psi (their primitive
+    // and conserved values are the same, rounded to R like the tiles') and
+    // the energy and momentum adjoints are read
+    const size_t cells = (size_t)BX * BX * a.bnz;
 #pragma unroll
     for (int n = 0; n < 2; ++n) {
-      R w[P::NW];
-      load_prim<P, R, NDIM>(a, fp, b, tx, ty, k + 2 * n - 1, w);
-      double l[P::NW];
-      load_lam<P::NW, NDIM>(a, g.lam_out, b, tx, ty, k + 2 * n - 1, l);
+      int x = tx, y = ty, z = k + 2 * n - 1;
+      double sign;
+      const int sb = locate<NDIM>(a, b, x, y, z, sign);
+      const size_t off = (size_t)sb * P::NW * cells + ((size_t)z * BX + y) * BX + x;
 #pragma unroll
       for (int q = 0; q < ND; ++q) {
-        Bn[2][n][q] = double(w[MAG + q]);
-        lm[2][n][q] = l[MOM + q];
+        Bn[2][n][q] = double(R(sign * a.src[off + (MAG + q) * cells]));
+        lm[2][n][q] = g.lam_out[off + (MOM + q) * cells];
       }
-      psin[2][n] = double(w[S]);
-      le[2][n] = l[E];
+      psin[2][n] = double(R(sign * a.src[off + S * cells]));
+      le[2][n] = g.lam_out[off + E * cells];
     }
   }
   double gp[3], divb = 0.0;
