@@ -90,8 +90,37 @@ class SSPRK3Run:
                                          device=like.device)
         return self._tape_buf
 
+    def _on_own_stream(self, fn, arg: torch.Tensor, *rest):
+        """fn(arg, *rest) on the run's own stream (default priority), ordered
+        after the caller's stream and before its later work.  The library
+        and the torch operations between its calls (the psi decay, copies)
+        then share one stream: on the legacy default stream those
+        operations would serialise with every other stream, the
+        recompute's included."""
+        if not self.device_backend:
+            return fn(arg, *rest)
+        caller = torch.cuda.current_stream()
+        if getattr(self, "_own", None) is None:
+            self._own = torch.cuda.Stream()
+        own = self._own
+        own.wait_stream(caller)
+        arg.record_stream(own)
+        with torch.cuda.stream(own):
+            out = fn(arg, *rest)
+        caller.wait_stream(own)
+        out.record_stream(caller)
+        return out
+
     def forward(self, u0: torch.Tensor, dts: list[float] | None = None,
                 chs: list[float] | None = None) -> torch.Tensor:
+        """Final state; keeps checkpoints for backward.  With ``dts`` the
+        given time steps are replayed instead of the CFL ones, and with
+        them the GLM speeds ``chs`` (by default those of the previous
+        forward pass)."""
+        return self._on_own_stream(self._forward, u0, dts, chs)
+
+    def _forward(self, u0: torch.Tensor, dts: list[float] | None = None,
+                 chs: list[float] | None = None) -> torch.Tensor:
         """Final state; keeps checkpoints for backward.  With ``dts`` the
         given time steps are replayed instead of the CFL ones, and with
         them the GLM speeds ``chs`` (by default those of the previous
@@ -189,6 +218,10 @@ class SSPRK3Run:
                         add_a=lam.data_ptr(), ca=c["alfa31"], add_b=l2.data_ptr(), cb=c["alfa21"], ch=ch)
 
     def backward(self, lam: torch.Tensor) -> torch.Tensor:
+        """Adjoint of the whole run: d(final)/d(initial)^T lam."""
+        return self._on_own_stream(self._backward, lam)
+
+    def _backward(self, lam: torch.Tensor) -> torch.Tensor:
         """Adjoint of the whole run: d(final)/d(initial)^T lam.
 
         On the device the recompute of the next segment (on a second stream
