@@ -119,6 +119,12 @@ struct amrvac_fv_ctx {
   double* d_adjG = nullptr;  // 3-D adjoint: summed z-face contributions per cell [nw][nb*cells]
   // device regrid: new-leaf state, plan, Löhner flags and weights (kept across
   // regrids, grown on demand)
+  // element capacities of the level's buffers: an AMR regrid keeps every
+  // buffer that still fits the new level (no free / allocate per regrid)
+  size_t slot_cap = 0, nbr_cap = 0, red_cap = 0, dtp_cap = 0, stage_cap = 0;
+  int nreg = 0;  // refluxing registers of the level
+  size_t level_cap = 0, ntype_cap = 0, nchild_cap = 0, rtask_cap = 0, ptask_cap = 0, ptask2_cap = 0;
+  size_t ftask_cap = 0, fregoff_cap = 0, freg_cap = 0, xlo_cap = 0;
   double* d_remap = nullptr;
   size_t remap_cap = 0;
   int* d_plan = nullptr;
@@ -880,6 +886,26 @@ bool check_slot(int s) { return s >= 0 && s <= 3; }
 
 }  // namespace
 
+// grow a device buffer of the context to hold n elements
+template <class T>
+static bool ensure_buf(T** buf, size_t* cap, size_t n) {
+  if (*buf && *cap >= n) return true;
+  if (*buf) (void)hipFree(*buf);
+  *buf = nullptr;
+  *cap = 0;
+  if (hipMalloc(buf, n * sizeof(T)) != hipSuccess) return false;
+  *cap = n;
+  return true;
+}
+
+// a table of the level into its buffer (grown when it does not fit)
+template <class T>
+static bool upload_vec(amrvac_fv_ctx* c, T** dst, size_t* cap, const std::vector<T>& v) {
+  if (v.empty()) return true;
+  if (!ensure_buf(dst, cap, v.size())) return false;
+  return hipMemcpyAsync(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream) == hipSuccess;
+}
+
 extern "C" {
 
 const char* amrvac_fv_last_error(void) { return g_err.c_str(); }
@@ -954,6 +980,7 @@ int amrvac_fv_init(const amrvac_fv_config* cfg, amrvac_fv_ctx** out) {
   return AMRVAC_FV_OK;
 }
 
+
 static void free_level(amrvac_fv_ctx* c) {
   for (int s = 0; s < 4; ++s) { if (c->slot[s]) (void)hipFree(c->slot[s]); c->slot[s] = nullptr; }
   for (int s = 0; s < 4; ++s) { if (c->wsl[s]) (void)hipFree(c->wsl[s]); c->wsl[s] = nullptr; }
@@ -977,9 +1004,24 @@ static void free_level(amrvac_fv_ctx* c) {
   c->d_level = nullptr; c->d_ntype = nullptr; c->d_nchild = nullptr; c->d_rtask = nullptr;
   c->d_ptask = nullptr; c->d_ptask2 = nullptr; c->d_ftask = nullptr; c->d_fregoff = nullptr;
   c->d_freg = nullptr; c->d_xlo = nullptr; c->d_adjG = nullptr;
+  c->slot_cap = c->nbr_cap = c->red_cap = c->dtp_cap = c->stage_cap = 0;
+  c->level_cap = c->ntype_cap = c->nchild_cap = c->rtask_cap = c->ptask_cap = c->ptask2_cap = 0;
+  c->ftask_cap = c->fregoff_cap = c->freg_cap = c->xlo_cap = 0;
   c->amr = false;
   c->nv = 0;
-  c->nrt = c->npt = c->nft = 0;
+  c->nrt = c->npt = c->nft = c->nreg = 0;
+  c->h_level.clear();
+}
+
+// an AMR level replaced by the next one (regrid): the old level's state is
+// dropped, its buffers are kept for alloc_level to reuse where they fit
+static void reset_level(amrvac_fv_ctx* c) {
+  if (c->d_adjG) (void)hipFree(c->d_adjG);
+  c->d_adjG = nullptr;
+  c->dt_valid = false;
+  c->amr = false;
+  c->nv = 0;
+  c->nrt = c->npt = c->nft = c->nreg = 0;
   c->h_level.clear();
 }
 
@@ -1047,13 +1089,20 @@ static int alloc_level(amrvac_fv_ctx* c, int nblocks, int capacity, const double
   for (int d = 0; d < 3; ++d) c->dx[d] = d < nd ? dx[d] : 1.0;
   for (int s = 0; s < 6; ++s) c->bc[s] = s < 2 * nd ? bc[s] : 5;
   const size_t n = (size_t)capacity * c->nw * c->cells;
-  for (int s = 0; s < 4; ++s) {
-    if (hipMalloc(&c->slot[s], n * sizeof(double)) != hipSuccess) {
-      free_level(c);
-      return fail(AMRVAC_FV_ENOMEM, "state slot allocation failed");
+  if (c->slot_cap < n) {
+    for (int s = 0; s < 4; ++s) {
+      if (c->slot[s]) (void)hipFree(c->slot[s]);
+      c->slot[s] = nullptr;
     }
-    HIPCHK(hipMemsetAsync(c->slot[s], 0, n * sizeof(double), c->stream));
+    c->slot_cap = 0;
+    for (int s = 0; s < 4; ++s)
+      if (hipMalloc(&c->slot[s], n * sizeof(double)) != hipSuccess) {
+        free_level(c);
+        return fail(AMRVAC_FV_ENOMEM, "state slot allocation failed");
+      }
+    c->slot_cap = n;
   }
+  for (int s = 0; s < 4; ++s) HIPCHK(hipMemsetAsync(c->slot[s], 0, n * sizeof(double), c->stream));
   if (c->nws > 0) {
     const size_t nf = (size_t)capacity * c->nws * c->cells;
     for (int s = 0; s < 4; ++s) {
@@ -1070,12 +1119,12 @@ static int alloc_level(amrvac_fv_ctx* c, int nblocks, int capacity, const double
       return fail(AMRVAC_FV_ENOMEM, "constrained-transport buffers allocation failed");
     }
   }
-  if (hipMalloc(&c->d_nbr, sizeof(int) * nblocks * c->nnb) != hipSuccess ||
-      hipMalloc(&c->d_crash, sizeof(int)) != hipSuccess ||
-      hipMalloc(&c->d_red, sizeof(double) * (size_t)nblocks * 2 * (c->nw > 1 ? c->nw : 1)) != hipSuccess ||
-      hipMalloc(&c->d_red2, sizeof(double) * 3) != hipSuccess ||
-      hipMalloc(&c->d_dtp, sizeof(double) * (size_t)nblocks * 2) != hipSuccess ||
-      hipMalloc(&c->d_stage, sizeof(double) * (size_t)nblocks * c->nw * c->cells_g) != hipSuccess) {
+  if (!ensure_buf(&c->d_nbr, &c->nbr_cap, (size_t)nblocks * c->nnb) ||
+      (!c->d_crash && hipMalloc(&c->d_crash, sizeof(int)) != hipSuccess) ||
+      !ensure_buf(&c->d_red, &c->red_cap, (size_t)nblocks * 2 * (c->nw > 1 ? c->nw : 1)) ||
+      (!c->d_red2 && hipMalloc(&c->d_red2, sizeof(double) * 3) != hipSuccess) ||
+      !ensure_buf(&c->d_dtp, &c->dtp_cap, (size_t)nblocks * 2) ||
+      !ensure_buf(&c->d_stage, &c->stage_cap, (size_t)nblocks * c->nw * c->cells_g)) {
     free_level(c);
     return fail(AMRVAC_FV_ENOMEM, "work buffer allocation failed");
   }
@@ -1204,25 +1253,6 @@ static int fix_conserve_amr(amrvac_fv_ctx* c, int slot) {
   return AMRVAC_FV_OK;
 }
 
-template <class T>
-static bool upload_vec(amrvac_fv_ctx* c, T** dst, const std::vector<T>& v) {
-  if (v.empty()) return true;
-  if (hipMalloc(dst, sizeof(T) * v.size()) != hipSuccess) return false;
-  return hipMemcpyAsync(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream) == hipSuccess;
-}
-
-// grow a device buffer of the context to hold n elements
-template <class T>
-static bool ensure_buf(T** buf, size_t* cap, size_t n) {
-  if (*buf && *cap >= n) return true;
-  if (*buf) (void)hipFree(*buf);
-  *buf = nullptr;
-  *cap = 0;
-  if (hipMalloc(buf, n * sizeof(T)) != hipSuccess) return false;
-  *cap = n;
-  return true;
-}
-
 // reduction of the per-block CFL maxima part[2*nb]: their max and the
 // small-value flag into fin[3], copied to the host; dt and cmax from them
 static int getdt_reduce(amrvac_fv_ctx* c, double courantpar, double* dtmin_out, double* cmax_out, double* part,
@@ -1302,7 +1332,7 @@ static int stage_core(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
       a.glm_decay_lev[l] = std::exp(-qdt * cmax_global * c->cfg.glm_alpha / mindx);
       mindx = mindx * 0.5;
     }
-    if (c->fix_conserve && c->d_freg && riem != fv::RIEM_HANCOCK) {
+    if (c->fix_conserve && c->nreg > 0 && riem != fv::RIEM_HANCOCK) {
       a.freg = c->d_freg;
       a.fregoff = c->d_fregoff;
       a.nface = c->nface;
@@ -1696,7 +1726,9 @@ int amrvac_fv_set_mesh_amr(amrvac_fv_ctx* c, int nblocks, const double* dx1, con
   for (int d = 0; d < nd; ++d)
     if (!(dx1[d] > 0.0)) return fail(AMRVAC_FV_EINVAL, "dx must be positive");
   HIPCHK(hipStreamSynchronize(c->stream));
-  free_level(c);
+  // a new AMR level over an AMR level keeps the buffers that still fit
+  if (c->amr) reset_level(c);
+  else free_level(c);
   // virtual blocks and the sweep's neighbour table
   std::vector<int> snbr((size_t)nblocks * nnb);
   std::vector<int4> rt, pt;
@@ -1760,11 +1792,13 @@ int amrvac_fv_set_mesh_amr(amrvac_fv_ctx* c, int nblocks, const double* dx1, con
   std::vector<double> xlo((size_t)nblocks * nd);
   for (int b = 0; b < nblocks; ++b)
     for (int d = 0; d < nd; ++d) xlo[(size_t)b * nd + d] = xlohi[((size_t)b * nd + d) * 2];
-  bool ok = upload_vec(c, &c->d_level, lev0) && upload_vec(c, &c->d_ntype, vtype) &&
-            upload_vec(c, &c->d_nchild, vchild) && upload_vec(c, &c->d_rtask, rt) && upload_vec(c, &c->d_ptask, pt) &&
-            upload_vec(c, &c->d_ptask2, pt2) && upload_vec(c, &c->d_ftask, ftask) &&
-            upload_vec(c, &c->d_fregoff, fregoff) && upload_vec(c, &c->d_xlo, xlo);
-  if (ok && nreg > 0) ok = hipMalloc(&c->d_freg, sizeof(double) * (size_t)nreg * c->nw * c->nface) == hipSuccess;
+  bool ok = upload_vec(c, &c->d_level, &c->level_cap, lev0) && upload_vec(c, &c->d_ntype, &c->ntype_cap, vtype) &&
+            upload_vec(c, &c->d_nchild, &c->nchild_cap, vchild) && upload_vec(c, &c->d_rtask, &c->rtask_cap, rt) &&
+            upload_vec(c, &c->d_ptask, &c->ptask_cap, pt) && upload_vec(c, &c->d_ptask2, &c->ptask2_cap, pt2) &&
+            upload_vec(c, &c->d_ftask, &c->ftask_cap, ftask) &&
+            upload_vec(c, &c->d_fregoff, &c->fregoff_cap, fregoff) && upload_vec(c, &c->d_xlo, &c->xlo_cap, xlo);
+  if (ok && nreg > 0) ok = ensure_buf(&c->d_freg, &c->freg_cap, (size_t)nreg * c->nw * c->nface);
+  c->nreg = nreg;
   if (!ok) {
     free_level(c);
     return fail(AMRVAC_FV_ENOMEM, "AMR table allocation failed");

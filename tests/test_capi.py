@@ -70,11 +70,16 @@ def test_init_rejects_unsupported_constrained_transport():
     cfg = fv.FVConfig()
     cfg.ndim, cfg.ndir, cfg.physics, cfg.gamma = 3, 3, 2, 5.0 / 3.0
     cfg.limiter, cfg.boundspeed, cfg.nghost, cfg.precision = 9, 1, 2, 0
-    cfg.block_nx[0], cfg.block_nx[1], cfg.block_nx[2] = 16, 16, 16
+    cfg.block_nx[0], cfg.block_nx[1], cfg.block_nx[2] = 16, 16, 8
     cfg.divb, cfg.type_ct = 8, 2
     h = ctypes.c_void_p()
+    # 3-D constrained transport needs 16^3 blocks
     assert L.amrvac_fv_init(ctypes.byref(cfg), ctypes.byref(h)) == -1
-    assert b"ndim = 2" in L.amrvac_fv_last_error()
-    cfg.ndim, cfg.block_nx[2], cfg.type_ct = 2, 1, 4
+    assert b"block_nx = 16" in L.amrvac_fv_last_error()
+    cfg.block_nx[2], cfg.ndir = 16, 2
+    cfg.ndim = 2
+    assert L.amrvac_fv_init(ctypes.byref(cfg), ctypes.byref(h)) == -1
+    assert b"ndir = 3" in L.amrvac_fv_last_error()
+    cfg.ndir, cfg.block_nx[2], cfg.type_ct = 3, 1, 4
     assert L.amrvac_fv_init(ctypes.byref(cfg), ctypes.byref(h)) == -1
     assert b"type_ct" in L.amrvac_fv_last_error()
