@@ -47,7 +47,7 @@ class SSPRK3Run:
         self.every = every or max(1, int(math.sqrt(nsteps)))
         # the backend: DeviceFV (the HIP kernels), or an object with the same
         # stage / adjoint1 / getdt_state / set_stream methods on the
-        # tensors' memory (the driver tests use a CPU one)
+        # tensors' memory (the checkpoint-schedule tests use a CPU one)
         self.be = backend if backend is not None else DeviceFV(par, mesh, precision=precision)
         self.device_backend = backend is None
         self.precision = precision

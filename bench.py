@@ -402,7 +402,7 @@ def main() -> int:
     if args.cpu_seconds > 0:
         try:
             result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
-        except Exception as exc:  # the oracle is optional on a GPU box
+        except Exception as exc:  # the oracle library is optional on the measuring machine
             result["cpu_baseline"] = {"value": None, "error": str(exc)}
     print(json.dumps(result))
     return 0

@@ -5,7 +5,7 @@ state-dependent map through the same stage / adjoint1 entry points:
   stage:    u_out = init + qdt * K(u_in),   K(u) = S u + c u^2  (elementwise c),
   adjoint1: lam_in = coef*lam_out + ca*add_a + cb*add_b + qdt * (S^T + 2 c u_in) lam_out,
 
-with S a fixed periodic stencil along x.  The driver's gradient must equal
+with S a fixed periodic stencil along x.  The checkpointed run's gradient must equal
 torch.autograd through the same ssprk3 steps, for any checkpoint spacing:
 the RK reverse, the stage tape, the segment recompute and the forward's last
 segment on the tape are all exercised (a wrong stage state changes the

@@ -201,7 +201,7 @@ def test_mhd_adjoint_config_128():
                                               ("glm", "hll", "vanleer")])
 def test_adjoint_stage_matches_torch_autograd(divb, riemann, lim):
     """One stage at one point: the device adjoint (J^T lam at u_in, with the
-    psi row of lam scaled by the stage's decay as the driver does) against
+    psi row of lam scaled by the stage's decay as the checkpointed run does) against
     torch autograd of the FP64 PyTorch stage at the same u_in, 32^3.  No
     trajectory lies between the two, so limiter and Riemann branches are
     those of the same state."""
