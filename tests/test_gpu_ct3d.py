@@ -129,3 +129,12 @@ def test_ct3d_field_loop_exact_kernels_match_oracle_bitwise():
     (wo, dto), (wd, dtd) = runs["oracle"], runs["device"]
     assert dto == dtd
     assert np.array_equal(mesh.interior(wd), mesh.interior(wo))
+
+
+def test_ct3d_exact_kernels_64_cubed_10_steps():
+    """A larger level (64^3, 64 blocks) and a longer run (10 steps): uct_hll
+    with HLLD, whose normal field comes from ps, bit for bit."""
+    print("[ct3d] 64^3, 10 steps, uct_hll + hlld", flush=True)
+    par = dataclasses.replace(ct_par(3, "uct_hll"), flux_scheme="hlld")
+    mesh = UniformMesh(3, (64, 64, 64), (16, 16, 16), (0.0,) * 3, (1.0,) * 3, ("periodic",) * 6)
+    _exact_vs_oracle(par, mesh, sheared_state(64, 0.3), nsteps=10)
