@@ -40,9 +40,12 @@ constexpr size_t G3 = (size_t)G * G * G;
 template <class P>
 struct Ct3Layout {
   static constexpr int FK = P::NW + 5;
-  static constexpr int VN = P::NW, CBMIN = P::NW + 1, CBMAX = P::NW + 2, VB1 = P::NW + 3, VB2 = P::NW + 4;
-  // scratch per block, in doubles: faces [3][FK][G3], primitives [NW][G3]
-  // (FP64 or FP32), b x v [3][G3], edge fields [3][G3]
+  // face data of the edge fields (FP64), after the fluxes
+  static constexpr int VN = 0, CBMIN = 1, CBMAX = 2, VB1 = 3, VB2 = 4;
+  // scratch per block, in doubles: faces [3][FK][G3] (the fluxes [3][NW][G3]
+  // in the face precision, whose values they are, then the face data
+  // [3][5][G3] in FP64), primitives [NW][G3] (FP64 or FP32), b x v [3][G3],
+  // edge fields [3][G3]
   static constexpr size_t faces = 3 * FK * G3;
   static constexpr size_t prim = P::NW * G3;
   static constexpr size_t ecc = 3 * G3;
@@ -92,10 +95,19 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct3_prim_kernel(SweepArgs a, Ct
   }
 }
 
-template <class P>
-__device__ __forceinline__ double* face_ptr(const CtArgs& t, int b, int d, int k) {
+// flux iw through the d faces, in the face precision R (an FP32 flux is
+// stored as the FP32 value it is)
+template <class P, class R>
+__device__ __forceinline__ R* flux_ptr(const CtArgs& t, int b, int d, int iw) {
   using L = Ct3Layout<P>;
-  return t.faces + (size_t)b * L::faces + ((size_t)d * L::FK + k) * G3;
+  return reinterpret_cast<R*>(t.faces + (size_t)b * L::faces) + ((size_t)d * P::NW + iw) * G3;
+}
+
+// face datum k (VN .. VB2) of the d faces, FP64
+template <class P>
+__device__ __forceinline__ double* ctd_ptr(const CtArgs& t, int b, int d, int k) {
+  using L = Ct3Layout<P>;
+  return t.faces + (size_t)b * L::faces + (size_t)3 * P::NW * G3 + ((size_t)d * 5 + k) * G3;
 }
 
 template <class P, class R, int RIEM, int LIM, int ID>
@@ -119,9 +131,10 @@ __device__ void ct3_one_face(const SweepArgs& a, const CtArgs& t, const FVParams
   R fC[NW];
   face_flux<P, R, RIEM, ID, LIM, 2, true>(st, fp, R(0), fC, crash, &io);
   const size_t c = gidx(i, j, k);
-  double* f = face_ptr<P>(t, b, ID, 0) + c;
+  R* fr = flux_ptr<P, R>(t, b, ID, 0) + c;
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) f[iw * G3] = double(fC[iw]);
+  for (int iw = 0; iw < NW; ++iw) fr[iw * G3] = fC[iw];
+  double* f = ctd_ptr<P>(t, b, ID, 0) + c;
   if (t.ct == CT_CONTACT) f[L::VN * G3] = 0.5 * (io.vL[ID] + io.vR[ID]);
   if (t.ct == CT_HLL) {
     const double cbmax = io.cmax > 0.0 ? io.cmax : 0.0;
@@ -169,7 +182,7 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct3_face_kernel(SweepArgs a, Ct
 // blockIdx.y: edge direction idir, blockIdx.z: block; edges at the upper
 // (idim1, idim2) corner of cells -1..15 across and 0..15 along idir, with
 // (idim1, idim2, idir) cyclic (lvc = 1)
-template <class P, int LIM>
+template <class P, class R, int LIM>
 __global__ void __launch_bounds__(fvc::NTHREADS) ct3_edge_kernel(SweepArgs a, CtArgs t, FVParams fp) {
   using L = Ct3Layout<P>;
   const int idir = blockIdx.y, b = blockIdx.z;
@@ -189,8 +202,8 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct3_edge_kernel(SweepArgs a, Ct
                                               c[2] + s1 * e1[2] + s2 * e2[2]); };
   // FM(cell, component, face direction): flux of B_component through the
   // upper face of the cell
-  auto FM = [&](size_t cell, int comp, int fd) { return face_ptr<P>(t, b, fd, P::MAG + comp)[cell]; };
-  auto FQ = [&](size_t cell, int k, int fd) { return face_ptr<P>(t, b, fd, k)[cell]; };
+  auto FM = [&](size_t cell, int comp, int fd) { return double(flux_ptr<P, R>(t, b, fd, P::MAG + comp)[cell]); };
+  auto FQ = [&](size_t cell, int k, int fd) { return ctd_ptr<P>(t, b, fd, k)[cell]; };
   double e;
   if (t.ct != CT_HLL) {
     const double fy0 = FM(at(0, 0), idim1, idim2), fy1 = FM(at(1, 0), idim1, idim2);
@@ -258,7 +271,7 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct3_edge_kernel(SweepArgs a, Ct
 }
 
 // one thread per interior cell, blockIdx.y: block
-template <class P>
+template <class P, class R>
 __global__ void __launch_bounds__(fvc::NTHREADS) ct3_update_kernel(SweepArgs a, CtArgs t, FVParams fp) {
   using L = Ct3Layout<P>;
   constexpr int NW = P::NW;
@@ -308,11 +321,11 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct3_update_kernel(SweepArgs a, 
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const double* f = face_ptr<P>(t, b, d, 0);
+    const R* f = flux_ptr<P, R>(t, b, d, 0);
     const size_t c1 = gidx(i, j, k), c0 = gidx(i - (d == 0), j - (d == 1), k - (d == 2));
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
-      const double f1 = a.dq[d] * f[iw * G3 + c1], f0 = a.dq[d] * f[iw * G3 + c0];
+      const double f1 = a.dq[d] * double(f[iw * G3 + c1]), f0 = a.dq[d] * double(f[iw * G3 + c0]);
       acc[iw] = acc[iw] + f1 - f0;
     }
   }
@@ -335,15 +348,15 @@ hipError_t launch_ct3_lim(int mixed, const SweepArgs& a, const CtArgs& t, const 
     else {
       hipLaunchKernelGGL((ct3_prim_kernel<P, float>), gp, dim3(fvc::NTHREADS), 0, s, a, t, fp);
       hipLaunchKernelGGL((ct3_face_kernel<P, float, RIEM, LIM>), gf, dim3(fvc::NTHREADS), 0, s, a, t, fp);
+      hipLaunchKernelGGL((ct3_edge_kernel<P, float, LIM>), ge, dim3(fvc::NTHREADS), 0, s, a, t, fp);
+      hipLaunchKernelGGL((ct3_update_kernel<P, float>), gu, dim3(fvc::NTHREADS), 0, s, a, t, fp);
     }
   } else {
     hipLaunchKernelGGL((ct3_prim_kernel<P, double>), gp, dim3(fvc::NTHREADS), 0, s, a, t, fp);
     hipLaunchKernelGGL((ct3_face_kernel<P, double, RIEM, LIM>), gf, dim3(fvc::NTHREADS), 0, s, a, t, fp);
+    hipLaunchKernelGGL((ct3_edge_kernel<P, double, LIM>), ge, dim3(fvc::NTHREADS), 0, s, a, t, fp);
+    hipLaunchKernelGGL((ct3_update_kernel<P, double>), gu, dim3(fvc::NTHREADS), 0, s, a, t, fp);
   }
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((ct3_edge_kernel<P, LIM>), ge, dim3(fvc::NTHREADS), 0, s, a, t, fp);
-  hipLaunchKernelGGL((ct3_update_kernel<P>), gu, dim3(fvc::NTHREADS), 0, s, a, t, fp);
   return hipGetLastError();
 }
 
