@@ -25,12 +25,15 @@ namespace {
 
 using fvc::BX;
 constexpr int CTG = 20;            // face grid edge
+// per block, in doubles: the fluxes [2][NW][20][20] in the face precision
+// (an FP32 flux is stored as the FP32 value it is), then the face data of
+// the edge fields [2][5][20][20] in FP64
 template <class P>
 struct CtLayout {
   static constexpr int FK = P::NW + 5;
-  static constexpr int VN = P::NW, CBMIN = P::NW + 1, CBMAX = P::NW + 2, VB1 = P::NW + 3, VB2 = P::NW + 4;
-  static constexpr size_t per_dim = (size_t)FK * CTG * CTG;
-  static constexpr size_t per_block = 2 * per_dim;
+  static constexpr int VN = 0, CBMIN = 1, CBMAX = 2, VB1 = 3, VB2 = 4;
+  static constexpr size_t K = (size_t)CTG * CTG;
+  static constexpr size_t per_block = 2 * (size_t)FK * K;
 };
 
 __device__ __forceinline__ double ws_at(const SweepArgs& a, const double* ws, int b, int d, int x, int y) {
@@ -89,10 +92,12 @@ __device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams&
     io.bx = t.bx_mode < 0 ? ws_at(a, t.ws_block, b, ID, i, j) : ws_comb(a, t, t.bx_mode, b, ID, i, j);
   R fC[NW];
   face_flux<P, R, RIEM, ID, LIM, 2, true>(st, fp, R(0), fC, crash, &io);
-  double* f = t.faces + (size_t)b * L::per_block + (size_t)ID * L::per_dim + (size_t)(j + 2) * CTG + (i + 2);
-  constexpr size_t K = (size_t)CTG * CTG;
+  constexpr size_t K = L::K;
+  const size_t c = (size_t)(j + 2) * CTG + (i + 2);
+  R* fr = reinterpret_cast<R*>(t.faces + (size_t)b * L::per_block) + (size_t)ID * NW * K + c;
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) f[iw * K] = double(fC[iw]);
+  for (int iw = 0; iw < NW; ++iw) fr[iw * K] = fC[iw];
+  double* f = t.faces + (size_t)b * L::per_block + 2 * NW * K + (size_t)ID * 5 * K + c;
   if (t.ct == CT_CONTACT) f[L::VN * K] = 0.5 * (io.vL[ID] + io.vR[ID]);
   if (t.ct == CT_HLL) {
     const double cbmax = io.cmax > 0.0 ? io.cmax : 0.0;
@@ -135,14 +140,23 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct_face_kernel(SweepArgs a, CtA
   if (crash) *a.crash = 1;
 }
 
-template <class P>
-__device__ __forceinline__ double fget(const CtArgs& t, int b, int dim, int k, int i, int j) {
+// flux iw through the dim faces (stored in the face precision R)
+template <class P, class R>
+__device__ __forceinline__ double fget(const CtArgs& t, int b, int dim, int iw, int i, int j) {
   using L = CtLayout<P>;
-  return t.faces[(size_t)b * L::per_block + (size_t)dim * L::per_dim + (size_t)k * CTG * CTG + (size_t)(j + 2) * CTG +
+  const R* f = reinterpret_cast<const R*>(t.faces + (size_t)b * L::per_block);
+  return double(f[((size_t)dim * P::NW + iw) * L::K + (size_t)(j + 2) * CTG + (i + 2)]);
+}
+
+// face datum k (VN .. VB2) of the dim faces (FP64)
+template <class P>
+__device__ __forceinline__ double dget(const CtArgs& t, int b, int dim, int k, int i, int j) {
+  using L = CtLayout<P>;
+  return t.faces[(size_t)b * L::per_block + 2 * P::NW * L::K + ((size_t)dim * 5 + k) * L::K + (size_t)(j + 2) * CTG +
                  (i + 2)];
 }
 
-template <class P, int LIM>
+template <class P, class R, int LIM>
 __global__ void __launch_bounds__(fvc::NTHREADS) ct_update_kernel(SweepArgs a, CtArgs t, FVParams fp) {
   using L = CtLayout<P>;
   constexpr int NW = P::NW, B1 = P::MAG, B2 = P::MAG + 1;
@@ -173,13 +187,13 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct_update_kernel(SweepArgs a, C
     double e;
     if (t.ct != CT_HLL) {
       // fC(b1) through y faces, fC(b2) through x faces
-      const double fy0 = fget<P>(t, b, 1, B1, i, j), fy1 = fget<P>(t, b, 1, B1, i + 1, j);
-      const double fx0 = fget<P>(t, b, 0, B2, i, j), fx1 = fget<P>(t, b, 0, B2, i, j + 1);
+      const double fy0 = fget<P, R>(t, b, 1, B1, i, j), fy1 = fget<P, R>(t, b, 1, B1, i + 1, j);
+      const double fx0 = fget<P, R>(t, b, 0, B2, i, j), fx1 = fget<P, R>(t, b, 0, B2, i, j + 1);
       e = 0.25 * (fy0 + fy1 - fx0 - fx1);
       if (t.ct == CT_CONTACT) {
         auto E = [&](int x, int y) { return ecc[(y + 1) * (BX + 2) + (x + 1)]; };
-        const double vx0 = fget<P>(t, b, 0, L::VN, i, j), vx1 = fget<P>(t, b, 0, L::VN, i, j + 1);
-        const double vy0 = fget<P>(t, b, 1, L::VN, i, j), vy1 = fget<P>(t, b, 1, L::VN, i + 1, j);
+        const double vx0 = dget<P>(t, b, 0, L::VN, i, j), vx1 = dget<P>(t, b, 0, L::VN, i, j + 1);
+        const double vy0 = dget<P>(t, b, 1, L::VN, i, j), vy1 = dget<P>(t, b, 1, L::VN, i + 1, j);
         // first pair: y-face fluxes of b1 at (i, j), (i+1, j)
         {
           const double ELc = fy0 - E(i, j), ELp = fy1 - E(i + 1, j);
@@ -213,21 +227,21 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct_update_kernel(SweepArgs a, C
       const double rx = fp.rdelinv[0], ry = fp.rdelinv[1];
       double vtL1, vtR1, vtL2, vtR2, btLx, btRx, btLy, btRy;
       // vbarC(x faces, 1) along y -> vtil(:, 2)
-      ct_recon<LIM>(fp, ry, fget<P>(t, b, 0, L::VB1, i, j - 1), fget<P>(t, b, 0, L::VB1, i, j),
-                    fget<P>(t, b, 0, L::VB1, i, j + 1), fget<P>(t, b, 0, L::VB1, i, j + 2), vtL2, vtR2);
+      ct_recon<LIM>(fp, ry, dget<P>(t, b, 0, L::VB1, i, j - 1), dget<P>(t, b, 0, L::VB1, i, j),
+                    dget<P>(t, b, 0, L::VB1, i, j + 1), dget<P>(t, b, 0, L::VB1, i, j + 2), vtL2, vtR2);
       // vbarC(y faces, 2) along x -> vtil(:, 1)
-      ct_recon<LIM>(fp, rx, fget<P>(t, b, 1, L::VB2, i - 1, j), fget<P>(t, b, 1, L::VB2, i, j),
-                    fget<P>(t, b, 1, L::VB2, i + 1, j), fget<P>(t, b, 1, L::VB2, i + 2, j), vtL1, vtR1);
+      ct_recon<LIM>(fp, rx, dget<P>(t, b, 1, L::VB2, i - 1, j), dget<P>(t, b, 1, L::VB2, i, j),
+                    dget<P>(t, b, 1, L::VB2, i + 1, j), dget<P>(t, b, 1, L::VB2, i + 2, j), vtL1, vtR1);
       // face field of the stage input: x faces along y, y faces along x
       ct_recon<LIM>(fp, ry, wsl_at(wsl, 0, i, j - 1), wsl_at(wsl, 0, i, j), wsl_at(wsl, 0, i, j + 1),
                     wsl_at(wsl, 0, i, j + 2), btLx, btRx);
       ct_recon<LIM>(fp, rx, wsl_at(wsl, 1, i - 1, j), wsl_at(wsl, 1, i, j), wsl_at(wsl, 1, i + 1, j),
                     wsl_at(wsl, 1, i + 2, j), btLy, btRy);
       auto mx = [](double u, double v) { return u > v ? u : v; };
-      const double cm1 = mx(fget<P>(t, b, 0, L::CBMIN, i, j + 1), fget<P>(t, b, 0, L::CBMIN, i, j));
-      const double cp1 = mx(fget<P>(t, b, 0, L::CBMAX, i, j + 1), fget<P>(t, b, 0, L::CBMAX, i, j));
-      const double cm2 = mx(fget<P>(t, b, 1, L::CBMIN, i + 1, j), fget<P>(t, b, 1, L::CBMIN, i, j));
-      const double cp2 = mx(fget<P>(t, b, 1, L::CBMAX, i + 1, j), fget<P>(t, b, 1, L::CBMAX, i, j));
+      const double cm1 = mx(dget<P>(t, b, 0, L::CBMIN, i, j + 1), dget<P>(t, b, 0, L::CBMIN, i, j));
+      const double cp1 = mx(dget<P>(t, b, 0, L::CBMAX, i, j + 1), dget<P>(t, b, 0, L::CBMAX, i, j));
+      const double cm2 = mx(dget<P>(t, b, 1, L::CBMIN, i + 1, j), dget<P>(t, b, 1, L::CBMIN, i, j));
+      const double cp2 = mx(dget<P>(t, b, 1, L::CBMAX, i + 1, j), dget<P>(t, b, 1, L::CBMAX, i, j));
       e = -(cp1 * vtL1 * btLy + cm1 * vtR1 * btRy - cp1 * cm1 * (btRy - btLy)) / (cp1 + cm1) +
           (cp2 * vtL2 * btLx + cm2 * vtR2 * btRx - cp2 * cm2 * (btRx - btLx)) / (cp2 + cm2);
       e = a.qdt * e;
@@ -273,12 +287,12 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct_update_kernel(SweepArgs a, C
   }
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
-    const double fx1 = a.dq[0] * fget<P>(t, b, 0, iw, i, j), fx0 = a.dq[0] * fget<P>(t, b, 0, iw, i - 1, j);
+    const double fx1 = a.dq[0] * fget<P, R>(t, b, 0, iw, i, j), fx0 = a.dq[0] * fget<P, R>(t, b, 0, iw, i - 1, j);
     acc[iw] = acc[iw] + fx1 - fx0;
   }
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
-    const double fy1 = a.dq[1] * fget<P>(t, b, 1, iw, i, j), fy0 = a.dq[1] * fget<P>(t, b, 1, iw, i, j - 1);
+    const double fy1 = a.dq[1] * fget<P, R>(t, b, 1, iw, i, j), fy0 = a.dq[1] * fget<P, R>(t, b, 1, iw, i, j - 1);
     acc[iw] = acc[iw] + fy1 - fy0;
   }
   acc[B1] = 0.5 / sCx * (wsx[j * NE + (i + 1)] * sCx + wsx[j * NE + i] * sCx);
@@ -291,16 +305,17 @@ template <class P, int RIEM, int LIM>
 hipError_t launch_ct_lim(int mixed, const SweepArgs& a, const CtArgs& t, const FVParams& fp, hipStream_t s) {
   if (mixed) {
     if constexpr (kExact) return hipErrorInvalidValue;  // the bit-reproducible build is FP64 only
-    else
+    else {
       hipLaunchKernelGGL((ct_face_kernel<P, float, RIEM, LIM>), dim3(8 * a.per_xcd), dim3(fvc::NTHREADS), 0, s, a,
                          t, fp);
+      hipLaunchKernelGGL((ct_update_kernel<P, float, LIM>), dim3(8 * a.per_xcd), dim3(fvc::NTHREADS), 0, s, a, t,
+                         fp);
+    }
   } else {
     hipLaunchKernelGGL((ct_face_kernel<P, double, RIEM, LIM>), dim3(8 * a.per_xcd), dim3(fvc::NTHREADS), 0, s, a,
                        t, fp);
+    hipLaunchKernelGGL((ct_update_kernel<P, double, LIM>), dim3(8 * a.per_xcd), dim3(fvc::NTHREADS), 0, s, a, t, fp);
   }
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((ct_update_kernel<P, LIM>), dim3(8 * a.per_xcd), dim3(fvc::NTHREADS), 0, s, a, t, fp);
   return hipGetLastError();
 }
 
