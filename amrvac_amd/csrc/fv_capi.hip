@@ -133,6 +133,13 @@ struct amrvac_fv_ctx {
   size_t flags_cap = 0;
   int* d_iws = nullptr;
   double* d_wts = nullptr;
+  std::vector<int> h_iws;     // the estimator's variables and weights as
+  std::vector<double> h_wts;  // last uploaded
+  // pinned host buffer of the results read back: the dt reduction's three
+  // doubles at 0, the small-value flag at kPinCrash, the regrid flags from
+  // kPinFlags (copies into it need no staging and share one synchronisation)
+  char* h_pin = nullptr;
+  size_t pin_cap = 0;
   int nrt = 0, npt = 0, nft = 0;
   std::vector<int> h_level;
   double* scratch = nullptr; // this device's copy of `scratch` (no-allocation dt)
@@ -688,6 +695,64 @@ hipError_t launch_linde(const SweepArgs& a, const FVParams& fp, const int* ntype
 // evaluated in that order without contraction.  flags[b]: bit 0 when some
 // cell has error >= thr, bit 1 when every cell has error < thr and
 // error <= cthr (the coarsening test of errest, :10-61).
+// In 2-D (16^2 blocks, one cell per lane) each variable's block with the
+// cells the stencil reaches (two along each axis, one diagonally) is staged
+// in LDS once, from the same cell values the 3-D path reads one by one.
+constexpr int kLohnerT = 16 + 4;  // 2-D tile edge: block_nx + 2 + 2
+__global__ void __launch_bounds__(NTHREADS) lohner2_kernel(SweepArgs a, int nw, int nwt, const int* __restrict__ iws,
+                                                           const double* __restrict__ wts, double wavefilter,
+                                                           double thr, double cthr, unsigned char* __restrict__ flags) {
+  constexpr int T = kLohnerT, N = T - 4;
+  __shared__ double tile[T * T];
+  const int b = blockIdx.x;
+  const int x = threadIdx.x % N, y = threadIdx.x / N;
+  auto Q = [&](int dx, int dy) { return tile[(y + 2 + dy) * T + (x + 2 + dx)]; };
+  double error = 0.0;
+  for (int t = 0; t < nwt; ++t) {
+    const int iw = iws[t];
+    for (int c = threadIdx.x; c < T * T; c += NTHREADS) {
+      const int X = c % T - 2, Y = c / T - 2;
+      const int ox = X < 0 ? -X : (X >= N ? X - N + 1 : 0), oy = Y < 0 ? -Y : (Y >= N ? Y - N + 1 : 0);
+      if ((ox == 0 || oy == 0) || (ox == 1 && oy == 1)) tile[c] = cell_value<2>(a, nw, b, X, Y, 0, iw);
+    }
+    __syncthreads();
+    // q at p + s1 e_d1 + s2 e_d2, as the 3-D path's q()
+    auto q = [&](int d1, int s1, int d2, int s2) {
+      int o[2] = {0, 0};
+      o[d1] += s1;
+      o[d2] += s2;
+      return Q(o[0], o[1]);
+    };
+    double num = 0.0, den = 0.0;
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+#pragma unroll
+      for (int d2 = 0; d2 < 2; ++d2) {
+        const double tj = q(d2, 1, d, 1) - q(d2, 1, d, -1);
+        const double th = q(d2, -1, d, 1) - q(d2, -1, d, -1);
+        const double diff = tj - th;
+        num = num + diff * diff;
+      }
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      const double q0 = q(d, 0, d, 0);
+      const double tt = fabs(q(d, 2, d, 0) - q0) + fabs(q0 - q(d, -2, d, 0));
+#pragma unroll
+      for (int d2 = 0; d2 < 2; ++d2) {
+        const double aj = fabs(q(d2, 1, d, 1)) + fabs(q(d2, 1, d, -1));
+        const double ah = fabs(q(d2, -1, d, 1)) + fabs(q(d2, -1, d, -1));
+        const double v = tt + wavefilter * (aj + ah);
+        den = den + v * v;
+      }
+    }
+    error = error + wts[t] * sqrt(num / (den > 1.0e-6 ? den : 1.0e-6));
+    __syncthreads();  // the tile is rewritten for the next variable
+  }
+  const int any_r = __syncthreads_or(error >= thr ? 1 : 0);
+  const int all_c = __syncthreads_and((error < thr && error <= cthr) ? 1 : 0);
+  if (threadIdx.x == 0) flags[b] = (unsigned char)((any_r ? 1 : 0) | (all_c ? 2 : 0));
+}
+
 template <int NDIM>
 __global__ void __launch_bounds__(NTHREADS) lohner_kernel(SweepArgs a, int nw, int nwt, const int* __restrict__ iws,
                                                           const double* __restrict__ wts, double wavefilter, double thr,
@@ -898,6 +963,31 @@ static bool ensure_buf(T** buf, size_t* cap, size_t n) {
   return true;
 }
 
+constexpr size_t kPinCrash = 24, kPinFlags = 32;
+
+static bool ensure_pin(amrvac_fv_ctx* c, size_t bytes) {
+  if (c->h_pin && c->pin_cap >= bytes) return true;
+  if (c->h_pin) (void)hipHostFree(c->h_pin);
+  c->h_pin = nullptr;
+  c->pin_cap = 0;
+  const size_t n = bytes + bytes / 4 + 64;
+  if (hipHostMalloc((void**)&c->h_pin, n) != hipSuccess) {
+    c->h_pin = nullptr;
+    return false;
+  }
+  c->pin_cap = n;
+  return true;
+}
+
+// the small-value flag, read back through the pinned buffer
+static hipError_t read_crash(amrvac_fv_ctx* c, int* out) {
+  int* h = reinterpret_cast<int*>(c->h_pin + kPinCrash);
+  hipError_t e = hipMemcpyAsync(h, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) *out = *h;
+  return e;
+}
+
 // a table of the level into its buffer (grown when it does not fit)
 template <class T>
 static bool upload_vec(amrvac_fv_ctx* c, T** dst, size_t* cap, const std::vector<T>& v) {
@@ -1032,6 +1122,7 @@ int amrvac_fv_finalize(amrvac_fv_ctx* c) {
   void* regrid_bufs[] = {c->d_remap, c->d_plan, c->d_flags, c->d_iws, c->d_wts};
   for (void* p : regrid_bufs)
     if (p) (void)hipFree(p);
+  if (c->h_pin) (void)hipHostFree(c->h_pin);
   for (auto& ev : c->events) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1122,7 +1213,7 @@ static int alloc_level(amrvac_fv_ctx* c, int nblocks, int capacity, const double
   if (!ensure_buf(&c->d_nbr, &c->nbr_cap, (size_t)nblocks * c->nnb) ||
       (!c->d_crash && hipMalloc(&c->d_crash, sizeof(int)) != hipSuccess) ||
       !ensure_buf(&c->d_red, &c->red_cap, (size_t)nblocks * 2 * (c->nw > 1 ? c->nw : 1)) ||
-      (!c->d_red2 && hipMalloc(&c->d_red2, sizeof(double) * 3) != hipSuccess) ||
+      (!c->d_red2 && hipMalloc(&c->d_red2, sizeof(double) * 3) != hipSuccess) || !ensure_pin(c, kPinFlags + nblocks) ||
       !ensure_buf(&c->d_dtp, &c->dtp_cap, (size_t)nblocks * 2) ||
       !ensure_buf(&c->d_stage, &c->stage_cap, (size_t)nblocks * c->nw * c->cells_g)) {
     free_level(c);
@@ -1272,8 +1363,8 @@ static int getdt_reduce(amrvac_fv_ctx* c, double courantpar, double* dtmin_out, 
   hipLaunchKernelGGL(fv::reduce_max2_kernel, dim3(1), dim3(fvc::NTHREADS), 0, c->stream, part, c->nb, fin,
                      (const int*)c->d_crash);
   HIPCHK(hipGetLastError());
-  double r[3];
-  HIPCHK(hipMemcpyAsync(r, fin, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  const double* r = reinterpret_cast<const double*>(c->h_pin);
+  HIPCHK(hipMemcpyAsync(c->h_pin, fin, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   // the small-value flag as of this reduction; valid until the next launch
   // that may set it
@@ -1386,12 +1477,7 @@ static int stage_core(amrvac_fv_ctx* c, int method, double dtfactor, double dt, 
     const int rc = getbc_amr(c, dst_slot);
     if (rc != AMRVAC_FV_OK) return rc;
   }
-  if (crash_out) {
-    int h = 0;
-    HIPCHK(hipMemcpyAsync(&h, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    *crash_out = h;
-  }
+  if (crash_out) HIPCHK(read_crash(c, crash_out));
   return AMRVAC_FV_OK;
 }
 
@@ -1455,12 +1541,7 @@ static int ct_stage(amrvac_fv_ctx* c, int method, double dtfactor, double dt, in
   // the update kernel wrote every upper face of every block: the buffer
   // becomes psb's face field and psb's old array the next stage's buffer
   std::swap(c->wsl[psb], c->d_wsn);
-  if (crash_out) {
-    int h = 0;
-    HIPCHK(hipMemcpyAsync(&h, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    *crash_out = h;
-  }
+  if (crash_out) HIPCHK(read_crash(c, crash_out));
   return AMRVAC_FV_OK;
 }
 
@@ -1517,8 +1598,7 @@ int amrvac_fv_crash(amrvac_fv_ctx* c, int clear, int* crash_out) {
   if (!c->d_crash) return fail(AMRVAC_FV_ESTATE, "no mesh");
   if (!c->crash_fresh) {
     int h = 0;
-    HIPCHK(hipMemcpyAsync(&h, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(read_crash(c, &h));
     c->crash_seen = h ? 1 : 0;
     c->crash_fresh = true;
   }
@@ -1819,24 +1899,38 @@ int amrvac_fv_errest(amrvac_fv_ctx* c, int slot, int nwt, const int* iws, const 
     if (hipMalloc(&c->d_iws, 64 * sizeof(int)) != hipSuccess || hipMalloc(&c->d_wts, 64 * sizeof(double)) != hipSuccess)
       return fail(AMRVAC_FV_ENOMEM, "errest buffers");
   }
-  if (!ensure_buf(&c->d_flags, &c->flags_cap, (size_t)c->nb)) return fail(AMRVAC_FV_ENOMEM, "errest flags");
-  HIPCHK(hipMemcpyAsync(c->d_iws, iws, nwt * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_wts, weights, nwt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (!ensure_buf(&c->d_flags, &c->flags_cap, (size_t)c->nb) || !ensure_pin(c, kPinFlags + c->nb))
+    return fail(AMRVAC_FV_ENOMEM, "errest flags");
+  // the variables and weights go up when they change (a run passes the
+  // same ones every time)
+  const std::vector<int> iv(iws, iws + nwt);
+  const std::vector<double> wv(weights, weights + nwt);
+  if (iv != c->h_iws || wv != c->h_wts) {
+    HIPCHK(hipMemcpyAsync(c->d_iws, iws, nwt * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_wts, weights, nwt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // pageable sources: the copies have read them when the calls return
+    c->h_iws = iv;
+    c->h_wts = wv;
+  }
   fvc::SweepArgs a = base_args(c);
   a.src = c->slot[slot];
-  if (c->cfg.ndim == 2)
+  if (c->cfg.ndim == 2 && c->bx * c->bx == fvc::NTHREADS && c->bx + 4 == fv::kLohnerT)
+    hipLaunchKernelGGL(fv::lohner2_kernel, dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, nwt, c->d_iws,
+                       c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags);
+  else if (c->cfg.ndim == 2)
     hipLaunchKernelGGL((fv::lohner_kernel<2>), dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, nwt,
                        c->d_iws, c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags);
   else
     hipLaunchKernelGGL((fv::lohner_kernel<3>), dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, nwt,
                        c->d_iws, c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(flags, c->d_flags, (size_t)c->nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_pin + kPinFlags, c->d_flags, (size_t)c->nb, hipMemcpyDeviceToHost, c->stream));
   // the small-value flag rides the same synchronisation (amrvac_fv_crash
   // then reads it without another one)
-  int h = 0;
-  HIPCHK(hipMemcpyAsync(&h, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_pin + kPinCrash, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  memcpy(flags, c->h_pin + kPinFlags, (size_t)c->nb);
+  const int h = *reinterpret_cast<const int*>(c->h_pin + kPinCrash);
   c->crash_seen = h ? 1 : 0;
   c->crash_fresh = true;
   return AMRVAC_FV_OK;
