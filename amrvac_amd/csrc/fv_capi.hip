@@ -137,7 +137,8 @@ struct amrvac_fv_ctx {
   std::vector<double> h_wts;  // last uploaded
   // pinned host buffer of the results read back: the dt reduction's three
   // doubles at 0, the small-value flag at kPinCrash, the regrid flags from
-  // kPinFlags (copies into it need no staging and share one synchronisation)
+  // kPinFlags followed (4-byte aligned) by the small-value flag copied with
+  // them (copies into it need no staging and share one synchronisation)
   char* h_pin = nullptr;
   size_t pin_cap = 0;
   int nrt = 0, npt = 0, nft = 0;
@@ -695,62 +696,72 @@ hipError_t launch_linde(const SweepArgs& a, const FVParams& fp, const int* ntype
 // evaluated in that order without contraction.  flags[b]: bit 0 when some
 // cell has error >= thr, bit 1 when every cell has error < thr and
 // error <= cthr (the coarsening test of errest, :10-61).
-// In 2-D (16^2 blocks, one cell per lane) each variable's block with the
-// cells the stencil reaches (two along each axis, one diagonally) is staged
-// in LDS once, from the same cell values the 3-D path reads one by one.
+// In 2-D (16^2 blocks, one cell per lane) the weighted variables' block,
+// with the cells the stencil reaches (two along each axis, one diagonally),
+// is staged in LDS, up to kLohnerV variables at a time, from the same cell
+// values the 3-D path reads one by one.
 constexpr int kLohnerT = 16 + 4;  // 2-D tile edge: block_nx + 2 + 2
+constexpr int kLohnerV = 4;
 __global__ void __launch_bounds__(NTHREADS) lohner2_kernel(SweepArgs a, int nw, int nwt, const int* __restrict__ iws,
                                                            const double* __restrict__ wts, double wavefilter,
-                                                           double thr, double cthr, unsigned char* __restrict__ flags) {
+                                                           double thr, double cthr, unsigned char* __restrict__ flags,
+                                                           const int* __restrict__ crash, int* __restrict__ crash_copy) {
   constexpr int T = kLohnerT, N = T - 4;
-  __shared__ double tile[T * T];
+  __shared__ double tile[kLohnerV][T * T];
   const int b = blockIdx.x;
   const int x = threadIdx.x % N, y = threadIdx.x / N;
-  auto Q = [&](int dx, int dy) { return tile[(y + 2 + dy) * T + (x + 2 + dx)]; };
   double error = 0.0;
-  for (int t = 0; t < nwt; ++t) {
-    const int iw = iws[t];
-    for (int c = threadIdx.x; c < T * T; c += NTHREADS) {
-      const int X = c % T - 2, Y = c / T - 2;
+  for (int t0 = 0; t0 < nwt; t0 += kLohnerV) {
+    const int nv = nwt - t0 < kLohnerV ? nwt - t0 : kLohnerV;
+    for (int c = threadIdx.x; c < nv * T * T; c += NTHREADS) {
+      const int v = c / (T * T), e = c % (T * T);
+      const int X = e % T - 2, Y = e / T - 2;
       const int ox = X < 0 ? -X : (X >= N ? X - N + 1 : 0), oy = Y < 0 ? -Y : (Y >= N ? Y - N + 1 : 0);
-      if ((ox == 0 || oy == 0) || (ox == 1 && oy == 1)) tile[c] = cell_value<2>(a, nw, b, X, Y, 0, iw);
+      if ((ox == 0 || oy == 0) || (ox == 1 && oy == 1)) tile[v][e] = cell_value<2>(a, nw, b, X, Y, 0, iws[t0 + v]);
     }
     __syncthreads();
-    // q at p + s1 e_d1 + s2 e_d2, as the 3-D path's q()
-    auto q = [&](int d1, int s1, int d2, int s2) {
-      int o[2] = {0, 0};
-      o[d1] += s1;
-      o[d2] += s2;
-      return Q(o[0], o[1]);
-    };
-    double num = 0.0, den = 0.0;
+    for (int v = 0; v < nv; ++v) {
+      // q at p + s1 e_d1 + s2 e_d2, as the 3-D path's q()
+      auto q = [&](int d1, int s1, int d2, int s2) {
+        int o[2] = {0, 0};
+        o[d1] += s1;
+        o[d2] += s2;
+        return tile[v][(y + 2 + o[1]) * T + (x + 2 + o[0])];
+      };
+      double num = 0.0, den = 0.0;
 #pragma unroll
-    for (int d = 0; d < 2; ++d)
+      for (int d = 0; d < 2; ++d)
 #pragma unroll
-      for (int d2 = 0; d2 < 2; ++d2) {
-        const double tj = q(d2, 1, d, 1) - q(d2, 1, d, -1);
-        const double th = q(d2, -1, d, 1) - q(d2, -1, d, -1);
-        const double diff = tj - th;
-        num = num + diff * diff;
+        for (int d2 = 0; d2 < 2; ++d2) {
+          const double tj = q(d2, 1, d, 1) - q(d2, 1, d, -1);
+          const double th = q(d2, -1, d, 1) - q(d2, -1, d, -1);
+          const double diff = tj - th;
+          num = num + diff * diff;
+        }
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        const double q0 = q(d, 0, d, 0);
+        const double tt = fabs(q(d, 2, d, 0) - q0) + fabs(q0 - q(d, -2, d, 0));
+#pragma unroll
+        for (int d2 = 0; d2 < 2; ++d2) {
+          const double aj = fabs(q(d2, 1, d, 1)) + fabs(q(d2, 1, d, -1));
+          const double ah = fabs(q(d2, -1, d, 1)) + fabs(q(d2, -1, d, -1));
+          const double vv = tt + wavefilter * (aj + ah);
+          den = den + vv * vv;
+        }
       }
-#pragma unroll
-    for (int d = 0; d < 2; ++d) {
-      const double q0 = q(d, 0, d, 0);
-      const double tt = fabs(q(d, 2, d, 0) - q0) + fabs(q0 - q(d, -2, d, 0));
-#pragma unroll
-      for (int d2 = 0; d2 < 2; ++d2) {
-        const double aj = fabs(q(d2, 1, d, 1)) + fabs(q(d2, 1, d, -1));
-        const double ah = fabs(q(d2, -1, d, 1)) + fabs(q(d2, -1, d, -1));
-        const double v = tt + wavefilter * (aj + ah);
-        den = den + v * v;
-      }
+      error = error + wts[t0 + v] * sqrt(num / (den > 1.0e-6 ? den : 1.0e-6));
     }
-    error = error + wts[t] * sqrt(num / (den > 1.0e-6 ? den : 1.0e-6));
-    __syncthreads();  // the tile is rewritten for the next variable
+    __syncthreads();  // the tile is rewritten for the next variables
   }
   const int any_r = __syncthreads_or(error >= thr ? 1 : 0);
   const int all_c = __syncthreads_and((error < thr && error <= cthr) ? 1 : 0);
-  if (threadIdx.x == 0) flags[b] = (unsigned char)((any_r ? 1 : 0) | (all_c ? 2 : 0));
+  if (threadIdx.x == 0) {
+    flags[b] = (unsigned char)((any_r ? 1 : 0) | (all_c ? 2 : 0));
+    // the small-value flag of the stages before, so that one copy brings
+    // both to the host
+    if (b == 0) *crash_copy = *crash;
+  }
 }
 
 template <int NDIM>
@@ -1899,7 +1910,9 @@ int amrvac_fv_errest(amrvac_fv_ctx* c, int slot, int nwt, const int* iws, const 
     if (hipMalloc(&c->d_iws, 64 * sizeof(int)) != hipSuccess || hipMalloc(&c->d_wts, 64 * sizeof(double)) != hipSuccess)
       return fail(AMRVAC_FV_ENOMEM, "errest buffers");
   }
-  if (!ensure_buf(&c->d_flags, &c->flags_cap, (size_t)c->nb) || !ensure_pin(c, kPinFlags + c->nb))
+  // flags, then (4-byte aligned) the copy of the small-value flag
+  const size_t nbf = ((size_t)c->nb + 3) & ~(size_t)3;
+  if (!ensure_buf(&c->d_flags, &c->flags_cap, nbf + sizeof(int)) || !ensure_pin(c, kPinFlags + nbf + sizeof(int)))
     return fail(AMRVAC_FV_ENOMEM, "errest flags");
   // the variables and weights go up when they change (a run passes the
   // same ones every time)
@@ -1914,9 +1927,12 @@ int amrvac_fv_errest(amrvac_fv_ctx* c, int slot, int nwt, const int* iws, const 
   }
   fvc::SweepArgs a = base_args(c);
   a.src = c->slot[slot];
-  if (c->cfg.ndim == 2 && c->bx * c->bx == fvc::NTHREADS && c->bx + 4 == fv::kLohnerT)
+  const bool tiled = c->cfg.ndim == 2 && c->bx * c->bx == fvc::NTHREADS && c->bx + 4 == fv::kLohnerT;
+  int* crash_copy = reinterpret_cast<int*>(c->d_flags + nbf);
+  if (tiled)
     hipLaunchKernelGGL(fv::lohner2_kernel, dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, nwt, c->d_iws,
-                       c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags);
+                       c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags, (const int*)c->d_crash,
+                       crash_copy);
   else if (c->cfg.ndim == 2)
     hipLaunchKernelGGL((fv::lohner_kernel<2>), dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, nwt,
                        c->d_iws, c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags);
@@ -1924,13 +1940,18 @@ int amrvac_fv_errest(amrvac_fv_ctx* c, int slot, int nwt, const int* iws, const 
     hipLaunchKernelGGL((fv::lohner_kernel<3>), dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, nwt,
                        c->d_iws, c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(c->h_pin + kPinFlags, c->d_flags, (size_t)c->nb, hipMemcpyDeviceToHost, c->stream));
   // the small-value flag rides the same synchronisation (amrvac_fv_crash
-  // then reads it without another one)
-  HIPCHK(hipMemcpyAsync(c->h_pin + kPinCrash, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  // then reads it without another one): copied behind the flags by the
+  // tiled kernel, else on its own
+  if (tiled) {
+    HIPCHK(hipMemcpyAsync(c->h_pin + kPinFlags, c->d_flags, nbf + sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  } else {
+    HIPCHK(hipMemcpyAsync(c->h_pin + kPinFlags, c->d_flags, (size_t)c->nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_pin + kPinFlags + nbf, c->d_crash, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  }
   HIPCHK(hipStreamSynchronize(c->stream));
   memcpy(flags, c->h_pin + kPinFlags, (size_t)c->nb);
-  const int h = *reinterpret_cast<const int*>(c->h_pin + kPinCrash);
+  const int h = *reinterpret_cast<const int*>(c->h_pin + kPinFlags + nbf);
   c->crash_seen = h ? 1 : 0;
   c->crash_fresh = true;
   return AMRVAC_FV_OK;
