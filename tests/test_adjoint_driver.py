@@ -13,6 +13,7 @@ segment on the tape are all exercised (a wrong stage state changes the
 from __future__ import annotations
 
 import ctypes
+import math
 
 import numpy as np
 import pytest
@@ -107,6 +108,31 @@ def test_driver_gradient_equals_autograd(nsteps, every):
     last = nsteps - (nseg - 1) * every
     assert be.calls["adjoint1"] == 3 * nsteps
     assert be.calls["stage"] == 3 * nsteps + 3 * (nsteps - last) - (nseg - 1)
+
+
+def test_whole_run_tape_when_it_fits():
+    """The default schedule keeps every stage state when the tape fits in
+    tape_bytes: no recompute, and the gradient bit for bit that of the
+    checkpointed schedule (the recompute replays the same stages)."""
+    mesh = UniformMesh(2, (32, 16), (16, 16), (0.0, 0.0), (1.0, 0.5), ("periodic",) * 4)
+    par = Par(physics="hd", ndim=2, ndir=2, time_stepper="threestep", flux_scheme="hllc", limiter="vanleer")
+    shape = (mesh.nblocks, par.nw) + tuple(mesh.block_nx[::-1])
+    nsteps = 9
+    tape = (nsteps + 1) * 3 * math.prod(shape) * 8
+    assert adjoint.SSPRK3Run(par, mesh, nsteps, backend=FakeBackend(shape), tape_bytes=tape - 1).every == 3
+    be = FakeBackend(shape)
+    run = adjoint.SSPRK3Run(par, mesh, nsteps, backend=be, tape_bytes=tape)
+    assert run.every == nsteps
+    gen = torch.Generator().manual_seed(5)
+    u0 = 1.0 + 0.1 * torch.rand(shape, generator=gen, dtype=torch.float64)
+    w = torch.randn(shape, generator=gen, dtype=torch.float64)
+    x = u0.clone().requires_grad_(True)
+    g, = torch.autograd.grad((adjoint.run_differentiable(x, run) * w).sum(), x)
+    assert be.calls["stage"] == 3 * nsteps  # the forward's stages only
+    run3 = adjoint.SSPRK3Run(par, mesh, nsteps, every=3, backend=FakeBackend(shape))
+    x3 = u0.clone().requires_grad_(True)
+    g3, = torch.autograd.grad((adjoint.run_differentiable(x3, run3) * w).sum(), x3)
+    assert torch.equal(g, g3)
 
 
 def test_driver_replays_given_dts():
