@@ -168,18 +168,17 @@ class AMRForest:
         self.ig = np.array([n.ig for n in self.leaves], dtype=np.int32).reshape(nb, nd)
         self.icpos = np.array([n.ic() for n in self.leaves], dtype=np.int32).reshape(nb, nd)
         self.ntype, self.nbr, self.nchild = amr_native.connectivity(nd, self.ng1, self.periodic, self.level, self.ig)
-        # block extents (alloc_node, src/amr/mod_amr_solution_node.t:150-160)
-        self.xlohi = np.zeros((nb, nd, 2), dtype=np.float64)
-        self.dxb = np.zeros((nb, nd), dtype=np.float64)
-        for n in self.leaves:
-            dg = self.dg[n.level]
-            for d in range(nd):
-                lo = self.xprobmin[d] + float(n.ig[d] - 1) * dg[d]
-                hi = self.xprobmin[d] + float(n.ig[d]) * dg[d]
-                if hi > self.xprobmax[d]:
-                    hi = self.xprobmax[d]
-                self.xlohi[n.block, d] = (lo, hi)
-                self.dxb[n.block, d] = self.dx[n.level][d]
+        # block extents (alloc_node, src/amr/mod_amr_solution_node.t:150-160),
+        # elementwise in float64 as the scalar formulas round
+        levs = sorted(self.dg)
+        dg = np.array([self.dg[lv] for lv in levs], dtype=np.float64)[self.level - levs[0]]
+        dxl = np.array([self.dx[lv] for lv in levs], dtype=np.float64)[self.level - levs[0]]
+        xmin = np.array(self.xprobmin, dtype=np.float64)
+        xmax = np.array(self.xprobmax, dtype=np.float64)
+        self.xlohi = np.empty((nb, nd, 2), dtype=np.float64)
+        self.xlohi[:, :, 0] = xmin + (self.ig - 1).astype(np.float64) * dg
+        self.xlohi[:, :, 1] = np.minimum(xmin + self.ig.astype(np.float64) * dg, xmax)
+        self.dxb = np.ascontiguousarray(dxl)
 
     # -- geometry of the leaves (the UniformMesh interface) ----------------
     @property

@@ -262,17 +262,17 @@ class DeviceFV:
         amr.apply_regrid), then install the new connectivity and ghost
         cells."""
         nd = forest.ndim
-        S = 2 + (1 << nd)
-        P = np.zeros((forest.nblocks, S), dtype=np.int32)
-        for b, org in enumerate(plan):
+        nc = 1 << nd
+        pad = [0] * (nc - 1)
+        rows = []
+        for org in plan:
             if org[0] == "copy":
-                P[b, 0], P[b, 1] = 0, org[1]
+                rows.append([0, org[1]] + pad + [0])
             elif org[0] == "coarsen":
-                P[b, 0] = 1
-                P[b, 1:1 + (1 << nd)] = org[1]
+                rows.append([1] + list(org[1]) + [0])
             else:
-                P[b, 0], P[b, 1] = 2, org[1]
-                P[b, S - 1] = sum((org[2][d] - 1) << d for d in range(nd))
+                rows.append([2, org[1]] + pad + [sum((org[2][d] - 1) << d for d in range(nd))])
+        P = np.array(rows, dtype=np.int32).reshape(forest.nblocks, 2 + nc)
         xl = np.ascontiguousarray(forest.xlohi, dtype=np.float64)
         bc = np.ascontiguousarray(forest.bc_codes, dtype=np.int32)
         dx1 = np.ascontiguousarray(forest.dx[1], dtype=np.float64)
