@@ -764,6 +764,83 @@ __global__ void __launch_bounds__(NTHREADS) lohner2_kernel(SweepArgs a, int nw, 
   }
 }
 
+// In 3-D the block with its stencil rim ((bx+4)^2 (bnz+4) cells, at most
+// kLohner3Tile) is staged in LDS one variable at a time; a lane keeps the
+// error sums of its (at most kLohner3Cells) cells in registers.
+constexpr int kLohner3Tile = 8000, kLohner3Cells = 16;
+__global__ void __launch_bounds__(NTHREADS) lohner3_kernel(SweepArgs a, int nw, int nwt, const int* __restrict__ iws,
+                                                           const double* __restrict__ wts, double wavefilter,
+                                                           double thr, double cthr, unsigned char* __restrict__ flags,
+                                                           const int* __restrict__ crash, int* __restrict__ crash_copy) {
+  __shared__ double tile[kLohner3Tile];
+  const int b = blockIdx.x;
+  const int n0 = a.bx, n2 = a.bnz;
+  const int T0 = n0 + 4, T2 = n2 + 4;
+  const int cells = n0 * n0 * n2, tcells = T0 * T0 * T2;
+  double err[kLohner3Cells];
+#pragma unroll
+  for (int j = 0; j < kLohner3Cells; ++j) err[j] = 0.0;
+  for (int t = 0; t < nwt; ++t) {
+    const int iw = iws[t];
+    for (int c = threadIdx.x; c < tcells; c += NTHREADS) {
+      const int X = c % T0 - 2, Y = (c / T0) % T0 - 2, Z = c / (T0 * T0) - 2;
+      const int o0 = X < 0 ? -X : (X >= n0 ? X - n0 + 1 : 0);
+      const int o1 = Y < 0 ? -Y : (Y >= n0 ? Y - n0 + 1 : 0);
+      const int o2 = Z < 0 ? -Z : (Z >= n2 ? Z - n2 + 1 : 0);
+      const int nout = (o0 > 0) + (o1 > 0) + (o2 > 0);
+      // the stencil reaches two cells out along an axis and one diagonally
+      if (nout <= 1 || (nout == 2 && o0 <= 1 && o1 <= 1 && o2 <= 1)) tile[c] = cell_value<3>(a, nw, b, X, Y, Z, iw);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kLohner3Cells; ++j) {
+      const int c = threadIdx.x + j * NTHREADS;
+      if (c >= cells) break;
+      const int X[3] = {c % n0, (c / n0) % n0, c / (n0 * n0)};
+      // q at p + s1 e_d1 + s2 e_d2, as the general path's q()
+      auto q = [&](int d1, int s1, int d2, int s2) {
+        int p[3] = {X[0] + 2, X[1] + 2, X[2] + 2};
+        p[d1] += s1;
+        p[d2] += s2;
+        return tile[(p[2] * T0 + p[1]) * T0 + p[0]];
+      };
+      double num = 0.0, den = 0.0;
+      for (int d = 0; d < 3; ++d)
+        for (int d2 = 0; d2 < 3; ++d2) {
+          const double tj = q(d2, 1, d, 1) - q(d2, 1, d, -1);
+          const double th = q(d2, -1, d, 1) - q(d2, -1, d, -1);
+          const double diff = tj - th;
+          num = num + diff * diff;
+        }
+      for (int d = 0; d < 3; ++d) {
+        const double q0 = q(d, 0, d, 0);
+        const double tt = fabs(q(d, 2, d, 0) - q0) + fabs(q0 - q(d, -2, d, 0));
+        for (int d2 = 0; d2 < 3; ++d2) {
+          const double aj = fabs(q(d2, 1, d, 1)) + fabs(q(d2, 1, d, -1));
+          const double ah = fabs(q(d2, -1, d, 1)) + fabs(q(d2, -1, d, -1));
+          const double v = tt + wavefilter * (aj + ah);
+          den = den + v * v;
+        }
+      }
+      err[j] = err[j] + wts[t] * sqrt(num / (den > 1.0e-6 ? den : 1.0e-6));
+    }
+    __syncthreads();  // the tile is rewritten for the next variable
+  }
+  int any_r = 0, all_c = 1;
+#pragma unroll
+  for (int j = 0; j < kLohner3Cells; ++j) {
+    if (threadIdx.x + j * NTHREADS >= cells) break;
+    if (err[j] >= thr) any_r = 1;
+    if (!(err[j] < thr && err[j] <= cthr)) all_c = 0;
+  }
+  any_r = __syncthreads_or(any_r);
+  all_c = __syncthreads_and(all_c);
+  if (threadIdx.x == 0) {
+    flags[b] = (unsigned char)((any_r ? 1 : 0) | (all_c ? 2 : 0));
+    if (b == 0) *crash_copy = *crash;
+  }
+}
+
 template <int NDIM>
 __global__ void __launch_bounds__(NTHREADS) lohner_kernel(SweepArgs a, int nw, int nwt, const int* __restrict__ iws,
                                                           const double* __restrict__ wts, double wavefilter, double thr,
@@ -1927,10 +2004,17 @@ int amrvac_fv_errest(amrvac_fv_ctx* c, int slot, int nwt, const int* iws, const 
   }
   fvc::SweepArgs a = base_args(c);
   a.src = c->slot[slot];
-  const bool tiled = c->cfg.ndim == 2 && c->bx * c->bx == fvc::NTHREADS && c->bx + 4 == fv::kLohnerT;
+  const bool tiled2 = c->cfg.ndim == 2 && c->bx * c->bx == fvc::NTHREADS && c->bx + 4 == fv::kLohnerT;
+  const bool tiled3 = c->cfg.ndim == 3 && (c->bx + 4) * (c->bx + 4) * (c->bnz + 4) <= fv::kLohner3Tile &&
+                      c->bx * c->bx * c->bnz <= fv::kLohner3Cells * fvc::NTHREADS;
+  const bool tiled = tiled2 || tiled3;
   int* crash_copy = reinterpret_cast<int*>(c->d_flags + nbf);
-  if (tiled)
+  if (tiled2)
     hipLaunchKernelGGL(fv::lohner2_kernel, dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, nwt, c->d_iws,
+                       c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags, (const int*)c->d_crash,
+                       crash_copy);
+  else if (tiled3)
+    hipLaunchKernelGGL(fv::lohner3_kernel, dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, nwt, c->d_iws,
                        c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags, (const int*)c->d_crash,
                        crash_copy);
   else if (c->cfg.ndim == 2)

@@ -140,20 +140,22 @@ def test_amr_3d_exact_matches_oracle_bitwise():
     assert np.array_equal(f_dev.interior(ti_dev.be.get_state(0)), f_orc.interior(ti_orc.be.get_state(0)))
 
 
-@pytest.mark.parametrize("ndim", [2, 3])
-def test_device_errest_and_regrid_match_host(ndim):
+@pytest.mark.parametrize("ndim,bx", [(2, 16), (3, 16), (3, 8), (3, 10)])
+def test_device_errest_and_regrid_match_host(ndim, bx):
     """The device Löhner flags and remap (coarsen_grid, prolong_2nd) against
     the host's on the same ghosted state: flags identical, new leaves bit
-    for bit."""
+    for bit (3-D: block_nx 16, 8 and 10, each estimated from an LDS tile of
+    the block and its stencil rim)."""
     from amrvac_amd.amr import apply_regrid, coarsen_child, errest_flags, prolong_2nd, proper_nesting
     from amrvac_amd.fv import DeviceFV
-    print(f"[amr] device errest/regrid vs host, {ndim}-D", flush=True)
+    print(f"[amr] device errest/regrid vs host, {ndim}-D, block_nx {bx}", flush=True)
     if ndim == 2:
         par, forest, ap = _blast("3step_hll_cada", 0.02)
     else:
         par = Par(physics="hd", ndim=3, ndir=3, gamma=5.0 / 3.0, typeboundspeed="cmaxmean", courantpar=0.9,
                   time_stepper="twostep", flux_scheme="tvdlf", limiter="minmod")
-        forest = AMRForest(3, (32, 32, 32), (16, 16, 16), (0.0,) * 3, (2.0,) * 3, ("cont",) * 6, max_level=3)
+        n = 2 * bx if bx == 16 else 4 * bx
+        forest = AMRForest(3, (n, n, n), (bx, bx, bx), (0.0,) * 3, (2.0,) * 3, ("cont",) * 6, max_level=3)
         ap = AMRPar(refine_max_level=3, w_refine_weight={0: 0.5, 4: 0.5})
     init = lambda f, b: problems.blast_wave_blocks(f, b, radius=0.3)
     ti = AMRIntegrator(par, forest, ap, lambda p, f: DeviceFV(p, f, precision="fp64_exact"), init)
