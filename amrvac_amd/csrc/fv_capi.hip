@@ -2018,8 +2018,7 @@ int amrvac_fv_errest(amrvac_fv_ctx* c, int slot, int nwt, const int* iws, const 
                        c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags, (const int*)c->d_crash,
                        crash_copy);
   else if (c->cfg.ndim == 2)
-    hipLaunchKernelGGL((fv::lohner_kernel<2>), dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, nwt,
-                       c->d_iws, c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags);
+    return fail(AMRVAC_FV_EINVAL, "the 2-D Löhner estimate is built for 16^2 blocks");
   else
     hipLaunchKernelGGL((fv::lohner_kernel<3>), dim3(c->nb), dim3(fvc::NTHREADS), 0, c->stream, a, c->nw, nwt,
                        c->d_iws, c->d_wts, wavefilter, threshold, coarsen_threshold, c->d_flags);
