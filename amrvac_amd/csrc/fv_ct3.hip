@@ -132,8 +132,11 @@ __device__ void ct3_one_face(const SweepArgs& a, const CtArgs& t, const FVParams
   face_flux<P, R, RIEM, ID, LIM, 2, true>(st, fp, R(0), fC, crash, &io);
   const size_t c = gidx(i, j, k);
   R* fr = flux_ptr<P, R>(t, b, ID, 0) + c;
+  // the normal field's flux is read by nothing: the edge fields take the
+  // tangential components and the cell field is the face average
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) fr[iw * G3] = fC[iw];
+  for (int iw = 0; iw < NW; ++iw)
+    if (iw != P::MAG + ID) fr[iw * G3] = fC[iw];
   double* f = ctd_ptr<P>(t, b, ID, 0) + c;
   if (t.ct == CT_CONTACT) f[L::VN * G3] = 0.5 * (io.vL[ID] + io.vR[ID]);
   if (t.ct == CT_HLL) {
@@ -325,6 +328,7 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct3_update_kernel(SweepArgs a, 
     const size_t c1 = gidx(i, j, k), c0 = gidx(i - (d == 0), j - (d == 1), k - (d == 2));
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
+      if (iw >= P::MAG && iw < P::MAG + 3) continue;  // the field comes from the faces below
       const double f1 = a.dq[d] * double(f[iw * G3 + c1]), f0 = a.dq[d] * double(f[iw * G3 + c0]);
       acc[iw] = acc[iw] + f1 - f0;
     }
