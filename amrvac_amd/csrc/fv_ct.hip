@@ -96,7 +96,10 @@ __device__ void ct_one_face(const SweepArgs& a, const CtArgs& t, const FVParams&
   const size_t c = (size_t)(j + 2) * CTG + (i + 2);
   R* fr = reinterpret_cast<R*>(t.faces + (size_t)b * L::per_block) + (size_t)ID * NW * K + c;
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) fr[iw * K] = fC[iw];
+  // the normal field's flux is read by nothing: the edge fields take the
+  // tangential components and the cell field is the face average
+  for (int iw = 0; iw < NW; ++iw)
+    if (iw != P::MAG + ID) fr[iw * K] = fC[iw];
   double* f = t.faces + (size_t)b * L::per_block + 2 * NW * K + (size_t)ID * 5 * K + c;
   if (t.ct == CT_CONTACT) f[L::VN * K] = 0.5 * (io.vL[ID] + io.vR[ID]);
   if (t.ct == CT_HLL) {
@@ -285,13 +288,16 @@ __global__ void __launch_bounds__(fvc::NTHREADS) ct_update_kernel(SweepArgs a, C
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) acc[iw] = a.src[o + iw * cells];
   }
+  // B1 and B2 come from the faces below, B3 from its fluxes
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
+    if (iw == B1 || iw == B2) continue;
     const double fx1 = a.dq[0] * fget<P, R>(t, b, 0, iw, i, j), fx0 = a.dq[0] * fget<P, R>(t, b, 0, iw, i - 1, j);
     acc[iw] = acc[iw] + fx1 - fx0;
   }
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
+    if (iw == B1 || iw == B2) continue;
     const double fy1 = a.dq[1] * fget<P, R>(t, b, 1, iw, i, j), fy0 = a.dq[1] * fget<P, R>(t, b, 1, iw, i, j - 1);
     acc[iw] = acc[iw] + fy1 - fy0;
   }
