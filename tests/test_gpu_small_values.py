@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from amrvac_amd import problems
+from amrvac_amd.amr import AMRForest, AMRIntegrator, AMRPar
 from amrvac_amd.host import Par, TimeIntegrator
 from amrvac_amd.mesh import UniformMesh
 from oracle.oracle import OracleFV
@@ -87,3 +88,37 @@ def test_no_floor_no_crash_and_the_flag_clears():
     crashed, _, _ = _run(be2, par2, mesh)
     assert crashed and be2.crash()
     assert be2.crash(clear=True) and not be2.crash()
+
+
+def _amr_init(forest, blocks):
+    x = forest.cell_centers(blocks)
+    vx = 0.6 * np.tanh((x[:, 0] - 0.5) / 0.05)
+    one = np.ones_like(vx)
+    return problems.hd_to_conserved(np.stack([one, vx, 0 * one, 0.1 * one], axis=1), GAMMA, 2)
+
+
+def _amr_run(make_backend, nmax=300):
+    """The rarefaction on a two-level forest refined on m1 (Löhner every
+    step): the crash flag reaches the host with the estimate's flags."""
+    par = CASES["hd_fix_small_values"]
+    forest = AMRForest(2, (64, 16), (16, 16), (0.0, 0.0), (1.0, 0.25), ("cont", "cont", "periodic", "periodic"),
+                       max_level=2)
+    ti = AMRIntegrator(par, forest, AMRPar(refine_max_level=2, w_refine_weight={1: 1.0}), make_backend, _amr_init)
+    try:
+        ti.run(nsteps=nmax)
+    except RuntimeError as e:
+        assert "small value" in str(e)
+        return True, ti.state.it, [n.ig for n in forest.leaves], forest.interior(ti.be.get_state(0))
+    return False, ti.state.it, [n.ig for n in forest.leaves], forest.interior(ti.be.get_state(0))
+
+
+def test_amr_device_stops_at_the_oracles_crash_step():
+    from amrvac_amd.fv import DeviceFV
+    print("[small values] AMR, two levels", flush=True)
+    crashed_o, it_o, leaves_o, w_o = _amr_run(lambda p, f: OracleFV(p, f, threads=4))
+    assert crashed_o and 0 < it_o < 300
+    crashed_d, it_d, leaves_d, w_d = _amr_run(lambda p, f: DeviceFV(p, f, precision="fp64_exact"))
+    assert crashed_d and it_d == it_o and leaves_d == leaves_o
+    assert np.array_equal(w_d, w_o)
+    crashed_p, it_p, _, _ = _amr_run(lambda p, f: DeviceFV(p, f, precision="fp64"))
+    assert crashed_p and it_p == it_o
