@@ -94,15 +94,17 @@ def _amr_init(forest, blocks):
     x = forest.cell_centers(blocks)
     vx = 0.6 * np.tanh((x[:, 0] - 0.5) / 0.05)
     one = np.ones_like(vx)
-    return problems.hd_to_conserved(np.stack([one, vx, 0 * one, 0.1 * one], axis=1), GAMMA, 2)
+    zero = 0 * one
+    prim = [one, vx] + [zero] * (forest.ndim - 1) + [0.1 * one]
+    return problems.hd_to_conserved(np.stack(prim, axis=1), GAMMA, forest.ndim)
 
 
-def _amr_run(make_backend, nmax=300):
+def _amr_run(ndim, make_backend, nmax=300):
     """The rarefaction on a two-level forest refined on m1 (Löhner every
     step): the crash flag reaches the host with the estimate's flags."""
-    par = CASES["hd_fix_small_values"]
-    forest = AMRForest(2, (64, 16), (16, 16), (0.0, 0.0), (1.0, 0.25), ("cont", "cont", "periodic", "periodic"),
-                       max_level=2)
+    par = dataclasses.replace(CASES["hd_fix_small_values"], ndim=ndim, ndir=ndim)
+    forest = AMRForest(ndim, (64,) + (16,) * (ndim - 1), (16,) * ndim, (0.0,) * ndim, (1.0,) + (0.25,) * (ndim - 1),
+                       ("cont", "cont") + ("periodic", "periodic") * (ndim - 1), max_level=2)
     ti = AMRIntegrator(par, forest, AMRPar(refine_max_level=2, w_refine_weight={1: 1.0}), make_backend, _amr_init)
     try:
         ti.run(nsteps=nmax)
@@ -112,13 +114,14 @@ def _amr_run(make_backend, nmax=300):
     return False, ti.state.it, [n.ig for n in forest.leaves], forest.interior(ti.be.get_state(0))
 
 
-def test_amr_device_stops_at_the_oracles_crash_step():
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_amr_device_stops_at_the_oracles_crash_step(ndim):
     from amrvac_amd.fv import DeviceFV
-    print("[small values] AMR, two levels", flush=True)
-    crashed_o, it_o, leaves_o, w_o = _amr_run(lambda p, f: OracleFV(p, f, threads=4))
+    print(f"[small values] AMR, two levels, {ndim}-D", flush=True)
+    crashed_o, it_o, leaves_o, w_o = _amr_run(ndim, lambda p, f: OracleFV(p, f, threads=4))
     assert crashed_o and 0 < it_o < 300
-    crashed_d, it_d, leaves_d, w_d = _amr_run(lambda p, f: DeviceFV(p, f, precision="fp64_exact"))
+    crashed_d, it_d, leaves_d, w_d = _amr_run(ndim, lambda p, f: DeviceFV(p, f, precision="fp64_exact"))
     assert crashed_d and it_d == it_o and leaves_d == leaves_o
     assert np.array_equal(w_d, w_o)
-    crashed_p, it_p, _, _ = _amr_run(lambda p, f: DeviceFV(p, f, precision="fp64"))
+    crashed_p, it_p, _, _ = _amr_run(ndim, lambda p, f: DeviceFV(p, f, precision="fp64"))
     assert crashed_p and it_p == it_o
