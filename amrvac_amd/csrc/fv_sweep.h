@@ -437,7 +437,11 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
       constexpr int NXF = BX * (BX + 1);             // x faces (272 for 16^2)
       constexpr int XSLOTS = (NXF + 63) / 64 * 64;   // padded to a wave multiple
       constexpr int NSLOT = XSLOTS + NXF;            // + the y faces
-      for (int t = tid; t < NSLOT; t += NTHREADS) {
+      // FP64: the waves that take the third, partial pass change from plane
+      // to plane, so that the co-resident workgroups' heavy waves spread
+      // over the SIMDs (measured 1.6% faster; the FP32 form is not helped)
+      const int t0 = std::is_same<R, double>::value ? (tid + 64 * (k & 3)) % NTHREADS : tid;
+      for (int t = t0; t < NSLOT; t += NTHREADS) {
         if (t < NXF) {
           const int y = t / (BX + 1), f = t % (BX + 1);  // face between f-1 and f
           R st[2 * NG][NW];
