@@ -398,8 +398,10 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
         for (int iw = 0; iw < NW; ++iw) sp[iw][y + NG][x + NG] = hring[slot][iw][t];
       }
     } else {
-      // halo strips of plane k
-      for (int t = tid; t < NHALO; t += NTHREADS) {
+      // halo strips of plane k (FP64: loaded by alternate wave pairs from
+      // plane to plane)
+      const int th = std::is_same<R, double>::value && NTHREADS == 256 ? (tid + 128 * (k & 1)) % NTHREADS : tid;
+      for (int t = th; t < NHALO; t += NTHREADS) {
         int x, y;
         strip_cell<NG, BX>(t, x, y);
         R w[NW];
