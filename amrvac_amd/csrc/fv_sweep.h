@@ -415,7 +415,9 @@ sweep_kernel(SweepArgs a, FVParams fp0) {
       // halo of plane k+NG into the ring slot just copied out
       if (k + NG < nz) {
         const int slot = k % NG;
-        for (int t = tid; t < NSTRIP; t += NTHREADS) {
+        // by the upper waves: the lower ones take the partial face pass
+        const int th = NTHREADS == 256 ? (tid + 128) % NTHREADS : tid;
+        for (int t = th; t < NSTRIP; t += NTHREADS) {
           int x, y;
           strip_cell<NG, BX>(t, x, y);
           R w[NW];
